@@ -1,0 +1,138 @@
+/*
+ * spand.h — C ABI of the MI355X (gfx950) spaND factorization and preconditioner.
+ *
+ * This is the drop-in boundary for the reference's Tree API and its Krylov callers
+ * (leopoldcambier/spaND_public). Each entry point names the reference interface it
+ * replaces. Plain pointers and sizes only; no C++ exceptions cross this boundary: every
+ * call returns a spand_status and spand_last_error() holds the message of the last
+ * failure on that handle (the reference throws std::runtime_error or asserts).
+ *
+ * Threading: a tree handle is not reentrant (like the reference, whose const solve()
+ * mutates per-cluster buffers, src/cluster.cpp:82-96). Use one handle per host thread.
+ * Matrices are CSC with 0-based int32 indices and sorted rows, the layout of
+ * Eigen::SparseMatrix<double, ColMajor, int> (include/spaND.h:59). Right-hand sides are
+ * column-major with a leading dimension.
+ */
+#ifndef SPAND_H
+#define SPAND_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SPAND_OK = 0,
+    SPAND_ERR_ARG = 1,
+    SPAND_ERR_NOT_SPD = 2,        /* non-SPD pivot: "Error: Non-SPD Pivot" (src/tree.cpp:587-590) */
+    SPAND_ERR_SINGULAR = 3,       /* singular pivot (src/tree.cpp:625-628) */
+    SPAND_ERR_HIP = 4,            /* HIP runtime failure */
+    SPAND_ERR_STATE = 5,          /* call out of order (partition -> assemble -> factorize -> solve) */
+    SPAND_ERR_NOMEM = 6,          /* device allocation failed */
+    SPAND_ERR_NOT_CONVERGED = 7,  /* Krylov iteration cap reached before the tolerance */
+    SPAND_ERR_UNSUPPORTED = 8     /* option not built (algebraic partitioning, GEN/PLU, ...) */
+} spand_status;
+
+enum { SPAND_SYMM_SPD = 0, SPAND_SYMM_SYM = 1, SPAND_SYMM_GEN = 2 }; /* SymmKind, include/spaND.h:21 */
+enum { SPAND_F64 = 0, SPAND_F32 = 1 };                               /* factor storage / arithmetic */
+
+typedef struct spand_tree spand_tree;
+typedef struct spand_matrix spand_matrix;
+
+/* Tree::Tree(int nlevels)                          include/tree.h:163 */
+spand_status spand_create(int nlevels, spand_tree** out);
+void spand_destroy(spand_tree* t);
+/* Message of the last failing call on t (or of the last failing spand_create/matrix call
+ * when t is NULL). */
+const char* spand_last_error(const spand_tree* t);
+
+/* Tree::set_tol / set_skip / set_verb / set_symm_kind / set_use_geo / set_Xcoo /
+ * set_use_sparsify                                  include/tree.h:133-147 */
+spand_status spand_set_tol(spand_tree* t, double tol);
+spand_status spand_set_skip(spand_tree* t, int skip);
+spand_status spand_set_verb(spand_tree* t, int verb);
+spand_status spand_set_symm_kind(spand_tree* t, int symm_kind);
+spand_status spand_set_use_geo(spand_tree* t, int use_geo);
+/* X is dim x N, column-major (the reference's Eigen::MatrixXd Xcoo). Copied. */
+spand_status spand_set_Xcoo(spand_tree* t, int dim, int N, const double* X);
+spand_status spand_set_use_sparsify(spand_tree* t, int use);
+/* No reference counterpart: element type of the factorization (SPAND_F64 / SPAND_F32).
+ * FP32 factors are corrected by the FP64 Krylov iteration. */
+spand_status spand_set_precision(spand_tree* t, int precision);
+
+/* Tree::partition(SpMat&) (include/tree.h:169, src/tree.cpp:306-419). The pattern must be
+ * symmetric (pass |A|+|A^T|+I as tests/spaND.cpp:237 does). part_out (nullable) receives
+ * 6 ints per dof: self.lvl, self.sep, left.lvl, left.sep, right.lvl, right.sep
+ * (the returned std::vector<ClusterID>). */
+spand_status spand_partition(spand_tree* t, int N, const int* colptr, const int* rowval, int* part_out);
+/* Tree::assemble(SpMat&)                           include/tree.h:175, src/tree.cpp:505-575 */
+spand_status spand_assemble(spand_tree* t, int N, const int* colptr, const int* rowval, const double* vals);
+/* Tree::factorize()                                include/tree.h:179, src/tree.cpp:1447-1551
+ * May be called again after a completed factorization (refactorization of the assembled
+ * matrix; no device allocation once the arenas have grown). */
+spand_status spand_factorize(spand_tree* t);
+/* Tree::solve(VectorXd&) const                     include/tree.h:185, src/tree.cpp:1610-1635
+ * x <- M^{-1} x in place for nrhs columns of a host array (ld >= N). */
+spand_status spand_solve(spand_tree* t, double* x, int nrhs, int ld);
+/* Same on device memory, ordered on `stream` (a hipStream_t; NULL = default stream). */
+spand_status spand_solve_device(spand_tree* t, double* d_x, int nrhs, int ld, void* stream);
+
+/* Tree::get_N / get_nlevels / get_stop / nnz / get_assembly_perm   include/tree.h:150-160 */
+spand_status spand_get_N(const spand_tree* t, int* N);
+spand_status spand_get_nlevels(const spand_tree* t, int* nlevels);
+spand_status spand_get_stop(const spand_tree* t, int* stop);
+spand_status spand_nnz(const spand_tree* t, long long* nnz);
+spand_status spand_get_assembly_perm(const spand_tree* t, int* perm);
+/* Tree::get_trailing_mat (src/tree.cpp:1730-1763) as COO of the lower triangle (SPD).
+ * Call with I == NULL to get the count in *nnz first. */
+spand_status spand_get_trailing_coo(const spand_tree* t, long long* nnz, int* I, int* J, double* V);
+
+/* Per-level log (Tree::log, include/util.h:366-401 and print_log, src/tree.cpp:1645-1727):
+ * 20 doubles per level:
+ *  0 dofs_nd, 1 dofs_left_nd, 2 dofs_left_elim, 3 dofs_left_spars, 4 fact_nnz,
+ *  5 rank_before (sum), 6 rank_after (sum), 7 #sparsified, 8 #ignored, 9 #dependency waves,
+ * 10 t_plan, 11 t_elim+scale+rrqr (device), 12 t_rank_apply, 13 t_merge (seconds),
+ * 14 potrf flops, 15 trsm flops, 16 schur flops, 17 rrqr flops, 18 reserved, 19 reserved */
+spand_status spand_get_level_stats(const spand_tree* t, double* out);
+/* Wall seconds of the last spand_factorize, device bytes of the stored factor, of the arena. */
+spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long long* factor_bytes,
+                                   long long* arena_bytes);
+
+/* Recorded operations (the Operation list, include/operations.h), for op-by-op comparison.
+ * 8 ints per op: type (0 ScalingLLT, 2 GemmSymmOut, 3 GemmSymmIn, 6 Orthogonal, 7 Merge,
+ * 8 Split), order(a), order(b), size(a), size(b), rows, cols, level. */
+spand_status spand_num_ops(const spand_tree* t, int* nops);
+spand_status spand_ops_info(const spand_tree* t, int* info);
+/* Copies op i's matrix (rows x cols, column-major, as FP64) and its tau (Orthogonal). */
+spand_status spand_op_data(const spand_tree* t, int i, double* M, double* tau);
+
+/* Device copy of a square CSC matrix, used as the Krylov operator (the SpMat argument of
+ * cg / gmres, include/is.h:12-14). */
+spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, const double* vals,
+                                 spand_matrix** out);
+void spand_matrix_destroy(spand_matrix* A);
+
+typedef struct {
+    int iterations;       /* max over right-hand sides of the reference's count (i+1) */
+    int converged;        /* number of columns that met the tolerance */
+    double relres_max;    /* max over columns of the true ||b - A x|| / ||b|| (FP64) */
+    double seconds;       /* wall time of the solve */
+    double seconds_precond;
+    double seconds_spmv;
+    int applies;          /* preconditioner applications */
+} spand_krylov_stats;
+
+/* cg(mat, rhs, x, precond, iters, tol)             include/is.h:13, src/is.cpp:39-121
+ * Preconditioned CG on nrhs right-hand sides at once, FP64 throughout (the
+ * preconditioner's own precision is the tree's). Stops a column when its recursive
+ * residual satisfies ||r||^2 < tol^2 ||b||^2, as the reference does. b and x (in: x0,
+ * out: solution) are column-major with leading dimension ld, on the host (on_device = 0)
+ * or on the device (on_device = 1). iters_out (nullable, nrhs ints) receives each
+ * column's count. */
+spand_status spand_pcg(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld,
+                       int maxit, double tol, int on_device, int* iters_out, spand_krylov_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPAND_H */

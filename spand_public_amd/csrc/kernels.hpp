@@ -1,0 +1,64 @@
+// Host-side launchers of the HIP kernels (kernels_factor.hip, kernels_solve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "tasks.hpp"
+
+namespace spand {
+
+// Dynamic LDS granted to the staged factorization kernels (bytes). Static LDS of the
+// RRQR kernel (about 27 KB) comes on top; the total stays below 64 KB so that two
+// workgroups can share a CU.
+constexpr int LDS_BYTES = 32768;
+
+template <typename T> hipError_t launch_fill(T* p, size_t n, T v, hipStream_t st);
+// base[off[k]] = (T) val[k]: level-0 assembly of the permuted matrix into its blocks.
+template <typename T>
+hipError_t launch_scatter(const long long* off, const double* val, long long n, T* base, hipStream_t st);
+template <typename T> hipError_t launch_potrf(const PotrfTask* d_tasks, int ntasks, int* d_info, hipStream_t st);
+template <typename T> hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st);
+template <typename T>
+hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib* d_contribs, hipStream_t st);
+template <typename T>
+hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
+                       hipStream_t st);
+template <typename T> hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st);
+
+// Preconditioner sweeps. ws holds the per-cluster segments, interleaved over nrhs.
+// Element (i, r) of the caller's block is at b[i * sx + r * sr].
+template <typename T>
+hipError_t launch_perm_in(const double* b, size_t sx, size_t sr, int nrhs, const int* perm, int N, T* ws,
+                          hipStream_t st);
+template <typename T>
+hipError_t launch_perm_out(const T* ws, const int* perm, int N, int nrhs, double* x, size_t sx, size_t sr,
+                           hipStream_t st);
+hipError_t launch_cm2il(const double* src, int ld, int N, int nrhs, double* dst, hipStream_t st);
+hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld, hipStream_t st);
+template <typename T>
+hipError_t launch_trsv(const TrsvTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
+template <typename T>
+hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, const GemvTerm* d_terms, T* ws, int nrhs,
+                       hipStream_t st);
+template <typename T>
+hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
+template <typename T>
+hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, T* ws, int nrhs, int fwd,
+                        hipStream_t st);
+
+// FP64 Krylov vector kernels (interleaved N x nrhs).
+hipError_t launch_spmv(int N, const int* rowptr, const int* colidx, const double* val, const double* x, double* y,
+                       int nrhs, hipStream_t st);
+// partial[b * nrhs + r] = sum over block b of a[i,r] * b[i,r]; out[r] = sum_b partial
+hipError_t launch_dot(int N, int nrhs, const double* a, const double* b, double* partial, double* out,
+                      hipStream_t st);
+// y[:, r] = x[:, r] * sx + y[:, r] * sy with per-column device scalars (nullptr => 1).
+hipError_t launch_axpby(int N, int nrhs, const double* x, const double* ax, double sgn_ax, double* y,
+                        const double* by, hipStream_t st);
+// alpha = active ? num / den : 0
+hipError_t launch_ratio(int nrhs, const double* num, const double* den, const int* active, double* out,
+                        hipStream_t st);
+hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t st);
+
+}  // namespace spand

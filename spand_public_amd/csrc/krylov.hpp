@@ -1,0 +1,34 @@
+// Device FP64 Krylov solvers preconditioned by the spaND tree (src/is.cpp).
+#pragma once
+#include <vector>
+
+#include "tree.hpp"
+
+namespace spand {
+
+// Square sparse operator on the device (CSR of A built from the caller's CSC).
+struct DevMatrix {
+    int N = 0;
+    long long nnz = 0;
+    int* rowptr = nullptr;
+    int* colidx = nullptr;
+    double* val = nullptr;
+    DevMatrix(int n, const int* colptr, const int* rowval, const double* vals);
+    ~DevMatrix();
+    DevMatrix(const DevMatrix&) = delete;
+    DevMatrix& operator=(const DevMatrix&) = delete;
+};
+
+struct KrylovResult {
+    std::vector<int> iters;
+    std::vector<double> relres;
+    int converged = 0;
+    double seconds = 0, seconds_precond = 0, seconds_spmv = 0;
+    int applies = 0;
+};
+
+// cg (src/is.cpp:39-121) on nrhs columns; b, x column-major with leading dimension ld.
+KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, double tol,
+                 bool on_device);
+
+}  // namespace spand

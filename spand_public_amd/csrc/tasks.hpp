@@ -1,0 +1,112 @@
+// Task descriptors shared by the host level planner (tree.cpp) and the level-batched HIP
+// kernels (kernels.hip). Every kernel launch processes an array of these, one workgroup per
+// task (or per task tile), so one launch covers every cluster / edge of one ND level, of
+// uneven sizes. Blocks are column-major with ld = rows, in the element type of the
+// factorization (float or double); pointers are device addresses in the block arena.
+#pragma once
+#include <cstdint>
+
+namespace spand {
+
+// In-place lower Cholesky of an n x n pivot block (reference: potf, src/util.cpp:158-165).
+struct PotrfTask {
+    void* A;
+    int n;
+    int cluster;  // reported through the info array on a non-SPD pivot
+};
+
+// Triangular solve of an edge block against a pivot factor L (n x n, lower):
+//   side 0: B (n x m, ld n) := L^{-1} B      (trsm_left,  src/util.cpp:266-274)
+//   side 1: B (m x n, ld m) := B L^{-T}      (trsm_right, src/util.cpp:256-264)
+struct TrsmTask {
+    const void* L;
+    void* B;
+    int n;
+    int m;
+    int side;
+    int pad;
+};
+
+// One Schur-complement target tile: C[r0:r0+tm, c0:c0+tn] of a block C (ldc rows) receives
+// contributions [k0, k1) in order, each C -= opA(A) opB(B) (gemm / syrk, src/util.cpp:122-156).
+// lower = 1 restricts the update to the lower triangle (syrk on a diagonal block).
+struct SchurTile {
+    void* C;
+    int ldc;
+    int r0, c0, tm, tn;
+    int lower;
+    int k0, k1;
+};
+struct SchurContrib {
+    const void* A;  // opA(A) is m x k: ta ? A^T (A is k x m, ld lda) : A (m x k, ld lda)
+    const void* B;  // opB(B) is k x n: tb ? B^T (B is n x k, ld ldb) : B (k x n, ld ldb)
+    int k;
+    int lda, ldb;
+    int ta, tb;
+    int pad;
+};
+
+// Column-pivoted QR sparsification of one cluster (sparsify_adaptive_only,
+// src/tree.cpp:1292-1347). Its Asn = [A_sn(in edges) | A_ns^T(out edges)] is gathered
+// from segs[seg0:seg0+nseg] using the live sizes in dsize[], factored (dgeqp3 order),
+// truncated by choose_rank and scattered back into the edge blocks.
+struct RrqrTask {
+    int cluster;
+    int seg0, nseg;
+    int max_rows, max_cols;  // capacity of the scratch (rows x cols) and Q (rows x rows)
+    int pad;
+    void* scratch;     // T[max_rows * max_cols]
+    double* norms;     // double[2 * max_cols]
+    int* jpvt;         // int[max_cols]
+    void* Q;           // T[max_rows * min(max_rows, max_cols)] (Householder vectors)
+    void* tau;         // T[min(max_rows, max_cols)]
+};
+struct RrqrSeg {
+    void* E;   // edge block
+    int nbr;   // neighbour cluster (its live size gives the segment width)
+    int out;   // 0: in-edge, block is s x n; 1: out-edge, block is n x s (transposed)
+};
+
+// Merge copy of one child block into a parent block (update_edges, src/tree.cpp:1133-1184).
+struct CopyTask {
+    const void* src;  // nullptr => identity block
+    void* dst;
+    int rows, cols;
+    int ldd;
+    int pad;
+};
+
+// ---------------------------------------------------------------------------------------
+// Preconditioner apply (Tree::solve, src/tree.cpp:1610-1635). Vectors are stored
+// interleaved: element i of right-hand side r lives at x[(xoff + i) * nrhs + r].
+struct TrsvTask {  // x := L^{-1} x (fwd) / L^{-T} x (bwd), ScalingLLT (src/operations.cpp:95-103)
+    const void* L;
+    int n;
+    int xoff;
+};
+// Gemm ops (src/operations.cpp:31-93) grouped by the segment they modify:
+// x[xdst] -= sum_k op(A_k) x[xsrc_k]   with op = A (trans 0) or A^T (trans 1).
+struct GemvGroup {
+    int xdst, n;  // destination segment and length
+    int k0, k1;   // term range
+};
+struct GemvTerm {
+    const void* A;
+    int rows, cols;  // A is rows x cols (ld rows)
+    int xsrc;
+    int trans;
+};
+struct QTask {  // Orthogonal (src/operations.cpp:149-158): x := Q^T x (fwd) / Q x (bwd)
+    const void* V;
+    const void* tau;
+    int m, k;
+    int xoff;
+    int pad;
+};
+struct MergeTask {  // Merge (src/operations.cpp:166-192)
+    int pxoff;
+    int c0, c1;  // range into (child xoff, child size) pairs
+    int pad;
+};
+
+}  // namespace spand

@@ -1,0 +1,966 @@
+// spaND factorization and preconditioner apply on one GPU.
+//
+// Host side of the reference's Tree::assemble / factorize / solve (src/tree.cpp:505-575,
+// 1447-1551, 1610-1635). The host mirrors the reference's structural bookkeeping exactly
+// (edge list order, fill-in creation order, cluster orders, sibling creation), which fixes
+// the column order of every RRQR and the op order of the solve. Numerical work is planned
+// per ND level and launched as level-batched kernels:
+//   Elim:   potrf(all pivots) -> trsm(all panels) -> Schur(all targets, ordered accumulation)
+//   Scale:  potrf(all pivots) -> trsm by the lower-order end of every edge -> trsm by the
+//           higher-order end (the reference's in-order rounding, src/tree.cpp:1512-1516)
+//   Sparsify: dependency waves (a cluster waits for its earlier-ordered sparsified
+//           neighbours, src/tree.cpp:1013-1017,1041-1045), one RRQR launch per wave
+//   Merge:  zero-fill + block copies into the parents' blocks
+// with one host<->device synchronization per level (the ranks decide the next level's sizes).
+#include "tree.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "kernels.hpp"
+
+namespace spand {
+
+static double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) throw Error(ERR_NOMEM, std::string(what) + ": out of device memory");
+        throw Error(ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+size_t arena_chunk_bytes(size_t dflt) {
+    const char* e = std::getenv("SPAND_ARENA_CHUNK_BYTES");
+    if (!e || !*e) return dflt;
+    return (size_t)std::strtoull(e, nullptr, 10);
+}
+
+DeviceArena::~DeviceArena() {
+    for (auto& c : chunks_) (void)hipFree(c.p);
+}
+void* DeviceArena::alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    if (chunk_bytes_ == 0) {
+        // one allocation per block (bounds-checking builds); kept until the arena dies
+        Chunk c;
+        hip_check(hipMalloc((void**)&c.p, bytes), "hipMalloc(arena block)");
+        c.size = c.used = bytes;
+        chunks_.push_back(c);
+        in_use_ += bytes;
+        return c.p;
+    }
+    while (cur_ < chunks_.size()) {
+        Chunk& c = chunks_[cur_];
+        if (c.used + bytes <= c.size) {
+            void* p = c.p + c.used;
+            c.used += bytes;
+            in_use_ += bytes;
+            return p;
+        }
+        cur_++;
+    }
+    Chunk c;
+    c.size = std::max(chunk_bytes_, bytes);
+    hip_check(hipMalloc((void**)&c.p, c.size), "hipMalloc(arena chunk)");
+    c.used = bytes;
+    chunks_.push_back(c);
+    cur_ = chunks_.size() - 1;
+    in_use_ += bytes;
+    return c.p;
+}
+void DeviceArena::reset() {
+    if (chunk_bytes_ == 0) {
+        for (auto& c : chunks_) (void)hipFree(c.p);
+        chunks_.clear();
+    }
+    for (auto& c : chunks_) c.used = 0;
+    cur_ = 0;
+    in_use_ = 0;
+}
+size_t DeviceArena::bytes_reserved() const {
+    size_t s = 0;
+    for (auto& c : chunks_) s += c.size;
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------
+Tree::Tree(int nlevels) : nlevels_(nlevels), stats_(nlevels) {
+    if (nlevels <= 0) throw Error(ERR_ARG, "nlevels must be positive");
+}
+
+Tree::~Tree() {
+    (void)hipFree(d_scatter_off_);
+    (void)hipFree(d_scatter_val_);
+    (void)hipFree(d_size_);
+    (void)hipFree(d_info_);
+    (void)hipFree(d_perm_);
+    (void)hipFree(d_ws_);
+}
+
+void Tree::set_symm_kind(int k) {
+    if (k != 0) throw Error(ERR_UNSUPPORTED, "only the SPD / LLT path is available");
+    symm_ = k;
+}
+void Tree::set_Xcoo(int dim, int N, const double* X) {
+    if (dim <= 0 || N <= 0 || !X) throw Error(ERR_ARG, "set_Xcoo: bad arguments");
+    dim_ = dim;
+    X_.assign(X, X + (size_t)dim * N);
+}
+void Tree::set_precision(int p) {
+    if (p != 0 && p != 1) throw Error(ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+    prec_ = p;
+}
+
+int Tree::find_out_edge(int from, int to) const {
+    for (int e : cls_[from].out)
+        if (edges_[e].n2 == to) return e;
+    return -1;
+}
+
+// Cluster::add_edge (src/cluster.cpp:113-123): the pivot goes first, others are appended.
+int Tree::new_edge(int n1, int n2, void* A, bool original) {
+    int id = (int)edges_.size();
+    HEdge e;
+    e.n1 = n1;
+    e.n2 = n2;
+    e.A = A;
+    e.original = original;
+    edges_.push_back(e);
+    if (n1 == n2) {
+        cls_[n1].out.insert(cls_[n1].out.begin(), id);
+    } else {
+        cls_[n1].out.push_back(id);
+        cls_[n2].in.push_back(id);
+    }
+    return id;
+}
+
+// Cluster::set_eliminated (src/cluster.cpp:32-45)
+void Tree::set_eliminated(int c) {
+    HCluster& s = cls_[c];
+    for (int e : s.out) {
+        int n = edges_[e].n2;
+        if (n == c) continue;
+        auto& v = cls_[n].in;
+        v.erase(std::remove_if(v.begin(), v.end(), [&](int x) { return edges_[x].n1 == c; }), v.end());
+    }
+    for (int e : s.in) {
+        int n = edges_[e].n1;
+        auto& v = cls_[n].out;
+        v.erase(std::remove_if(v.begin(), v.end(), [&](int x) { return edges_[x].n2 == c; }), v.end());
+    }
+    s.out.clear();
+    s.in.clear();
+    s.elim = true;
+}
+
+void* Tree::upload_tasks(const std::vector<char>& host) {
+    if (host.empty()) return nullptr;
+    void* d = task_arena_.alloc(host.size());
+    hip_check(hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice), "upload tasks");
+    return d;
+}
+
+// Packs several task arrays into one host buffer (one upload per batch of launches).
+struct Packer {
+    std::vector<char> buf;
+    template <class X>
+    size_t add(const std::vector<X>& v) {
+        size_t off = (buf.size() + 255) & ~(size_t)255;
+        buf.resize(off + v.size() * sizeof(X));
+        if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(X));
+        return off;
+    }
+    template <class X>
+    static X* at(void* base, size_t off) {
+        return reinterpret_cast<X*>(reinterpret_cast<char*>(base) + off);
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// Tree::assemble (src/tree.cpp:505-575): P^T A P cut into dense blocks, one edge per
+// (leaf cluster, neighbour cluster) pair with order(nbr) >= order(self) (symmetric). The
+// dense blocks are never materialized on the host: the device zero-fills the level-0
+// region and scatters the nonzeros to precomputed offsets.
+void Tree::assemble(int N, const int* colptr, const int* rowval, const double* vals) {
+    if (state_ != State::Partitioned) throw Error(ERR_STATE, "assemble() must follow partition()");
+    if (N != N_) throw Error(ERR_ARG, "assemble: matrix size differs from the partitioned one");
+    std::vector<int> pinv(N);
+    for (int i = 0; i < N; i++) pinv[perm_[i]] = i;
+    const long long nnz = colptr[N];
+    // permuted CSC with sorted rows
+    std::vector<long long> cp(N + 1, 0);
+    for (int j = 0; j < N; j++) cp[pinv[j] + 1] += colptr[j + 1] - colptr[j];
+    for (int j = 0; j < N; j++) cp[j + 1] += cp[j];
+    std::vector<std::pair<int, double>> ent(nnz);
+    {
+        std::vector<long long> fill(cp.begin(), cp.end() - 1);
+        for (int j = 0; j < N; j++) {
+            int pj = pinv[j];
+            for (int k = colptr[j]; k < colptr[j + 1]; k++) ent[fill[pj]++] = {pinv[rowval[k]], vals[k]};
+        }
+        for (int j = 0; j < N; j++)
+            std::sort(ent.begin() + cp[j], ent.begin() + cp[j + 1],
+                      [](const auto& a, const auto& b) { return a.first < b.first; });
+    }
+    std::vector<int> cmap(N);
+    for (int c : bottoms_[0])
+        for (int k = cls_[c].start; k < cls_[c].start + cls_[c].size; k++) cmap[k] = c;
+    const bool symmetric = symm_ == 0;
+    edges_.clear();
+    for (auto& c : cls_) {
+        c.out.clear();
+        c.in.clear();
+    }
+    edge0_off_.clear();
+    level0_elems_ = 0;
+    std::vector<long long> soff;
+    std::vector<double> sval;
+    soff.reserve(nnz);
+    sval.reserve(nnz);
+    std::vector<int> edge_to(cls_.size(), -1);
+    for (int self : bottoms_[0]) {
+        HCluster& s = cls_[self];
+        std::vector<int> nbrs{self};
+        for (int j = s.start; j < s.start + s.size; j++)
+            for (long long k = cp[j]; k < cp[j + 1]; k++) {
+                int i = ent[k].first;
+                if (symmetric && i < j) continue;
+                nbrs.push_back(cmap[i]);
+            }
+        std::sort(nbrs.begin(), nbrs.end());
+        nbrs.erase(std::unique(nbrs.begin(), nbrs.end()), nbrs.end());
+        for (int n : nbrs) {
+            int e = new_edge(self, n, nullptr, true);
+            edge_to[n] = e;
+            edge0_off_.push_back(level0_elems_);
+            level0_elems_ += (long long)cls_[n].size * s.size;
+        }
+        // Cluster::sort_edges (src/cluster.cpp:162-176)
+        auto& out = s.out;
+        std::stable_sort(out.begin() + 1, out.end(), [&](int a, int b) { return edges_[a].n2 < edges_[b].n2; });
+        std::stable_sort(s.in.begin(), s.in.end(), [&](int a, int b) { return edges_[a].n1 < edges_[b].n1; });
+        for (int j = s.start; j < s.start + s.size; j++)
+            for (long long k = cp[j]; k < cp[j + 1]; k++) {
+                int i = ent[k].first;
+                int n = cmap[i];
+                if (symmetric && i < j && n != self) continue;
+                int e = edge_to[n];
+                if (e < 0 || edges_[e].n1 != self) continue;
+                soff.push_back(edge0_off_[e] + (i - cls_[n].start) + (long long)(j - s.start) * cls_[n].size);
+                sval.push_back(ent[k].second);
+            }
+        for (int n : nbrs) edge_to[n] = -1;
+    }
+    nscatter_ = (long long)soff.size();
+    h_scatter_off_.swap(soff);
+    h_scatter_val_.swap(sval);
+    device_inputs_ = false;
+    // ND statistics (src/tree.cpp:336-342)
+    for (auto& st : stats_) st = LevelStats();
+    for (int i = 0; i < N; i++) stats_[part_[(size_t)6 * i]].dofs_nd++;
+    for (int l = nlevels_ - 2; l >= 0; l--) stats_[l].dofs_left_nd = stats_[l + 1].dofs_nd + stats_[l + 1].dofs_left_nd;
+    cls0_ = cls_;
+    edges0_ = edges_;
+    state_ = State::Assembled;
+}
+
+// ---------------------------------------------------------------------------------------
+// Device copies of the assembly scatter map and of the permutation (first factorize()).
+void Tree::upload_inputs() {
+    if (device_inputs_) return;
+    (void)hipFree(d_scatter_off_);
+    (void)hipFree(d_scatter_val_);
+    (void)hipFree(d_perm_);
+    d_scatter_off_ = nullptr;
+    d_scatter_val_ = nullptr;
+    d_perm_ = nullptr;
+    if (nscatter_ > 0) {
+        hip_check(hipMalloc((void**)&d_scatter_off_, nscatter_ * sizeof(long long)), "hipMalloc(scatter)");
+        hip_check(hipMalloc((void**)&d_scatter_val_, nscatter_ * sizeof(double)), "hipMalloc(scatter)");
+        hip_check(hipMemcpy(d_scatter_off_, h_scatter_off_.data(), nscatter_ * sizeof(long long),
+                            hipMemcpyHostToDevice),
+                  "upload scatter");
+        hip_check(hipMemcpy(d_scatter_val_, h_scatter_val_.data(), nscatter_ * sizeof(double), hipMemcpyHostToDevice),
+                  "upload scatter");
+    }
+    hip_check(hipMalloc((void**)&d_perm_, (size_t)N_ * sizeof(int)), "hipMalloc(perm)");
+    hip_check(hipMemcpy(d_perm_, perm_.data(), (size_t)N_ * sizeof(int), hipMemcpyHostToDevice), "upload perm");
+    device_inputs_ = true;
+}
+
+void Tree::factorize() {
+    if (state_ != State::Assembled && state_ != State::Factorized)
+        throw Error(ERR_STATE, "factorize() must follow assemble()");
+    upload_inputs();
+    if (prec_ == 0)
+        factorize_impl<double>();
+    else
+        factorize_impl<float>();
+    state_ = State::Factorized;
+}
+
+static inline double potrf_flops(double n) { return n * n * n / 3.0; }
+
+template <typename T>
+void Tree::factorize_impl() {
+    const double t_start = wall();
+    const hipStream_t st = 0;
+    arena_.reset();
+    task_arena_.reset();
+    ops_.clear();
+    cls_ = cls0_;
+    edges_ = edges0_;
+    others_.clear();
+    current_bottom_ = 0;
+    solve_plan_.assign(nlevels_, LevelSolve());
+    for (auto& s : stats_) {
+        int nd = s.dofs_nd, nl = s.dofs_left_nd;
+        s = LevelStats();
+        s.dofs_nd = nd;
+        s.dofs_left_nd = nl;
+    }
+    factor_bytes_ = 0;
+    // level-0 blocks
+    T* base0 = reinterpret_cast<T*>(arena_.alloc((size_t)std::max<long long>(level0_elems_, 1) * sizeof(T)));
+    hip_check(launch_fill<T>(base0, (size_t)level0_elems_, T(0), st), "fill level0");
+    if (nscatter_ > 0) hip_check(launch_scatter<T>(d_scatter_off_, d_scatter_val_, nscatter_, base0, st), "scatter");
+    for (size_t e = 0; e < edges_.size(); e++) edges_[e].A = base0 + edge0_off_[e];
+    // device size table (room for one sibling per cluster)
+    const int cap = (int)(2 * cls_.size() + 16);
+    if (cap > d_size_cap_) {
+        (void)hipFree(d_size_);
+        hip_check(hipMalloc((void**)&d_size_, cap * sizeof(int)), "hipMalloc(size)");
+        d_size_cap_ = cap;
+    }
+    if (!d_info_) hip_check(hipMalloc((void**)&d_info_, sizeof(int)), "hipMalloc(info)");
+    hip_check(hipMemset(d_info_, 0, sizeof(int)), "memset info");
+    std::vector<int> hsize(cap, 0);
+    for (int c : bottoms_[0]) cls_[c].xoff = cls_[c].start;
+    ws_len_ = N_;
+    const int lds_elems = LDS_BYTES / (int)sizeof(T);
+
+    for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
+        LevelStats& lst = stats_[ilvl_];
+        LevelSolve& lsv = solve_plan_[ilvl_];
+        const double t_lvl0 = wall();
+        const std::vector<int>& bot = bottoms_[current_bottom_];
+        // ---------------- Elim (src/tree.cpp:1494-1506, 895-967) ----------------
+        std::vector<PotrfTask> potrfE, potrfS;
+        std::vector<TrsmTask> trsmE, trsmA, trsmB;
+        struct PC { int target; SchurContrib c; };
+        std::vector<PC> pcs;
+        std::vector<int> fill_edges;
+        long long fill_elems = 0;
+        std::vector<long long> fill_off;  // by position in fill_edges
+        std::vector<TrsvTask> sv_elim, sv_scale;
+        std::unordered_map<int, std::vector<GemvTerm>> fwd_terms;  // by destination cluster
+        std::vector<int> fwd_order;
+        std::vector<GemvGroup> bwd_groups;
+        std::vector<GemvTerm> bwd_terms;
+        auto target_edge = [&](int n1, int n2) {
+            int e = find_out_edge(n2, n1);
+            if (e < 0) {
+                e = new_edge(n2, n1, nullptr, false);
+                fill_edges.push_back(e);
+                fill_off.push_back(fill_elems);
+                fill_elems += (long long)size(n1) * size(n2);
+            }
+            return e;
+        };
+        for (int c : bot) {
+            if (cls_[c].level != ilvl_) continue;
+            HCluster& s = cls_[c];
+            const int n = s.size;
+            // edges_ may grow (fill-in) below: keep the pivot block pointer, not a reference
+            const HEdge piv = edges_[s.out[0]];
+            if (piv.ident) throw Error(ERR_STATE, "internal: identity pivot on an eliminated cluster");
+            potrfE.push_back({piv.A, n, c});
+            lst.flops_potrf += potrf_flops(n);
+            std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
+            for (int e : in) {
+                trsmE.push_back({piv.A, edges_[e].A, n, size(edges_[e].n1), 0, 0});
+                lst.flops_trsm += (double)n * n * size(edges_[e].n1);
+            }
+            for (int e : out) {
+                trsmE.push_back({piv.A, edges_[e].A, n, size(edges_[e].n2), 1, 0});
+                lst.flops_trsm += (double)n * n * size(edges_[e].n2);
+            }
+            // schur_symmetric (src/tree.cpp:862-883) with gemm_edges (:748-793)
+            for (int e1 : out)
+                for (int e2 : out) {
+                    int n1 = edges_[e1].n2, n2 = edges_[e2].n2;
+                    if (cls_[n1].order < cls_[n2].order) continue;
+                    int tgt = target_edge(n1, n2);
+                    SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), size(n2), 0, 1, 0};
+                    pcs.push_back({tgt, sc});
+                    lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
+                }
+            for (int e1 : in)
+                for (int e2 : in) {
+                    int n1 = edges_[e1].n1, n2 = edges_[e2].n1;
+                    if (cls_[n1].order < cls_[n2].order) continue;
+                    int tgt = target_edge(n1, n2);
+                    SchurContrib sc{edges_[e1].A, edges_[e2].A, n, n, n, 1, 0, 0};
+                    pcs.push_back({tgt, sc});
+                    lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
+                }
+            for (int e1 : out)
+                for (int e2 : in) {
+                    int n1 = edges_[e1].n2, n2 = edges_[e2].n1;
+                    int tgt = target_edge(n1, n2);
+                    SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
+                    pcs.push_back({tgt, sc});
+                    lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
+                }
+            // record ScalingLLT + GemmSymmOut/In (src/tree.cpp:909-910, 885-892)
+            ops_.push_back({OP_SCALING_LLT, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr});
+            if (n > 0) sv_elim.push_back({piv.A, n, (int)s.xoff});
+            factor_bytes_ += (size_t)n * (n + 1) / 2 * sizeof(T);
+            std::vector<GemvTerm> mine;
+            for (int e : out) {
+                int nb = edges_[e].n2;
+                ops_.push_back({OP_GEMM_SYMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
+                factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+                if (n > 0 && size(nb) > 0) {
+                    if (!fwd_terms.count(nb)) fwd_order.push_back(nb);
+                    fwd_terms[nb].push_back({edges_[e].A, size(nb), n, (int)s.xoff, 0});
+                    mine.push_back({edges_[e].A, size(nb), n, (int)cls_[nb].xoff, 1});
+                }
+            }
+            for (int e : in) {
+                int nb = edges_[e].n1;
+                ops_.push_back({OP_GEMM_SYMM_IN, c, nb, n, size(nb), n, size(nb), ilvl_, edges_[e].A, nullptr});
+                factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+                if (n > 0 && size(nb) > 0) {
+                    if (!fwd_terms.count(nb)) fwd_order.push_back(nb);
+                    fwd_terms[nb].push_back({edges_[e].A, n, size(nb), (int)s.xoff, 1});
+                    mine.push_back({edges_[e].A, n, size(nb), (int)cls_[nb].xoff, 0});
+                }
+            }
+            if (!mine.empty()) {
+                int k0 = (int)bwd_terms.size();
+                for (auto it = mine.rbegin(); it != mine.rend(); ++it) bwd_terms.push_back(*it);
+                bwd_groups.push_back({(int)s.xoff, n, k0, (int)bwd_terms.size()});
+            }
+            set_eliminated(c);
+        }
+        // fill-in blocks: the Schur targets created at this level, zero-initialized
+        T* fill_base = nullptr;
+        if (fill_elems > 0) {
+            fill_base = reinterpret_cast<T*>(arena_.alloc((size_t)fill_elems * sizeof(T)));
+            for (size_t k = 0; k < fill_edges.size(); k++) edges_[fill_edges[k]].A = fill_base + fill_off[k];
+        }
+        // Schur tiles grouped by target, contributions in recorded order (built before the
+        // scale phase below turns the neighbours' pivots into identities)
+        std::vector<SchurTile> tiles;
+        std::vector<SchurContrib> contribs;
+        {
+            std::unordered_map<int, std::vector<int>> by_target;
+            std::vector<int> torder;
+            for (size_t k = 0; k < pcs.size(); k++) {
+                auto it = by_target.find(pcs[k].target);
+                if (it == by_target.end()) {
+                    torder.push_back(pcs[k].target);
+                    by_target[pcs[k].target].push_back((int)k);
+                } else {
+                    it->second.push_back((int)k);
+                }
+            }
+            for (int tg : torder) {
+                const HEdge& e = edges_[tg];
+                const int m = size(e.n2), nn = size(e.n1);
+                const int k0 = (int)contribs.size();
+                for (int k : by_target[tg]) contribs.push_back(pcs[k].c);
+                const int k1 = (int)contribs.size();
+                const int lower = e.n1 == e.n2 ? 1 : 0;
+                for (int c0 = 0; c0 < nn; c0 += 32)
+                    for (int r0 = 0; r0 < m; r0 += 32) {
+                        int tm = std::min(32, m - r0), tn = std::min(32, nn - c0);
+                        if (lower && r0 + tm - 1 < c0) continue;
+                        tiles.push_back({e.A, m, r0, c0, tm, tn, lower, k0, k1});
+                    }
+            }
+        }
+        int dofs_left = 0;
+        for (int c : bot)
+            if (!cls_[c].elim) dofs_left += cls_[c].size;
+        lst.dofs_left_elim = dofs_left;
+        // ---------------- Scale (src/tree.cpp:1510-1519, 796-856) ----------------
+        const bool do_sparsify_level = ilvl_ >= skip_;
+        std::vector<int> live;
+        for (int c : bot)
+            if (cls_[c].level > ilvl_) live.push_back(c);
+        if (do_sparsify_level) {
+            for (int c : live) {
+                HCluster& s = cls_[c];
+                const HEdge& piv = edges_[s.out[0]];
+                potrfS.push_back({piv.A, s.size, c});
+                lst.flops_potrf += potrf_flops(s.size);
+                ops_.push_back({OP_SCALING_LLT, c, -1, s.size, 0, s.size, s.size, ilvl_, piv.A, nullptr});
+                factor_bytes_ += (size_t)s.size * (s.size + 1) / 2 * sizeof(T);
+                if (s.size > 0) sv_scale.push_back({piv.A, s.size, (int)s.xoff});
+            }
+            for (int c : live) {
+                for (size_t k = 1; k < cls_[c].out.size(); k++) {
+                    const HEdge& e = edges_[cls_[c].out[k]];
+                    const int a = e.n1, b = e.n2;  // block is size(b) x size(a)
+                    TrsmTask right{edges_[cls_[a].out[0]].A, e.A, size(a), size(b), 1, 0};
+                    TrsmTask left{edges_[cls_[b].out[0]].A, e.A, size(b), size(a), 0, 0};
+                    lst.flops_trsm += (double)size(a) * size(b) * (size(a) + size(b));
+                    if (cls_[a].order < cls_[b].order) {
+                        trsmA.push_back(right);
+                        trsmB.push_back(left);
+                    } else {
+                        trsmA.push_back(left);
+                        trsmB.push_back(right);
+                    }
+                }
+            }
+            for (int c : live) {
+                HEdge& piv = edges_[cls_[c].out[0]];
+                piv.A = nullptr;
+                piv.ident = true;
+            }
+        }
+        // ---------------- Sparsify plan (src/tree.cpp:1521-1533, 1292-1347) ----------------
+        std::vector<int> plist;
+        std::vector<int> wave, widx;
+        int nwaves = 0;
+        std::vector<std::vector<RrqrTask>> wtasks;
+        std::vector<RrqrSeg> segs;
+        if (do_sparsify_level) {
+            std::vector<int> inP(cls_.size(), -1);
+            for (int c : live) {
+                if (use_want_sparsify_ && !cls_[c].spars) {
+                    lst.nignored++;
+                    continue;
+                }
+                inP[c] = (int)plist.size();
+                plist.push_back(c);
+            }
+            wave.assign(plist.size(), 0);
+            for (size_t p = 0; p < plist.size(); p++) {
+                const int c = plist[p];
+                int w = 0;
+                auto visit = [&](int nb) {
+                    if (inP[nb] >= 0 && cls_[nb].order < cls_[c].order) w = std::max(w, wave[inP[nb]] + 1);
+                };
+                for (int e : cls_[c].in) visit(edges_[e].n1);
+                for (size_t k = 1; k < cls_[c].out.size(); k++) visit(edges_[cls_[c].out[k]].n2);
+                wave[p] = w;
+                nwaves = std::max(nwaves, w + 1);
+            }
+            wtasks.assign(nwaves, {});
+            widx.assign(plist.size(), 0);
+            for (size_t p = 0; p < plist.size(); p++) {
+                const int c = plist[p];
+                const HCluster& s = cls_[c];
+                RrqrTask t{};
+                t.cluster = c;
+                t.seg0 = (int)segs.size();
+                int cols = 0;
+                for (int e : s.in) {
+                    segs.push_back({edges_[e].A, edges_[e].n1, 0});
+                    cols += size(edges_[e].n1);
+                }
+                for (size_t k = 1; k < s.out.size(); k++) {
+                    const HEdge& e = edges_[s.out[k]];
+                    segs.push_back({e.A, e.n2, 1});
+                    cols += size(e.n2);
+                }
+                t.nseg = (int)segs.size() - t.seg0;
+                if (t.nseg > 1024) throw Error(ERR_UNSUPPORTED, "cluster with more than 1024 neighbours");
+                t.max_rows = s.size;
+                t.max_cols = cols;
+                const int mn = std::min(s.size, cols);
+                if ((long long)s.size * cols > lds_elems)
+                    t.scratch = arena_.alloc((size_t)s.size * cols * sizeof(T));
+                if (cols > 1024) {
+                    t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * cols * sizeof(double)));
+                    t.jpvt = reinterpret_cast<int*>(arena_.alloc((size_t)cols * sizeof(int)));
+                }
+                t.Q = arena_.alloc((size_t)std::max(1, s.size * mn) * sizeof(T));
+                t.tau = arena_.alloc((size_t)std::max(1, mn) * sizeof(T));
+                lst.flops_rrqr += mn > 0 ? 2.0 * s.size * cols * mn - (2.0 / 3.0) * (double)mn * mn * mn : 0.0;
+                widx[p] = (int)wtasks[wave[p]].size();
+                wtasks[wave[p]].push_back(t);
+            }
+        }
+        lst.nwaves = nwaves;
+        // ---------------- upload + launch Elim / Scale / Sparsify ----------------
+        for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
+        Packer pk;
+        size_t o_pe = pk.add(potrfE), o_te = pk.add(trsmE), o_ti = pk.add(tiles), o_co = pk.add(contribs);
+        size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
+        std::vector<size_t> o_w(nwaves);
+        for (int w = 0; w < nwaves; w++) o_w[w] = pk.add(wtasks[w]);
+        const double t_plan1 = wall();
+        void* dt = upload_tasks(pk.buf);
+        hip_check(hipMemcpy(d_size_, hsize.data(), cls_.size() * sizeof(int), hipMemcpyHostToDevice), "upload sizes");
+        if (fill_elems > 0) hip_check(launch_fill<T>(fill_base, (size_t)fill_elems, T(0), st), "fill fill-in");
+        hip_check(launch_potrf<T>(Packer::at<PotrfTask>(dt, o_pe), (int)potrfE.size(), d_info_, st), "potrf elim");
+        hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_te), (int)trsmE.size(), st), "trsm elim");
+        hip_check(launch_schur<T>(Packer::at<SchurTile>(dt, o_ti), (int)tiles.size(),
+                                  Packer::at<SchurContrib>(dt, o_co), st),
+                  "schur");
+        hip_check(launch_potrf<T>(Packer::at<PotrfTask>(dt, o_ps), (int)potrfS.size(), d_info_, st), "potrf scale");
+        hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_ta), (int)trsmA.size(), st), "trsm scale A");
+        hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_tb), (int)trsmB.size(), st), "trsm scale B");
+        for (int w = 0; w < nwaves; w++)
+            hip_check(launch_rrqr<T>(Packer::at<RrqrTask>(dt, o_w[w]), (int)wtasks[w].size(),
+                                     Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st),
+                      "rrqr");
+        int info = 0;
+        hip_check(hipMemcpy(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost), "read info");
+        if (info != 0)
+            throw Error(ERR_NOT_SPD, "Error: Non-SPD Pivot (cluster order " + std::to_string(info - 1) + ")");
+        if (!plist.empty())
+            hip_check(hipMemcpy(hsize.data(), d_size_, cls_.size() * sizeof(int), hipMemcpyDeviceToHost), "read ranks");
+        const double t_lvl1 = wall();
+        lst.t_plan = t_plan1 - t_lvl0;
+        lst.t_elim_scale = t_lvl1 - t_plan1;
+        // ---------------- apply ranks: Orthogonal, Split, sibling (src/tree.cpp:1317-1342) ----------------
+        std::vector<QTask> qt;
+        for (size_t p = 0; p < plist.size(); p++) {
+            const int c = plist[p];
+            const int r = hsize[c];
+            const int s0 = cls_[c].size;
+            lst.rank_before += s0;
+            lst.nsparsified++;
+            if (r < s0) {
+                const RrqrTask* t = &wtasks[wave[p]][widx[p]];
+                ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
+                factor_bytes_ += ((size_t)s0 * r + r) * sizeof(T);
+                if (r > 0) qt.push_back({t->Q, t->tau, s0, r, (int)cls_[c].xoff, 0});
+                HCluster sib;
+                sib.start = cls_[c].start + r;
+                sib.size = sib.osize = s0 - r;
+                sib.level = cls_[c].level;
+                sib.spars = cls_[c].spars;
+                sib.elim = true;
+                sib.order = (int)cls_.size();
+                sib.xoff = cls_[c].xoff + r;
+                const int sid = (int)cls_.size();
+                cls_.push_back(sib);
+                others_.push_back(sid);
+                cls_[c].size = r;
+                ops_.push_back({OP_SPLIT, c, sid, r, s0 - r, 0, 0, ilvl_, nullptr, nullptr});
+                ops_.push_back({OP_SCALING_LLT, sid, -1, s0 - r, 0, s0 - r, s0 - r, ilvl_, nullptr, nullptr});
+                // the sibling's pivot is the identity and is never stored (src/tree.cpp:1076-1099)
+            }
+            lst.rank_after += cls_[c].size;
+        }
+        const double t_lvl2 = wall();
+        lst.t_spars = t_lvl2 - t_lvl1;
+        // ---------------- Merge (src/tree.cpp:1435-1445, 1106-1184) ----------------
+        std::vector<MergeTask> mt;
+        std::vector<int2> mchildren;
+        if (ilvl_ < nlevels_ - 1) {
+            current_bottom_++;
+            const std::vector<int>& nb = bottoms_[current_bottom_];
+            std::unordered_map<int, int> pos;
+            for (int p : nb) {
+                HCluster& P = cls_[p];
+                int off = 0;
+                MergeTask m{(int)ws_len_, (int)mchildren.size(), 0, 0};
+                for (int ch : P.children) {
+                    pos[ch] = off;
+                    mchildren.push_back(make_int2((int)cls_[ch].xoff, cls_[ch].size));
+                    off += cls_[ch].size;
+                }
+                m.c1 = (int)mchildren.size();
+                P.size = P.osize = off;
+                P.xoff = ws_len_;
+                ws_len_ += off;
+                ops_.push_back({OP_MERGE, p, -1, off, 0, 0, 0, ilvl_, nullptr, nullptr});
+                if (off > 0) mt.push_back(m);
+            }
+            long long melems = 0;
+            std::vector<long long> moff;   // by new edge id - first_new
+            const int first_new = (int)edges_.size();
+            struct CT { int dst_edge; long long dst_add; const void* src; int rows, cols; };
+            std::vector<CT> cts;
+            for (int p : nb) {
+                std::vector<int> nnews;
+                for (int ch : cls_[p].children)
+                    for (int e : cls_[ch].out) {
+                        int nn = cls_[edges_[e].n2].parent;
+                        if (std::find(nnews.begin(), nnews.end(), nn) == nnews.end()) nnews.push_back(nn);
+                    }
+                std::unordered_map<int, int> eid;
+                for (int nn : nnews) {
+                    bool orig = false;
+                    for (int ch : cls_[p].children)
+                        for (int e : cls_[ch].out)
+                            if (cls_[edges_[e].n2].parent == nn && edges_[e].original) orig = true;
+                    int e = new_edge(p, nn, nullptr, orig);
+                    eid[nn] = e;
+                    moff.push_back(melems);
+                    melems += (long long)size(nn) * size(p);
+                }
+                auto& out = cls_[p].out;
+                std::stable_sort(out.begin() + 1, out.end(),
+                                 [&](int a, int b) { return cls_[edges_[a].n2].order < cls_[edges_[b].n2].order; });
+                auto& in = cls_[p].in;
+                std::stable_sort(in.begin(), in.end(),
+                                 [&](int a, int b) { return cls_[edges_[a].n1].order < cls_[edges_[b].n1].order; });
+                for (int ch : cls_[p].children) {
+                    for (int e : cls_[ch].out) {
+                        int nold = edges_[e].n2, nn = cls_[nold].parent;
+                        int de = eid[nn];
+                        long long add = pos[nold] + (long long)pos[ch] * size(nn);
+                        if (size(nold) > 0 && size(ch) > 0)
+                            cts.push_back({de, add, edges_[e].ident ? nullptr : edges_[e].A, size(nold), size(ch)});
+                    }
+                    cls_[ch].out.clear();
+                    cls_[ch].in.clear();
+                }
+            }
+            T* mbase = reinterpret_cast<T*>(arena_.alloc((size_t)std::max<long long>(melems, 1) * sizeof(T)));
+            for (size_t k = first_new; k < edges_.size(); k++) edges_[k].A = mbase + moff[k - first_new];
+            std::vector<CopyTask> copies;
+            copies.reserve(cts.size());
+            for (auto& c : cts) {
+                const HEdge& d = edges_[c.dst_edge];
+                copies.push_back({c.src, reinterpret_cast<T*>(d.A) + c.dst_add, c.rows, c.cols, size(d.n2), 0});
+            }
+            Packer pm;
+            size_t o_c = pm.add(copies);
+            void* dm = upload_tasks(pm.buf);
+            if (melems > 0) hip_check(launch_fill<T>(mbase, (size_t)melems, T(0), st), "fill merge");
+            hip_check(launch_copy<T>(Packer::at<CopyTask>(dm, o_c), (int)copies.size(), st), "merge copy");
+        }
+        lst.t_merge = wall() - t_lvl2;
+        int left = 0;
+        for (int c : bottoms_[current_bottom_])
+            if (!cls_[c].elim) left += cls_[c].size;
+        lst.dofs_left_spars = left;
+        lst.fact_nnz = nnz();
+        // ---------------- solve plan for this level ----------------
+        std::vector<GemvGroup> fwd_groups;
+        std::vector<GemvTerm> terms;
+        for (int nbc : fwd_order) {
+            auto& v = fwd_terms[nbc];
+            int k0 = (int)terms.size();
+            terms.insert(terms.end(), v.begin(), v.end());
+            int n = v[0].trans ? v[0].cols : v[0].rows;
+            fwd_groups.push_back({(int)cls_[nbc].xoff, n, k0, (int)terms.size()});
+        }
+        const int bwd_base = (int)terms.size();
+        for (auto& g : bwd_groups) {
+            g.k0 += bwd_base;
+            g.k1 += bwd_base;
+        }
+        terms.insert(terms.end(), bwd_terms.begin(), bwd_terms.end());
+        Packer ps;
+        size_t s_et = ps.add(sv_elim), s_fg = ps.add(fwd_groups), s_bg = ps.add(bwd_groups), s_tm = ps.add(terms);
+        size_t s_st = ps.add(sv_scale), s_q = ps.add(qt), s_m = ps.add(mt), s_mc = ps.add(mchildren);
+        void* dsv = upload_tasks(ps.buf);
+        lsv.elim_trsv = Packer::at<TrsvTask>(dsv, s_et);
+        lsv.n_elim_trsv = (int)sv_elim.size();
+        lsv.fwd_groups = Packer::at<GemvGroup>(dsv, s_fg);
+        lsv.n_fwd_groups = (int)fwd_groups.size();
+        lsv.bwd_groups = Packer::at<GemvGroup>(dsv, s_bg);
+        lsv.n_bwd_groups = (int)bwd_groups.size();
+        lsv.terms = Packer::at<GemvTerm>(dsv, s_tm);
+        lsv.scale_trsv = Packer::at<TrsvTask>(dsv, s_st);
+        lsv.n_scale_trsv = (int)sv_scale.size();
+        lsv.q = Packer::at<QTask>(dsv, s_q);
+        lsv.n_q = (int)qt.size();
+        lsv.merge = Packer::at<MergeTask>(dsv, s_m);
+        lsv.n_merge = (int)mt.size();
+        lsv.children = Packer::at<int2>(dsv, s_mc);
+        if (verb_)
+            printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, potrfE.size(),
+                   potrfS.size(), plist.size(), nwaves, left);
+    }
+    hip_check(hipDeviceSynchronize(), "factorize");
+    t_factor_ = wall() - t_start;
+}
+
+// ---------------------------------------------------------------------------------------
+int Tree::get_stop() const {
+    int stop = N_;
+    for (auto& l : stats_) {
+        if (l.dofs_left_elim > 0) stop = std::min(stop, l.dofs_left_elim);
+        if (l.dofs_left_spars > 0) stop = std::min(stop, l.dofs_left_spars);
+    }
+    return stop;
+}
+
+// Tree::nnz (src/tree.cpp:1637-1643) with the reference's per-op counts.
+long long Tree::nnz() const {
+    long long n = 0;
+    for (auto& o : ops_) {
+        switch (o.type) {
+            case OP_SCALING_LLT: n += (long long)o.sa * (o.sa + 1) / 2; break;
+            case OP_GEMM_SYMM_OUT: case OP_GEMM_SYMM_IN: n += (long long)o.sa * o.sb; break;
+            case OP_ORTHOGONAL: n += (long long)o.sa * o.sa; break;
+            default: break;
+        }
+    }
+    return n;
+}
+
+template <typename T>
+static void d2h_as_double(const void* src, size_t n, double* dst) {
+    std::vector<T> tmp(n);
+    if (n) hip_check(hipMemcpy(tmp.data(), src, n * sizeof(T), hipMemcpyDeviceToHost), "read block");
+    for (size_t i = 0; i < n; i++) dst[i] = (double)tmp[i];
+}
+
+void Tree::op_data(int i, double* M, double* tau) const {
+    if (i < 0 || i >= (int)ops_.size()) throw Error(ERR_ARG, "op index out of range");
+    const OpRec& o = ops_[i];
+    const size_t n = (size_t)o.rows * o.cols;
+    if (M && n) {
+        if (o.M == nullptr) {
+            for (size_t k = 0; k < n; k++) M[k] = 0.0;
+            for (int k = 0; k < std::min(o.rows, o.cols); k++) M[k + (size_t)k * o.rows] = 1.0;
+        } else if (prec_ == 0) {
+            d2h_as_double<double>(o.M, n, M);
+        } else {
+            d2h_as_double<float>(o.M, n, M);
+        }
+    }
+    if (tau && o.tau && o.cols > 0) {
+        if (prec_ == 0) d2h_as_double<double>(o.tau, o.cols, tau);
+        else d2h_as_double<float>(o.tau, o.cols, tau);
+    }
+}
+
+long long Tree::trailing_coo(int* I, int* J, double* V) const {
+    // Before factorize(): the assembled matrix, from the scatter map. After: the current
+    // bottom's blocks (empty once every level is eliminated).
+    if (state_ == State::Assembled || state_ == State::Factorized) {
+        if (state_ == State::Factorized) {
+            long long cnt = 0;
+            for (int c : bottoms_[current_bottom_]) {
+                for (int e : cls_[c].out) {
+                    const HEdge& E = edges_[e];
+                    const int m = size(E.n2), n = size(E.n1);
+                    std::vector<double> blk((size_t)m * n);
+                    if (E.ident) {
+                        for (int k = 0; k < std::min(m, n); k++) blk[k + (size_t)k * m] = 1.0;
+                    } else if (prec_ == 0) {
+                        d2h_as_double<double>(E.A, blk.size(), blk.data());
+                    } else {
+                        d2h_as_double<float>(E.A, blk.size(), blk.data());
+                    }
+                    for (int jj = 0; jj < n; jj++)
+                        for (int ii = 0; ii < m; ii++) {
+                            int gi = cls_[E.n2].start + ii, gj = cls_[c].start + jj;
+                            if (symm_ == 0 && gi < gj) continue;
+                            if (I) { I[cnt] = gi; J[cnt] = gj; V[cnt] = blk[ii + (size_t)jj * m]; }
+                            cnt++;
+                        }
+                }
+            }
+            return cnt;
+        }
+        const std::vector<long long>& off = h_scatter_off_;
+        const std::vector<double>& val = h_scatter_val_;
+        // map offsets back to (edge, i, j)
+        std::vector<std::pair<long long, int>> starts;
+        for (size_t e = 0; e < edges0_.size(); e++) starts.push_back({edge0_off_[e], (int)e});
+        std::sort(starts.begin(), starts.end());
+        long long cnt = 0;
+        for (long long k = 0; k < nscatter_; k++) {
+            auto it = std::upper_bound(starts.begin(), starts.end(), std::make_pair(off[k], INT32_MAX));
+            --it;
+            const HEdge& E = edges0_[it->second];
+            const int m = cls0_[E.n2].size;
+            long long rel = off[k] - it->first;
+            int ii = (int)(rel % m), jj = (int)(rel / m);
+            int gi = cls0_[E.n2].start + ii, gj = cls0_[E.n1].start + jj;
+            if (symm_ == 0 && gi < gj) continue;
+            if (I) { I[cnt] = gi; J[cnt] = gj; V[cnt] = val[k]; }
+            cnt++;
+        }
+        return cnt;
+    }
+    throw Error(ERR_STATE, "trailing matrix needs assemble()");
+}
+
+// ---------------------------------------------------------------------------------------
+void Tree::solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
+    if (state_ != State::Factorized) throw Error(ERR_STATE, "solve() must follow factorize()");
+    if (nrhs <= 0 || nrhs > 64 || ldx < N_) throw Error(ERR_ARG, "solve: need 1 <= nrhs <= 64 and ld >= N");
+    double* dx = x;
+    double* tmp = nullptr;
+    int ld = ldx;
+    if (!on_device) {
+        hip_check(hipMalloc((void**)&tmp, (size_t)N_ * nrhs * sizeof(double)), "hipMalloc(solve x)");
+        hip_check(hipMemcpy2D(tmp, (size_t)N_ * sizeof(double), x, (size_t)ldx * sizeof(double),
+                              (size_t)N_ * sizeof(double), nrhs, hipMemcpyHostToDevice),
+                  "upload x");
+        dx = tmp;
+        ld = N_;
+    }
+    if (prec_ == 0)
+        apply_impl<double>(dx, nrhs, 1, (size_t)ld, st);
+    else
+        apply_impl<float>(dx, nrhs, 1, (size_t)ld, st);
+    if (!on_device) {
+        hip_check(hipStreamSynchronize(st), "solve");
+        hip_check(hipMemcpy2D(x, (size_t)ldx * sizeof(double), tmp, (size_t)N_ * sizeof(double),
+                              (size_t)N_ * sizeof(double), nrhs, hipMemcpyDeviceToHost),
+                  "download x");
+        (void)hipFree(tmp);
+    }
+}
+
+void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st) {
+    if (state_ != State::Factorized) throw Error(ERR_STATE, "solve() must follow factorize()");
+    if (nrhs <= 0 || nrhs > 64) throw Error(ERR_ARG, "solve: need 1 <= nrhs <= 64");
+    if (prec_ == 0)
+        apply_impl<double>(d_x, nrhs, (size_t)nrhs, 1, st);
+    else
+        apply_impl<float>(d_x, nrhs, (size_t)nrhs, 1, st);
+}
+
+// Tree::solve (src/tree.cpp:1610-1635): permute in, forward sweep over the levels, backward
+// sweep in reverse, permute out. Split ops need no work: a sibling's segment aliases the
+// tail of its original cluster's segment.
+template <typename T>
+void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st) {
+    const size_t need = (size_t)ws_len_ * nrhs * sizeof(T);
+    if (need > ws_cap_) {
+        (void)hipFree(d_ws_);
+        d_ws_ = nullptr;
+        hip_check(hipMalloc(&d_ws_, need), "hipMalloc(solve workspace)");
+        ws_cap_ = need;
+    }
+    T* ws = reinterpret_cast<T*>(d_ws_);
+    hip_check(launch_perm_in<T>(dx, sx, sr, nrhs, d_perm_, N_, ws, st), "perm in");
+    for (int l = 0; l < nlevels_; l++) {
+        const LevelSolve& p = solve_plan_[l];
+        hip_check(launch_trsv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 0, st), "trsv");
+        hip_check(launch_gemv<T>(p.fwd_groups, p.n_fwd_groups, p.terms, ws, nrhs, st), "gemv");
+        hip_check(launch_trsv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 0, st), "trsv");
+        hip_check(launch_qapply<T>(p.q, p.n_q, ws, nrhs, 1, st), "qapply");
+        hip_check(launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 1, st), "merge");
+    }
+    for (int l = nlevels_ - 1; l >= 0; l--) {
+        const LevelSolve& p = solve_plan_[l];
+        hip_check(launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 0, st), "merge");
+        hip_check(launch_qapply<T>(p.q, p.n_q, ws, nrhs, 0, st), "qapply");
+        hip_check(launch_trsv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 1, st), "trsv");
+        hip_check(launch_gemv<T>(p.bwd_groups, p.n_bwd_groups, p.terms, ws, nrhs, st), "gemv");
+        hip_check(launch_trsv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 1, st), "trsv");
+    }
+    hip_check(launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st), "perm out");
+}
+
+}  // namespace spand

@@ -1,0 +1,231 @@
+// Host-side spaND tree for one GPU: the reference's Tree / Cluster / Edge API
+// (include/tree.h:130-197, include/cluster.h:16-117, include/edge.h:32-44) with the
+// numerical blocks moved into device arenas.
+//
+// The host keeps only the structure (cluster sizes, orders, edge lists in the reference's
+// list order) and plans every ND level as a handful of level-batched kernel launches; the
+// dense blocks (pivots, edges, factors, Householder vectors) live in device memory, in the
+// element type chosen by set_precision (FP64 or FP32).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tasks.hpp"
+
+namespace spand {
+
+enum Status {
+    OK = 0,
+    ERR_ARG = 1,
+    ERR_NOT_SPD = 2,
+    ERR_SINGULAR = 3,
+    ERR_HIP = 4,
+    ERR_STATE = 5,
+    ERR_NOMEM = 6,
+    ERR_NOT_CONVERGED = 7,
+    ERR_UNSUPPORTED = 8,
+};
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// Chunked bump allocator of device memory. reset() keeps the chunks for the next
+// factorization, so refactorizations of the same structure do not call hipMalloc.
+// A chunk size of 0 (environment SPAND_ARENA_CHUNK_BYTES=0) gives one exact-size
+// allocation per block, for bounds-checking builds.
+size_t arena_chunk_bytes(size_t dflt);
+class DeviceArena {
+  public:
+    explicit DeviceArena(size_t chunk_bytes = (size_t)256 << 20) : chunk_bytes_(arena_chunk_bytes(chunk_bytes)) {}
+    ~DeviceArena();
+    void* alloc(size_t bytes);
+    void reset();
+    size_t bytes_in_use() const { return in_use_; }
+    size_t bytes_reserved() const;
+
+  private:
+    struct Chunk {
+        char* p;
+        size_t size, used;
+    };
+    std::vector<Chunk> chunks_;
+    size_t cur_ = 0;
+    size_t chunk_bytes_;
+    size_t in_use_ = 0;
+};
+
+struct HCluster {
+    int start = 0, size = 0, osize = 0, level = 0, order = 0;
+    bool elim = false, spars = false;
+    int parent = -1;
+    std::vector<int> children;
+    std::vector<int> out;  // edge ids; out[0] is the pivot (self edge)
+    std::vector<int> in;   // edge ids of edges (n -> self), n != self
+    long long xoff = -1;   // segment of the cluster's x in the solve workspace
+};
+
+struct HEdge {
+    int n1 = -1, n2 = -1;  // block is A[n2, n1]: size(n2) x size(n1), ld = size(n2)
+    void* A = nullptr;     // device block; nullptr together with ident => identity pivot
+    bool ident = false;
+    bool original = false;
+};
+
+// One recorded op, kept for introspection (op-by-op comparison with the CPU oracle).
+// Types follow the reference's Operation classes (include/operations.h).
+enum OpType {
+    OP_SCALING_LLT = 0,
+    OP_SCALING_PLUQ = 1,
+    OP_GEMM_SYMM_OUT = 2,
+    OP_GEMM_SYMM_IN = 3,
+    OP_GEMM_OUT = 4,
+    OP_GEMM_IN = 5,
+    OP_ORTHOGONAL = 6,
+    OP_MERGE = 7,
+    OP_SPLIT = 8,
+};
+struct OpRec {
+    int type, a, b, sa, sb, rows, cols, level;
+    const void* M;
+    const void* tau;
+};
+
+// Per-level solve plan (device task arrays) for the preconditioner apply.
+struct LevelSolve {
+    TrsvTask* elim_trsv = nullptr; int n_elim_trsv = 0;
+    GemvGroup* fwd_groups = nullptr; int n_fwd_groups = 0;
+    GemvGroup* bwd_groups = nullptr; int n_bwd_groups = 0;
+    GemvTerm* terms = nullptr;
+    TrsvTask* scale_trsv = nullptr; int n_scale_trsv = 0;
+    QTask* q = nullptr; int n_q = 0;
+    MergeTask* merge = nullptr; int n_merge = 0;
+    int2* children = nullptr;
+};
+
+struct LevelStats {
+    int dofs_nd = 0, dofs_left_nd = 0, dofs_left_elim = 0, dofs_left_spars = 0;
+    long long fact_nnz = 0;
+    int rank_before = 0, rank_after = 0, nsparsified = 0, nignored = 0, nwaves = 0;
+    double t_plan = 0, t_elim_scale = 0, t_spars = 0, t_merge = 0;
+    double flops_potrf = 0, flops_trsm = 0, flops_schur = 0, flops_rrqr = 0;
+};
+
+class Tree {
+  public:
+    explicit Tree(int nlevels);
+    ~Tree();
+
+    // options (include/tree.h:133-147)
+    void set_tol(double t) { tol_ = t; }
+    void set_skip(int s) { skip_ = s; }
+    void set_verb(bool v) { verb_ = v; }
+    void set_symm_kind(int k);
+    void set_use_geo(bool g) { geo_ = g; }
+    void set_Xcoo(int dim, int N, const double* X);
+    void set_precision(int p);
+    void set_use_sparsify(bool u) { use_want_sparsify_ = u; }
+
+    void partition(int N, const int* colptr, const int* rowval);
+    void assemble(int N, const int* colptr, const int* rowval, const double* vals);
+    void factorize();
+    // M^{-1} applied in place to nrhs columns (ld >= N) of x, on device or host memory.
+    void solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
+    // M^{-1} applied in place to an interleaved device block (element (i, r) at x[i*nrhs+r]).
+    void solve_interleaved(double* d_x, int nrhs, hipStream_t st);
+
+    int N() const { return N_; }
+    int nlevels() const { return nlevels_; }
+    int precision() const { return prec_; }
+    int get_stop() const;
+    long long nnz() const;
+    const std::vector<int>& perm() const { return perm_; }
+    const std::vector<int>& part() const { return part_; }
+    const std::vector<HCluster>& clusters() const { return cls_; }
+    const std::vector<int>& bottom(int l) const { return bottoms_[l]; }
+    const std::vector<LevelStats>& stats() const { return stats_; }
+    const std::vector<OpRec>& ops() const { return ops_; }
+    double factor_seconds() const { return t_factor_; }
+    size_t factor_bytes() const { return factor_bytes_; }
+    size_t arena_bytes() const { return arena_.bytes_in_use(); }
+    // Copies op i's block (rows x cols, column-major) and tau (if any) to host, as FP64.
+    void op_data(int i, double* M, double* tau) const;
+    // Trailing (current) matrix as COO triplets; returns the count (arrays may be null).
+    long long trailing_coo(int* I, int* J, double* V) const;
+
+  private:
+    enum class State { Created, Partitioned, Assembled, Factorized };
+    template <typename T> void factorize_impl();
+    template <typename T> void apply_impl(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st);
+    int size(int c) const { return cls_[c].size; }
+    int find_out_edge(int from, int to) const;
+    int new_edge(int n1, int n2, void* A, bool original);
+    void set_eliminated(int c);
+    void* upload_tasks(const std::vector<char>& host);
+
+    // options
+    int nlevels_;
+    double tol_ = 10.0;  // src/tree.cpp:94
+    int skip_ = 0;
+    bool verb_ = false;
+    int symm_ = 0;       // 0 SPD (LLT); 2 GEN (PLU) is not built yet
+    bool geo_ = false;
+    int dim_ = 0;
+    std::vector<double> X_;
+    int prec_ = 0;       // 0 FP64, 1 FP32
+    bool use_want_sparsify_ = true;
+
+    State state_ = State::Created;
+    int N_ = 0;
+    std::vector<int> part_;
+    std::vector<int> perm_;
+    // structure after assembly (restored at every factorize())
+    std::vector<HCluster> cls0_;
+    std::vector<HEdge> edges0_;
+    // live structure
+    std::vector<HCluster> cls_;
+    std::vector<HEdge> edges_;
+    std::vector<std::vector<int>> bottoms_;
+    std::vector<int> others_;
+    int current_bottom_ = 0;
+    int ilvl_ = 0;
+
+    // assembly: level-0 blocks live in one region of `level0_elems` elements; nonzero k of
+    // the permuted matrix lands at offset scatter_off[k] of that region
+    std::vector<long long> edge0_off_;  // element offset of each assembled edge block
+    long long level0_elems_ = 0;
+    std::vector<long long> h_scatter_off_;
+    std::vector<double> h_scatter_val_;
+    long long* d_scatter_off_ = nullptr;
+    double* d_scatter_val_ = nullptr;
+    long long nscatter_ = 0;
+    bool device_inputs_ = false;  // scatter map and permutation uploaded
+    void upload_inputs();
+
+    // device state
+    DeviceArena arena_;
+    DeviceArena task_arena_{(size_t)64 << 20};
+    int* d_size_ = nullptr;
+    int* d_info_ = nullptr;
+    int d_size_cap_ = 0;
+    int* d_perm_ = nullptr;
+    void* d_ws_ = nullptr;
+    size_t ws_cap_ = 0;
+    long long ws_len_ = 0;   // segments in the workspace (per right-hand side)
+
+    std::vector<LevelSolve> solve_plan_;
+    std::vector<OpRec> ops_;
+    std::vector<LevelStats> stats_;
+    double t_factor_ = 0;
+    size_t factor_bytes_ = 0;
+};
+
+}  // namespace spand

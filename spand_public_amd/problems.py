@@ -1,0 +1,145 @@
+"""Host-side problem generators and sparse-matrix helpers.
+
+These produce the matrices spaND is run on, as CSC arrays (``colptr``, ``rowval``,
+``nzval``) with sorted row indices and int32 indices, the layout of
+``Eigen::SparseMatrix<double, ColMajor, int>`` that the reference's ``Tree::partition`` /
+``Tree::assemble`` consume (include/spaND.h:59).
+
+* ``neglapl(n, d)``: Dirichlet negative Laplacian, 2D 5-point (4/-1) or 3D 7-point
+  (6/-1); identical entries to the reference fixtures ``mats/neglapl_{d}_{n}.mm``.
+* ``linspace_nd(n, d)``: grid coordinates with dimension 0 slowest
+  (src/util.cpp:488-517). Dof ``id`` of ``neglapl`` sits at ``linspace_nd[:, id]``; the
+  Laplacian is axis-symmetric, so the reference's pairing of ``mats/*.mm`` (coordinate 0
+  fastest) with ``linspace_nd`` describes the same matrix.
+* ``symmetric_graph(A)``: |A| + |A^T| + I pattern (src/util.cpp:47-63).
+* ``mmread(path)``: MatrixMarket coordinate reader with the reference's symmetric /
+  hermitian expansion (include/mmio.hpp:160-219).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+class CSC:
+    """Square sparse matrix in CSC form (sorted rows, int32 indices, float64 values)."""
+
+    __slots__ = ("n", "colptr", "rowval", "nzval")
+
+    def __init__(self, n, colptr, rowval, nzval):
+        self.n = int(n)
+        self.colptr = np.ascontiguousarray(colptr, dtype=np.int32)
+        self.rowval = np.ascontiguousarray(rowval, dtype=np.int32)
+        self.nzval = np.ascontiguousarray(nzval, dtype=np.float64)
+
+    @property
+    def nnz(self):
+        return int(self.colptr[-1])
+
+    @staticmethod
+    def from_coo(n, rows, cols, vals):
+        rows = np.asarray(rows, dtype=np.int64)
+        cols = np.asarray(cols, dtype=np.int64)
+        vals = np.asarray(vals, dtype=np.float64)
+        key = cols * n + rows
+        order = np.argsort(key, kind="stable")
+        key = key[order]
+        vals = vals[order]
+        # sum duplicates (Eigen setFromTriplets semantics)
+        uniq, start = np.unique(key, return_index=True)
+        sums = np.add.reduceat(vals, start) if len(vals) else vals
+        r = (uniq % n).astype(np.int32)
+        c = (uniq // n).astype(np.int64)
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        np.add.at(colptr, c + 1, 1)
+        colptr = np.cumsum(colptr)
+        return CSC(n, colptr.astype(np.int32), r, sums)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+
+        return sp.csc_matrix((self.nzval, self.rowval, self.colptr), shape=(self.n, self.n))
+
+    def to_csr_arrays(self):
+        """CSR of the same matrix (rowptr, colval, nzval), used by the device SpMV."""
+        t = self.transpose()
+        return t.colptr, t.rowval, t.nzval
+
+    def transpose(self):
+        cols = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.colptr))
+        return CSC.from_coo(self.n, cols, self.rowval, self.nzval)
+
+    def matvec(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        cols = np.repeat(np.arange(self.n), np.diff(self.colptr))
+        y = np.zeros((self.n,) + x.shape[1:], dtype=np.float64)
+        np.add.at(y, self.rowval, self.nzval[:, None] * x[cols] if x.ndim > 1 else self.nzval * x[cols])
+        return y
+
+
+def linspace_nd(n: int, d: int) -> np.ndarray:
+    """Coordinates (d x n^d), dimension 0 slowest (src/util.cpp:488-517)."""
+    grids = np.meshgrid(*[np.arange(n, dtype=np.float64)] * d, indexing="ij")
+    return np.ascontiguousarray(np.stack([g.reshape(-1) for g in grids], axis=0))
+
+
+def neglapl(n: int, d: int) -> CSC:
+    """Dirichlet negative Laplacian on an n^d grid: diagonal 2d, -1 per grid neighbour."""
+    N = n ** d
+    idx = np.arange(N, dtype=np.int64)
+    coords = np.stack(np.unravel_index(idx, (n,) * d), axis=0)
+    rows = [idx]
+    cols = [idx]
+    vals = [np.full(N, 2.0 * d)]
+    stride = 1
+    for axis in range(d - 1, -1, -1):
+        c = coords[axis]
+        for sgn in (-1, 1):
+            ok = (c + sgn >= 0) & (c + sgn < n)
+            rows.append(idx[ok])
+            cols.append(idx[ok] + sgn * stride)
+            vals.append(np.full(int(ok.sum()), -1.0))
+        stride *= n
+    return CSC.from_coo(N, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+
+
+def symmetric_graph(A: CSC) -> CSC:
+    """|A| + |A^T| + I (src/util.cpp:47-63)."""
+    n = A.n
+    cols = np.repeat(np.arange(n, dtype=np.int64), np.diff(A.colptr))
+    r = np.concatenate([np.arange(n), cols, A.rowval])
+    c = np.concatenate([np.arange(n), A.rowval, cols])
+    v = np.concatenate([np.ones(n), np.abs(A.nzval), np.abs(A.nzval)])
+    return CSC.from_coo(n, r, c, v)
+
+
+def mmread(path: str) -> CSC:
+    """MatrixMarket coordinate reader (general / symmetric / hermitian / skew)."""
+    with open(path) as f:
+        header = f.readline().lower().split()
+        assert header[0] == "%%matrixmarket" and header[2] == "coordinate", header
+        prop = header[4] if len(header) > 4 else "general"
+        line = f.readline()
+        while line.startswith("%") or not line.strip():
+            line = f.readline()
+        M, N, K = (int(t) for t in line.split())
+        data = np.loadtxt(f, ndmin=2)
+    assert data.shape[0] == K and M == N
+    r = data[:, 0].astype(np.int64) - 1
+    c = data[:, 1].astype(np.int64) - 1
+    v = data[:, 2].astype(np.float64)
+    if prop in ("symmetric", "hermitian", "skew-symmetric"):
+        off = r != c
+        sgn = -1.0 if prop == "skew-symmetric" else 1.0
+        r, c, v = (np.concatenate([r, c[off]]), np.concatenate([c, r[off]]),
+                   np.concatenate([v, sgn * v[off]]))
+    return CSC.from_coo(N, r, c, v)
+
+
+def auto_nlevels(N: int, l: int = 0) -> int:
+    """The CLI's level rule round(log2(N/64)) + l for l <= 0 (tests/spaND.cpp:137-140)."""
+    return int(round(np.log(N / 64.0) / np.log(2.0))) + l
+
+
+def seeded_rhs(N: int, nrhs: int, seed: int) -> np.ndarray:
+    """b ~ N(0,1), N x nrhs, from numpy's PCG64 with the given seed."""
+    return np.random.default_rng(seed).standard_normal((N, nrhs))
