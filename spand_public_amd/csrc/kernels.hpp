@@ -24,6 +24,10 @@ hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib
 template <typename T>
 hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
                        hipStream_t st);
+// Stepped RRQR of the large clusters of one wave: init, then steps 0..nsteps-1, then finish.
+template <typename T>
+hipError_t launch_rrqr_big(const RrqrBig* d_bigs, int nbig, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
+                           int* d_size, double tol, int nsteps, int max_rows, hipStream_t st);
 template <typename T> hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st);
 
 // Preconditioner sweeps. ws holds the per-cluster segments, interleaved over nrhs.

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -39,112 +40,145 @@ __global__ void __launch_bounds__(NT) k_scatter(const long long* off, const doub
 }
 
 // ---------------------------------------------------------------------------------------
-// potrf: right-looking column Cholesky of the lower triangle; every element receives its
-// updates in k order as A(a,b) = fma(-L(a,k), L(b,k), A(a,b)). The block is staged in LDS
-// when it fits, otherwise the same loop runs on the global copy (L2 resident).
+// potrf: right-looking Cholesky of the lower triangle in 32-column blocks. Every element
+// receives its updates in k order as A(a,b) = fma(-L(a,k), L(b,k), A(a,b)), column j is
+// scaled by 1/L(j,j) after its own updates: the element recurrence of the oracle, so FP64
+// results are bit-identical to it. The diagonal block lives in LDS; panel and trailing
+// updates read the block from global memory (L2 resident).
+constexpr int PB = 32;
 template <typename T>
-__global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info, int lds_elems) {
+__global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info) {
 #pragma clang fp contract(off)
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* S = reinterpret_cast<T*>(smem_raw);
+    __shared__ T D[PB][PB + 1];
+    __shared__ T rdiag[PB];
     __shared__ int bad;
     const PotrfTask t = tasks[blockIdx.x];
     const int n = t.n;
     const int tid = threadIdx.x;
     if (n == 0) return;
-    T* G = reinterpret_cast<T*>(t.A);
-    const bool lds = (long long)n * n <= lds_elems;
-    T* M = lds ? S : G;
-    if (lds) {
-        for (int i = tid; i < n * n; i += NT) S[i] = G[i];
-    }
+    T* A = reinterpret_cast<T*>(t.A);
     if (tid == 0) bad = 0;
-    __syncthreads();
-    for (int j = 0; j < n; j++) {
-        if (tid == 0) {
-            T ajj = M[j + (size_t)j * n];
-            if (!(ajj > T(0))) {
-                bad = j + 1;
-            } else {
-                M[j + (size_t)j * n] = sqrt(ajj);
-            }
+    for (int kb = 0; kb < n; kb += PB) {
+        const int nb = min(PB, n - kb);
+        for (int idx = tid; idx < nb * nb; idx += NT) {
+            int i = idx % nb, k = idx / nb;
+            D[i][k] = A[(kb + i) + (size_t)(kb + k) * n];
         }
         __syncthreads();
+        // diagonal block: unblocked right-looking
+        for (int j = 0; j < nb; j++) {
+            if (tid == 0) {
+                T d = D[j][j];
+                if (!(d > T(0))) {
+                    if (!bad) bad = kb + j + 1;
+                } else {
+                    D[j][j] = sqrt(d);
+                }
+                rdiag[j] = T(1) / D[j][j];
+            }
+            __syncthreads();
+            if (bad) break;
+            const T r = rdiag[j];
+            for (int i = j + 1 + tid; i < nb; i += NT) D[i][j] *= r;
+            __syncthreads();
+            const int mm = nb - j - 1;
+            for (int idx = tid; idx < mm * mm; idx += NT) {
+                int a = j + 1 + idx % mm, b = j + 1 + idx / mm;
+                if (a >= b) D[a][b] = fma(-D[a][j], D[b][j], D[a][b]);
+            }
+            __syncthreads();
+        }
         if (bad) break;
-        const T r = T(1) / M[j + (size_t)j * n];
-        for (int i = j + 1 + tid; i < n; i += NT) M[i + (size_t)j * n] *= r;
-        __syncthreads();
-        const int m = n - j - 1;
-        // trailing lower update: M(a,b) -= M(a,j) M(b,j), a >= b > j
-        for (int idx = tid; idx < m * m; idx += NT) {
-            int a = idx % m, b = idx / m;
-            if (a >= b) {
-                int ra = j + 1 + a, cb = j + 1 + b;
-                M[ra + (size_t)cb * n] = fma(-M[ra + (size_t)j * n], M[cb + (size_t)j * n], M[ra + (size_t)cb * n]);
+        for (int idx = tid; idx < nb * nb; idx += NT) {
+            int i = idx % nb, k = idx / nb;
+            if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = D[i][k];
+        }
+        // panel rows i >= kb + nb: scale / update across the block's columns in order
+        for (int i = kb + nb + tid; i < n; i += NT) {
+            for (int k = 0; k < nb; k++) {
+                T a = A[i + (size_t)(kb + k) * n] * rdiag[k];
+                A[i + (size_t)(kb + k) * n] = a;
+                for (int k2 = k + 1; k2 < nb; k2++)
+                    A[i + (size_t)(kb + k2) * n] = fma(-a, D[k2][k], A[i + (size_t)(kb + k2) * n]);
             }
         }
         __syncthreads();
+        // trailing lower update by the block's columns, k in order
+        const int mt = n - kb - nb;
+        for (long long idx = tid; idx < (long long)mt * mt; idx += NT) {
+            int a = kb + nb + (int)(idx % mt), b = kb + nb + (int)(idx / mt);
+            if (a < b) continue;
+            T acc = A[a + (size_t)b * n];
+            for (int k = 0; k < nb; k++) acc = fma(-A[a + (size_t)(kb + k) * n], A[b + (size_t)(kb + k) * n], acc);
+            A[a + (size_t)b * n] = acc;
+        }
+        __syncthreads();
     }
-    if (bad) {
-        if (tid == 0) atomicCAS(info, 0, t.cluster + 1);
-        return;
-    }
-    if (lds) {
-        for (int i = tid; i < n * n; i += NT) G[i] = S[i];
-    }
+    if (bad && tid == 0) atomicCAS(info, 0, t.cluster + 1);
 }
 
 // ---------------------------------------------------------------------------------------
-// trsm: substitution in the reference BLAS element order, with explicit fused
-// multiply-adds (the FP64 result is bit-identical to the oracle's). B is staged in LDS
-// when it fits.
+// trsm on a chunk of <= 64 independent columns (left) or rows (right), in 32-row blocks of
+// L: the diagonal triangle is solved by one thread per independent index, the rows below
+// by the whole workgroup as an ordered-FMA update. Every element sees the reference BLAS
+// operations in the same order (FP64 bit-identical to the oracle).
 template <typename T>
-__global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks, int lds_elems) {
+__global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
 #pragma clang fp contract(off)
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* S = reinterpret_cast<T*>(smem_raw);
+    __shared__ T Bk[PB][64 + 1];
+    __shared__ T Lk[PB][PB + 1];
     const TrsmTask t = tasks[blockIdx.x];
-    const int n = t.n, m = t.m;
+    const int n = t.n, m = t.m, ld = t.ldb;
     const int tid = threadIdx.x;
     if (n == 0 || m == 0) return;
     const T* L = reinterpret_cast<const T*>(t.L);
     T* G = reinterpret_cast<T*>(t.B);
-    const long long nb = (long long)n * m;
-    const bool lds = nb <= lds_elems;
-    T* B = lds ? S : G;
-    if (lds) {
-        for (int i = tid; i < nb; i += NT) S[i] = G[i];
+    const bool left = t.side == 0;
+    // element (triangular index i, independent index c)
+    auto at = [&](int i, int c) -> T& { return left ? G[i + (size_t)c * ld] : G[c + (size_t)i * ld]; };
+    for (int kb = 0; kb < n; kb += PB) {
+        const int nb = min(PB, n - kb);
+        for (int idx = tid; idx < nb * m; idx += NT) {
+            int k, c;
+            if (left) { k = idx % nb; c = idx / nb; } else { c = idx % m; k = idx / m; }
+            Bk[k][c] = at(kb + k, c);
+        }
+        for (int idx = tid; idx < nb * nb; idx += NT) {
+            int i = idx % nb, k = idx / nb;
+            Lk[i][k] = L[(kb + i) + (size_t)(kb + k) * n];
+        }
         __syncthreads();
-    }
-    if (t.side == 0) {
-        // B (n x m, ld n) := L^{-1} B
-        for (int k = 0; k < n; k++) {
-            const T lkk = L[k + (size_t)k * n];
-            for (int c = tid; c < m; c += NT) B[k + (size_t)c * n] /= lkk;
-            __syncthreads();
-            const int rem = n - k - 1;
-            for (int idx = tid; idx < rem * m; idx += NT) {
-                int i = k + 1 + idx % rem, c = idx / rem;
-                B[i + (size_t)c * n] = fma(-B[k + (size_t)c * n], L[i + (size_t)k * n], B[i + (size_t)c * n]);
+        if (tid < m) {
+            const int c = tid;
+            for (int k = 0; k < nb; k++) {
+                T b;
+                if (left) {
+                    b = Bk[k][c] / Lk[k][k];
+                } else {
+                    const T r = T(1) / Lk[k][k];
+                    b = Bk[k][c] * r;
+                }
+                Bk[k][c] = b;
+                for (int i = k + 1; i < nb; i++) Bk[i][c] = fma(-b, Lk[i][k], Bk[i][c]);
             }
-            __syncthreads();
         }
-    } else {
-        // B (m x n, ld m) := B L^{-T}
-        for (int k = 0; k < n; k++) {
-            const T r = T(1) / L[k + (size_t)k * n];
-            for (int i = tid; i < m; i += NT) B[i + (size_t)k * m] *= r;
-            __syncthreads();
-            const int rem = n - k - 1;
-            for (int idx = tid; idx < rem * m; idx += NT) {
-                int i = idx % m, j = k + 1 + idx / m;
-                B[i + (size_t)j * m] = fma(-L[j + (size_t)k * n], B[i + (size_t)k * m], B[i + (size_t)j * m]);
-            }
-            __syncthreads();
+        __syncthreads();
+        for (int idx = tid; idx < nb * m; idx += NT) {
+            int k, c;
+            if (left) { k = idx % nb; c = idx / nb; } else { c = idx % m; k = idx / m; }
+            at(kb + k, c) = Bk[k][c];
         }
-    }
-    if (lds) {
-        for (int i = tid; i < nb; i += NT) G[i] = S[i];
+        const int rest = n - kb - nb;
+        for (long long idx = tid; idx < (long long)rest * m; idx += NT) {
+            int i, c;
+            if (left) { i = kb + nb + (int)(idx % rest); c = (int)(idx / rest); }
+            else { c = (int)(idx % m); i = kb + nb + (int)(idx / m); }
+            T acc = at(i, c);
+            const T* li = L + i + (size_t)kb * n;
+            for (int k = 0; k < nb; k++) acc = fma(-Bk[k][c], li[(size_t)k * n], acc);
+            at(i, c) = acc;
+        }
+        __syncthreads();
     }
 }
 
@@ -447,6 +481,383 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
 }
 
 // ---------------------------------------------------------------------------------------
+// Stepped RRQR for clusters too large for one workgroup. Launch sequence per wave:
+// k_rrqr_init (gather + norms), k_rrqr_step for every pivot step, k_rrqr_finish (rank,
+// Q, scatter). Each step: every slab reads the candidates of all slabs and picks the
+// pivot (first max by position), recomputes the pivot's dlarfg scalars redundantly
+// (identical in every workgroup), and updates / downdates its own free columns. The
+// pivot column is only read during its step; its owner writes the scaled reflector one
+// step later, so no data crosses workgroups inside a launch.
+constexpr int SLAB = 64;
+
+// dlarfg scalars for the column segment [alpha; x(0..n1)): tau, beta, the final scale and
+// the number of safe-minimum rescalings (reference dlarfg). Single thread.
+template <typename T>
+__device__ void larfg_scalars(double alpha, const T* x, int n1, double& tau, double& beta_out, double& scal,
+                              int& knt) {
+#pragma clang fp contract(off)
+    tau = 0.0;
+    scal = 1.0;
+    knt = 0;
+    beta_out = alpha;
+    if (n1 < 1) return;
+    double xs = 0.0;
+    for (int r = 0; r < n1; r++) {
+        double v = (double)x[r];
+        xs = fma(v, v, xs);
+    }
+    double xnorm = sqrt(xs);
+    if (xnorm == 0.0) return;
+    const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+    const double rs = 1.0 / safmin;
+    double beta = -copysign(lapy2_d(alpha, xnorm), alpha);
+    if (fabs(beta) < safmin) {
+        do {
+            knt++;
+            beta *= rs;
+            alpha *= rs;
+        } while (fabs(beta) < safmin && knt < 20);
+        xs = 0.0;
+        for (int r = 0; r < n1; r++) {
+            T xv = x[r];
+            for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
+            double v = (double)xv;
+            xs = fma(v, v, xs);
+        }
+        beta = -copysign(lapy2_d(alpha, sqrt(xs)), alpha);
+    }
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+    for (int k = 0; k < knt; k++) beta *= safmin;
+    beta_out = beta;
+}
+
+template <typename T>
+__device__ inline T larfg_scale(T xv, int knt, double scal) {
+    const double rs = 1.0 / (DBL_MIN / (DBL_EPSILON * 0.5));
+    for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
+    return xv * (T)scal;
+}
+
+// Stepped-path storage: 64-column slabs, row-major inside a slab, so a wavefront reading
+// one row of its slab touches one contiguous segment.
+__device__ inline size_t slab_idx(int rows, int r, int j) {
+    return ((size_t)(j / SLAB) * rows + r) * SLAB + (j % SLAB);
+}
+
+// Writes the reflector of step k (column `col`, rows k..rows-1) in place: beta on the
+// diagonal, the scaled vector below, tau[k]. All threads of the block take part; `col_buf`
+// is LDS scratch of at least rows elements.
+template <typename T>
+__device__ void write_reflector(T* A, int rows, int col, int k, T* tau_out, double* sh, T* col_buf) {
+    for (int r = k + threadIdx.x; r < rows; r += blockDim.x) col_buf[r] = A[slab_idx(rows, r, col)];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tau, beta, scal;
+        int knt;
+        larfg_scalars<T>((double)col_buf[k], col_buf + k + 1, rows - k - 1, tau, beta, scal, knt);
+        sh[0] = tau;
+        sh[1] = beta;
+        sh[2] = scal;
+        sh[3] = (double)knt;
+    }
+    __syncthreads();
+    const double tau = sh[0];
+    if (tau != 0.0) {
+        const int knt = (int)sh[3];
+        for (int r = k + 1 + threadIdx.x; r < rows; r += blockDim.x)
+            A[slab_idx(rows, r, col)] = larfg_scale<T>(col_buf[r], knt, sh[2]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (tau != 0.0) A[slab_idx(rows, k, col)] = (T)sh[1];
+        tau_out[k] = (T)tau;
+    }
+    __syncthreads();
+}
+
+__device__ inline int seg_find(const int* col0, int nseg, int j) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (col0[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// (value desc, position asc) reduction over the block into r_val[0], r_pos[0]
+__device__ inline void reduce_best(double* r_val, int* r_pos, double v, int p) {
+    const int tid = threadIdx.x;
+    r_val[tid] = v;
+    r_pos[tid] = p;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+            double v2 = r_val[tid + off];
+            int p2 = r_pos[tid + off];
+            if (v2 > r_val[tid] || (v2 == r_val[tid] && p2 < r_pos[tid])) {
+                r_val[tid] = v2;
+                r_pos[tid] = p2;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SLAB) k_rrqr_init(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
+                                                    const int* dsize) {
+#pragma clang fp contract(off)
+    __shared__ int s_col0[RRQR_MAXSEG + 1];
+    __shared__ double r_val[SLAB];
+    __shared__ int r_pos[SLAB];
+    const RrqrSlab sl = slabs[blockIdx.x];
+    const RrqrBig B = bigs[sl.big];
+    const RrqrSeg* sg = segs + B.t.seg0;
+    const int tid = threadIdx.x;
+    const int rows = dsize[B.t.cluster];
+    if (tid == 0) {
+        int c = 0;
+        for (int k = 0; k < B.t.nseg; k++) {
+            s_col0[k] = c;
+            c += dsize[sg[k].nbr];
+        }
+        s_col0[B.t.nseg] = c;
+    }
+    __syncthreads();
+    const int cols = s_col0[B.t.nseg];
+    if (sl.slab == 0 && tid == 0) {
+        B.dims[0] = rows;
+        B.dims[1] = cols;
+        B.dims[2] = min(rows, cols);
+    }
+    T* A = reinterpret_cast<T*>(B.t.scratch);
+    const int mc = B.t.max_cols;
+    double* vn1 = B.t.norms;
+    double* vn2 = B.t.norms + mc;
+    int* p2c = B.map;
+    int* c2p = B.map + 2 * mc;
+    const int j = sl.slab * SLAB + tid;
+    double best = -1.0;
+    int bpos = INT32_MAX;
+    if (j < cols && rows > 0) {
+        const int k = seg_find(s_col0, B.t.nseg, j);
+        const int jj = j - s_col0[k], w = s_col0[k + 1] - s_col0[k];
+        const T* E = reinterpret_cast<const T*>(sg[k].E);
+        T* a = A + (size_t)sl.slab * rows * SLAB + tid;
+        double sum = 0.0;
+        for (int i = 0; i < rows; i++) {
+            T v = sg[k].out ? E[jj + (size_t)i * w] : E[i + (size_t)jj * rows];
+            a[(size_t)i * SLAB] = v;
+            double dv = (double)v;
+            sum = fma(dv, dv, sum);
+        }
+        vn1[j] = sqrt(sum);
+        vn2[j] = vn1[j];
+        p2c[j] = j;
+        c2p[j] = j;
+        best = vn1[j];
+        bpos = j;
+    }
+    reduce_best(r_val, r_pos, best, bpos);
+    if (tid == 0) {
+        B.cand_v[sl.slab] = r_val[0];
+        B.cand_p[sl.slab] = r_pos[0];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SLAB) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* s_v = reinterpret_cast<T*>(smem_raw);
+    __shared__ double r_val[SLAB];
+    __shared__ int r_pos[SLAB];
+    __shared__ double sh[4];
+    __shared__ int s_p, s_ca, s_cb;
+    const RrqrSlab sl = slabs[blockIdx.x];
+    const RrqrBig B = bigs[sl.big];
+    const int tid = threadIdx.x;
+    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
+    const int i = step;
+    if (rows == 0 || i >= mn) return;
+    const int cur = i & 1, nxt = cur ^ 1;
+    const int mc = B.t.max_cols, ns = B.nslab;
+    const int* p2c_cur = B.map + cur * mc;
+    int* p2c_nxt = B.map + nxt * mc;
+    const int* c2p_cur = B.map + 2 * mc + cur * mc;
+    int* c2p_nxt = B.map + 2 * mc + nxt * mc;
+    T* A = reinterpret_cast<T*>(B.t.scratch);
+    double* vn1 = B.t.norms;
+    double* vn2 = B.t.norms + mc;
+    if (tid == 0) {
+        double bv = -1.0;
+        int bp = INT32_MAX;
+        for (int k = 0; k < ns; k++) {
+            double v = B.cand_v[cur * ns + k];
+            int pp = B.cand_p[cur * ns + k];
+            if (v > bv || (v == bv && pp < bp)) {
+                bv = v;
+                bp = pp;
+            }
+        }
+        s_p = bp;
+        s_ca = (bp >= i && bp < cols) ? p2c_cur[bp] : 0;
+        s_cb = p2c_cur[i];
+    }
+    __syncthreads();
+    const int p = s_p, ca = s_ca, cb = s_cb;
+    if (p < i || p >= cols) return;  // cannot happen for i < mn: some column is still free
+    if (sl.slab == 0) {
+        for (int j = tid; j < cols; j += SLAB) {
+            p2c_nxt[j] = p2c_cur[j];
+            c2p_nxt[j] = c2p_cur[j];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            p2c_nxt[i] = ca;
+            p2c_nxt[p] = cb;
+            c2p_nxt[ca] = i;
+            c2p_nxt[cb] = p;
+        }
+    }
+    // deferred store of the previous step's reflector by the owner of its column
+    T* s_col = s_v + rows;  // LDS copy of the pivot column
+    if (i >= 1) {
+        const int cprev = p2c_cur[i - 1];
+        if (cprev / SLAB == sl.slab) write_reflector<T>(A, rows, cprev, i - 1, reinterpret_cast<T*>(B.t.tau), sh, s_col);
+    }
+    // this step's reflector, recomputed identically in every slab from an LDS copy
+    for (int r = i + tid; r < rows; r += SLAB) s_col[r] = A[slab_idx(rows, r, ca)];
+    __syncthreads();
+    if (tid == 0) {
+        double tau, beta, scal;
+        int knt;
+        larfg_scalars<T>((double)s_col[i], s_col + i + 1, rows - i - 1, tau, beta, scal, knt);
+        sh[0] = tau;
+        sh[2] = scal;
+        sh[3] = (double)knt;
+    }
+    __syncthreads();
+    const T tau = (T)sh[0];
+    if (sh[0] != 0.0) {
+        const int knt = (int)sh[3];
+        for (int r = i + 1 + tid; r < rows; r += SLAB) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
+    }
+    __syncthreads();
+    const double tol3z = sqrt(DBL_EPSILON * 0.5);
+    const int j = sl.slab * SLAB + tid;
+    double best = -1.0;
+    int bpos = INT32_MAX;
+    if (j < cols && j != ca) {
+        const int pos = (j == cb) ? p : c2p_cur[j];
+        if (pos > i) {
+            T* cj = A + (size_t)sl.slab * rows * SLAB + tid;  // element r at cj[r * SLAB]
+            if (tau != T(0)) {
+                T w = cj[(size_t)i * SLAB];
+                for (int r = i + 1; r < rows; r++) w = fma(cj[(size_t)r * SLAB], s_v[r], w);
+                T tm = -tau * w;
+                cj[(size_t)i * SLAB] = cj[(size_t)i * SLAB] + tm;
+                for (int r = i + 1; r < rows; r++) cj[(size_t)r * SLAB] = fma(s_v[r], tm, cj[(size_t)r * SLAB]);
+            }
+            if (vn1[j] != 0.0) {
+                double q = fabs((double)cj[(size_t)i * SLAB]) / vn1[j];
+                double temp = 1.0 - q * q;
+                temp = fmax(temp, 0.0);
+                double rr = vn1[j] / vn2[j];
+                double temp2 = temp * rr * rr;
+                if (temp2 <= tol3z) {
+                    if (i < rows - 1) {
+                        double sum = 0.0;
+                        for (int r = i + 1; r < rows; r++) {
+                            double v = (double)cj[(size_t)r * SLAB];
+                            sum = fma(v, v, sum);
+                        }
+                        vn1[j] = sqrt(sum);
+                        vn2[j] = vn1[j];
+                    } else {
+                        vn1[j] = 0.0;
+                        vn2[j] = 0.0;
+                    }
+                } else {
+                    vn1[j] = vn1[j] * sqrt(temp);
+                }
+            }
+            best = vn1[j];
+            bpos = pos;
+        }
+    }
+    reduce_best(r_val, r_pos, best, bpos);
+    if (tid == 0) {
+        B.cand_v[nxt * ns + sl.slab] = r_val[0];
+        B.cand_p[nxt * ns + sl.slab] = r_pos[0];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const RrqrSeg* segs, int* dsize, double tol) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* s_col = reinterpret_cast<T*>(smem_raw);
+    __shared__ int s_col0[RRQR_MAXSEG + 1];
+    __shared__ double sh[4];
+    __shared__ int s_rank;
+    const RrqrBig B = bigs[blockIdx.x];
+    const RrqrSeg* sg = segs + B.t.seg0;
+    const int tid = threadIdx.x;
+    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
+    if (rows == 0) return;
+    if (tid == 0) {
+        int c = 0;
+        for (int k = 0; k < B.t.nseg; k++) {
+            s_col0[k] = c;
+            c += dsize[sg[k].nbr];
+        }
+        s_col0[B.t.nseg] = c;
+    }
+    const int mc = B.t.max_cols;
+    const int fin = mn & 1;
+    const int* p2c = B.map + fin * mc;
+    const int* c2p = B.map + 2 * mc + fin * mc;
+    T* A = reinterpret_cast<T*>(B.t.scratch);
+    __syncthreads();
+    if (mn >= 1) write_reflector<T>(A, rows, p2c[mn - 1], mn - 1, reinterpret_cast<T*>(B.t.tau), sh, s_col);
+    if (tid == 0) {
+        int rank;
+        if (tol == 0.0) rank = mn;
+        else if (tol >= 1.0) rank = 0;
+        else if (mn <= 1) rank = mn;
+        else {
+            double sref = fabs((double)A[slab_idx(rows, 0, p2c[0])]);
+            rank = 1;
+            while (rank < mn && fabs((double)A[slab_idx(rows, rank, p2c[rank])]) / sref >= tol) rank++;
+        }
+        s_rank = rank;
+    }
+    __syncthreads();
+    const int rank = s_rank;
+    if (rank >= rows) return;
+    T* Q = reinterpret_cast<T*>(B.t.Q);
+    for (long long idx = tid; idx < (long long)rows * rank; idx += NT) {
+        int r = (int)(idx % rows), k = (int)(idx / rows);
+        Q[idx] = A[slab_idx(rows, r, p2c[k])];
+    }
+    for (int j = tid; j < cols; j += NT) {
+        const int pos = c2p[j];
+        const int k = seg_find(s_col0, B.t.nseg, j);
+        const int jj = j - s_col0[k], w = s_col0[k + 1] - s_col0[k];
+        T* E = reinterpret_cast<T*>(sg[k].E);
+        if (sg[k].out) {
+            for (int r = 0; r < rank; r++) E[jj + (size_t)r * w] = (r <= pos) ? A[slab_idx(rows, r, j)] : T(0);
+        } else {
+            for (int r = 0; r < rank; r++) E[r + (size_t)jj * rank] = (r <= pos) ? A[slab_idx(rows, r, j)] : T(0);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) dsize[B.t.cluster] = rank;
+}
+
+// ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(NT) k_copy(const CopyTask* tasks) {
     const CopyTask t = tasks[blockIdx.x];
@@ -483,16 +894,14 @@ hipError_t launch_scatter(const long long* off, const double* val, long long n, 
 template <typename T>
 hipError_t launch_potrf(const PotrfTask* d_tasks, int ntasks, int* d_info, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    const int bytes = LDS_BYTES;
-    k_potrf<T><<<ntasks, NT, bytes, st>>>(d_tasks, d_info, bytes / (int)sizeof(T));
+    k_potrf<T><<<ntasks, NT, 0, st>>>(d_tasks, d_info);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    const int bytes = LDS_BYTES;
-    k_trsm<T><<<ntasks, NT, bytes, st>>>(d_tasks, bytes / (int)sizeof(T));
+    k_trsm<T><<<ntasks, NT, 0, st>>>(d_tasks);
     return hipGetLastError();
 }
 
@@ -513,6 +922,23 @@ hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_seg
 }
 
 template <typename T>
+hipError_t launch_rrqr_big(const RrqrBig* d_bigs, int nbig, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
+                           int* d_size, double tol, int nsteps, int max_rows, hipStream_t st) {
+    if (nbig == 0) return hipSuccess;
+    k_rrqr_init<T><<<nslabs, SLAB, 0, st>>>(d_bigs, d_slabs, d_segs, d_size);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t vbytes = (size_t)std::max(max_rows, 1) * sizeof(T);
+    for (int i = 0; i < nsteps; i++) {
+        k_rrqr_step<T><<<nslabs, SLAB, 2 * vbytes, st>>>(d_bigs, d_slabs, i);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    k_rrqr_finish<T><<<nbig, NT, vbytes, st>>>(d_bigs, d_segs, d_size, tol);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
     k_copy<T><<<ntasks, NT, 0, st>>>(d_tasks);
@@ -526,7 +952,9 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_trsm<T>(const TrsmTask*, int, hipStream_t);                        \
     template hipError_t launch_schur<T>(const SchurTile*, int, const SchurContrib*, hipStream_t); \
     template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, hipStream_t); \
-    template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);
+    template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);                        \
+    template hipError_t launch_rrqr_big<T>(const RrqrBig*, int, const RrqrSlab*, int, const RrqrSeg*, int*, double, \
+                                           int, int, hipStream_t);
 SPAND_INST(float)
 SPAND_INST(double)
 

@@ -15,16 +15,18 @@ struct PotrfTask {
     int cluster;  // reported through the info array on a non-SPD pivot
 };
 
-// Triangular solve of an edge block against a pivot factor L (n x n, lower):
-//   side 0: B (n x m, ld n) := L^{-1} B      (trsm_left,  src/util.cpp:266-274)
-//   side 1: B (m x n, ld m) := B L^{-T}      (trsm_right, src/util.cpp:256-264)
+// Triangular solve of (a 64-wide chunk of) an edge block against a pivot factor L
+// (n x n, lower, ld n):
+//   side 0: B (n x m, ld ldb) := L^{-1} B      (trsm_left,  src/util.cpp:266-274)
+//   side 1: B (m x n, ld ldb) := B L^{-T}      (trsm_right, src/util.cpp:256-264)
+// m <= 64: the host splits wide blocks into independent column (left) / row (right) chunks.
 struct TrsmTask {
     const void* L;
     void* B;
     int n;
     int m;
     int side;
-    int pad;
+    int ldb;
 };
 
 // One Schur-complement target tile: C[r0:r0+tm, c0:c0+tn] of a block C (ldc rows) receives
@@ -65,6 +67,23 @@ struct RrqrSeg {
     void* E;   // edge block
     int nbr;   // neighbour cluster (its live size gives the segment width)
     int out;   // 0: in-edge, block is s x n; 1: out-edge, block is n x s (transposed)
+};
+
+// Large clusters use the stepped RRQR (one launch per pivot step, columns split into
+// 64-wide slabs, one wavefront each). Columns never move: a double-buffered
+// position <-> column map reproduces LAPACK's column swaps and its first-max tie-break.
+struct RrqrBig {
+    RrqrTask t;      // scratch holds A (rows x cols, ld rows); norms holds vn1 | vn2 by column
+    int* map;        // p2c[2][max_cols] then c2p[2][max_cols]
+    double* cand_v;  // [2][nslab] per-slab best partial norm for the next step
+    int* cand_p;     // [2][nslab] its position
+    int* dims;       // rows, cols, min(rows, cols)
+    int nslab;
+    int pad;
+};
+struct RrqrSlab {
+    int big;   // index into the RrqrBig array
+    int slab;  // columns [64 * slab, 64 * slab + 64)
 };
 
 // Merge copy of one child block into a parent block (update_edges, src/tree.cpp:1133-1184).

@@ -312,6 +312,25 @@ void Tree::factorize() {
 
 static inline double potrf_flops(double n) { return n * n * n / 3.0; }
 
+// Splits a triangular solve into independent chunks of at most 64 columns (left solve,
+// block n x m, ld n) or 64 rows (right solve, block m x n, ld m).
+static void push_trsm(std::vector<TrsmTask>& v, const void* L, void* B, int n, int m, int side, size_t esz) {
+    if (n == 0 || m == 0) return;
+    for (int c0 = 0; c0 < m; c0 += 64) {
+        const int w = std::min(64, m - c0);
+        char* b = reinterpret_cast<char*>(B) + (side == 0 ? (size_t)c0 * n : (size_t)c0) * esz;
+        v.push_back({L, b, n, w, side, side == 0 ? n : m});
+    }
+}
+
+// Clusters with at least this many rows and more Asn elements than this take the stepped
+// multi-workgroup RRQR; smaller ones run in one workgroup. Both paths compute the same
+// numbers; SPAND_RRQR_STEPPED_MIN_ROWS / SPAND_RRQR_STEPPED_MIN_ELEMS override the split.
+static long long env_or(const char* name, long long dflt) {
+    const char* e = std::getenv(name);
+    return (e && *e) ? std::strtoll(e, nullptr, 10) : dflt;
+}
+
 template <typename T>
 void Tree::factorize_impl() {
     const double t_start = wall();
@@ -349,6 +368,8 @@ void Tree::factorize_impl() {
     for (int c : bottoms_[0]) cls_[c].xoff = cls_[c].start;
     ws_len_ = N_;
     const int lds_elems = LDS_BYTES / (int)sizeof(T);
+    const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 48);
+    const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 16384);
 
     for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
         LevelStats& lst = stats_[ilvl_];
@@ -389,11 +410,11 @@ void Tree::factorize_impl() {
             lst.flops_potrf += potrf_flops(n);
             std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
             for (int e : in) {
-                trsmE.push_back({piv.A, edges_[e].A, n, size(edges_[e].n1), 0, 0});
+                push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n1), 0, sizeof(T));
                 lst.flops_trsm += (double)n * n * size(edges_[e].n1);
             }
             for (int e : out) {
-                trsmE.push_back({piv.A, edges_[e].A, n, size(edges_[e].n2), 1, 0});
+                push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
                 lst.flops_trsm += (double)n * n * size(edges_[e].n2);
             }
             // schur_symmetric (src/tree.cpp:862-883) with gemm_edges (:748-793)
@@ -515,16 +536,13 @@ void Tree::factorize_impl() {
                 for (size_t k = 1; k < cls_[c].out.size(); k++) {
                     const HEdge& e = edges_[cls_[c].out[k]];
                     const int a = e.n1, b = e.n2;  // block is size(b) x size(a)
-                    TrsmTask right{edges_[cls_[a].out[0]].A, e.A, size(a), size(b), 1, 0};
-                    TrsmTask left{edges_[cls_[b].out[0]].A, e.A, size(b), size(a), 0, 0};
+                    const void* La = edges_[cls_[a].out[0]].A;
+                    const void* Lb = edges_[cls_[b].out[0]].A;
                     lst.flops_trsm += (double)size(a) * size(b) * (size(a) + size(b));
-                    if (cls_[a].order < cls_[b].order) {
-                        trsmA.push_back(right);
-                        trsmB.push_back(left);
-                    } else {
-                        trsmA.push_back(left);
-                        trsmB.push_back(right);
-                    }
+                    const bool a_first = cls_[a].order < cls_[b].order;
+                    // right solve by L_a (B is size(b) x size(a)), left solve by L_b
+                    push_trsm(a_first ? trsmA : trsmB, La, e.A, size(a), size(b), 1, sizeof(T));
+                    push_trsm(a_first ? trsmB : trsmA, Lb, e.A, size(b), size(a), 0, sizeof(T));
                 }
             }
             for (int c : live) {
@@ -538,6 +556,10 @@ void Tree::factorize_impl() {
         std::vector<int> wave, widx;
         int nwaves = 0;
         std::vector<std::vector<RrqrTask>> wtasks;
+        std::vector<std::vector<RrqrBig>> wbig;       // stepped path, per wave
+        std::vector<std::vector<RrqrSlab>> wslab;
+        std::vector<int> wsteps, wmaxrows;
+        std::vector<std::pair<int, int>> where;      // (0 small | 1 big, index) by plist position
         std::vector<RrqrSeg> segs;
         if (do_sparsify_level) {
             std::vector<int> inP(cls_.size(), -1);
@@ -562,6 +584,11 @@ void Tree::factorize_impl() {
                 nwaves = std::max(nwaves, w + 1);
             }
             wtasks.assign(nwaves, {});
+            wbig.assign(nwaves, {});
+            wslab.assign(nwaves, {});
+            wsteps.assign(nwaves, 0);
+            wmaxrows.assign(nwaves, 0);
+            where.assign(plist.size(), {0, 0});
             widx.assign(plist.size(), 0);
             for (size_t p = 0; p < plist.size(); p++) {
                 const int c = plist[p];
@@ -584,17 +611,36 @@ void Tree::factorize_impl() {
                 t.max_rows = s.size;
                 t.max_cols = cols;
                 const int mn = std::min(s.size, cols);
-                if ((long long)s.size * cols > lds_elems)
-                    t.scratch = arena_.alloc((size_t)s.size * cols * sizeof(T));
-                if (cols > 1024) {
-                    t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * cols * sizeof(double)));
-                    t.jpvt = reinterpret_cast<int*>(arena_.alloc((size_t)cols * sizeof(int)));
-                }
                 t.Q = arena_.alloc((size_t)std::max(1, s.size * mn) * sizeof(T));
                 t.tau = arena_.alloc((size_t)std::max(1, mn) * sizeof(T));
                 lst.flops_rrqr += mn > 0 ? 2.0 * s.size * cols * mn - (2.0 / 3.0) * (double)mn * mn * mn : 0.0;
-                widx[p] = (int)wtasks[wave[p]].size();
-                wtasks[wave[p]].push_back(t);
+                const int w = wave[p];
+                if (s.size >= big_min_rows && (long long)s.size * cols > big_min_elems) {
+                    // stepped multi-workgroup path
+                    RrqrBig b{};
+                    b.nslab = std::max(1, (cols + 63) / 64);
+                    t.scratch = arena_.alloc((size_t)s.size * b.nslab * 64 * sizeof(T));
+                    t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * std::max(cols, 1) * sizeof(double)));
+                    b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
+                    b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)2 * b.nslab * sizeof(double)));
+                    b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
+                    b.dims = reinterpret_cast<int*>(arena_.alloc(4 * sizeof(int)));
+                    b.t = t;
+                    where[p] = {1, (int)wbig[w].size()};
+                    for (int k = 0; k < b.nslab; k++) wslab[w].push_back({(int)wbig[w].size(), k});
+                    wbig[w].push_back(b);
+                    wsteps[w] = std::max(wsteps[w], mn);
+                    wmaxrows[w] = std::max(wmaxrows[w], s.size);
+                } else {
+                    if ((long long)s.size * cols > lds_elems)
+                        t.scratch = arena_.alloc((size_t)s.size * cols * sizeof(T));
+                    if (cols > 1024) {
+                        t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * cols * sizeof(double)));
+                        t.jpvt = reinterpret_cast<int*>(arena_.alloc((size_t)cols * sizeof(int)));
+                    }
+                    where[p] = {0, (int)wtasks[w].size()};
+                    wtasks[w].push_back(t);
+                }
             }
         }
         lst.nwaves = nwaves;
@@ -603,8 +649,12 @@ void Tree::factorize_impl() {
         Packer pk;
         size_t o_pe = pk.add(potrfE), o_te = pk.add(trsmE), o_ti = pk.add(tiles), o_co = pk.add(contribs);
         size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
-        std::vector<size_t> o_w(nwaves);
-        for (int w = 0; w < nwaves; w++) o_w[w] = pk.add(wtasks[w]);
+        std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves);
+        for (int w = 0; w < nwaves; w++) {
+            o_w[w] = pk.add(wtasks[w]);
+            o_b[w] = pk.add(wbig[w]);
+            o_s[w] = pk.add(wslab[w]);
+        }
         const double t_plan1 = wall();
         void* dt = upload_tasks(pk.buf);
         hip_check(hipMemcpy(d_size_, hsize.data(), cls_.size() * sizeof(int), hipMemcpyHostToDevice), "upload sizes");
@@ -617,10 +667,15 @@ void Tree::factorize_impl() {
         hip_check(launch_potrf<T>(Packer::at<PotrfTask>(dt, o_ps), (int)potrfS.size(), d_info_, st), "potrf scale");
         hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_ta), (int)trsmA.size(), st), "trsm scale A");
         hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_tb), (int)trsmB.size(), st), "trsm scale B");
-        for (int w = 0; w < nwaves; w++)
+        for (int w = 0; w < nwaves; w++) {
             hip_check(launch_rrqr<T>(Packer::at<RrqrTask>(dt, o_w[w]), (int)wtasks[w].size(),
                                      Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st),
                       "rrqr");
+            hip_check(launch_rrqr_big<T>(Packer::at<RrqrBig>(dt, o_b[w]), (int)wbig[w].size(),
+                                         Packer::at<RrqrSlab>(dt, o_s[w]), (int)wslab[w].size(),
+                                         Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, wsteps[w], wmaxrows[w], st),
+                      "rrqr (stepped)");
+        }
         int info = 0;
         hip_check(hipMemcpy(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost), "read info");
         if (info != 0)
@@ -639,7 +694,8 @@ void Tree::factorize_impl() {
             lst.rank_before += s0;
             lst.nsparsified++;
             if (r < s0) {
-                const RrqrTask* t = &wtasks[wave[p]][widx[p]];
+                const RrqrTask* t = where[p].first ? &wbig[wave[p]][where[p].second].t
+                                                   : &wtasks[wave[p]][where[p].second];
                 ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
                 factor_bytes_ += ((size_t)s0 * r + r) * sizeof(T);
                 if (r > 0) qt.push_back({t->Q, t->tau, s0, r, (int)cls_[c].xoff, 0});

@@ -1,0 +1,61 @@
+"""The C ABI library loads on a machine without a GPU and exports every function that
+include/spand.h declares; argument and state errors come back as status codes."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import spand_public_amd as S
+from spand_public_amd import _lib
+from spand_public_amd.problems import linspace_nd, neglapl, symmetric_graph
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "spand.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(spand_[A-Za-z_0-9]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = _lib.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), s
+    # and the Python binding declares a signature for each of them
+    assert set(syms) == set(_lib.SIGNATURES)
+
+
+def test_status_codes_without_gpu():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.spand_create(0, C.byref(h)) == 1  # SPAND_ERR_ARG
+    assert L.spand_create(3, C.byref(h)) == 0
+    assert L.spand_factorize(h) == 5  # SPAND_ERR_STATE: factorize before assemble
+    assert L.spand_set_symm_kind(h, 1) == 8  # SPAND_ERR_UNSUPPORTED (SYM / LDLT not built)
+    assert L.spand_set_tol(h, -1.0) == 1
+    assert b"tol" in L.spand_last_error(h)
+    A = neglapl(5, 2)
+    # partition without coordinates: only geometric partitioning is available
+    assert L.spand_partition(h, A.n, A.colptr, A.rowval, None) == 8
+    L.spand_destroy(h)
+
+
+def test_python_mirror_errors():
+    t = S.Tree(3)
+    with pytest.raises(S.SpandError):
+        t.factorize()
+    with pytest.raises(S.SpandError):
+        S.Tree(0)
+    A, X = neglapl(5, 2), linspace_nd(5, 2)
+    t.set_use_geo(True)
+    t.set_Xcoo(X)
+    t.partition(symmetric_graph(A))
+    t.assemble(A)
+    assert t.get_N() == 25
+    with pytest.raises(S.SpandError):
+        t.solve(np.ones(25))  # solve before factorize

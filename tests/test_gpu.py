@@ -1,0 +1,185 @@
+"""GPU parity tests: the HIP factorization and solve (through the C ABI, include/spand.h)
+against the CPU oracle on the same inputs.
+
+FP64: identical op sequence (types, cluster orders, sizes - i.e. identical kept ranks),
+every stored block within 1e-12 relative of the oracle's (the kernels follow the oracle's
+per-element FMA order, so the observed difference is at the ulp level), solves within
+1e-12. FP32: solves within 1e-4 of the FP64 oracle. PCG: iteration counts within 10% of
+the oracle's CG and true ||b - Ax|| / ||b|| <= 1e-12 (src/is.cpp:39-121)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import spand_public_amd as S
+from spand_public_amd.problems import linspace_nd, neglapl, symmetric_graph
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(5, 2, 3, 0.0), (20, 2, 5, 1e-2), (10, 3, 5, 1e-2), (15, 3, 6, 0.0), (16, 3, 7, 1e-2), (128, 2, 7, 1e-2)]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def gpu_tree(A, X, nl, tol, prec=S.FP64, skip=0):
+    t = S.Tree(nl)
+    t.set_use_geo(True)
+    t.set_Xcoo(X)
+    t.set_tol(tol)
+    t.set_skip(skip)
+    t.set_precision(prec)
+    t.partition(symmetric_graph(A))
+    t.assemble(A)
+    return t
+
+
+@pytest.fixture
+def stepped_rrqr(request, monkeypatch):
+    if request.param:
+        monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
+        monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
+    return request.param
+
+
+@pytest.mark.parametrize("stepped_rrqr", [False, True], indirect=True, ids=["wg-rrqr", "stepped-rrqr"])
+@pytest.mark.parametrize("n,d,nl,tol", CASES)
+def test_fp64_factor_matches_oracle(n, d, nl, tol, stepped_rrqr):
+    say(f"case fp64 {d}D n={n} levels={nl} tol={tol} stepped={stepped_rrqr}")
+    A, X = neglapl(n, d), linspace_nd(n, d)
+    o, _ = oracle.setup_tree(A, X, nl, tol)
+    o.factorize()
+    g = gpu_tree(A, X, nl, tol)
+    g.factorize()
+    oi, gi = o.ops(), g.ops()
+    assert np.array_equal(oi, gi), "op sequence / kept ranks differ from the oracle"
+    worst = 0.0
+    for i in range(len(oi)):
+        t, r, c = oi[i, 0], oi[i, 5], oi[i, 6]
+        if t in (7, 8) or r * c == 0:
+            continue
+        od, gd = o.op_data(i, oi[i]), g.op_data(i, gi[i])
+        M0, M1 = od["M"], gd["M"]
+        if t == 0:
+            M0, M1 = np.tril(M0), np.tril(M1)
+        rel = np.linalg.norm(M0 - M1) / max(np.linalg.norm(M0), 1e-300)
+        if t == 6:
+            rel = max(rel, np.abs(od["tau"] - gd["tau"]).max())
+        worst = max(worst, rel)
+    assert worst <= 1e-12, worst
+    b = np.random.default_rng(1).standard_normal(A.n)
+    xo, xg = o.solve(b), g.solve(b)
+    assert np.linalg.norm(xo - xg) <= 1e-12 * np.linalg.norm(xo)
+    assert g.nnz() == o.nnz() and g.get_stop() == o.get_stop()
+
+
+@pytest.mark.parametrize("n,d,nl,tol", [(20, 2, 5, 1e-2), (16, 3, 7, 1e-2), (15, 3, 6, 0.0)])
+def test_fp32_factor_solve_close_to_oracle(n, d, nl, tol):
+    say(f"case fp32 {d}D n={n} levels={nl} tol={tol}")
+    A, X = neglapl(n, d), linspace_nd(n, d)
+    o, _ = oracle.setup_tree(A, X, nl, tol)
+    o.factorize()
+    g = gpu_tree(A, X, nl, tol, prec=S.FP32)
+    g.factorize()
+    b = np.random.default_rng(2).standard_normal(A.n)
+    xo, xg = o.solve(b), g.solve(b)
+    assert np.linalg.norm(xo - xg) <= 1e-4 * np.linalg.norm(xo)
+    # kept ranks: same total within 1% (FP32 rounding can move a singular value across tol)
+    ro = o.get_log()[:, 6].sum()
+    rg = g.level_stats()[:, 6].sum()
+    assert abs(ro - rg) <= 0.01 * max(ro, 1)
+
+
+@pytest.mark.parametrize("n,d,nl", [(128, 2, 7), (24, 3, 8)])
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32], ids=["fp64", "fp32"])
+def test_pcg_iterations_and_residual(n, d, nl, prec):
+    say(f"case pcg {d}D n={n} levels={nl} prec={prec}")
+    A, X = neglapl(n, d), linspace_nd(n, d)
+    B = np.random.default_rng(2019).standard_normal((A.n, 4))
+    o, _ = oracle.setup_tree(A, X, nl, 1e-2)
+    o.factorize()
+    its_o = [o.cg(A, B[:, j], iters=200, tol=1e-12)[1] for j in range(4)]
+    g = gpu_tree(A, X, nl, 1e-2, prec=prec)
+    g.factorize()
+    x, info = S.cg(A, B, g, iters=200, tol=1e-12)
+    res = np.linalg.norm(A.matvec(x) - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert res.max() <= 1e-12, res
+    for io, ig in zip(its_o, info["iters"]):
+        assert abs(int(ig) - io) <= max(1, 0.1 * io), (its_o, info["iters"])
+
+
+@pytest.mark.parametrize("n,d", [(20, 2), (10, 3)])
+def test_approx_tolerances(n, d):
+    # ApproxTest.Approx (tests/tests.cpp:799-856) on the GPU path
+    A, X = neglapl(n, d), linspace_nd(n, d)
+    b = np.random.default_rng(3).uniform(-1, 1, A.n)
+    for nl in (2, 4, 6):
+        for tol in (0.0, 1e-10, 1e-6, 1e-2, 10.0):
+            for skip in (0, 1, 2):
+                say(f"case approx {d}D n={n} levels={nl} tol={tol} skip={skip}")
+                g = gpu_tree(A, X, nl, tol, skip=skip)
+                g.factorize()
+                err = np.linalg.norm(A.matvec(g.solve(b)) - b) / np.linalg.norm(b)
+                assert err <= (5e-12 if tol == 0 else tol * 2e2), (nl, tol, skip, err)
+
+
+def test_repro_and_refactorization_bitwise():
+    # ApproxTest.Repro (tests/tests.cpp:917-994): reruns and refactorizations are bitwise equal
+    A, X = neglapl(16, 3), linspace_nd(16, 3)
+    b = np.random.default_rng(4).standard_normal(A.n)
+    g = gpu_tree(A, X, 7, 1e-2)
+    ref = None
+    for rep in range(3):
+        say(f"case repro rep={rep}")
+        g.factorize()
+        x = g.solve(b)
+        if ref is None:
+            ref = x
+        assert np.array_equal(ref, x)
+    g2 = gpu_tree(A, X, 7, 1e-2)
+    g2.factorize()
+    assert np.array_equal(ref, g2.solve(b))
+
+
+def test_multi_rhs_and_device_solve():
+    torch = pytest.importorskip("torch")
+    A, X = neglapl(12, 3), linspace_nd(12, 3)
+    g = gpu_tree(A, X, 6, 1e-2)
+    g.factorize()
+    B = np.random.default_rng(6).standard_normal((A.n, 8))
+    Xh = g.solve(B)
+    for j in range(8):
+        assert np.array_equal(Xh[:, j], g.solve(B[:, j]))
+    d = torch.tensor(np.asfortranarray(B).T.copy(), dtype=torch.float64, device="cuda:0")  # column-major N x 8
+    g.solve_device(d.data_ptr(), 8, A.n)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy().T, Xh)
+
+
+def test_edge_cases():
+    say("case edge: one level, tiny grid, zero rhs")
+    A, X = neglapl(3, 2), linspace_nd(3, 2)
+    g = gpu_tree(A, X, 1, 1e-2)
+    g.factorize()
+    b = np.arange(9.0)
+    assert np.allclose(A.matvec(g.solve(b)), b, atol=1e-12)
+    A, X = neglapl(10, 2), linspace_nd(10, 2)
+    g = gpu_tree(A, X, 4, 1e-2)
+    g.factorize()
+    B = np.zeros((A.n, 2))
+    B[:, 1] = 1.0
+    x, info = S.cg(A, B, g, iters=50, tol=1e-12)
+    assert info["iters"][0] == 0 and np.all(x[:, 0] == 0)
+    assert np.linalg.norm(A.matvec(x[:, 1]) - B[:, 1]) <= 1e-12 * np.linalg.norm(B[:, 1])
+
+
+def test_not_spd_reports_status():
+    say("case not-spd")
+    A, X = neglapl(10, 2), linspace_nd(10, 2)
+    A.nzval = -A.nzval
+    g = gpu_tree(A, X, 3, 1e-2)
+    with pytest.raises(S.SpandError, match="NOT_SPD"):
+        g.factorize()
