@@ -97,6 +97,19 @@ spand_status spand_get_level_stats(const spand_tree* t, double* out);
 spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long long* factor_bytes,
                                    long long* arena_bytes);
 
+/* Kernel timing (the reference's timer log, include/util.h:366-401, at kernel granularity):
+ * when enabled, every kernel launch of the tree (factorization, preconditioner apply) and of
+ * spand_pcg is bracketed by two HIP events on its stream. Off by default; enabling it adds
+ * two event records per launch. spand_kernel_profile waits for the recorded events and
+ * returns, for kernel family `family` (0 <= family < spand_kernel_family_count()): launches,
+ * summed event time (ms), algorithmic bytes and flops of those launches. */
+spand_status spand_set_profiling(spand_tree* t, int on);
+spand_status spand_profile_reset(spand_tree* t);
+int spand_kernel_family_count(void);
+const char* spand_kernel_family_name(int family);
+spand_status spand_kernel_profile(spand_tree* t, int family, long long* launches, double* ms, double* bytes,
+                                  double* flops);
+
 /* Recorded operations (the Operation list, include/operations.h), for op-by-op comparison.
  * 8 ints per op: type (0 ScalingLLT, 2 GemmSymmOut, 3 GemmSymmIn, 6 Orthogonal, 7 Merge,
  * 8 Split), order(a), order(b), size(a), size(b), rows, cols, level. */

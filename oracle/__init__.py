@@ -73,6 +73,10 @@ def lib():
                                C.c_int]),
             "or_gemm": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _f64pF, C.c_int, _f64pF,
                                C.c_int, C.c_double, _f64pF, C.c_int]),
+            "or_schur": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f64pF, C.c_int, _f64pF,
+                                C.c_int, _f64pF, C.c_int]),
+            "or_sumsq16": (C.c_double, [C.c_int, _f64p]),
+            "or_dot16": (C.c_double, [C.c_int, _f64p, _f64p]),
             "or_geqp3": (None, [C.c_int, C.c_int, _f64pF, C.c_int, _i32p, _f64p]),
             "or_ormqr_vec": (None, [C.c_int, C.c_int, C.c_int, _f64pF, C.c_int, _f64p, _f64p]),
             "or_getrf": (C.c_int, [C.c_int, _f64pF, C.c_int, _i32p]),

@@ -19,6 +19,28 @@ double nrm2(int n, const double* x, int incx) {
     return std::sqrt(s);
 }
 
+double tree16(double c[kChains]) {
+    for (int w = 0; w < 8; w++) c[w] = c[w] + c[w + 8];
+    for (int w = 0; w < 4; w++) c[w] = c[w] + c[w + 4];
+    for (int w = 0; w < 2; w++) c[w] = c[w] + c[w + 2];
+    return c[0] + c[1];
+}
+
+double sumsq16(int n, const double* x, int incx) {
+    double c[kChains] = {0.0};
+    for (int t = 0; t < n; t++) {
+        const double a = x[(size_t)t * incx];
+        c[t % kChains] = std::fma(a, a, c[t % kChains]);
+    }
+    return tree16(c);
+}
+
+double dot16(int n, const double* a, const double* b) {
+    double c[kChains] = {0.0};
+    for (int t = 0; t < n; t++) c[t % kChains] = std::fma(a[t], b[t], c[t % kChains]);
+    return tree16(c);
+}
+
 // Right-looking column Cholesky of the lower triangle with explicit fused multiply-adds:
 // column j is scaled by 1/L(j,j), then every trailing lower element receives
 // A(a,b) = fma(-L(a,j), L(b,j), A(a,b)). Each element therefore sees its updates in the
@@ -210,7 +232,7 @@ void larfg(int n, double* alpha, double* x, int incx, double* tau) {
         *tau = 0.0;
         return;
     }
-    double xnorm = nrm2(n - 1, x, incx);
+    double xnorm = std::sqrt(sumsq16(n - 1, x, incx));
     if (xnorm == 0.0) {
         *tau = 0.0;
         return;
@@ -226,7 +248,7 @@ void larfg(int n, double* alpha, double* x, int incx, double* tau) {
             beta *= rsafmn;
             *alpha *= rsafmn;
         } while (std::fabs(beta) < safmin && knt < 20);
-        xnorm = nrm2(n - 1, x, incx);
+        xnorm = std::sqrt(sumsq16(n - 1, x, incx));
         beta = -std::copysign(lapy2(*alpha, xnorm), *alpha);
     }
     *tau = (beta - *alpha) / beta;
@@ -239,13 +261,8 @@ void larfg(int n, double* alpha, double* x, int incx, double* tau) {
 void larf_left(int m, int n, const double* v, double tau, double* C, int ldc, double* work) {
     if (tau == 0.0 || m == 0 || n == 0) return;
     // w := C^T v  (dgemv Trans, beta = 0)
-    for (int j = 0; j < n; j++) {
-        const double* c = C + (size_t)j * ldc;
-        double t = 0.0;
-        for (int i = 0; i < m; i++) t = std::fma(c[i], v[i], t);
-        work[j] = t;
-    }
-    // C := C - tau v w^T  (dger, alpha = -tau)
+    for (int j = 0; j < n; j++) work[j] = dot16(m, C + (size_t)j * ldc, v);
+    // C := C - tau v w^T  (dger, alpha = -tau; reference dger skips columns with w_j = 0)
     for (int j = 0; j < n; j++) {
         if (work[j] != 0.0) {
             double t = -tau * work[j];
@@ -262,7 +279,7 @@ void geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau) {
     std::vector<double> vn1(n), vn2(n), work(n), vcol(m);
     for (int j = 0; j < n; j++) {
         jpvt[j] = j;
-        vn1[j] = nrm2(m, A + (size_t)j * lda, 1);
+        vn1[j] = std::sqrt(sumsq16(m, A + (size_t)j * lda, 1));
         vn2[j] = vn1[j];
     }
     for (int i = 0; i < mn; i++) {
@@ -301,7 +318,7 @@ void geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau) {
                 double temp2 = temp * r * r;
                 if (temp2 <= tol3z) {
                     if (i < m - 1) {
-                        vn1[j] = nrm2(m - i - 1, &IDX(A, i + 1, j, lda), 1);
+                        vn1[j] = std::sqrt(sumsq16(m - i - 1, &IDX(A, i + 1, j, lda), 1));
                         vn2[j] = vn1[j];
                     } else {
                         vn1[j] = 0.0;

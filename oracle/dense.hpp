@@ -12,8 +12,20 @@
 namespace oracle {
 
 // dnrm2-equivalent: sequential sum of squares in index order with fused multiply-add,
-// then sqrt. (The same order is used by the GPU RRQR column-norm threads.)
+// then sqrt.
 double nrm2(int n, const double* x, int incx);
+
+// Reduction order of the column-pivoted QR (geqp3 below; the GPU RRQR kernels follow it
+// term for term). The n terms are dealt round-robin to 16 chains (term t to chain t % 16);
+// each chain accumulates its terms in increasing t with fused multiply-adds, starting from
+// 0; the 16 chain values are combined by the halving tree
+//   c[w] += c[w + 8] (w < 8), c[w] += c[w + 4] (w < 4), c[w] += c[w + 2] (w < 2), c[0] + c[1].
+// BLAS leaves the order of dnrm2 / ddot / dgemv sums to the implementation; this one is
+// chosen so that a 16-wavefront workgroup computes every sum in parallel.
+constexpr int kChains = 16;
+double tree16(double c[kChains]);
+double sumsq16(int n, const double* x, int incx);           // sum x[t]^2
+double dot16(int n, const double* a, const double* b);      // sum a[t] * b[t]
 
 // dpotrf('L') semantics: A = L L^T in place (lower triangle), returns info (0 or j+1 of
 // the first non-positive pivot). Reference call site: src/util.cpp:158-165.
@@ -53,12 +65,16 @@ void gemv_sub_notrans(int m, int n, const double* A, int lda, const double* x, d
 // y := y - A^T x (dgemv Trans, alpha=-1, beta=1), A is m x n. src/util.cpp:317-326.
 void gemv_sub_trans(int m, int n, const double* A, int lda, const double* x, double* y);
 
-// dlarfg: generates H = I - tau v v^T with v(0)=1 such that H [alpha; x] = [beta; 0].
+// dlarfg: generates H = I - tau v v^T with v(0)=1 such that H [alpha; x] = [beta; 0];
+// ||x|| by sumsq16.
 void larfg(int n, double* alpha, double* x, int incx, double* tau);
-// dlarf 'Left': C := (I - tau v v^T) C, C is m x n, v has v(0) explicitly stored.
+// dlarf 'Left': C := (I - tau v v^T) C, C is m x n, v has v(0) explicitly stored;
+// w = C^T v by dot16.
 void larf_left(int m, int n, const double* v, double tau, double* C, int ldc, double* work);
 
-// dgeqp3 (all columns free): column-pivoted Householder QR, unblocked dlaqp2 order.
+// dgeqp3 (all columns free): column-pivoted Householder QR in the unblocked dlaqp2 step
+// order (pivot = first column of largest partial norm, LAWN 176 norm downdating with
+// recomputation below tol3z = sqrt(eps/2)), every sum in the sumsq16 / dot16 order.
 // jpvt is returned 0-based. tau has min(m,n) entries. src/util.cpp:381-394.
 void geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau);
 

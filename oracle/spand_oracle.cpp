@@ -1160,6 +1160,12 @@ void or_gemm(int tA, int tB, int m, int n, int k, double alpha, const double* A,
              double beta, double* C, int ldc) {
     gemm(tA != 0, tB != 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
+void or_schur(int tA, int tB, int lower, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+              double* C, int ldc) {
+    schur_update(tA != 0, tB != 0, lower != 0, m, n, k, A, lda, B, ldb, C, ldc);
+}
+double or_sumsq16(int n, const double* x) { return sumsq16(n, x, 1); }
+double or_dot16(int n, const double* a, const double* b) { return dot16(n, a, b); }
 void or_geqp3(int m, int n, double* A, int lda, int* jpvt, double* tau) { geqp3(m, n, A, lda, jpvt, tau); }
 void or_ormqr_vec(int trans, int m, int k, const double* V, int ldv, const double* tau, double* x) {
     ormqr_vec(trans != 0, m, k, V, ldv, tau, x);

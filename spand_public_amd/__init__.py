@@ -20,7 +20,7 @@ from . import problems
 from ._lib import KrylovStats, SpandError, lib
 from .problems import CSC
 
-__all__ = ["Tree", "Matrix", "cg", "problems", "SpandError", "CSC", "FP64", "FP32"]
+__all__ = ["Tree", "Matrix", "cg", "cg_device", "problems", "SpandError", "CSC", "FP64", "FP32"]
 
 FP64 = 0
 FP32 = 1
@@ -155,6 +155,24 @@ class Tree:
         _check(self._h, self._L.spand_get_factor_info(self._h, C.byref(s), C.byref(fb), C.byref(ab)))
         return {"seconds": s.value, "factor_bytes": fb.value, "arena_bytes": ab.value}
 
+    # -- kernel timing (HIP events per launch) --
+    def set_profiling(self, on: bool):
+        _check(self._h, self._L.spand_set_profiling(self._h, int(bool(on))))
+
+    def profile_reset(self):
+        _check(self._h, self._L.spand_profile_reset(self._h))
+
+    def kernel_profile(self) -> dict:
+        """{family: {"launches", "ms", "bytes", "flops"}} accumulated since the last reset."""
+        out = {}
+        for f in range(self._L.spand_kernel_family_count()):
+            n, ms, by, fl = C.c_longlong(), C.c_double(), C.c_double(), C.c_double()
+            _check(self._h, self._L.spand_kernel_profile(self._h, f, C.byref(n), C.byref(ms), C.byref(by), C.byref(fl)))
+            if n.value:
+                out[self._L.spand_kernel_family_name(f).decode()] = {"launches": n.value, "ms": ms.value,
+                                                                      "bytes": by.value, "flops": fl.value}
+        return out
+
     def ops(self) -> np.ndarray:
         n = C.c_int()
         _check(self._h, self._L.spand_num_ops(self._h, C.byref(n)))
@@ -192,6 +210,19 @@ class Matrix:
             except Exception:
                 pass
             self._h = None
+
+
+def cg_device(A: "Matrix", precond: Tree, d_b: int, d_x: int, nrhs: int, ld: int, iters: int = 100,
+              tol: float = 1e-12):
+    """PCG on device-resident column-major blocks (pointers d_b, d_x with leading dimension
+    ld; d_x holds x0 on entry and the solution on return). Returns (iters per column, stats)."""
+    its = np.zeros(nrhs, dtype=np.int32)
+    st = KrylovStats()
+    status = lib().spand_pcg(precond._h, A._h, C.c_void_p(d_b), C.c_void_p(d_x), int(nrhs), int(ld), int(iters),
+                             float(tol), 1, its.ctypes.data, C.byref(st))
+    if status not in (0, 7):
+        _check(precond._h, status)
+    return its, {"status": status, **{f: getattr(st, f) for f, _ in KrylovStats._fields_}}
 
 
 def cg(A, b, precond: Tree, x=None, iters: int = 100, tol: float = 1e-12, raise_on_fail: bool = False):

@@ -53,6 +53,12 @@ SIGNATURES = {
     "spand_get_level_stats": (C.c_int, [vp, f64p]),
     "spand_get_factor_info": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),
+    "spand_set_profiling": (C.c_int, [vp, C.c_int]),
+    "spand_profile_reset": (C.c_int, [vp]),
+    "spand_kernel_family_count": (C.c_int, []),
+    "spand_kernel_family_name": (C.c_char_p, [C.c_int]),
+    "spand_kernel_profile": (C.c_int, [vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "spand_num_ops": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "spand_ops_info": (C.c_int, [vp, i32p]),
     "spand_op_data": (C.c_int, [vp, C.c_int, vp, vp]),

@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspand.so")
 
-SOURCES = ["kernels_factor.hip", "kernels_solve.hip", "partition.cpp", "tree.cpp", "krylov.cpp", "capi.cpp"]
+SOURCES = ["kernels_factor.hip", "kernels_solve.hip", "partition.cpp", "tree.cpp", "krylov.cpp", "profile.cpp",
+           "capi.cpp"]
 ARCH = "gfx950"
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -34,9 +35,14 @@ def hipcc():
     raise RuntimeError("hipcc not found: the spaND HIP library cannot be built")
 
 
+# The factorization kernels reproduce the CPU oracle's rounding step for step: every fused
+# multiply-add there is an explicit fma(), and nothing else may be contracted.
+PER_FILE = {"kernels_factor.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [hipcc(), *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *FLAGS, *PER_FILE.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -183,6 +183,35 @@ spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long lo
     if (ab) *ab = (long long)t->t->arena_bytes();
     return SPAND_OK;
 }
+spand_status spand_set_profiling(spand_tree* t, int on) {
+    CHECK_T(t);
+    return guarded(t, [&] {
+        t->t->profiler().collect();
+        t->t->profiler().enable(on != 0);
+    });
+}
+spand_status spand_profile_reset(spand_tree* t) {
+    CHECK_T(t);
+    return guarded(t, [&] { t->t->profiler().reset(); });
+}
+int spand_kernel_family_count(void) { return spand::KF_COUNT; }
+const char* spand_kernel_family_name(int family) { return spand::kernel_family_name(family); }
+spand_status spand_kernel_profile(spand_tree* t, int family, long long* launches, double* ms, double* bytes,
+                                  double* flops) {
+    CHECK_T(t);
+    if (family < 0 || family >= spand::KF_COUNT) {
+        t->err = "kernel family out of range";
+        return SPAND_ERR_ARG;
+    }
+    return guarded(t, [&] {
+        t->t->profiler().collect();
+        spand::KernelTotals k = t->t->profiler().totals(family);
+        if (launches) *launches = k.launches;
+        if (ms) *ms = k.ms;
+        if (bytes) *bytes = k.bytes;
+        if (flops) *flops = k.flops;
+    });
+}
 spand_status spand_num_ops(const spand_tree* t, int* n) {
     CHECK_T(t);
     *n = (int)t->t->ops().size();

@@ -25,9 +25,19 @@ template <typename T>
 hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
                        hipStream_t st);
 // Stepped RRQR of the large clusters of one wave: init, then steps 0..nsteps-1, then finish.
+// lds_bytes is the dynamic LDS of the pivot-column / reflector buffers (2 x max_rows
+// elements for a step, max_rows for finish), or 0 when the tasks carry global buffers
+// (RrqrBig::vbuf) because they exceed STEP_LDS_MAX.
+constexpr int STEP_LDS_MAX = 40960;
 template <typename T>
-hipError_t launch_rrqr_big(const RrqrBig* d_bigs, int nbig, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
-                           int* d_size, double tol, int nsteps, int max_rows, hipStream_t st);
+hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
+                            const int* d_size, double tol, hipStream_t st);
+template <typename T>
+hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol,
+                            size_t lds_bytes, hipStream_t st);
+template <typename T>
+hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, int nbig, const RrqrSeg* d_segs, int* d_size, double tol,
+                              size_t lds_bytes, hipStream_t st);
 template <typename T> hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st);
 
 // Preconditioner sweeps. ws holds the per-cluster segments, interleaved over nrhs.
@@ -40,11 +50,17 @@ hipError_t launch_perm_out(const T* ws, const int* perm, int N, int nrhs, double
                            hipStream_t st);
 hipError_t launch_cm2il(const double* src, int ld, int N, int nrhs, double* dst, hipStream_t st);
 hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld, hipStream_t st);
+// x := Linv x (trans 0) or Linv^T x (trans 1) with the explicit inverse factor.
 template <typename T>
-hipError_t launch_trsv(const TrsvTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
+hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
+// compact-WY factors of the Orthogonal ops
+template <typename T> hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st);
 template <typename T>
 hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, const GemvTerm* d_terms, T* ws, int nrhs,
                        hipStream_t st);
+// LDS of one solve workgroup; a segment of m rows with k reflectors needs
+// (m + 2k) * sizeof(T) bytes per right-hand side in flight.
+constexpr int SOLVE_LDS_BYTES = 65536;
 template <typename T>
 hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
 template <typename T>

@@ -243,13 +243,22 @@ __global__ void __launch_bounds__(NT) k_schur(const SchurTile* tiles, const Schu
 }
 
 // ---------------------------------------------------------------------------------------
-// RRQR: gather Asn, dgeqp3 (dlaqp2 order: first-max pivot, LAWN-176 norm downdate with
-// recomputation below sqrt(eps)), choose_rank, Q/tau store and scatter of R[:r,:] P^T into
-// the edge blocks. Column norms, the downdate and the rank test run in FP64 for both
-// element types. Scalar steps follow the reference LAPACK formulas one-for-one so that an
-// FP64 run on the same input reproduces LAPACK's pivot sequence.
+// RRQR: gather Asn, dgeqp3 (dlaqp2 step order: first-max pivot, LAWN-176 norm downdate with
+// recomputation below sqrt(eps/2)), choose_rank, Q/tau store and scatter of R[:r,:] P^T
+// into the edge blocks. Column norms, the downdate and the rank test run in FP64 for both
+// element types; scalar steps follow the reference LAPACK formulas one-for-one.
+//
+// Every sum (column norms, the dlarfg norm, w = v^T a_j) is taken in one fixed order: term
+// t goes to chain t % 16, each chain accumulates in increasing t with fused multiply-adds
+// from 0, and the 16 chains are combined by a halving tree (+8, +4, +2, +1). The CPU oracle
+// uses the same order, so FP64 runs agree bit for bit; sixteen wavefronts (one per chain)
+// compute each sum in parallel.
+//
+// The factorization stops at the rank: once |R_ii| / |R_00| < tol (choose_rank,
+// src/util.cpp:434-452) nothing of step i or later is kept, so those steps are skipped.
 constexpr int RRQR_MAXSEG = 1024;
 constexpr int RRQR_LDS_COLS = 1024;
+constexpr int NCH = 16;
 
 __device__ inline double lapy2_d(double x, double y) {
     double xa = fabs(x), ya = fabs(y);
@@ -257,6 +266,142 @@ __device__ inline double lapy2_d(double x, double y) {
     if (z == 0.0) return w;
     double q = z / w;
     return w * sqrt(1.0 + q * q);
+}
+
+// halving tree over the 16 chain values (c is overwritten)
+template <typename A>
+__device__ inline A tree16(A* c) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int w = 0; w < 8; w++) c[w] = c[w] + c[w + 8];
+#pragma unroll
+    for (int w = 0; w < 4; w++) c[w] = c[w] + c[w + 4];
+#pragma unroll
+    for (int w = 0; w < 2; w++) c[w] = c[w] + c[w + 2];
+    return c[0] + c[1];
+}
+
+// sum over t < n of g(t)^2 in the chain order, by one thread
+template <typename G>
+__device__ inline double sumsq16_serial(int n, G g) {
+#pragma clang fp contract(off)
+    double c[NCH];
+#pragma unroll
+    for (int w = 0; w < NCH; w++) c[w] = 0.0;
+    int t0 = 0;
+    for (; t0 + NCH <= n; t0 += NCH) {
+#pragma unroll
+        for (int w = 0; w < NCH; w++) {
+            const double v = g(t0 + w);
+            c[w] = fma(v, v, c[w]);
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < NCH; w++)
+        if (t0 + w < n) {
+            const double v = g(t0 + w);
+            c[w] = fma(v, v, c[w]);
+        }
+    return tree16(c);
+}
+
+// sum over t < n of a(t) * b(t) in the chain order, in T, by one thread
+template <typename T, typename GA, typename GB>
+__device__ inline T dot16_serial(int n, GA a, GB b) {
+#pragma clang fp contract(off)
+    T c[NCH];
+#pragma unroll
+    for (int w = 0; w < NCH; w++) c[w] = T(0);
+    int t0 = 0;
+    for (; t0 + NCH <= n; t0 += NCH) {
+#pragma unroll
+        for (int w = 0; w < NCH; w++) c[w] = fma(a(t0 + w), b(t0 + w), c[w]);
+    }
+#pragma unroll
+    for (int w = 0; w < NCH; w++)
+        if (t0 + w < n) c[w] = fma(a(t0 + w), b(t0 + w), c[w]);
+    return tree16(c);
+}
+
+// Same sum of squares by threads 0..15 of the block (one chain each); every thread calls,
+// the result is valid in thread 0. s_ch: 16 doubles of LDS.
+template <typename G>
+__device__ inline double sumsq16_block(int n, G g, double* s_ch) {
+#pragma clang fp contract(off)
+    if (threadIdx.x < NCH) {
+        double c = 0.0;
+        for (int t = threadIdx.x; t < n; t += NCH) {
+            const double v = g(t);
+            c = fma(v, v, c);
+        }
+        s_ch[threadIdx.x] = c;
+    }
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        double c[NCH];
+#pragma unroll
+        for (int w = 0; w < NCH; w++) c[w] = s_ch[w];
+        r = tree16(c);
+    }
+    return r;
+}
+
+// dlarfg scalars of the column segment [alpha; x(0..n1)): tau, beta, the final scale and
+// the number of safe-minimum rescalings (reference dlarfg). Every thread calls; sh[0..3] =
+// tau, beta, scal, knt on return (after a barrier).
+template <typename T>
+__device__ void larfg_block(double alpha, const T* x, int n1, double* sh, double* s_ch) {
+#pragma clang fp contract(off)
+    const double xs0 = sumsq16_block(n1, [&](int t) { return (double)x[t]; }, s_ch);
+    if (threadIdx.x == 0) {
+        double tau = 0.0, scal = 1.0, beta = alpha;
+        int knt = 0;
+        if (n1 >= 1) {
+            const double xnorm = sqrt(xs0);
+            if (xnorm != 0.0) {
+                const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+                const double rs = 1.0 / safmin;
+                beta = -copysign(lapy2_d(alpha, xnorm), alpha);
+                if (fabs(beta) < safmin) {
+                    do {
+                        knt++;
+                        beta *= rs;
+                        alpha *= rs;
+                    } while (fabs(beta) < safmin && knt < 20);
+                    const double xs = sumsq16_serial(n1, [&](int t) {
+                        T xv = x[t];
+                        for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
+                        return (double)xv;
+                    });
+                    beta = -copysign(lapy2_d(alpha, sqrt(xs)), alpha);
+                }
+                tau = (beta - alpha) / beta;
+                scal = 1.0 / (alpha - beta);
+                for (int k = 0; k < knt; k++) beta *= safmin;
+            }
+        }
+        sh[0] = tau;
+        sh[1] = beta;
+        sh[2] = scal;
+        sh[3] = (double)knt;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ inline T larfg_scale(T xv, int knt, double scal) {
+    const double rs = 1.0 / (DBL_MIN / (DBL_EPSILON * 0.5));
+    for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
+    return xv * (T)scal;
+}
+
+// choose_rank (src/util.cpp:434-452) decided during the factorization: -1 while undecided.
+__device__ inline int rank_fixed(double tol, int mn) {
+    if (tol == 0.0) return mn;
+    if (tol >= 1.0) return 0;
+    if (mn <= 1) return mn;
+    return -1;
 }
 
 template <typename T>
@@ -269,8 +414,7 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
     __shared__ int r_idx[NT];
     __shared__ double s_vn1[RRQR_LDS_COLS], s_vn2[RRQR_LDS_COLS];
     __shared__ int s_jp[RRQR_LDS_COLS];
-    __shared__ int s_rank;
-    __shared__ double s_tau, s_scal;
+    __shared__ double sh[4], s_ch[NCH];
 
     const RrqrTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x;
@@ -311,20 +455,17 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
     }
     __syncthreads();
     for (int j = tid; j < cols; j += NT) {
-        double s = 0.0;
         const T* a = A + (size_t)j * rows;
-        for (int i = 0; i < rows; i++) {
-            double v = (double)a[i];
-            s = fma(v, v, s);
-        }
         jp[j] = j;
-        vn1[j] = sqrt(s);
+        vn1[j] = sqrt(sumsq16_serial(rows, [&](int r) { return (double)a[r]; }));
         vn2[j] = vn1[j];
     }
     __syncthreads();
     const double tol3z = sqrt(DBL_EPSILON * 0.5);
-    const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
-    for (int i = 0; i < mn; i++) {
+    const int fixed = rank_fixed(tol, mn);
+    const int nsteps = fixed == 0 ? 0 : mn;
+    int rank = fixed >= 0 ? fixed : mn;
+    for (int i = 0; i < nsteps; i++) {
         // pivot = first index of the max of vn1[i:cols]
         double bv = -1.0;
         int bi = cols;
@@ -362,60 +503,34 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
         __syncthreads();
         // dlarfg on A(i:rows, i)
         T* ci = A + (size_t)i * rows;
-        if (tid == 0) {
-            double tau = 0.0, scal = 1.0;
-            const int nh = rows - i;
-            if (nh > 1) {
-                double xs = 0.0;
-                for (int r = i + 1; r < rows; r++) {
-                    double v = (double)ci[r];
-                    xs = fma(v, v, xs);
-                }
-                double xnorm = sqrt(xs);
-                if (xnorm != 0.0) {
-                    double alpha = (double)ci[i];
-                    double beta = -copysign(lapy2_d(alpha, xnorm), alpha);
-                    int knt = 0;
-                    if (fabs(beta) < safmin) {
-                        const double rs = 1.0 / safmin;
-                        do {
-                            knt++;
-                            for (int r = i + 1; r < rows; r++) ci[r] = (T)((double)ci[r] * rs);
-                            beta *= rs;
-                            alpha *= rs;
-                        } while (fabs(beta) < safmin && knt < 20);
-                        xs = 0.0;
-                        for (int r = i + 1; r < rows; r++) {
-                            double v = (double)ci[r];
-                            xs = fma(v, v, xs);
-                        }
-                        beta = -copysign(lapy2_d(alpha, sqrt(xs)), alpha);
-                    }
-                    tau = (beta - alpha) / beta;
-                    scal = 1.0 / (alpha - beta);
-                    for (int j = 0; j < knt; j++) beta *= safmin;
-                    ci[i] = (T)beta;
-                }
-            }
-            s_tau = tau;
-            s_scal = scal;
+        larfg_block<T>((double)ci[i], ci + i + 1, rows - i - 1, sh, s_ch);
+        const double taud = sh[0];
+        if (taud != 0.0) {
+            const int knt = (int)sh[3];
+            const double scal = sh[2];
+            for (int r = i + 1 + tid; r < rows; r += NT) ci[r] = larfg_scale<T>(ci[r], knt, scal);
         }
         __syncthreads();
-        const T tau = (T)s_tau;
-        if (s_tau != 0.0) {
-            const T scal = (T)s_scal;
-            for (int r = i + 1 + tid; r < rows; r += NT) ci[r] *= scal;
-        }
+        if (tid == 0 && taud != 0.0) ci[i] = (T)sh[1];
         __syncthreads();
-        // apply H(i) to A(i:rows, i+1:cols) (dlarf) and downdate the partial norms
+        // rank reached: |R_ii| / |R_00| < tol at i >= 1
+        if (fixed < 0 && i >= 1 && !(fabs((double)ci[i]) / fabs((double)A[0]) >= tol)) {
+            rank = i;
+            break;
+        }
+        const T tau = (T)taud;
+        // apply H(i) to A(i:rows, i+1:cols) (dlarf: w = v^T a_j, skip w = 0 as dger does)
+        // and downdate the partial norms
         for (int j = i + 1 + tid; j < cols; j += NT) {
             T* cj = A + (size_t)j * rows;
             if (tau != T(0)) {
-                T w = cj[i];
-                for (int r = i + 1; r < rows; r++) w = fma(cj[r], ci[r], w);
-                T tm = -tau * w;
-                cj[i] = cj[i] + tm;
-                for (int r = i + 1; r < rows; r++) cj[r] = fma(ci[r], tm, cj[r]);
+                const T w = dot16_serial<T>(
+                    rows - i, [&](int q) { return cj[i + q]; }, [&](int q) { return q == 0 ? T(1) : ci[i + q]; });
+                if (w != T(0)) {
+                    const T tm = -tau * w;
+                    cj[i] = cj[i] + tm;
+                    for (int r = i + 1; r < rows; r++) cj[r] = fma(ci[r], tm, cj[r]);
+                }
             }
             if (vn1[j] != 0.0) {
                 double q = fabs((double)cj[i]) / vn1[j];
@@ -425,12 +540,7 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
                 double temp2 = temp * rr * rr;
                 if (temp2 <= tol3z) {
                     if (i < rows - 1) {
-                        double s = 0.0;
-                        for (int r = i + 1; r < rows; r++) {
-                            double v = (double)cj[r];
-                            s = fma(v, v, s);
-                        }
-                        vn1[j] = sqrt(s);
+                        vn1[j] = sqrt(sumsq16_serial(rows - i - 1, [&](int q) { return (double)cj[i + 1 + q]; }));
                         vn2[j] = vn1[j];
                     } else {
                         vn1[j] = 0.0;
@@ -444,21 +554,6 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
         if (tid == 0) reinterpret_cast<T*>(t.tau)[i] = tau;
         __syncthreads();
     }
-    // choose_rank (src/util.cpp:434-452) on diag(R), FP64
-    if (tid == 0) {
-        int rank;
-        if (tol == 0.0) rank = mn;
-        else if (tol >= 1.0) rank = 0;
-        else if (mn <= 1) rank = mn;
-        else {
-            double sref = fabs((double)A[0]);
-            rank = 1;
-            while (rank < mn && fabs((double)A[rank + (size_t)rank * rows]) / sref >= tol) rank++;
-        }
-        s_rank = rank;
-    }
-    __syncthreads();
-    const int rank = s_rank;
     if (rank >= rows) return;
     // Householder vectors v = A[:, :rank] (rows x rank, ld rows)
     T* Q = reinterpret_cast<T*>(t.Q);
@@ -483,61 +578,15 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
 // ---------------------------------------------------------------------------------------
 // Stepped RRQR for clusters too large for one workgroup. Launch sequence per wave:
 // k_rrqr_init (gather + norms), k_rrqr_step for every pivot step, k_rrqr_finish (rank,
-// Q, scatter). Each step: every slab reads the candidates of all slabs and picks the
+// Q, scatter). The columns are split into 64-wide slabs, one 1024-thread workgroup each:
+// lane l of wavefront w handles column l of the slab and chain w (rows i+w, i+w+16, ...)
+// of every sum. Each step: every slab reads the candidates of all slabs and picks the
 // pivot (first max by position), recomputes the pivot's dlarfg scalars redundantly
 // (identical in every workgroup), and updates / downdates its own free columns. The
 // pivot column is only read during its step; its owner writes the scaled reflector one
 // step later, so no data crosses workgroups inside a launch.
 constexpr int SLAB = 64;
-
-// dlarfg scalars for the column segment [alpha; x(0..n1)): tau, beta, the final scale and
-// the number of safe-minimum rescalings (reference dlarfg). Single thread.
-template <typename T>
-__device__ void larfg_scalars(double alpha, const T* x, int n1, double& tau, double& beta_out, double& scal,
-                              int& knt) {
-#pragma clang fp contract(off)
-    tau = 0.0;
-    scal = 1.0;
-    knt = 0;
-    beta_out = alpha;
-    if (n1 < 1) return;
-    double xs = 0.0;
-    for (int r = 0; r < n1; r++) {
-        double v = (double)x[r];
-        xs = fma(v, v, xs);
-    }
-    double xnorm = sqrt(xs);
-    if (xnorm == 0.0) return;
-    const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
-    const double rs = 1.0 / safmin;
-    double beta = -copysign(lapy2_d(alpha, xnorm), alpha);
-    if (fabs(beta) < safmin) {
-        do {
-            knt++;
-            beta *= rs;
-            alpha *= rs;
-        } while (fabs(beta) < safmin && knt < 20);
-        xs = 0.0;
-        for (int r = 0; r < n1; r++) {
-            T xv = x[r];
-            for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
-            double v = (double)xv;
-            xs = fma(v, v, xs);
-        }
-        beta = -copysign(lapy2_d(alpha, sqrt(xs)), alpha);
-    }
-    tau = (beta - alpha) / beta;
-    scal = 1.0 / (alpha - beta);
-    for (int k = 0; k < knt; k++) beta *= safmin;
-    beta_out = beta;
-}
-
-template <typename T>
-__device__ inline T larfg_scale(T xv, int knt, double scal) {
-    const double rs = 1.0 / (DBL_MIN / (DBL_EPSILON * 0.5));
-    for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
-    return xv * (T)scal;
-}
+constexpr int STEP_NT = SLAB * NCH;
 
 // Stepped-path storage: 64-column slabs, row-major inside a slab, so a wavefront reading
 // one row of its slab touches one contiguous segment.
@@ -545,30 +594,26 @@ __device__ inline size_t slab_idx(int rows, int r, int j) {
     return ((size_t)(j / SLAB) * rows + r) * SLAB + (j % SLAB);
 }
 
+// pivot column and reflector buffers (2 x max_rows): LDS, or the task's global buffer
+template <typename T>
+__device__ inline T* step_buf(const RrqrBig& B, unsigned char* smem, int slab) {
+    return B.vbuf ? reinterpret_cast<T*>(B.vbuf) + (size_t)slab * 2 * B.t.max_rows : reinterpret_cast<T*>(smem);
+}
+
 // Writes the reflector of step k (column `col`, rows k..rows-1) in place: beta on the
 // diagonal, the scaled vector below, tau[k]. All threads of the block take part; `col_buf`
-// is LDS scratch of at least rows elements.
+// holds at least rows elements.
 template <typename T>
-__device__ void write_reflector(T* A, int rows, int col, int k, T* tau_out, double* sh, T* col_buf) {
+__device__ void write_reflector(T* A, int rows, int col, int k, T* tau_out, double* sh, double* s_ch, T* col_buf) {
     for (int r = k + threadIdx.x; r < rows; r += blockDim.x) col_buf[r] = A[slab_idx(rows, r, col)];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double tau, beta, scal;
-        int knt;
-        larfg_scalars<T>((double)col_buf[k], col_buf + k + 1, rows - k - 1, tau, beta, scal, knt);
-        sh[0] = tau;
-        sh[1] = beta;
-        sh[2] = scal;
-        sh[3] = (double)knt;
-    }
-    __syncthreads();
+    larfg_block<T>((double)col_buf[k], col_buf + k + 1, rows - k - 1, sh, s_ch);
     const double tau = sh[0];
     if (tau != 0.0) {
         const int knt = (int)sh[3];
         for (int r = k + 1 + threadIdx.x; r < rows; r += blockDim.x)
             A[slab_idx(rows, r, col)] = larfg_scale<T>(col_buf[r], knt, sh[2]);
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
         if (tau != 0.0) A[slab_idx(rows, k, col)] = (T)sh[1];
         tau_out[k] = (T)tau;
@@ -585,36 +630,28 @@ __device__ inline int seg_find(const int* col0, int nseg, int j) {
     return lo;
 }
 
-// (value desc, position asc) reduction over the block into r_val[0], r_pos[0]
-__device__ inline void reduce_best(double* r_val, int* r_pos, double v, int p) {
-    const int tid = threadIdx.x;
-    r_val[tid] = v;
-    r_pos[tid] = p;
-    __syncthreads();
-    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (tid < off) {
-            double v2 = r_val[tid + off];
-            int p2 = r_pos[tid + off];
-            if (v2 > r_val[tid] || (v2 == r_val[tid] && p2 < r_pos[tid])) {
-                r_val[tid] = v2;
-                r_pos[tid] = p2;
-            }
+// (value desc, position asc) maximum over the 64 lanes of a wavefront; every lane gets it
+__device__ inline void wave_best(double& v, int& p) {
+    for (int off = SLAB / 2; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const int p2 = __shfl_xor(p, off);
+        if (v2 > v || (v2 == v && p2 < p)) {
+            v = v2;
+            p = p2;
         }
-        __syncthreads();
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(SLAB) k_rrqr_init(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
-                                                    const int* dsize) {
+__global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
+                                                       const int* dsize, double tol) {
 #pragma clang fp contract(off)
     __shared__ int s_col0[RRQR_MAXSEG + 1];
-    __shared__ double r_val[SLAB];
-    __shared__ int r_pos[SLAB];
+    __shared__ double s_part[NCH][SLAB];
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
     const RrqrSeg* sg = segs + B.t.seg0;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
     const int rows = dsize[B.t.cluster];
     if (tid == 0) {
         int c = 0;
@@ -627,60 +664,72 @@ __global__ void __launch_bounds__(SLAB) k_rrqr_init(const RrqrBig* bigs, const R
     __syncthreads();
     const int cols = s_col0[B.t.nseg];
     if (sl.slab == 0 && tid == 0) {
+        const int mn = min(rows, cols);
         B.dims[0] = rows;
         B.dims[1] = cols;
-        B.dims[2] = min(rows, cols);
+        B.dims[2] = mn;
+        B.dims[3] = rank_fixed(tol, mn) == 0 ? 0 : mn;  // steps to run (lowered at the rank)
     }
     T* A = reinterpret_cast<T*>(B.t.scratch);
     const int mc = B.t.max_cols;
-    double* vn1 = B.t.norms;
-    double* vn2 = B.t.norms + mc;
-    int* p2c = B.map;
-    int* c2p = B.map + 2 * mc;
-    const int j = sl.slab * SLAB + tid;
-    double best = -1.0;
-    int bpos = INT32_MAX;
-    if (j < cols && rows > 0) {
+    const int j = sl.slab * SLAB + lane;
+    const bool active = j < cols && rows > 0;
+    double c = 0.0;
+    if (active) {
         const int k = seg_find(s_col0, B.t.nseg, j);
         const int jj = j - s_col0[k], w = s_col0[k + 1] - s_col0[k];
         const T* E = reinterpret_cast<const T*>(sg[k].E);
-        T* a = A + (size_t)sl.slab * rows * SLAB + tid;
-        double sum = 0.0;
-        for (int i = 0; i < rows; i++) {
-            T v = sg[k].out ? E[jj + (size_t)i * w] : E[i + (size_t)jj * rows];
+        T* a = A + (size_t)sl.slab * rows * SLAB + lane;
+        for (int i = wv; i < rows; i += NCH) {
+            const T v = sg[k].out ? E[jj + (size_t)i * w] : E[i + (size_t)jj * rows];
             a[(size_t)i * SLAB] = v;
-            double dv = (double)v;
-            sum = fma(dv, dv, sum);
+            const double dv = (double)v;
+            c = fma(dv, dv, c);
         }
-        vn1[j] = sqrt(sum);
-        vn2[j] = vn1[j];
-        p2c[j] = j;
-        c2p[j] = j;
-        best = vn1[j];
-        bpos = j;
     }
-    reduce_best(r_val, r_pos, best, bpos);
-    if (tid == 0) {
-        B.cand_v[sl.slab] = r_val[0];
-        B.cand_p[sl.slab] = r_pos[0];
+    s_part[wv][lane] = c;
+    __syncthreads();
+    if (wv == 0) {
+        double best = -1.0;
+        int bpos = INT32_MAX;
+        if (active) {
+            double cc[NCH];
+#pragma unroll
+            for (int w = 0; w < NCH; w++) cc[w] = s_part[w][lane];
+            const double nrm = sqrt(tree16(cc));
+            B.t.norms[j] = nrm;
+            B.t.norms[mc + j] = nrm;
+            B.map[j] = j;
+            B.map[2 * mc + j] = j;
+            best = nrm;
+            bpos = j;
+        }
+        wave_best(best, bpos);
+        if (lane == 0) {
+            B.cand_v[sl.slab] = best;
+            B.cand_p[sl.slab] = bpos;
+        }
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(SLAB) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step) {
+__global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step, double tol) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* s_v = reinterpret_cast<T*>(smem_raw);
-    __shared__ double r_val[SLAB];
-    __shared__ int r_pos[SLAB];
-    __shared__ double sh[4];
-    __shared__ int s_p, s_ca, s_cb;
+    __shared__ double s_part[NCH][SLAB];
+    __shared__ double sh[4], s_ch[NCH];
+    __shared__ T s_tm[SLAB];
+    __shared__ int s_upd[SLAB];
+    __shared__ int s_p, s_ca, s_cb, s_stop;
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
-    const int tid = threadIdx.x;
-    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
+    const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
+    // dims[3] may be lowered by slab 0 during this launch: read it once per workgroup
+    if (tid == 0) s_stop = B.dims[3];
+    __syncthreads();
+    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2], stop = s_stop;
     const int i = step;
-    if (rows == 0 || i >= mn) return;
+    if (rows == 0 || i >= mn || i > stop) return;
     const int cur = i & 1, nxt = cur ^ 1;
     const int mc = B.t.max_cols, ns = B.nslab;
     const int* p2c_cur = B.map + cur * mc;
@@ -690,12 +739,20 @@ __global__ void __launch_bounds__(SLAB) k_rrqr_step(const RrqrBig* bigs, const R
     T* A = reinterpret_cast<T*>(B.t.scratch);
     double* vn1 = B.t.norms;
     double* vn2 = B.t.norms + mc;
+    T* s_col = step_buf<T>(B, smem_raw, sl.slab);  // the pivot column
+    T* s_v = s_col + B.t.max_rows;                 // its reflector, v[i] = 1
+    // deferred store of the previous step's reflector by the owner of its column
+    if (i >= 1) {
+        const int cprev = p2c_cur[i - 1];
+        if (cprev / SLAB == sl.slab) write_reflector<T>(A, rows, cprev, i - 1, reinterpret_cast<T*>(B.t.tau), sh, s_ch, s_col);
+    }
+    if (i == stop) return;
     if (tid == 0) {
         double bv = -1.0;
         int bp = INT32_MAX;
         for (int k = 0; k < ns; k++) {
-            double v = B.cand_v[cur * ns + k];
-            int pp = B.cand_p[cur * ns + k];
+            const double v = B.cand_v[cur * ns + k];
+            const int pp = B.cand_p[cur * ns + k];
             if (v > bv || (v == bv && pp < bp)) {
                 bv = v;
                 bp = pp;
@@ -709,7 +766,7 @@ __global__ void __launch_bounds__(SLAB) k_rrqr_step(const RrqrBig* bigs, const R
     const int p = s_p, ca = s_ca, cb = s_cb;
     if (p < i || p >= cols) return;  // cannot happen for i < mn: some column is still free
     if (sl.slab == 0) {
-        for (int j = tid; j < cols; j += SLAB) {
+        for (int j = tid; j < cols; j += STEP_NT) {
             p2c_nxt[j] = p2c_cur[j];
             c2p_nxt[j] = c2p_cur[j];
         }
@@ -721,59 +778,87 @@ __global__ void __launch_bounds__(SLAB) k_rrqr_step(const RrqrBig* bigs, const R
             c2p_nxt[cb] = p;
         }
     }
-    // deferred store of the previous step's reflector by the owner of its column
-    T* s_col = s_v + rows;  // LDS copy of the pivot column
-    if (i >= 1) {
-        const int cprev = p2c_cur[i - 1];
-        if (cprev / SLAB == sl.slab) write_reflector<T>(A, rows, cprev, i - 1, reinterpret_cast<T*>(B.t.tau), sh, s_col);
-    }
-    // this step's reflector, recomputed identically in every slab from an LDS copy
-    for (int r = i + tid; r < rows; r += SLAB) s_col[r] = A[slab_idx(rows, r, ca)];
+    // this step's reflector, recomputed identically in every slab
+    for (int r = i + tid; r < rows; r += STEP_NT) s_col[r] = A[slab_idx(rows, r, ca)];
     __syncthreads();
-    if (tid == 0) {
-        double tau, beta, scal;
-        int knt;
-        larfg_scalars<T>((double)s_col[i], s_col + i + 1, rows - i - 1, tau, beta, scal, knt);
-        sh[0] = tau;
-        sh[2] = scal;
-        sh[3] = (double)knt;
+    larfg_block<T>((double)s_col[i], s_col + i + 1, rows - i - 1, sh, s_ch);
+    // R_ii as stored (tau = 0 leaves alpha on the diagonal)
+    const double rii = fabs((double)(T)sh[1]);
+    if (i == 0) {
+        if (sl.slab == 0 && tid == 0) B.cand_v[2 * ns] = rii;
+    } else if (rank_fixed(tol, mn) < 0 && !(rii / B.cand_v[2 * ns] >= tol)) {
+        if (sl.slab == 0 && tid == 0) B.dims[3] = i;  // rank = i: step i and later are not kept
+        return;
     }
-    __syncthreads();
-    const T tau = (T)sh[0];
-    if (sh[0] != 0.0) {
+    const double taud = sh[0];
+    const T tau = (T)taud;
+    if (taud != 0.0) {
         const int knt = (int)sh[3];
-        for (int r = i + 1 + tid; r < rows; r += SLAB) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
+        for (int r = i + 1 + tid; r < rows; r += STEP_NT) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
     }
+    if (tid == 0) s_v[i] = T(1);
     __syncthreads();
-    const double tol3z = sqrt(DBL_EPSILON * 0.5);
-    const int j = sl.slab * SLAB + tid;
-    double best = -1.0;
-    int bpos = INT32_MAX;
-    if (j < cols && j != ca) {
-        const int pos = (j == cb) ? p : c2p_cur[j];
-        if (pos > i) {
-            T* cj = A + (size_t)sl.slab * rows * SLAB + tid;  // element r at cj[r * SLAB]
-            if (tau != T(0)) {
-                T w = cj[(size_t)i * SLAB];
-                for (int r = i + 1; r < rows; r++) w = fma(cj[(size_t)r * SLAB], s_v[r], w);
-                T tm = -tau * w;
-                cj[(size_t)i * SLAB] = cj[(size_t)i * SLAB] + tm;
-                for (int r = i + 1; r < rows; r++) cj[(size_t)r * SLAB] = fma(s_v[r], tm, cj[(size_t)r * SLAB]);
+    const int j = sl.slab * SLAB + lane;
+    const int pos = (j == cb) ? p : (j < cols ? c2p_cur[j] : 0);
+    const bool active = j < cols && j != ca && pos > i;
+    T* cj = A + (size_t)sl.slab * rows * SLAB + lane;  // element r of column j at cj[r * SLAB]
+    // w_j = v^T a_j: chain wv of the sum
+    if (taud != 0.0) {
+        T c = T(0);
+        if (active)
+            for (int r = i + wv; r < rows; r += NCH) c = fma(cj[(size_t)r * SLAB], s_v[r], c);
+        s_part[wv][lane] = (double)c;
+        __syncthreads();
+        if (wv == 0) {
+            T cc[NCH];
+#pragma unroll
+            for (int w = 0; w < NCH; w++) cc[w] = (T)s_part[w][lane];
+            const T w = tree16(cc);
+            s_tm[lane] = -tau * w;
+            s_upd[lane] = w != T(0);  // dger skips columns with w_j = 0
+        }
+        __syncthreads();
+    }
+    // update a_j := a_j + tm v and the chains of sum over rows > i of a_j^2 (the norm
+    // recomputation): row r > i belongs to chain (r - i - 1) % 16 = (wv + 15) % 16
+    T ai = T(0);
+    double c2 = 0.0;
+    if (active) {
+        const bool upd = taud != 0.0 && s_upd[lane];
+        const T tm = upd ? s_tm[lane] : T(0);
+        for (int r = i + wv; r < rows; r += NCH) {
+            T a = cj[(size_t)r * SLAB];
+            if (upd) {
+                a = (r == i) ? a + tm : fma(s_v[r], tm, a);
+                cj[(size_t)r * SLAB] = a;
             }
+            if (r == i) {
+                ai = a;
+            } else {
+                const double d = (double)a;
+                c2 = fma(d, d, c2);
+            }
+        }
+    }
+    s_part[(wv + NCH - 1) % NCH][lane] = c2;
+    __syncthreads();
+    if (wv == 0) {
+        const double tol3z = sqrt(DBL_EPSILON * 0.5);
+        double best = -1.0;
+        int bpos = INT32_MAX;
+        if (active) {
             if (vn1[j] != 0.0) {
-                double q = fabs((double)cj[(size_t)i * SLAB]) / vn1[j];
+                const double q = fabs((double)ai) / vn1[j];
                 double temp = 1.0 - q * q;
                 temp = fmax(temp, 0.0);
-                double rr = vn1[j] / vn2[j];
-                double temp2 = temp * rr * rr;
+                const double rr = vn1[j] / vn2[j];
+                const double temp2 = temp * rr * rr;
                 if (temp2 <= tol3z) {
                     if (i < rows - 1) {
-                        double sum = 0.0;
-                        for (int r = i + 1; r < rows; r++) {
-                            double v = (double)cj[(size_t)r * SLAB];
-                            sum = fma(v, v, sum);
-                        }
-                        vn1[j] = sqrt(sum);
+                        double cc[NCH];
+#pragma unroll
+                        for (int w = 0; w < NCH; w++) cc[w] = s_part[w][lane];
+                        vn1[j] = sqrt(tree16(cc));
                         vn2[j] = vn1[j];
                     } else {
                         vn1[j] = 0.0;
@@ -786,11 +871,11 @@ __global__ void __launch_bounds__(SLAB) k_rrqr_step(const RrqrBig* bigs, const R
             best = vn1[j];
             bpos = pos;
         }
-    }
-    reduce_best(r_val, r_pos, best, bpos);
-    if (tid == 0) {
-        B.cand_v[nxt * ns + sl.slab] = r_val[0];
-        B.cand_p[nxt * ns + sl.slab] = r_pos[0];
+        wave_best(best, bpos);
+        if (lane == 0) {
+            B.cand_v[nxt * ns + sl.slab] = best;
+            B.cand_p[nxt * ns + sl.slab] = bpos;
+        }
     }
 }
 
@@ -798,14 +883,12 @@ template <typename T>
 __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const RrqrSeg* segs, int* dsize, double tol) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* s_col = reinterpret_cast<T*>(smem_raw);
     __shared__ int s_col0[RRQR_MAXSEG + 1];
-    __shared__ double sh[4];
-    __shared__ int s_rank;
+    __shared__ double sh[4], s_ch[NCH];
     const RrqrBig B = bigs[blockIdx.x];
     const RrqrSeg* sg = segs + B.t.seg0;
     const int tid = threadIdx.x;
-    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
+    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2], stop = B.dims[3];
     if (rows == 0) return;
     if (tid == 0) {
         int c = 0;
@@ -816,26 +899,17 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
         s_col0[B.t.nseg] = c;
     }
     const int mc = B.t.max_cols;
-    const int fin = mn & 1;
+    const int fin = stop & 1;  // maps after `stop` steps
     const int* p2c = B.map + fin * mc;
     const int* c2p = B.map + 2 * mc + fin * mc;
     T* A = reinterpret_cast<T*>(B.t.scratch);
     __syncthreads();
-    if (mn >= 1) write_reflector<T>(A, rows, p2c[mn - 1], mn - 1, reinterpret_cast<T*>(B.t.tau), sh, s_col);
-    if (tid == 0) {
-        int rank;
-        if (tol == 0.0) rank = mn;
-        else if (tol >= 1.0) rank = 0;
-        else if (mn <= 1) rank = mn;
-        else {
-            double sref = fabs((double)A[slab_idx(rows, 0, p2c[0])]);
-            rank = 1;
-            while (rank < mn && fabs((double)A[slab_idx(rows, rank, p2c[rank])]) / sref >= tol) rank++;
-        }
-        s_rank = rank;
-    }
-    __syncthreads();
-    const int rank = s_rank;
+    // without an early stop the last reflector has no later step to store it
+    if (stop == mn && mn >= 1)
+        write_reflector<T>(A, rows, p2c[mn - 1], mn - 1, reinterpret_cast<T*>(B.t.tau), sh, s_ch,
+                           step_buf<T>(B, smem_raw, 0));
+    const int fixed = rank_fixed(tol, mn);
+    const int rank = fixed >= 0 ? fixed : stop;
     if (rank >= rows) return;
     T* Q = reinterpret_cast<T*>(B.t.Q);
     for (long long idx = tid; idx < (long long)rows * rank; idx += NT) {
@@ -922,19 +996,26 @@ hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_seg
 }
 
 template <typename T>
-hipError_t launch_rrqr_big(const RrqrBig* d_bigs, int nbig, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
-                           int* d_size, double tol, int nsteps, int max_rows, hipStream_t st) {
+hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
+                            const int* d_size, double tol, hipStream_t st) {
+    if (nslabs == 0) return hipSuccess;
+    k_rrqr_init<T><<<nslabs, STEP_NT, 0, st>>>(d_bigs, d_slabs, d_segs, d_size, tol);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol,
+                            size_t lds_bytes, hipStream_t st) {
+    if (nslabs == 0) return hipSuccess;
+    k_rrqr_step<T><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, int nbig, const RrqrSeg* d_segs, int* d_size, double tol,
+                              size_t lds_bytes, hipStream_t st) {
     if (nbig == 0) return hipSuccess;
-    k_rrqr_init<T><<<nslabs, SLAB, 0, st>>>(d_bigs, d_slabs, d_segs, d_size);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const size_t vbytes = (size_t)std::max(max_rows, 1) * sizeof(T);
-    for (int i = 0; i < nsteps; i++) {
-        k_rrqr_step<T><<<nslabs, SLAB, 2 * vbytes, st>>>(d_bigs, d_slabs, i);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    k_rrqr_finish<T><<<nbig, NT, vbytes, st>>>(d_bigs, d_segs, d_size, tol);
+    k_rrqr_finish<T><<<nbig, NT, lds_bytes, st>>>(d_bigs, d_segs, d_size, tol);
     return hipGetLastError();
 }
 
@@ -953,8 +1034,12 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_schur<T>(const SchurTile*, int, const SchurContrib*, hipStream_t); \
     template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, hipStream_t); \
     template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);                        \
-    template hipError_t launch_rrqr_big<T>(const RrqrBig*, int, const RrqrSlab*, int, const RrqrSeg*, int*, double, \
-                                           int, int, hipStream_t);
+    template hipError_t launch_rrqr_init<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, const int*,      \
+                                            double, hipStream_t);                                                  \
+    template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, double, size_t,            \
+                                            hipStream_t);                                                          \
+    template hipError_t launch_rrqr_finish<T>(const RrqrBig*, int, const RrqrSeg*, int*, double, size_t,          \
+                                              hipStream_t);
 SPAND_INST(float)
 SPAND_INST(double)
 

@@ -17,7 +17,10 @@
 namespace spand {
 
 constexpr int NT = 256;
-constexpr int SOLVE_LDS_ELEMS_BYTES = 32768;
+// Dynamic LDS of the solve kernels: a chunk of up to 8 right-hand sides of the segment (and
+// for the Householder apply two k-vectors per right-hand side); the chunk width is derived
+// from the segment size, and the host rejects segments that would not fit one column.
+constexpr int SOLVE_LDS_ELEMS_BYTES = SOLVE_LDS_BYTES;
 
 static inline int grid_for(size_t n) {
     size_t g = (n + NT - 1) / NT;
@@ -63,141 +66,204 @@ __global__ void __launch_bounds__(NT) k_il2cm(const double* src, int N, int nrhs
 }
 
 // ---------------------------------------------------------------------------------------
-// x := L^{-1} x (trans 0) or L^{-T} x (trans 1), both as column sweeps (the second on
-// U = L^T) with explicit fused multiply-adds; the segment is staged in LDS when it fits.
+// Block-of-right-hand-sides helpers. A segment's nrhs columns are processed in chunks of
+// RG <= 8 right-hand sides, staged in LDS as S[i * RG + r].
+constexpr int RG_MAX = 8;
+constexpr int WAVE = 64;
+
 template <typename T>
-__global__ void __launch_bounds__(NT) k_trsv(const TrsvTask* tasks, T* ws, int nrhs, int trans, int lds_elems) {
-#pragma clang fp contract(off)
+__device__ inline T wave_sum(T v) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// x := Linv x (trans 0: rows of Linv, threads over output rows, coalesced column reads)
+// x := Linv^T x (trans 1: one wavefront per output j, lanes over rows i >= j).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, T* ws, int nrhs, int trans) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* S = reinterpret_cast<T*>(smem_raw);
     const TrsvTask t = tasks[blockIdx.x];
     const int n = t.n, tid = threadIdx.x;
     if (n == 0) return;
-    const T* L = reinterpret_cast<const T*>(t.L);
-    T* G = ws + (size_t)t.xoff * nrhs;
-    const int nx = n * nrhs;
-    const bool lds = nx <= lds_elems;
-    T* x = lds ? S : G;
-    if (lds) {
-        for (int i = tid; i < nx; i += NT) S[i] = G[i];
+    const T* Li = reinterpret_cast<const T*>(t.L);
+    T* x = ws + (size_t)t.xoff * nrhs;
+    const int lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
+    const int step = min(RG_MAX, SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T) / n);
+    for (int r0 = 0; r0 < nrhs; r0 += step) {
+        const int rg = min(step, nrhs - r0);
+        for (int idx = tid; idx < n * rg; idx += NT) S[idx] = x[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
         __syncthreads();
-    }
-    if (!trans) {
-        for (int j = 0; j < n; j++) {
-            const T ljj = L[j + (size_t)j * n];
-            for (int r = tid; r < nrhs; r += NT) x[j * nrhs + r] /= ljj;
-            __syncthreads();
-            const int rem = (n - j - 1) * nrhs;
-            for (int idx = tid; idx < rem; idx += NT) {
-                int i = j + 1 + idx / nrhs, r = idx % nrhs;
-                x[i * nrhs + r] = fma(-x[j * nrhs + r], L[i + (size_t)j * n], x[i * nrhs + r]);
+        if (!trans) {
+            for (int i = tid; i < n; i += NT) {
+                T acc[RG_MAX];
+                for (int r = 0; r < rg; r++) acc[r] = T(0);
+                for (int j = 0; j <= i; j++) {
+                    const T l = Li[i + (size_t)j * n];
+                    for (int r = 0; r < rg; r++) acc[r] += l * S[j * rg + r];
+                }
+                for (int r = 0; r < rg; r++) x[(size_t)i * nrhs + r0 + r] = acc[r];
             }
-            __syncthreads();
-        }
-    } else {
-        for (int j = n - 1; j >= 0; j--) {
-            const T ljj = L[j + (size_t)j * n];
-            for (int r = tid; r < nrhs; r += NT) x[j * nrhs + r] /= ljj;
-            __syncthreads();
-            const int rem = j * nrhs;
-            for (int idx = tid; idx < rem; idx += NT) {
-                int i = idx / nrhs, r = idx % nrhs;
-                x[i * nrhs + r] = fma(-x[j * nrhs + r], L[j + (size_t)i * n], x[i * nrhs + r]);
+        } else {
+            for (int j = wv; j < n; j += nw) {
+                T acc[RG_MAX];
+                for (int r = 0; r < rg; r++) acc[r] = T(0);
+                const T* col = Li + (size_t)j * n;
+                for (int i = j + lane; i < n; i += WAVE) {
+                    const T l = col[i];
+                    for (int r = 0; r < rg; r++) acc[r] += l * S[i * rg + r];
+                }
+                for (int r = 0; r < rg; r++) {
+                    T v = wave_sum(acc[r]);
+                    if (lane == 0) x[(size_t)j * nrhs + r0 + r] = v;
+                }
             }
-            __syncthreads();
         }
-    }
-    if (lds) {
-        for (int i = tid; i < nx; i += NT) G[i] = S[i];
+        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// x[dst] -= sum_k op(A_k) x[src_k], terms applied in recorded order by the thread that owns
-// each output element.
+// x[dst] -= sum_k op(A_k) x[src_k], terms in recorded order. The destination chunk lives in
+// LDS; op = A terms use one thread per output row, op = A^T terms one wavefront per output.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const GemvTerm* terms, T* ws, int nrhs) {
-#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* S = reinterpret_cast<T*>(smem_raw);
     const GemvGroup g = groups[blockIdx.x];
-    const int tid = threadIdx.x;
-    const int nout = g.n * nrhs;
+    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
     T* xd = ws + (size_t)g.xdst * nrhs;
-    for (int idx = tid; idx < nout; idx += NT) {
-        const int i = idx / nrhs, r = idx % nrhs;
-        T acc_out = xd[idx];
+    const int step = min(RG_MAX, SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T) / max(g.n, 1));
+    for (int r0 = 0; r0 < nrhs; r0 += step) {
+        const int rg = min(step, nrhs - r0);
+        for (int idx = tid; idx < g.n * rg; idx += NT) S[idx] = xd[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
+        __syncthreads();
         for (int k = g.k0; k < g.k1; k++) {
             const GemvTerm tm = terms[k];
             const T* A = reinterpret_cast<const T*>(tm.A);
-            const T* xs = ws + (size_t)tm.xsrc * nrhs + r;
+            const T* xs = ws + (size_t)tm.xsrc * nrhs + r0;
             if (!tm.trans) {
-                for (int j = 0; j < tm.cols; j++) acc_out = fma(-A[i + (size_t)j * tm.rows], xs[(size_t)j * nrhs], acc_out);
+                // output i (rows of A), sum over j < cols
+                for (int i = tid; i < tm.rows; i += NT) {
+                    T acc[RG_MAX];
+                    for (int r = 0; r < rg; r++) acc[r] = T(0);
+                    for (int j = 0; j < tm.cols; j++) {
+                        const T a = A[i + (size_t)j * tm.rows];
+                        for (int r = 0; r < rg; r++) acc[r] += a * xs[(size_t)j * nrhs + r];
+                    }
+                    for (int r = 0; r < rg; r++) S[i * rg + r] -= acc[r];
+                }
             } else {
-                const T* a = A + (size_t)i * tm.rows;
-                T acc = T(0);
-                for (int j = 0; j < tm.rows; j++) acc = fma(a[j], xs[(size_t)j * nrhs], acc);
-                acc_out -= acc;
+                // output j (columns of A), sum over rows i
+                for (int j = wv; j < tm.cols; j += nw) {
+                    T acc[RG_MAX];
+                    for (int r = 0; r < rg; r++) acc[r] = T(0);
+                    const T* col = A + (size_t)j * tm.rows;
+                    for (int i = lane; i < tm.rows; i += WAVE) {
+                        const T a = col[i];
+                        for (int r = 0; r < rg; r++) acc[r] += a * xs[(size_t)i * nrhs + r];
+                    }
+                    for (int r = 0; r < rg; r++) {
+                        T v = wave_sum(acc[r]);
+                        if (lane == 0) S[j * rg + r] -= v;
+                    }
+                }
             }
+            __syncthreads();
         }
-        xd[idx] = acc_out;
+        for (int idx = tid; idx < g.n * rg; idx += NT) xd[(size_t)(idx / rg) * nrhs + r0 + idx % rg] = S[idx];
+        __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// Householder sequence: x := H(k-1)..H(0) x (trans 1, Q^T x) or H(0)..H(k-1) x (trans 0).
+// Householder block in compact-WY form: Q^T x = x - V T^T V^T x (trans 1),
+// Q x = x - V T V^T x (trans 0). V has a unit diagonal and zeros above it implicitly.
 template <typename T>
-__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nrhs, int trans, int lds_elems) {
+__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nrhs, int trans) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* S = reinterpret_cast<T*>(smem_raw);
-    __shared__ T red[NT];
     const QTask t = tasks[blockIdx.x];
-    const int m = t.m, k = t.k, tid = threadIdx.x;
+    const int m = t.m, k = t.k, tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
     if (m == 0 || k == 0) return;
+    const int step = min(RG_MAX, SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T) / (m + 2 * k));
+    T* S = reinterpret_cast<T*>(smem_raw);  // x chunk, m * step
+    T* W = S + (size_t)m * step;            // V^T x, k * step
+    T* U = W + (size_t)k * step;            // T^(T) W, k * step
+    const T* V = reinterpret_cast<const T*>(t.V);
+    const T* Tm = reinterpret_cast<const T*>(t.Tm);
+    T* x = ws + (size_t)t.xoff * nrhs;
+    for (int r0 = 0; r0 < nrhs; r0 += step) {
+        const int rg = min(step, nrhs - r0);
+        for (int idx = tid; idx < m * rg; idx += NT) S[idx] = x[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
+        __syncthreads();
+        // W = V^T x: wavefront per reflector c, w_c = x_c + sum_{i > c} V(i,c) x_i
+        for (int c = wv; c < k; c += nw) {
+            T acc[RG_MAX];
+            for (int r = 0; r < rg; r++) acc[r] = T(0);
+            const T* col = V + (size_t)c * m;
+            for (int i = c + 1 + lane; i < m; i += WAVE) {
+                const T a = col[i];
+                for (int r = 0; r < rg; r++) acc[r] += a * S[i * rg + r];
+            }
+            for (int r = 0; r < rg; r++) {
+                T v = wave_sum(acc[r]);
+                if (lane == 0) W[c * rg + r] = S[c * rg + r] + v;
+            }
+        }
+        __syncthreads();
+        // U = T^T W (Q^T) or T W (Q); T is upper triangular, ld k
+        for (int idx = tid; idx < k * rg; idx += NT) {
+            const int a = idx / rg, r = idx % rg;
+            T acc = T(0);
+            if (trans) {
+                for (int b = 0; b <= a; b++) acc += Tm[b + (size_t)a * k] * W[b * rg + r];
+            } else {
+                for (int b = a; b < k; b++) acc += Tm[a + (size_t)b * k] * W[b * rg + r];
+            }
+            U[a * rg + r] = acc;
+        }
+        __syncthreads();
+        // x -= V U: row i gets U_i (unit diagonal, i < k) + sum_{c < min(i, k)} V(i,c) U_c
+        for (int idx = tid; idx < m * rg; idx += NT) {
+            const int i = idx / rg, r = idx % rg;
+            T acc = i < k ? U[i * rg + r] : T(0);
+            const int cmax = min(i, k);
+            for (int c = 0; c < cmax; c++) acc += V[i + (size_t)c * m] * U[c * rg + r];
+            x[(size_t)i * nrhs + r0 + r] = S[idx] - acc;
+        }
+        __syncthreads();
+    }
+}
+
+// dlarft (forward, columnwise): T(i,i) = tau_i; T(0:i, i) = -tau_i T(0:i,0:i) V(:,0:i)^T v_i.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_larft(const LarftTask* tasks) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* tcol = reinterpret_cast<T*>(smem_raw);  // k
+    const LarftTask t = tasks[blockIdx.x];
+    const int m = t.m, k = t.k, tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
     const T* V = reinterpret_cast<const T*>(t.V);
     const T* tau = reinterpret_cast<const T*>(t.tau);
-    T* G = ws + (size_t)t.xoff * nrhs;
-    const int nx = m * nrhs;
-    const bool lds = nx <= lds_elems;
-    T* x = lds ? S : G;
-    if (lds) {
-        for (int i = tid; i < nx; i += NT) S[i] = G[i];
-        __syncthreads();
-    }
-    int nrp = 1;
-    while (nrp < nrhs) nrp <<= 1;
-    const int groups = NT / nrp;
-    const int r = tid % nrp, gidx = tid / nrp;
-    for (int q = 0; q < k; q++) {
-        const int i = trans ? q : k - 1 - q;
-        const T* v = V + (size_t)i * m;
-        T part = T(0);
-        if (r < nrhs) {
-            for (int j = i + 1 + gidx; j < m; j += groups) part += v[j] * x[j * nrhs + r];
-        }
-        red[tid] = part;
-        __syncthreads();
-        for (int off = groups / 2; off > 0; off >>= 1) {
-            if (gidx < off) red[tid] += red[tid + off * nrp];
-            __syncthreads();
-        }
-        // w_r = x_i + red[r];  x_i -= tau w ; x_j -= tau v_j w
+    T* Tm = reinterpret_cast<T*>(t.Tm);
+    for (int i = 0; i < k; i++) {
         const T ti = tau[i];
-        for (int idx = tid; idx < (m - i) * nrhs; idx += NT) {
-            int j = i + idx / nrhs, rr = idx % nrhs;
-            T w = x[i * nrhs + rr] + red[rr];
-            T vj = (j == i) ? T(1) : v[j];
-            (void)w;
-            // defer the write of row i until every thread has read it
-            if (j != i) x[j * nrhs + rr] -= ti * vj * w;
+        // tcol[j] = -tau_i * (V(i, j) + sum_{r > i} V(r, j) V(r, i)), j < i
+        for (int j = wv; j < i; j += nw) {
+            T acc = T(0);
+            const T* vj = V + (size_t)j * m;
+            const T* vi = V + (size_t)i * m;
+            for (int r = i + 1 + lane; r < m; r += WAVE) acc += vj[r] * vi[r];
+            acc = wave_sum(acc);
+            if (lane == 0) tcol[j] = -ti * (vj[i] + acc);
         }
         __syncthreads();
-        for (int rr = tid; rr < nrhs; rr += NT) {
-            T w = x[i * nrhs + rr] + red[rr];
-            x[i * nrhs + rr] -= ti * w;
+        // T(0:i, i) = T(0:i, 0:i) tcol
+        for (int a = tid; a < i; a += NT) {
+            T acc = T(0);
+            for (int b = a; b < i; b++) acc += Tm[a + (size_t)b * k] * tcol[b];
+            Tm[a + (size_t)i * k] = acc;
         }
+        if (tid == 0) Tm[i + (size_t)i * k] = ti;
+        for (int a = i + 1 + tid; a < k; a += NT) Tm[a + (size_t)i * k] = T(0);
         __syncthreads();
-    }
-    if (lds) {
-        for (int i = tid; i < nx; i += NT) G[i] = S[i];
     }
 }
 
@@ -310,25 +376,31 @@ hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld,
     k_il2cm<<<grid_for((size_t)N * nrhs), NT, 0, st>>>(src, N, nrhs, dst, ld);
     return hipGetLastError();
 }
+// LDS for one RG chunk of the largest segment in a launch is sized by the caller through
+// max_n; the launchers take the largest dimension from the task arrays' host copies.
 template <typename T>
-hipError_t launch_trsv(const TrsvTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st) {
+hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    k_trsv<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks, ws, nrhs, trans,
-                                                         SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T));
+    k_trmv<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks, ws, nrhs, trans);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, const GemvTerm* d_terms, T* ws, int nrhs,
                        hipStream_t st) {
     if (ngroups == 0) return hipSuccess;
-    k_gemv<T><<<ngroups, NT, 0, st>>>(d_groups, d_terms, ws, nrhs);
+    k_gemv<T><<<ngroups, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_groups, d_terms, ws, nrhs);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    k_qapply<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks, ws, nrhs, trans,
-                                                           SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T));
+    k_qapply<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks, ws, nrhs, trans);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st) {
+    if (ntasks == 0) return hipSuccess;
+    k_larft<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks);
     return hipGetLastError();
 }
 template <typename T>
@@ -368,7 +440,8 @@ hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t
 #define SPAND_SOLVE_INST(T)                                                                                  \
     template hipError_t launch_perm_in<T>(const double*, size_t, size_t, int, const int*, int, T*, hipStream_t); \
     template hipError_t launch_perm_out<T>(const T*, const int*, int, int, double*, size_t, size_t, hipStream_t); \
-    template hipError_t launch_trsv<T>(const TrsvTask*, int, T*, int, int, hipStream_t);                    \
+    template hipError_t launch_trmv<T>(const TrsvTask*, int, T*, int, int, hipStream_t);                    \
+    template hipError_t launch_larft<T>(const LarftTask*, int, hipStream_t);                                \
     template hipError_t launch_gemv<T>(const GemvGroup*, int, const GemvTerm*, T*, int, hipStream_t);       \
     template hipError_t launch_qapply<T>(const QTask*, int, T*, int, int, hipStream_t);                     \
     template hipError_t launch_merge<T>(const MergeTask*, int, const int2*, T*, int, int, hipStream_t);

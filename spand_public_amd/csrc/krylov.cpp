@@ -39,7 +39,19 @@ DevMatrix::DevMatrix(int n, const int* colptr, const int* rowval, const double* 
     }
 }
 
+void* DevMatrix::workspace(size_t bytes) const {
+    if (bytes > ws.bytes) {
+        (void)hipFree(ws.p);
+        ws.p = nullptr;
+        ws.bytes = 0;
+        hip_check(hipMalloc(&ws.p, bytes), "hipMalloc(krylov workspace)");
+        ws.bytes = bytes;
+    }
+    return ws.p;
+}
+
 DevMatrix::~DevMatrix() {
+    (void)hipFree(ws.p);
     (void)hipFree(rowptr);
     (void)hipFree(colidx);
     (void)hipFree(val);
@@ -56,18 +68,15 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
     KrylovResult res;
     res.iters.assign(nrhs, 0);
     res.relres.assign(nrhs, 0.0);
-    // interleaved work vectors
-    double *db, *dx, *r, *p, *z, *tmp, *partial, *sc;
-    int* dact;
-    hip_check(hipMalloc((void**)&db, nv * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&dx, nv * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&r, nv * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&p, nv * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&z, nv * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&tmp, nv * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&partial, (size_t)1024 * nrhs * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&sc, (size_t)8 * nrhs * sizeof(double)), "hipMalloc(pcg)");
-    hip_check(hipMalloc((void**)&dact, nrhs * sizeof(int)), "hipMalloc(pcg)");
+    // interleaved work vectors (b, x, r, p, z, tmp, io), dot partials, scalars, active flags
+    const size_t n_vec = 7 * nv, n_part = (size_t)1024 * nrhs, n_sc = (size_t)8 * nrhs;
+    char* wsb = static_cast<char*>(A.workspace((n_vec + n_part + n_sc) * sizeof(double) + nrhs * sizeof(int)));
+    double* base = reinterpret_cast<double*>(wsb);
+    double *db = base, *dx = base + nv, *r = base + 2 * nv, *p = base + 3 * nv, *z = base + 4 * nv;
+    double *tmp = base + 5 * nv, *io = base + 6 * nv;
+    double* partial = base + n_vec;
+    double* sc = partial + n_part;
+    int* dact = reinterpret_cast<int*>(sc + n_sc);
     double* bn2 = sc;
     double* rn2 = sc + nrhs;
     double* absNew = sc + 2 * nrhs;
@@ -75,26 +84,20 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
     double* pAp = sc + 4 * nrhs;
     double* alpha = sc + 5 * nrhs;
     double* beta = sc + 6 * nrhs;
-    struct Guard {
-        std::vector<void*> p;
-        ~Guard() {
-            for (void* q : p) (void)hipFree(q);
-        }
-    } guard{{db, dx, r, p, z, tmp, partial, sc, dact}};
+    KernelProfiler& prof = t.profiler();
+    const double vbytes = (double)nv * sizeof(double);
+    auto vec = [&](double passes, const char* what, auto&& launch) {
+        hip_check(timed(prof, KF_KRYLOV_VEC, st, passes * vbytes, 0, launch), what);
+    };
     // column-major (ld) <-> interleaved, on the device
-    double* io = nullptr;
     auto to_inter = [&](const double* src, double* dst) {
         if (on_device) {
-            hip_check(launch_cm2il(src, ld, N, nrhs, dst, st), "cm2il");
+            vec(2, "cm2il", [&] { return launch_cm2il(src, ld, N, nrhs, dst, st); });
         } else {
-            if (!io) {
-                hip_check(hipMalloc((void**)&io, nv * sizeof(double)), "hipMalloc(pcg)");
-                guard.p.push_back(io);
-            }
             hip_check(hipMemcpy2D(io, (size_t)N * sizeof(double), src, (size_t)ld * sizeof(double),
                                   (size_t)N * sizeof(double), nrhs, hipMemcpyHostToDevice),
                       "pcg io");
-            hip_check(launch_cm2il(io, N, N, nrhs, dst, st), "cm2il");
+            vec(2, "cm2il", [&] { return launch_cm2il(io, N, N, nrhs, dst, st); });
         }
     };
     to_inter(b, db);
@@ -105,7 +108,9 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
     int applies = 0;
     auto spmv = [&](const double* in, double* out) {
         double a = wall();
-        hip_check(launch_spmv(N, A.rowptr, A.colidx, A.val, in, out, nrhs, st), "spmv");
+        hip_check(timed(prof, KF_SPMV, st, 12.0 * A.nnz + 4.0 * (N + 1) + 2.0 * vbytes, 2.0 * A.nnz * nrhs,
+                        [&] { return launch_spmv(N, A.rowptr, A.colidx, A.val, in, out, nrhs, st); }),
+                  "spmv");
         t_spmv += wall() - a;
     };
     auto precond = [&](double* v) {
@@ -115,12 +120,12 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
         t_prec += wall() - a;
     };
     auto dot = [&](const double* a, const double* c, double* out) {
-        hip_check(launch_dot(N, nrhs, a, c, partial, out, st), "dot");
+        vec(2, "dot", [&] { return launch_dot(N, nrhs, a, c, partial, out, st); });
     };
     // r = b - A x
     spmv(dx, tmp);
-    hip_check(launch_copy_f64(r, db, nv, st), "copy");
-    hip_check(launch_axpby(N, nrhs, tmp, nullptr, -1.0, r, nullptr, st), "axpby");
+    vec(2, "copy", [&] { return launch_copy_f64(r, db, nv, st); });
+    vec(3, "axpby", [&] { return launch_axpby(N, nrhs, tmp, nullptr, -1.0, r, nullptr, st); });
     dot(db, db, bn2);
     dot(r, r, rn2);
     hip_check(hipMemcpy(h_bn2.data(), bn2, nrhs * sizeof(double), hipMemcpyDeviceToHost), "pcg");
@@ -148,7 +153,7 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
     }
     hip_check(hipMemcpy(dact, act.data(), nrhs * sizeof(int), hipMemcpyHostToDevice), "pcg");
     if (nact > 0) {
-        hip_check(launch_copy_f64(p, r, nv, st), "copy");
+        vec(2, "copy", [&] { return launch_copy_f64(p, r, nv, st); });
         precond(p);
         dot(r, p, absNew);
         int i = 0;
@@ -156,8 +161,8 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
             spmv(p, tmp);
             dot(p, tmp, pAp);
             hip_check(launch_ratio(nrhs, absNew, pAp, dact, alpha, st), "ratio");
-            hip_check(launch_axpby(N, nrhs, p, alpha, 1.0, dx, nullptr, st), "axpby");
-            hip_check(launch_axpby(N, nrhs, tmp, alpha, -1.0, r, nullptr, st), "axpby");
+            vec(3, "axpby", [&] { return launch_axpby(N, nrhs, p, alpha, 1.0, dx, nullptr, st); });
+            vec(3, "axpby", [&] { return launch_axpby(N, nrhs, tmp, alpha, -1.0, r, nullptr, st); });
             dot(r, r, rn2);
             hip_check(hipMemcpy(h_rn2.data(), rn2, nrhs * sizeof(double), hipMemcpyDeviceToHost), "pcg");
             bool changed = false;
@@ -170,13 +175,13 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
                 }
             if (nact == 0) break;
             if (changed) hip_check(hipMemcpy(dact, act.data(), nrhs * sizeof(int), hipMemcpyHostToDevice), "pcg");
-            hip_check(launch_copy_f64(z, r, nv, st), "copy");
+            vec(2, "copy", [&] { return launch_copy_f64(z, r, nv, st); });
             precond(z);
             hip_check(launch_copy_f64(absOld, absNew, nrhs, st), "copy");
             dot(r, z, absNew);
             hip_check(launch_ratio(nrhs, absNew, absOld, dact, beta, st), "ratio");
             // p = z + beta p
-            hip_check(launch_axpby(N, nrhs, z, nullptr, 1.0, p, beta, st), "axpby");
+            vec(3, "axpby", [&] { return launch_axpby(N, nrhs, z, nullptr, 1.0, p, beta, st); });
             i++;
         }
         for (int c = 0; c < nrhs; c++)
@@ -184,8 +189,8 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
     }
     // true residual
     spmv(dx, tmp);
-    hip_check(launch_copy_f64(r, db, nv, st), "copy");
-    hip_check(launch_axpby(N, nrhs, tmp, nullptr, -1.0, r, nullptr, st), "axpby");
+    vec(2, "copy", [&] { return launch_copy_f64(r, db, nv, st); });
+    vec(3, "axpby", [&] { return launch_axpby(N, nrhs, tmp, nullptr, -1.0, r, nullptr, st); });
     dot(r, r, rn2);
     hip_check(hipMemcpy(h_rn2.data(), rn2, nrhs * sizeof(double), hipMemcpyDeviceToHost), "pcg");
     res.converged = 0;
@@ -203,6 +208,7 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
                               (size_t)N * sizeof(double), nrhs, hipMemcpyDeviceToHost),
                   "pcg io");
     }
+    prof.collect();
     res.seconds = wall() - t0;
     res.seconds_precond = t_prec;
     res.seconds_spmv = t_spmv;

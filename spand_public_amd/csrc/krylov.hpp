@@ -13,6 +13,14 @@ struct DevMatrix {
     int* rowptr = nullptr;
     int* colidx = nullptr;
     double* val = nullptr;
+    // Krylov work vectors, kept between calls (grown on demand): repeated solves with the
+    // same operator allocate nothing.
+    struct Workspace {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    mutable Workspace ws;
+    void* workspace(size_t bytes) const;
     DevMatrix(int n, const int* colptr, const int* rowval, const double* vals);
     ~DevMatrix();
     DevMatrix(const DevMatrix&) = delete;

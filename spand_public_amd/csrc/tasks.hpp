@@ -75,9 +75,10 @@ struct RrqrSeg {
 struct RrqrBig {
     RrqrTask t;      // scratch holds A (rows x cols, ld rows); norms holds vn1 | vn2 by column
     int* map;        // p2c[2][max_cols] then c2p[2][max_cols]
-    double* cand_v;  // [2][nslab] per-slab best partial norm for the next step
+    double* cand_v;  // [2][nslab] per-slab best partial norm for the next step, then |R_00|
     int* cand_p;     // [2][nslab] its position
-    int* dims;       // rows, cols, min(rows, cols)
+    int* dims;       // rows, cols, min(rows, cols), steps to run (the rank once it is known)
+    void* vbuf;      // nullptr, or [nslab][2][max_rows] pivot column / reflector buffers
     int nslab;
     int pad;
 };
@@ -98,8 +99,8 @@ struct CopyTask {
 // ---------------------------------------------------------------------------------------
 // Preconditioner apply (Tree::solve, src/tree.cpp:1610-1635). Vectors are stored
 // interleaved: element i of right-hand side r lives at x[(xoff + i) * nrhs + r].
-struct TrsvTask {  // x := L^{-1} x (fwd) / L^{-T} x (bwd), ScalingLLT (src/operations.cpp:95-103)
-    const void* L;
+struct TrsvTask {  // ScalingLLT (src/operations.cpp:95-103): x := L^{-1} x (fwd) / L^{-T} x (bwd)
+    const void* L;  // the explicit inverse L^{-1} (n x n, lower), computed at factorization
     int n;
     int xoff;
 };
@@ -116,11 +117,19 @@ struct GemvTerm {
     int trans;
 };
 struct QTask {  // Orthogonal (src/operations.cpp:149-158): x := Q^T x (fwd) / Q x (bwd)
-    const void* V;
+    const void* V;    // m x k Householder vectors (unit diagonal implicit), ld m
     const void* tau;
+    const void* Tm;   // k x k upper-triangular compact-WY factor, Q = I - V T V^T (dlarft)
     int m, k;
     int xoff;
     int pad;
+};
+// dlarft (forward, columnwise) for one Orthogonal op: T from V and tau.
+struct LarftTask {
+    const void* V;
+    const void* tau;
+    void* Tm;
+    int m, k;
 };
 struct MergeTask {  // Merge (src/operations.cpp:166-192)
     int pxoff;

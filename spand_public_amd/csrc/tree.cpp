@@ -323,6 +323,48 @@ static void push_trsm(std::vector<TrsmTask>& v, const void* L, void* B, int n, i
     }
 }
 
+// Algorithmic work of one launch over a task array: every block touched is read (and
+// written, if updated) once, in the factor's element type; flops by the LAPACK counts.
+struct Work {
+    double bytes = 0, flops = 0;
+};
+static Work work_potrf(const std::vector<PotrfTask>& v, size_t esz) {
+    Work w;
+    for (auto& t : v) {
+        w.bytes += (double)t.n * (t.n + 1) * esz;
+        w.flops += potrf_flops(t.n);
+    }
+    return w;
+}
+static Work work_trsm(const std::vector<TrsmTask>& v, size_t esz) {
+    Work w;
+    for (auto& t : v) {
+        w.bytes += ((double)t.n * (t.n + 1) / 2 + 2.0 * t.n * t.m) * esz;
+        w.flops += (double)t.m * t.n * t.n;
+    }
+    return w;
+}
+static Work work_schur(const std::vector<SchurTile>& tiles, const std::vector<SchurContrib>& cs, size_t esz) {
+    Work w;
+    for (auto& t : tiles) {
+        w.bytes += 2.0 * t.tm * t.tn * esz;
+        for (int k = t.k0; k < t.k1; k++) {
+            w.bytes += (double)(t.tm + t.tn) * cs[k].k * esz;
+            w.flops += 2.0 * t.tm * t.tn * cs[k].k;
+        }
+    }
+    return w;
+}
+static Work work_copy(const std::vector<CopyTask>& v, size_t esz) {
+    Work w;
+    for (auto& t : v) w.bytes += (t.src ? 2.0 : 1.0) * t.rows * t.cols * esz;
+    return w;
+}
+static double geqp3_flops(double m, double n) {
+    const double k = std::min(m, n);
+    return k > 0 ? 2.0 * m * n * k - (2.0 / 3.0) * k * k * k : 0.0;
+}
+
 // Clusters with at least this many rows and more Asn elements than this take the stepped
 // multi-workgroup RRQR; smaller ones run in one workgroup. Both paths compute the same
 // numbers; SPAND_RRQR_STEPPED_MIN_ROWS / SPAND_RRQR_STEPPED_MIN_ELEMS override the split.
@@ -352,8 +394,13 @@ void Tree::factorize_impl() {
     factor_bytes_ = 0;
     // level-0 blocks
     T* base0 = reinterpret_cast<T*>(arena_.alloc((size_t)std::max<long long>(level0_elems_, 1) * sizeof(T)));
-    hip_check(launch_fill<T>(base0, (size_t)level0_elems_, T(0), st), "fill level0");
-    if (nscatter_ > 0) hip_check(launch_scatter<T>(d_scatter_off_, d_scatter_val_, nscatter_, base0, st), "scatter");
+    hip_check(timed(prof_, KF_FILL, st, (double)level0_elems_ * sizeof(T), 0,
+                    [&] { return launch_fill<T>(base0, (size_t)level0_elems_, T(0), st); }),
+              "fill level0");
+    if (nscatter_ > 0)
+        hip_check(timed(prof_, KF_SCATTER, st, (double)nscatter_ * (8.0 + 8.0 + sizeof(T)), 0,
+                        [&] { return launch_scatter<T>(d_scatter_off_, d_scatter_val_, nscatter_, base0, st); }),
+                  "scatter");
     for (size_t e = 0; e < edges_.size(); e++) edges_[e].A = base0 + edge0_off_[e];
     // device size table (room for one sibling per cluster)
     const int cap = (int)(2 * cls_.size() + 16);
@@ -371,6 +418,12 @@ void Tree::factorize_impl() {
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 48);
     const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 16384);
 
+    // the solve kernels stage one right-hand side of a segment in LDS at least
+    const int max_segment = SOLVE_LDS_BYTES / (int)sizeof(T) / 3;
+    for (int c = 0; c < (int)cls_.size(); c++)
+        if (cls_[c].osize > max_segment)
+            throw Error(ERR_UNSUPPORTED, "cluster of " + std::to_string(cls_[c].osize) +
+                                             " unknowns exceeds the solve kernels' segment limit");
     for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
         LevelStats& lst = stats_[ilvl_];
         LevelSolve& lsv = solve_plan_[ilvl_];
@@ -385,6 +438,7 @@ void Tree::factorize_impl() {
         long long fill_elems = 0;
         std::vector<long long> fill_off;  // by position in fill_edges
         std::vector<TrsvTask> sv_elim, sv_scale;
+        std::vector<CopyTask> inv_init;  // identity blocks that become explicit inverses L^{-1}
         std::unordered_map<int, std::vector<GemvTerm>> fwd_terms;  // by destination cluster
         std::vector<int> fwd_order;
         std::vector<GemvGroup> bwd_groups;
@@ -446,7 +500,14 @@ void Tree::factorize_impl() {
                 }
             // record ScalingLLT + GemmSymmOut/In (src/tree.cpp:909-910, 885-892)
             ops_.push_back({OP_SCALING_LLT, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr});
-            if (n > 0) sv_elim.push_back({piv.A, n, (int)s.xoff});
+            if (n > 0) {
+                // explicit inverse for the solve: L^{-1} = L^{-1} I by the chunked trsm
+                void* Li = arena_.alloc((size_t)n * n * sizeof(T));
+                inv_init.push_back({nullptr, Li, n, n, n, 0});
+                push_trsm(trsmE, piv.A, Li, n, n, 0, sizeof(T));
+                lst.flops_trsm += (double)n * n * n;
+                sv_elim.push_back({Li, n, (int)s.xoff});
+            }
             factor_bytes_ += (size_t)n * (n + 1) / 2 * sizeof(T);
             std::vector<GemvTerm> mine;
             for (int e : out) {
@@ -530,7 +591,13 @@ void Tree::factorize_impl() {
                 lst.flops_potrf += potrf_flops(s.size);
                 ops_.push_back({OP_SCALING_LLT, c, -1, s.size, 0, s.size, s.size, ilvl_, piv.A, nullptr});
                 factor_bytes_ += (size_t)s.size * (s.size + 1) / 2 * sizeof(T);
-                if (s.size > 0) sv_scale.push_back({piv.A, s.size, (int)s.xoff});
+                if (s.size > 0) {
+                    void* Li = arena_.alloc((size_t)s.size * s.size * sizeof(T));
+                    inv_init.push_back({nullptr, Li, s.size, s.size, s.size, 0});
+                    push_trsm(trsmA, piv.A, Li, s.size, s.size, 0, sizeof(T));
+                    lst.flops_trsm += (double)s.size * s.size * s.size;
+                    sv_scale.push_back({Li, s.size, (int)s.xoff});
+                }
             }
             for (int c : live) {
                 for (size_t k = 1; k < cls_[c].out.size(); k++) {
@@ -622,7 +689,7 @@ void Tree::factorize_impl() {
                     t.scratch = arena_.alloc((size_t)s.size * b.nslab * 64 * sizeof(T));
                     t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * std::max(cols, 1) * sizeof(double)));
                     b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
-                    b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)2 * b.nslab * sizeof(double)));
+                    b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)(2 * b.nslab + 1) * sizeof(double)));
                     b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
                     b.dims = reinterpret_cast<int*>(arena_.alloc(4 * sizeof(int)));
                     b.t = t;
@@ -643,12 +710,26 @@ void Tree::factorize_impl() {
                 }
             }
         }
+        // pivot-column / reflector buffers of the stepped path: LDS unless a wave's largest
+        // cluster needs more than STEP_LDS_MAX, then global buffers for the whole wave
+        std::vector<size_t> wlds_step(nwaves, 0), wlds_fin(nwaves, 0);
+        for (int w = 0; w < nwaves; w++) {
+            const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
+            if (2 * vb <= (size_t)STEP_LDS_MAX) {
+                wlds_step[w] = 2 * vb;
+                wlds_fin[w] = vb;
+            } else {
+                for (auto& b : wbig[w])
+                    b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
+            }
+        }
         lst.nwaves = nwaves;
         // ---------------- upload + launch Elim / Scale / Sparsify ----------------
         for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
         Packer pk;
         size_t o_pe = pk.add(potrfE), o_te = pk.add(trsmE), o_ti = pk.add(tiles), o_co = pk.add(contribs);
         size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
+        size_t o_ii = pk.add(inv_init);
         std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves);
         for (int w = 0; w < nwaves; w++) {
             o_w[w] = pk.add(wtasks[w]);
@@ -658,23 +739,82 @@ void Tree::factorize_impl() {
         const double t_plan1 = wall();
         void* dt = upload_tasks(pk.buf);
         hip_check(hipMemcpy(d_size_, hsize.data(), cls_.size() * sizeof(int), hipMemcpyHostToDevice), "upload sizes");
-        if (fill_elems > 0) hip_check(launch_fill<T>(fill_base, (size_t)fill_elems, T(0), st), "fill fill-in");
-        hip_check(launch_potrf<T>(Packer::at<PotrfTask>(dt, o_pe), (int)potrfE.size(), d_info_, st), "potrf elim");
-        hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_te), (int)trsmE.size(), st), "trsm elim");
-        hip_check(launch_schur<T>(Packer::at<SchurTile>(dt, o_ti), (int)tiles.size(),
-                                  Packer::at<SchurContrib>(dt, o_co), st),
-                  "schur");
-        hip_check(launch_potrf<T>(Packer::at<PotrfTask>(dt, o_ps), (int)potrfS.size(), d_info_, st), "potrf scale");
-        hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_ta), (int)trsmA.size(), st), "trsm scale A");
-        hip_check(launch_trsm<T>(Packer::at<TrsmTask>(dt, o_tb), (int)trsmB.size(), st), "trsm scale B");
+        const size_t esz = sizeof(T);
+        if (fill_elems > 0)
+            hip_check(timed(prof_, KF_FILL, st, (double)fill_elems * esz, 0,
+                            [&] { return launch_fill<T>(fill_base, (size_t)fill_elems, T(0), st); }),
+                      "fill fill-in");
+        {
+            Work w = work_copy(inv_init, esz);
+            hip_check(timed(prof_, KF_COPY, st, w.bytes, 0,
+                            [&] { return launch_copy<T>(Packer::at<CopyTask>(dt, o_ii), (int)inv_init.size(), st); }),
+                      "identity blocks");
+        }
+        auto run_potrf = [&](const std::vector<PotrfTask>& v, size_t off, const char* what) {
+            if (v.empty()) return;
+            Work w = work_potrf(v, esz);
+            hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
+                            [&] { return launch_potrf<T>(Packer::at<PotrfTask>(dt, off), (int)v.size(), d_info_, st); }),
+                      what);
+        };
+        auto run_trsm = [&](const std::vector<TrsmTask>& v, size_t off, const char* what) {
+            if (v.empty()) return;
+            Work w = work_trsm(v, esz);
+            hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
+                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), st); }),
+                      what);
+        };
+        run_potrf(potrfE, o_pe, "potrf elim");
+        run_trsm(trsmE, o_te, "trsm elim");
+        if (!tiles.empty()) {
+            Work w = work_schur(tiles, contribs, esz);
+            hip_check(timed(prof_, KF_SCHUR, st, w.bytes, w.flops,
+                            [&] {
+                                return launch_schur<T>(Packer::at<SchurTile>(dt, o_ti), (int)tiles.size(),
+                                                       Packer::at<SchurContrib>(dt, o_co), st);
+                            }),
+                      "schur");
+        }
+        run_potrf(potrfS, o_ps, "potrf scale");
+        run_trsm(trsmA, o_ta, "trsm scale A");
+        run_trsm(trsmB, o_tb, "trsm scale B");
+        // stepped-path launch slots, for the post-hoc work attribution below
+        std::vector<int> slot_init(nwaves, -1), slot_finish(nwaves, -1);
+        std::vector<std::vector<int>> slot_step(nwaves);
         for (int w = 0; w < nwaves; w++) {
-            hip_check(launch_rrqr<T>(Packer::at<RrqrTask>(dt, o_w[w]), (int)wtasks[w].size(),
-                                     Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st),
-                      "rrqr");
-            hip_check(launch_rrqr_big<T>(Packer::at<RrqrBig>(dt, o_b[w]), (int)wbig[w].size(),
-                                         Packer::at<RrqrSlab>(dt, o_s[w]), (int)wslab[w].size(),
-                                         Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, wsteps[w], wmaxrows[w], st),
-                      "rrqr (stepped)");
+            if (!wtasks[w].empty()) {
+                Work wk;
+                for (auto& t : wtasks[w]) {
+                    wk.bytes += (2.0 * t.max_rows * t.max_cols + (double)t.max_rows * std::min(t.max_rows, t.max_cols)) * esz;
+                    wk.flops += geqp3_flops(t.max_rows, t.max_cols);
+                }
+                hip_check(timed(prof_, KF_RRQR, st, wk.bytes, wk.flops,
+                                [&] {
+                                    return launch_rrqr<T>(Packer::at<RrqrTask>(dt, o_w[w]), (int)wtasks[w].size(),
+                                                          Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st);
+                                }),
+                          "rrqr");
+            }
+            if (wbig[w].empty()) continue;
+            const RrqrBig* db = Packer::at<RrqrBig>(dt, o_b[w]);
+            const RrqrSlab* ds = Packer::at<RrqrSlab>(dt, o_s[w]);
+            const int nsl = (int)wslab[w].size();
+            hip_check(timed(prof_, KF_RRQR_INIT, st, 0, 0,
+                            [&] { return launch_rrqr_init<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st); },
+                            &slot_init[w]),
+                      "rrqr init");
+            slot_step[w].assign(wsteps[w], -1);
+            for (int i = 0; i < wsteps[w]; i++)
+                hip_check(timed(prof_, KF_RRQR_STEP, st, 0, 0,
+                                [&] { return launch_rrqr_step<T>(db, ds, nsl, i, tol_, wlds_step[w], st); }, &slot_step[w][i]),
+                          "rrqr step");
+            hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,
+                            [&] {
+                                return launch_rrqr_finish<T>(db, (int)wbig[w].size(), Packer::at<RrqrSeg>(dt, o_sg),
+                                                             d_size_, tol_, wlds_fin[w], st);
+                            },
+                            &slot_finish[w]),
+                      "rrqr finish");
         }
         int info = 0;
         hip_check(hipMemcpy(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost), "read info");
@@ -682,11 +822,43 @@ void Tree::factorize_impl() {
             throw Error(ERR_NOT_SPD, "Error: Non-SPD Pivot (cluster order " + std::to_string(info - 1) + ")");
         if (!plist.empty())
             hip_check(hipMemcpy(hsize.data(), d_size_, cls_.size() * sizeof(int), hipMemcpyDeviceToHost), "read ranks");
+        if (prof_.on()) {
+            // the stepped path's sizes are decided on the device (ranks of earlier waves)
+            for (int w = 0; w < nwaves; w++) {
+                if (wbig[w].empty()) continue;
+                std::vector<int> rows, cols, mns;
+                for (auto& b : wbig[w]) {
+                    int dims[4];
+                    hip_check(hipMemcpy(dims, b.dims, 4 * sizeof(int), hipMemcpyDeviceToHost), "read dims");
+                    rows.push_back(dims[0]);
+                    cols.push_back(dims[1]);
+                    mns.push_back(dims[3]);  // steps run: the QR stops at the rank
+                }
+                double ib = 0, fb = 0;
+                for (size_t k = 0; k < rows.size(); k++) {
+                    ib += ((double)rows[k] * cols[k] * 2 * esz) + 16.0 * cols[k];
+                    fb += ((double)rows[k] * cols[k] * 2 + (double)rows[k] * mns[k]) * esz;
+                }
+                prof_.set_work(slot_init[w], ib, 0);
+                prof_.set_work(slot_finish[w], fb, 0);
+                for (int i = 0; i < (int)slot_step[w].size(); i++) {
+                    double by = 0, fl = 0;
+                    for (size_t k = 0; k < rows.size(); k++) {
+                        if (i >= mns[k]) continue;
+                        const double m = rows[k] - i, n = cols[k] - i;
+                        by += (2.0 * m * n + m) * esz + 16.0 * n;
+                        fl += 4.0 * m * n;
+                    }
+                    prof_.set_work(slot_step[w][i], by, fl);
+                }
+            }
+        }
         const double t_lvl1 = wall();
         lst.t_plan = t_plan1 - t_lvl0;
         lst.t_elim_scale = t_lvl1 - t_plan1;
         // ---------------- apply ranks: Orthogonal, Split, sibling (src/tree.cpp:1317-1342) ----------------
         std::vector<QTask> qt;
+        std::vector<LarftTask> larft;
         for (size_t p = 0; p < plist.size(); p++) {
             const int c = plist[p];
             const int r = hsize[c];
@@ -697,8 +869,12 @@ void Tree::factorize_impl() {
                 const RrqrTask* t = where[p].first ? &wbig[wave[p]][where[p].second].t
                                                    : &wtasks[wave[p]][where[p].second];
                 ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
-                factor_bytes_ += ((size_t)s0 * r + r) * sizeof(T);
-                if (r > 0) qt.push_back({t->Q, t->tau, s0, r, (int)cls_[c].xoff, 0});
+                factor_bytes_ += ((size_t)s0 * r + (size_t)r * (r + 1) / 2) * sizeof(T);
+                if (r > 0) {
+                    void* Tm = arena_.alloc((size_t)r * r * sizeof(T));
+                    larft.push_back({t->Q, t->tau, Tm, s0, r});
+                    qt.push_back({t->Q, t->tau, Tm, s0, r, (int)cls_[c].xoff, 0});
+                }
                 HCluster sib;
                 sib.start = cls_[c].start + r;
                 sib.size = sib.osize = s0 - r;
@@ -794,8 +970,27 @@ void Tree::factorize_impl() {
             Packer pm;
             size_t o_c = pm.add(copies);
             void* dm = upload_tasks(pm.buf);
-            if (melems > 0) hip_check(launch_fill<T>(mbase, (size_t)melems, T(0), st), "fill merge");
-            hip_check(launch_copy<T>(Packer::at<CopyTask>(dm, o_c), (int)copies.size(), st), "merge copy");
+            if (melems > 0)
+                hip_check(timed(prof_, KF_FILL, st, (double)melems * sizeof(T), 0,
+                                [&] { return launch_fill<T>(mbase, (size_t)melems, T(0), st); }),
+                          "fill merge");
+            Work wc = work_copy(copies, sizeof(T));
+            hip_check(timed(prof_, KF_COPY, st, wc.bytes, 0,
+                            [&] { return launch_copy<T>(Packer::at<CopyTask>(dm, o_c), (int)copies.size(), st); }),
+                      "merge copy");
+        }
+        if (!larft.empty()) {
+            Packer pl;
+            size_t o_l = pl.add(larft);
+            void* dl = upload_tasks(pl.buf);
+            double lb = 0, lf = 0;
+            for (auto& l : larft) {
+                lb += ((double)l.m * l.k + (double)l.k * l.k) * sizeof(T);
+                lf += (double)l.m * l.k * l.k;
+            }
+            hip_check(timed(prof_, KF_LARFT, st, lb, lf,
+                            [&] { return launch_larft<T>(Packer::at<LarftTask>(dl, o_l), (int)larft.size(), st); }),
+                      "larft");
         }
         lst.t_merge = wall() - t_lvl2;
         int left = 0;
@@ -823,6 +1018,41 @@ void Tree::factorize_impl() {
         size_t s_et = ps.add(sv_elim), s_fg = ps.add(fwd_groups), s_bg = ps.add(bwd_groups), s_tm = ps.add(terms);
         size_t s_st = ps.add(sv_scale), s_q = ps.add(qt), s_m = ps.add(mt), s_mc = ps.add(mchildren);
         void* dsv = upload_tasks(ps.buf);
+        {
+            const double esz = sizeof(T);
+            auto trmv_work = [&](const std::vector<TrsvTask>& v) {
+                SweepWork w;
+                for (auto& t : v) {
+                    w.mat_bytes += (double)t.n * (t.n + 1) / 2 * esz;
+                    w.vec_elems += 2.0 * t.n;
+                    w.flops += (double)t.n * (t.n + 1);
+                }
+                return w;
+            };
+            auto gemv_work = [&](const std::vector<GemvGroup>& gs) {
+                SweepWork w;
+                for (auto& g : gs) {
+                    w.vec_elems += 2.0 * g.n;
+                    for (int k = g.k0; k < g.k1; k++) {
+                        const GemvTerm& t = terms[k];
+                        w.mat_bytes += (double)t.rows * t.cols * esz;
+                        w.vec_elems += t.trans ? t.rows : t.cols;
+                        w.flops += 2.0 * t.rows * t.cols;
+                    }
+                }
+                return w;
+            };
+            lsv.w_elim = trmv_work(sv_elim);
+            lsv.w_scale = trmv_work(sv_scale);
+            lsv.w_fwd = gemv_work(fwd_groups);
+            lsv.w_bwd = gemv_work(bwd_groups);
+            for (auto& q : qt) {
+                lsv.w_q.mat_bytes += ((double)q.m * q.k + (double)q.k * (q.k + 1) / 2) * esz;
+                lsv.w_q.vec_elems += 2.0 * q.m;
+                lsv.w_q.flops += 4.0 * q.m * q.k + (double)q.k * q.k;
+            }
+            for (auto& c : mchildren) lsv.w_merge.vec_elems += 2.0 * c.y;
+        }
         lsv.elim_trsv = Packer::at<TrsvTask>(dsv, s_et);
         lsv.n_elim_trsv = (int)sv_elim.size();
         lsv.fwd_groups = Packer::at<GemvGroup>(dsv, s_fg);
@@ -843,6 +1073,7 @@ void Tree::factorize_impl() {
     }
     hip_check(hipDeviceSynchronize(), "factorize");
     t_factor_ = wall() - t_start;
+    prof_.collect();
 }
 
 // ---------------------------------------------------------------------------------------
@@ -999,24 +1230,42 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
         ws_cap_ = need;
     }
     T* ws = reinterpret_cast<T*>(d_ws_);
-    hip_check(launch_perm_in<T>(dx, sx, sr, nrhs, d_perm_, N_, ws, st), "perm in");
+    const double esz = sizeof(T);
+    auto run = [&](int fam, const SweepWork& w, const char* what, auto&& launch) {
+        hip_check(timed(prof_, fam, st, w.mat_bytes + w.vec_elems * nrhs * esz, w.flops * nrhs, launch), what);
+    };
+    SweepWork wp;
+    wp.vec_elems = (double)N_ * (1.0 + 8.0 / esz) + (double)N_ * 4.0 / esz / nrhs;
+    run(KF_PERM, wp, "perm in", [&] { return launch_perm_in<T>(dx, sx, sr, nrhs, d_perm_, N_, ws, st); });
     for (int l = 0; l < nlevels_; l++) {
         const LevelSolve& p = solve_plan_[l];
-        hip_check(launch_trsv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 0, st), "trsv");
-        hip_check(launch_gemv<T>(p.fwd_groups, p.n_fwd_groups, p.terms, ws, nrhs, st), "gemv");
-        hip_check(launch_trsv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 0, st), "trsv");
-        hip_check(launch_qapply<T>(p.q, p.n_q, ws, nrhs, 1, st), "qapply");
-        hip_check(launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 1, st), "merge");
+        if (p.n_elim_trsv)
+            run(KF_TRMV, p.w_elim, "trmv", [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 0, st); });
+        if (p.n_fwd_groups)
+            run(KF_GEMV, p.w_fwd, "gemv",
+                [&] { return launch_gemv<T>(p.fwd_groups, p.n_fwd_groups, p.terms, ws, nrhs, st); });
+        if (p.n_scale_trsv)
+            run(KF_TRMV, p.w_scale, "trmv", [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 0, st); });
+        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 1, st); });
+        if (p.n_merge)
+            run(KF_MERGE, p.w_merge, "merge",
+                [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 1, st); });
     }
     for (int l = nlevels_ - 1; l >= 0; l--) {
         const LevelSolve& p = solve_plan_[l];
-        hip_check(launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 0, st), "merge");
-        hip_check(launch_qapply<T>(p.q, p.n_q, ws, nrhs, 0, st), "qapply");
-        hip_check(launch_trsv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 1, st), "trsv");
-        hip_check(launch_gemv<T>(p.bwd_groups, p.n_bwd_groups, p.terms, ws, nrhs, st), "gemv");
-        hip_check(launch_trsv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 1, st), "trsv");
+        if (p.n_merge)
+            run(KF_MERGE, p.w_merge, "merge",
+                [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 0, st); });
+        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 0, st); });
+        if (p.n_scale_trsv)
+            run(KF_TRMV, p.w_scale, "trmv", [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 1, st); });
+        if (p.n_bwd_groups)
+            run(KF_GEMV, p.w_bwd, "gemv",
+                [&] { return launch_gemv<T>(p.bwd_groups, p.n_bwd_groups, p.terms, ws, nrhs, st); });
+        if (p.n_elim_trsv)
+            run(KF_TRMV, p.w_elim, "trmv", [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 1, st); });
     }
-    hip_check(launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st), "perm out");
+    run(KF_PERM, wp, "perm out", [&] { return launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st); });
 }
 
 }  // namespace spand

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "profile.hpp"
 #include "tasks.hpp"
 
 namespace spand {
@@ -99,8 +100,15 @@ struct OpRec {
     const void* tau;
 };
 
+// Algorithmic work of one solve launch: matrix bytes read once per apply, vector elements
+// moved per right-hand side, flops per right-hand side.
+struct SweepWork {
+    double mat_bytes = 0, vec_elems = 0, flops = 0;
+};
+
 // Per-level solve plan (device task arrays) for the preconditioner apply.
 struct LevelSolve {
+    SweepWork w_elim, w_fwd, w_bwd, w_scale, w_q, w_merge;
     TrsvTask* elim_trsv = nullptr; int n_elim_trsv = 0;
     GemvGroup* fwd_groups = nullptr; int n_fwd_groups = 0;
     GemvGroup* bwd_groups = nullptr; int n_bwd_groups = 0;
@@ -160,6 +168,8 @@ class Tree {
     void op_data(int i, double* M, double* tau) const;
     // Trailing (current) matrix as COO triplets; returns the count (arrays may be null).
     long long trailing_coo(int* I, int* J, double* V) const;
+    // Per-launch kernel timing (HIP events on the launch stream); off by default.
+    KernelProfiler& profiler() { return prof_; }
 
   private:
     enum class State { Created, Partitioned, Assembled, Factorized };
@@ -221,6 +231,7 @@ class Tree {
     size_t ws_cap_ = 0;
     long long ws_len_ = 0;   // segments in the workspace (per right-hand side)
 
+    KernelProfiler prof_;
     std::vector<LevelSolve> solve_plan_;
     std::vector<OpRec> ops_;
     std::vector<LevelStats> stats_;

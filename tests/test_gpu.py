@@ -183,3 +183,29 @@ def test_not_spd_reports_status():
     g = gpu_tree(A, X, 3, 1e-2)
     with pytest.raises(S.SpandError, match="NOT_SPD"):
         g.factorize()
+
+
+def test_kernel_profile_and_device_pcg():
+    """Kernel timer totals are consistent with the launches, and PCG on device-resident
+    right-hand sides gives the same iterates as the host-array entry point."""
+    torch = pytest.importorskip("torch")
+    say("case profile + device pcg")
+    A, X = neglapl(12, 3), linspace_nd(12, 3)
+    g = gpu_tree(A, X, 6, 1e-2)
+    g.set_profiling(True)
+    g.factorize()
+    B = np.random.default_rng(8).standard_normal((A.n, 3))
+    xh, info = S.cg(A, B, g, iters=100, tol=1e-12)
+    prof = g.kernel_profile()
+    for fam in ("potrf", "trsm", "schur", "trmv", "gemv", "spmv"):
+        assert fam in prof and prof[fam]["launches"] > 0 and prof[fam]["ms"] > 0, (fam, prof.get(fam))
+        assert prof[fam]["bytes"] > 0
+    g.set_profiling(False)
+    g.profile_reset()
+    assert g.kernel_profile() == {}
+    d_b = torch.tensor(B.T.copy(), dtype=torch.float64, device="cuda:0")
+    d_x = torch.zeros_like(d_b)
+    its, dinfo = S.cg_device(S.Matrix(A), g, d_b.data_ptr(), d_x.data_ptr(), 3, A.n, iters=100, tol=1e-12)
+    torch.cuda.synchronize()
+    assert np.array_equal(its, info["iters"])
+    assert np.array_equal(d_x.cpu().numpy().T, xh)

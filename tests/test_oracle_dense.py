@@ -111,3 +111,71 @@ def test_trsv_and_gemv_closed_forms():
         x = b.copy()
         L.or_trsv(kind, n, Lm, n, x)
         assert np.allclose(M @ x, b, rtol=0, atol=1e-12)
+
+
+def test_geqp3_exact_ties_kat():
+    # Exactly tied column norms (orthogonal columns, all arithmetic exact): LAPACK's dgeqp3
+    # takes the first maximum by current position, after its column swaps. Answers from
+    # scipy.linalg.lapack.dgeqp3 (OpenBLAS), 0-based.
+    A = np.asfortranarray(np.diag([1.0, 2.0, 2.0, 1.0]))
+    jpvt, tau = np.zeros(4, dtype=np.int32), np.zeros(4)
+    L.or_geqp3(4, 4, A, 4, jpvt, tau)
+    assert jpvt.tolist() == [1, 2, 0, 3]
+    B = np.vstack([np.diag([1.0, 2.0, 2.0, 1.0]), np.zeros((2, 4))])
+    B[4, 0], B[5, 3] = 2.0, 3.0
+    B = np.asfortranarray(B)
+    L.or_geqp3(6, 4, B, 6, jpvt, tau)
+    assert jpvt.tolist() == [3, 0, 2, 1]
+    assert np.allclose(np.abs(np.diag(B)), [np.sqrt(10), np.sqrt(5), 2.0, 2.0], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [0, 1, 15, 16, 17, 33, 1000])
+def test_reduction16_sums(n):
+    # The RRQR's fixed 16-chain summation order: on integer data every partial sum is
+    # exact, so any dropped, repeated or misplaced term changes the result.
+    rng = np.random.default_rng(n)
+    a = rng.integers(-1000, 1000, n).astype(np.float64)
+    b = rng.integers(-1000, 1000, n).astype(np.float64)
+    assert L.or_sumsq16(n, a) == float(np.sum(a.astype(np.int64) ** 2))
+    assert L.or_dot16(n, a, b) == float(np.sum(a.astype(np.int64) * b.astype(np.int64)))
+    # and on random data it is a sum: relative error within n * eps
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    s, d = L.or_sumsq16(n, x), L.or_dot16(n, x, y)
+    exact_s = float(np.sum(np.float128(x) ** 2)) if hasattr(np, "float128") else float(np.sum(x * x))
+    assert abs(s - exact_s) <= (n + 1) * 2.3e-16 * exact_s + 1e-300
+    assert abs(d - float(np.dot(x, y))) <= (n + 1) * 2.3e-16 * float(np.dot(np.abs(x), np.abs(y))) + 1e-300
+
+
+def test_gemm_syrk_schur_closed_forms():
+    rng = np.random.default_rng(7)
+    m, n, k = 13, 9, 21
+    A = np.asfortranarray(rng.standard_normal((m, k)))
+    B = np.asfortranarray(rng.standard_normal((k, n)))
+    C0 = rng.standard_normal((m, n))
+    C = np.asfortranarray(C0.copy())
+    L.or_gemm(0, 0, m, n, k, -1.0, A, m, B, k, 1.0, C, m)
+    assert np.allclose(C, C0 - A @ B, rtol=0, atol=1e-12)
+    At, Bt = np.asfortranarray(A.T), np.asfortranarray(B.T)
+    C = np.asfortranarray(C0.copy())
+    L.or_gemm(1, 1, m, n, k, 2.0, At, k, Bt, n, 0.5, C, m)
+    assert np.allclose(C, 0.5 * C0 + 2.0 * A @ B, rtol=0, atol=1e-12)
+    # Schur update C -= op(A) op(B), lower triangle only when asked
+    for tA, tB in ((0, 1), (1, 0), (0, 0)):
+        Aop = np.asfortranarray(A.T) if tA else A
+        Bop = np.asfortranarray(B.T) if tB else B
+        C = np.asfortranarray(C0.copy())
+        L.or_schur(tA, tB, 0, m, n, k, Aop, Aop.shape[0], Bop, Bop.shape[0], C, m)
+        assert np.allclose(C, C0 - A @ B, rtol=0, atol=1e-12)
+    S0 = rng.standard_normal((m, m))
+    S = np.asfortranarray(S0.copy())
+    L.or_schur(0, 1, 1, m, m, k, A, m, A, m, S, m)
+    ref = S0 - A @ A.T
+    low = np.tril(np.ones((m, m), dtype=bool))
+    assert np.allclose(S[low], ref[low], rtol=0, atol=1e-12)
+    assert np.array_equal(S[~low], S0[~low])
+    # syrk lower (src/util.cpp:148-156): C := alpha A A^T + beta C on the lower triangle
+    S = np.asfortranarray(S0.copy())
+    L.or_syrk(0, m, k, -1.0, A, m, 1.0, S, m)
+    assert np.allclose(S[low], ref[low], rtol=0, atol=1e-12)
+    assert np.array_equal(S[~low], S0[~low])
