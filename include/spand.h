@@ -98,12 +98,14 @@ spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long lo
                                    long long* arena_bytes);
 
 /* Kernel timing (the reference's timer log, include/util.h:366-401, at kernel granularity):
- * when enabled, every kernel launch of the tree (factorization, preconditioner apply) and of
- * spand_pcg is bracketed by two HIP events on its stream. Off by default; enabling it adds
- * two event records per launch. spand_kernel_profile waits for the recorded events and
- * returns, for kernel family `family` (0 <= family < spand_kernel_family_count()): launches,
- * summed event time (ms), algorithmic bytes and flops of those launches. */
-spand_status spand_set_profiling(spand_tree* t, int on);
+ * for every kernel family whose bit is set in family_mask (bit f = family f; 0 turns timing
+ * off, the default), each launch of the tree (factorization, preconditioner apply) and of
+ * spand_pcg is bracketed by two HIP events on its stream; the stepped RRQR's steps are
+ * bracketed per dependency wave (one event pair around all steps of the wave).
+ * spand_kernel_profile waits for the recorded events and returns, for kernel family
+ * `family` (0 <= family < spand_kernel_family_count()): launches, summed event time (ms),
+ * algorithmic bytes and flops of those launches. */
+spand_status spand_set_profiling(spand_tree* t, unsigned long long family_mask);
 spand_status spand_profile_reset(spand_tree* t);
 int spand_kernel_family_count(void);
 const char* spand_kernel_family_name(int family);

@@ -156,8 +156,23 @@ class Tree:
         return {"seconds": s.value, "factor_bytes": fb.value, "arena_bytes": ab.value}
 
     # -- kernel timing (HIP events per launch) --
-    def set_profiling(self, on: bool):
-        _check(self._h, self._L.spand_set_profiling(self._h, int(bool(on))))
+    def kernel_families(self) -> list:
+        return [self._L.spand_kernel_family_name(f).decode() for f in range(self._L.spand_kernel_family_count())]
+
+    def set_profiling(self, on=True):
+        """True: time every kernel family; False: off; a family name or a list of names:
+        time only those families."""
+        if on is True:
+            mask = (1 << self._L.spand_kernel_family_count()) - 1
+        elif not on:
+            mask = 0
+        else:
+            names = [on] if isinstance(on, str) else list(on)
+            fams = self.kernel_families()
+            mask = 0
+            for n in names:
+                mask |= 1 << fams.index(n)
+        _check(self._h, self._L.spand_set_profiling(self._h, mask))
 
     def profile_reset(self):
         _check(self._h, self._L.spand_profile_reset(self._h))

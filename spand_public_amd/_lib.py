@@ -53,7 +53,7 @@ SIGNATURES = {
     "spand_get_level_stats": (C.c_int, [vp, f64p]),
     "spand_get_factor_info": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),
-    "spand_set_profiling": (C.c_int, [vp, C.c_int]),
+    "spand_set_profiling": (C.c_int, [vp, C.c_ulonglong]),
     "spand_profile_reset": (C.c_int, [vp]),
     "spand_kernel_family_count": (C.c_int, []),
     "spand_kernel_family_name": (C.c_char_p, [C.c_int]),
@@ -69,6 +69,17 @@ SIGNATURES = {
 }
 
 
+def _share_hip_runtime_with_torch():
+    """PyTorch-ROCm wheels carry their own libamdhip64.so.7. If libspand.so were loaded
+    first, the process would hold two HIP runtimes and the second one to initialize would
+    find no device. Loading torch first lets libspand.so bind to the runtime already in the
+    process (same soname). Without torch nothing changes."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Loads libspand.so (raises if it has not been built)."""
     global _lib
@@ -76,6 +87,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build the HIP library first (python -m spand_public_amd.build)")
+        _share_hip_runtime_with_torch()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -183,11 +183,11 @@ spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long lo
     if (ab) *ab = (long long)t->t->arena_bytes();
     return SPAND_OK;
 }
-spand_status spand_set_profiling(spand_tree* t, int on) {
+spand_status spand_set_profiling(spand_tree* t, unsigned long long family_mask) {
     CHECK_T(t);
     return guarded(t, [&] {
         t->t->profiler().collect();
-        t->t->profiler().enable(on != 0);
+        t->t->profiler().set_mask(family_mask);
     });
 }
 spand_status spand_profile_reset(spand_tree* t) {

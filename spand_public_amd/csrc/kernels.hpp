@@ -18,6 +18,13 @@ template <typename T> hipError_t launch_fill(T* p, size_t n, T v, hipStream_t st
 template <typename T>
 hipError_t launch_scatter(const long long* off, const double* val, long long n, T* base, hipStream_t st);
 template <typename T> hipError_t launch_potrf(const PotrfTask* d_tasks, int ntasks, int* d_info, hipStream_t st);
+// Pivots larger than POTRF_SMALL_MAX: one panel of 32 columns per call pair, kb = 0, 32, ...
+// (launch_potrf_trail's grid is sized by nmax, the largest pivot of the batch).
+constexpr int POTRF_SMALL_MAX = 128;
+template <typename T>
+hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int* d_info, hipStream_t st);
+template <typename T>
+hipError_t launch_potrf_trail(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, hipStream_t st);
 template <typename T> hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st);
 template <typename T>
 hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib* d_contribs, hipStream_t st);
@@ -36,8 +43,8 @@ template <typename T>
 hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol,
                             size_t lds_bytes, hipStream_t st);
 template <typename T>
-hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, int nbig, const RrqrSeg* d_segs, int* d_size, double tol,
-                              size_t lds_bytes, hipStream_t st);
+hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
+                              int* d_size, double tol, size_t lds_bytes, hipStream_t st);
 template <typename T> hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st);
 
 // Preconditioner sweeps. ws holds the per-cluster segments, interleaved over nrhs.
@@ -50,19 +57,22 @@ hipError_t launch_perm_out(const T* ws, const int* perm, int N, int nrhs, double
                            hipStream_t st);
 hipError_t launch_cm2il(const double* src, int ld, int N, int nrhs, double* dst, hipStream_t st);
 hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld, hipStream_t st);
+// The solve kernels stage one right-hand side of a segment in LDS at least: a segment of m
+// rows needs m * sizeof(T) bytes (m + 2 QB for the Householder apply). seg_max is the
+// largest segment of the launch; it sizes the dynamic LDS.
+constexpr int SOLVE_LDS_BYTES = 65536;
+// reflectors per compact-WY block
+constexpr int QB = 32;
 // x := Linv x (trans 0) or Linv^T x (trans 1) with the explicit inverse factor.
 template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
-// compact-WY factors of the Orthogonal ops
+hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st);
+// compact-WY factor of one block of reflectors per task
 template <typename T> hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st);
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, const GemvTerm* d_terms, T* ws, int nrhs,
+hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, int seg_max, const GemvTerm* d_terms, T* ws, int nrhs,
                        hipStream_t st);
-// LDS of one solve workgroup; a segment of m rows with k reflectors needs
-// (m + 2k) * sizeof(T) bytes per right-hand side in flight.
-constexpr int SOLVE_LDS_BYTES = 65536;
 template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
+hipError_t launch_qapply(const QTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st);
 template <typename T>
 hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, T* ws, int nrhs, int fwd,
                         hipStream_t st);

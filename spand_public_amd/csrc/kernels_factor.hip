@@ -117,6 +117,108 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
     if (bad && tid == 0) atomicCAS(info, 0, t.cluster + 1);
 }
 
+// Large pivots take the same blocked algorithm one panel per launch pair, so the trailing
+// update of every panel spreads over many workgroups: k_potrf_panel factors the diagonal
+// block kb and scales / updates the panel rows below it (one workgroup per pivot), then
+// k_potrf_trail applies the panel to the trailing lower triangle, one 32 x 32 tile per
+// workgroup. Every element sees the same operations in the same order as in k_potrf.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_potrf_panel(const PotrfTask* tasks, int kb, int* info) {
+#pragma clang fp contract(off)
+    __shared__ T D[PB][PB + 1];
+    __shared__ T rdiag[PB];
+    __shared__ int bad;
+    const PotrfTask t = tasks[blockIdx.x];
+    const int n = t.n;
+    const int tid = threadIdx.x;
+    if (kb >= n) return;
+    T* A = reinterpret_cast<T*>(t.A);
+    if (tid == 0) bad = 0;
+    const int nb = min(PB, n - kb);
+    for (int idx = tid; idx < nb * nb; idx += NT) {
+        int i = idx % nb, k = idx / nb;
+        D[i][k] = A[(kb + i) + (size_t)(kb + k) * n];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; j++) {
+        if (tid == 0) {
+            T d = D[j][j];
+            if (!(d > T(0))) {
+                if (!bad) bad = kb + j + 1;
+            } else {
+                D[j][j] = sqrt(d);
+            }
+            rdiag[j] = T(1) / D[j][j];
+        }
+        __syncthreads();
+        if (bad) break;
+        const T r = rdiag[j];
+        for (int i = j + 1 + tid; i < nb; i += NT) D[i][j] *= r;
+        __syncthreads();
+        const int mm = nb - j - 1;
+        for (int idx = tid; idx < mm * mm; idx += NT) {
+            int a = j + 1 + idx % mm, b = j + 1 + idx / mm;
+            if (a >= b) D[a][b] = fma(-D[a][j], D[b][j], D[a][b]);
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (tid == 0) atomicCAS(info, 0, t.cluster + 1);
+        return;
+    }
+    for (int idx = tid; idx < nb * nb; idx += NT) {
+        int i = idx % nb, k = idx / nb;
+        if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = D[i][k];
+    }
+    for (int i = kb + nb + tid; i < n; i += NT) {
+        for (int k = 0; k < nb; k++) {
+            T a = A[i + (size_t)(kb + k) * n] * rdiag[k];
+            A[i + (size_t)(kb + k) * n] = a;
+            for (int k2 = k + 1; k2 < nb; k2++)
+                A[i + (size_t)(kb + k2) * n] = fma(-a, D[k2][k], A[i + (size_t)(kb + k2) * n]);
+        }
+    }
+}
+
+// grid (tiles, tasks): tile index -> (bi, bj), bj <= bi, over the trailing lower triangle
+template <typename T>
+__global__ void __launch_bounds__(NT) k_potrf_trail(const PotrfTask* tasks, int kb) {
+#pragma clang fp contract(off)
+    __shared__ T As[PB][PB + 1];
+    __shared__ T Bs[PB][PB + 1];
+    const PotrfTask t = tasks[blockIdx.y];
+    const int n = t.n, tid = threadIdx.x;
+    const int r0 = kb + PB;  // a full panel precedes any trailing rows
+    if (r0 >= n) return;
+    const int nt = (n - r0 + PB - 1) / PB;
+    int p = blockIdx.x, bi = 0;
+    while (p > bi) {
+        p -= bi + 1;
+        bi++;
+    }
+    if (bi >= nt) return;
+    const int bj = p;
+    const int a0 = r0 + bi * PB, b0 = r0 + bj * PB;
+    const int na = min(PB, n - a0), nbb = min(PB, n - b0);
+    T* A = reinterpret_cast<T*>(t.A);
+    for (int idx = tid; idx < PB * PB; idx += NT) {
+        const int i = idx % PB, k = idx / PB;
+        As[k][i] = i < na ? A[(a0 + i) + (size_t)(kb + k) * n] : T(0);
+        Bs[k][i] = i < nbb ? A[(b0 + i) + (size_t)(kb + k) * n] : T(0);
+    }
+    __syncthreads();
+    const int ti = tid % PB, tj0 = (tid / PB) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int a = a0 + ti, b = b0 + tj0 + q;
+        if (ti < na && tj0 + q < nbb && a >= b) {
+            T acc = A[a + (size_t)b * n];
+            for (int k = 0; k < PB; k++) acc = fma(-As[k][ti], Bs[k][tj0 + q], acc);
+            A[a + (size_t)b * n] = acc;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // trsm on a chunk of <= 64 independent columns (left) or rows (right), in 32-row blocks of
 // L: the diagonal triangle is solved by one thread per independent index, the rows below
@@ -127,6 +229,7 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
 #pragma clang fp contract(off)
     __shared__ T Bk[PB][64 + 1];
     __shared__ T Lk[PB][PB + 1];
+    __shared__ T Lr[PB][64 + 1];
     const TrsmTask t = tasks[blockIdx.x];
     const int n = t.n, m = t.m, ld = t.ldb;
     const int tid = threadIdx.x;
@@ -168,17 +271,23 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
             if (left) { k = idx % nb; c = idx / nb; } else { c = idx % m; k = idx / m; }
             at(kb + k, c) = Bk[k][c];
         }
-        const int rest = n - kb - nb;
-        for (long long idx = tid; idx < (long long)rest * m; idx += NT) {
-            int i, c;
-            if (left) { i = kb + nb + (int)(idx % rest); c = (int)(idx / rest); }
-            else { c = (int)(idx % m); i = kb + nb + (int)(idx / m); }
-            T acc = at(i, c);
-            const T* li = L + i + (size_t)kb * n;
-            for (int k = 0; k < nb; k++) acc = fma(-Bk[k][c], li[(size_t)k * n], acc);
-            at(i, c) = acc;
+        // rows below the block, 64 at a time: L[i0:i0+64, kb:kb+nb] staged in LDS
+        for (int i0 = kb + nb; i0 < n; i0 += 64) {
+            const int ni = min(64, n - i0);
+            for (int idx = tid; idx < ni * nb; idx += NT) {
+                const int ii = idx % ni, k = idx / ni;
+                Lr[k][ii] = L[(i0 + ii) + (size_t)(kb + k) * n];
+            }
+            __syncthreads();
+            for (int idx = tid; idx < ni * m; idx += NT) {
+                int ii, c;
+                if (left) { ii = idx % ni; c = idx / ni; } else { c = idx % m; ii = idx / m; }
+                T acc = at(i0 + ii, c);
+                for (int k = 0; k < nb; k++) acc = fma(-Bk[k][c], Lr[k][ii], acc);
+                at(i0 + ii, c) = acc;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -805,8 +914,19 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     // w_j = v^T a_j: chain wv of the sum
     if (taud != 0.0) {
         T c = T(0);
-        if (active)
-            for (int r = i + wv; r < rows; r += NCH) c = fma(cj[(size_t)r * SLAB], s_v[r], c);
+        if (active) {
+            // four rows of the chain loaded ahead of the (ordered) accumulation
+            int r = i + wv;
+            for (; r + 3 * NCH < rows; r += 4 * NCH) {
+                const T a0 = cj[(size_t)r * SLAB], a1 = cj[(size_t)(r + NCH) * SLAB];
+                const T a2 = cj[(size_t)(r + 2 * NCH) * SLAB], a3 = cj[(size_t)(r + 3 * NCH) * SLAB];
+                c = fma(a0, s_v[r], c);
+                c = fma(a1, s_v[r + NCH], c);
+                c = fma(a2, s_v[r + 2 * NCH], c);
+                c = fma(a3, s_v[r + 3 * NCH], c);
+            }
+            for (; r < rows; r += NCH) c = fma(cj[(size_t)r * SLAB], s_v[r], c);
+        }
         s_part[wv][lane] = (double)c;
         __syncthreads();
         if (wv == 0) {
@@ -826,18 +946,41 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     if (active) {
         const bool upd = taud != 0.0 && s_upd[lane];
         const T tm = upd ? s_tm[lane] : T(0);
-        for (int r = i + wv; r < rows; r += NCH) {
+        int r = i + wv;
+        if (r == i && r < rows) {  // row i, chain 0's first row
             T a = cj[(size_t)r * SLAB];
             if (upd) {
-                a = (r == i) ? a + tm : fma(s_v[r], tm, a);
+                a = a + tm;
                 cj[(size_t)r * SLAB] = a;
             }
-            if (r == i) {
-                ai = a;
-            } else {
-                const double d = (double)a;
+            ai = a;
+            r += NCH;
+        }
+        for (; r + 3 * NCH < rows; r += 4 * NCH) {
+            T a[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[q] = cj[(size_t)(r + q * NCH) * SLAB];
+            if (upd) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    a[q] = fma(s_v[r + q * NCH], tm, a[q]);
+                    cj[(size_t)(r + q * NCH) * SLAB] = a[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double d = (double)a[q];
                 c2 = fma(d, d, c2);
             }
+        }
+        for (; r < rows; r += NCH) {
+            T a = cj[(size_t)r * SLAB];
+            if (upd) {
+                a = fma(s_v[r], tm, a);
+                cj[(size_t)r * SLAB] = a;
+            }
+            const double d = (double)a;
+            c2 = fma(d, d, c2);
         }
     }
     s_part[(wv + NCH - 1) % NCH][lane] = c2;
@@ -879,13 +1022,18 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     }
 }
 
+// One workgroup per slab: the owner of the last pivot column stores its reflector (when
+// the QR ran to min(rows, cols)), every slab copies the reflectors of its own columns into
+// Q and scatters R[:rank, j] of its columns into the shrunk edge blocks.
 template <typename T>
-__global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const RrqrSeg* segs, int* dsize, double tol) {
+__global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
+                                                    int* dsize, double tol) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_col0[RRQR_MAXSEG + 1];
     __shared__ double sh[4], s_ch[NCH];
-    const RrqrBig B = bigs[blockIdx.x];
+    const RrqrSlab sl = slabs[blockIdx.x];
+    const RrqrBig B = bigs[sl.big];
     const RrqrSeg* sg = segs + B.t.seg0;
     const int tid = threadIdx.x;
     const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2], stop = B.dims[3];
@@ -905,30 +1053,31 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
     T* A = reinterpret_cast<T*>(B.t.scratch);
     __syncthreads();
     // without an early stop the last reflector has no later step to store it
-    if (stop == mn && mn >= 1)
+    if (stop == mn && mn >= 1 && p2c[mn - 1] / SLAB == sl.slab)
         write_reflector<T>(A, rows, p2c[mn - 1], mn - 1, reinterpret_cast<T*>(B.t.tau), sh, s_ch,
-                           step_buf<T>(B, smem_raw, 0));
+                           step_buf<T>(B, smem_raw, sl.slab));
     const int fixed = rank_fixed(tol, mn);
     const int rank = fixed >= 0 ? fixed : stop;
     if (rank >= rows) return;
     T* Q = reinterpret_cast<T*>(B.t.Q);
-    for (long long idx = tid; idx < (long long)rows * rank; idx += NT) {
-        int r = (int)(idx % rows), k = (int)(idx / rows);
-        Q[idx] = A[slab_idx(rows, r, p2c[k])];
+    for (int k = 0; k < rank; k++) {
+        const int col = p2c[k];
+        if (col / SLAB != sl.slab) continue;
+        for (int r = tid; r < rows; r += NT) Q[r + (size_t)k * rows] = A[slab_idx(rows, r, col)];
     }
-    for (int j = tid; j < cols; j += NT) {
+    const int j0 = sl.slab * SLAB;
+    for (int idx = tid; idx < rank * SLAB; idx += NT) {
+        const int j = j0 + idx % SLAB, r = idx / SLAB;
+        if (j >= cols) continue;
         const int pos = c2p[j];
         const int k = seg_find(s_col0, B.t.nseg, j);
         const int jj = j - s_col0[k], w = s_col0[k + 1] - s_col0[k];
         T* E = reinterpret_cast<T*>(sg[k].E);
-        if (sg[k].out) {
-            for (int r = 0; r < rank; r++) E[jj + (size_t)r * w] = (r <= pos) ? A[slab_idx(rows, r, j)] : T(0);
-        } else {
-            for (int r = 0; r < rank; r++) E[r + (size_t)jj * rank] = (r <= pos) ? A[slab_idx(rows, r, j)] : T(0);
-        }
+        const T v = (r <= pos) ? A[slab_idx(rows, r, j)] : T(0);
+        if (sg[k].out) E[jj + (size_t)r * w] = v;
+        else E[r + (size_t)jj * rank] = v;
     }
-    __syncthreads();
-    if (tid == 0) dsize[B.t.cluster] = rank;
+    if (sl.slab == 0 && tid == 0) dsize[B.t.cluster] = rank;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -987,6 +1136,21 @@ hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib
 }
 
 template <typename T>
+hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int* d_info, hipStream_t st) {
+    if (ntasks == 0) return hipSuccess;
+    k_potrf_panel<T><<<ntasks, NT, 0, st>>>(d_tasks, kb, d_info);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_potrf_trail(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, hipStream_t st) {
+    const int nt = (nmax - kb - PB + PB - 1) / PB;
+    if (ntasks == 0 || nt <= 0) return hipSuccess;
+    k_potrf_trail<T><<<dim3(nt * (nt + 1) / 2, ntasks), NT, 0, st>>>(d_tasks, kb);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
                        hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
@@ -1012,10 +1176,10 @@ hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
 }
 
 template <typename T>
-hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, int nbig, const RrqrSeg* d_segs, int* d_size, double tol,
-                              size_t lds_bytes, hipStream_t st) {
-    if (nbig == 0) return hipSuccess;
-    k_rrqr_finish<T><<<nbig, NT, lds_bytes, st>>>(d_bigs, d_segs, d_size, tol);
+hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
+                              int* d_size, double tol, size_t lds_bytes, hipStream_t st) {
+    if (nslabs == 0) return hipSuccess;
+    k_rrqr_finish<T><<<nslabs, NT, lds_bytes, st>>>(d_bigs, d_slabs, d_segs, d_size, tol);
     return hipGetLastError();
 }
 
@@ -1030,6 +1194,8 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_fill<T>(T*, size_t, T, hipStream_t);                               \
     template hipError_t launch_scatter<T>(const long long*, const double*, long long, T*, hipStream_t); \
     template hipError_t launch_potrf<T>(const PotrfTask*, int, int*, hipStream_t);                \
+    template hipError_t launch_potrf_panel<T>(const PotrfTask*, int, int, int*, hipStream_t);     \
+    template hipError_t launch_potrf_trail<T>(const PotrfTask*, int, int, int, hipStream_t);      \
     template hipError_t launch_trsm<T>(const TrsmTask*, int, hipStream_t);                        \
     template hipError_t launch_schur<T>(const SchurTile*, int, const SchurContrib*, hipStream_t); \
     template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, hipStream_t); \
@@ -1038,8 +1204,8 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
                                             double, hipStream_t);                                                  \
     template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, double, size_t,            \
                                             hipStream_t);                                                          \
-    template hipError_t launch_rrqr_finish<T>(const RrqrBig*, int, const RrqrSeg*, int*, double, size_t,          \
-                                              hipStream_t);
+    template hipError_t launch_rrqr_finish<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, int*, double,  \
+                                              size_t, hipStream_t);
 SPAND_INST(float)
 SPAND_INST(double)
 

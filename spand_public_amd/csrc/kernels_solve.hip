@@ -11,6 +11,8 @@
 // The Krylov kernels back cg (src/is.cpp:39-121): CSR SpMV and deterministic two-pass dots.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "tasks.hpp"
 
@@ -20,7 +22,6 @@ constexpr int NT = 256;
 // Dynamic LDS of the solve kernels: a chunk of up to 8 right-hand sides of the segment (and
 // for the Householder apply two k-vectors per right-hand side); the chunk width is derived
 // from the segment size, and the host rejects segments that would not fit one column.
-constexpr int SOLVE_LDS_ELEMS_BYTES = SOLVE_LDS_BYTES;
 
 static inline int grid_for(size_t n) {
     size_t g = (n + NT - 1) / NT;
@@ -80,7 +81,7 @@ __device__ inline T wave_sum(T v) {
 // x := Linv x (trans 0: rows of Linv, threads over output rows, coalesced column reads)
 // x := Linv^T x (trans 1: one wavefront per output j, lanes over rows i >= j).
 template <typename T>
-__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, T* ws, int nrhs, int trans) {
+__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, T* ws, int nrhs, int trans, int lds_elems) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* S = reinterpret_cast<T*>(smem_raw);
     const TrsvTask t = tasks[blockIdx.x];
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, T* ws, int n
     const T* Li = reinterpret_cast<const T*>(t.L);
     T* x = ws + (size_t)t.xoff * nrhs;
     const int lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
-    const int step = min(RG_MAX, SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T) / n);
+    const int step = min(RG_MAX, lds_elems / n);
     for (int r0 = 0; r0 < nrhs; r0 += step) {
         const int rg = min(step, nrhs - r0);
         for (int idx = tid; idx < n * rg; idx += NT) S[idx] = x[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
@@ -98,7 +99,15 @@ __global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, T* ws, int n
             for (int i = tid; i < n; i += NT) {
                 T acc[RG_MAX];
                 for (int r = 0; r < rg; r++) acc[r] = T(0);
-                for (int j = 0; j <= i; j++) {
+                int j = 0;
+                for (; j + 3 <= i; j += 4) {
+                    const T l0 = Li[i + (size_t)j * n], l1 = Li[i + (size_t)(j + 1) * n];
+                    const T l2 = Li[i + (size_t)(j + 2) * n], l3 = Li[i + (size_t)(j + 3) * n];
+                    for (int r = 0; r < rg; r++)
+                        acc[r] += l0 * S[j * rg + r] + l1 * S[(j + 1) * rg + r] + l2 * S[(j + 2) * rg + r] +
+                                  l3 * S[(j + 3) * rg + r];
+                }
+                for (; j <= i; j++) {
                     const T l = Li[i + (size_t)j * n];
                     for (int r = 0; r < rg; r++) acc[r] += l * S[j * rg + r];
                 }
@@ -126,13 +135,14 @@ __global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, T* ws, int n
 // x[dst] -= sum_k op(A_k) x[src_k], terms in recorded order. The destination chunk lives in
 // LDS; op = A terms use one thread per output row, op = A^T terms one wavefront per output.
 template <typename T>
-__global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const GemvTerm* terms, T* ws, int nrhs) {
+__global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const GemvTerm* terms, T* ws, int nrhs,
+                                             int lds_elems) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* S = reinterpret_cast<T*>(smem_raw);
     const GemvGroup g = groups[blockIdx.x];
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
     T* xd = ws + (size_t)g.xdst * nrhs;
-    const int step = min(RG_MAX, SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T) / max(g.n, 1));
+    const int step = min(RG_MAX, lds_elems / max(g.n, 1));
     for (int r0 = 0; r0 < nrhs; r0 += step) {
         const int rg = min(step, nrhs - r0);
         for (int idx = tid; idx < g.n * rg; idx += NT) S[idx] = xd[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
@@ -146,7 +156,15 @@ __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const Gemv
                 for (int i = tid; i < tm.rows; i += NT) {
                     T acc[RG_MAX];
                     for (int r = 0; r < rg; r++) acc[r] = T(0);
-                    for (int j = 0; j < tm.cols; j++) {
+                    int j = 0;
+                    for (; j + 4 <= tm.cols; j += 4) {
+                        const T a0 = A[i + (size_t)j * tm.rows], a1 = A[i + (size_t)(j + 1) * tm.rows];
+                        const T a2 = A[i + (size_t)(j + 2) * tm.rows], a3 = A[i + (size_t)(j + 3) * tm.rows];
+                        for (int r = 0; r < rg; r++)
+                            acc[r] += a0 * xs[(size_t)j * nrhs + r] + a1 * xs[(size_t)(j + 1) * nrhs + r] +
+                                      a2 * xs[(size_t)(j + 2) * nrhs + r] + a3 * xs[(size_t)(j + 3) * nrhs + r];
+                    }
+                    for (; j < tm.cols; j++) {
                         const T a = A[i + (size_t)j * tm.rows];
                         for (int r = 0; r < rg; r++) acc[r] += a * xs[(size_t)j * nrhs + r];
                     }
@@ -175,95 +193,136 @@ __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const Gemv
     }
 }
 
-// Householder block in compact-WY form: Q^T x = x - V T^T V^T x (trans 1),
-// Q x = x - V T V^T x (trans 0). V has a unit diagonal and zeros above it implicitly.
+// Householder reflectors in blocked compact-WY form (the dormqr scheme): Q = Q_0 Q_1 ...
+// with Q_b = I - V_b T_b V_b^T over reflectors [32b, 32b + 32). Q^T x applies Q_0^T first,
+// Q x applies the last block first. V has a unit diagonal and zeros above it implicitly.
 template <typename T>
-__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nrhs, int trans) {
+__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nrhs, int trans, int lds_elems) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const QTask t = tasks[blockIdx.x];
     const int m = t.m, k = t.k, tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
     if (m == 0 || k == 0) return;
-    const int step = min(RG_MAX, SOLVE_LDS_ELEMS_BYTES / (int)sizeof(T) / (m + 2 * k));
+    const int step = min(RG_MAX, lds_elems / (m + 2 * QB));
     T* S = reinterpret_cast<T*>(smem_raw);  // x chunk, m * step
-    T* W = S + (size_t)m * step;            // V^T x, k * step
-    T* U = W + (size_t)k * step;            // T^(T) W, k * step
+    T* W = S + (size_t)m * step;            // V_b^T x, QB * step
+    T* U = W + (size_t)QB * step;           // T_b^(T) W, QB * step
     const T* V = reinterpret_cast<const T*>(t.V);
-    const T* Tm = reinterpret_cast<const T*>(t.Tm);
     T* x = ws + (size_t)t.xoff * nrhs;
+    const int nblk = (k + QB - 1) / QB;
     for (int r0 = 0; r0 < nrhs; r0 += step) {
         const int rg = min(step, nrhs - r0);
         for (int idx = tid; idx < m * rg; idx += NT) S[idx] = x[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
         __syncthreads();
-        // W = V^T x: wavefront per reflector c, w_c = x_c + sum_{i > c} V(i,c) x_i
-        for (int c = wv; c < k; c += nw) {
-            T acc[RG_MAX];
-            for (int r = 0; r < rg; r++) acc[r] = T(0);
-            const T* col = V + (size_t)c * m;
-            for (int i = c + 1 + lane; i < m; i += WAVE) {
-                const T a = col[i];
-                for (int r = 0; r < rg; r++) acc[r] += a * S[i * rg + r];
+        for (int bi = 0; bi < nblk; bi++) {
+            const int b = trans ? bi : nblk - 1 - bi;
+            const int b0 = b * QB, nb = min(QB, k - b0);
+            const T* Tb = reinterpret_cast<const T*>(t.Tm) + (size_t)b * QB * QB;
+            // W = V_b^T x: wavefront per reflector c, w_c = x_{b0+c} + sum_{i > b0+c} V(i, b0+c) x_i
+            for (int c = wv; c < nb; c += nw) {
+                const int cc = b0 + c;
+                T acc[RG_MAX];
+                for (int r = 0; r < rg; r++) acc[r] = T(0);
+                const T* col = V + (size_t)cc * m;
+                for (int i = cc + 1 + lane; i < m; i += WAVE) {
+                    const T a = col[i];
+                    for (int r = 0; r < rg; r++) acc[r] += a * S[i * rg + r];
+                }
+                for (int r = 0; r < rg; r++) {
+                    T v = wave_sum(acc[r]);
+                    if (lane == 0) W[c * rg + r] = S[cc * rg + r] + v;
+                }
             }
-            for (int r = 0; r < rg; r++) {
-                T v = wave_sum(acc[r]);
-                if (lane == 0) W[c * rg + r] = S[c * rg + r] + v;
+            __syncthreads();
+            // U = T_b^T W (Q^T) or T_b W (Q); T_b is nb x nb upper triangular, ld QB
+            for (int idx = tid; idx < nb * rg; idx += NT) {
+                const int a = idx / rg, r = idx % rg;
+                T acc = T(0);
+                if (trans) {
+                    for (int q = 0; q <= a; q++) acc += Tb[q + (size_t)a * QB] * W[q * rg + r];
+                } else {
+                    for (int q = a; q < nb; q++) acc += Tb[a + (size_t)q * QB] * W[q * rg + r];
+                }
+                U[a * rg + r] = acc;
             }
-        }
-        __syncthreads();
-        // U = T^T W (Q^T) or T W (Q); T is upper triangular, ld k
-        for (int idx = tid; idx < k * rg; idx += NT) {
-            const int a = idx / rg, r = idx % rg;
-            T acc = T(0);
-            if (trans) {
-                for (int b = 0; b <= a; b++) acc += Tm[b + (size_t)a * k] * W[b * rg + r];
-            } else {
-                for (int b = a; b < k; b++) acc += Tm[a + (size_t)b * k] * W[b * rg + r];
+            __syncthreads();
+            // x -= V_b U for rows i >= b0: U_{i-b0} on the unit diagonal + sum_{c < i-b0} V(i, b0+c) U_c
+            for (int idx = tid; idx < (m - b0) * rg; idx += NT) {
+                const int i = b0 + idx / rg, r = idx % rg;
+                const int li = i - b0;
+                T acc = li < nb ? U[li * rg + r] : T(0);
+                const int cmax = min(li, nb);
+                for (int c = 0; c < cmax; c++) acc += V[i + (size_t)(b0 + c) * m] * U[c * rg + r];
+                S[i * rg + r] -= acc;
             }
-            U[a * rg + r] = acc;
+            __syncthreads();
         }
-        __syncthreads();
-        // x -= V U: row i gets U_i (unit diagonal, i < k) + sum_{c < min(i, k)} V(i,c) U_c
-        for (int idx = tid; idx < m * rg; idx += NT) {
-            const int i = idx / rg, r = idx % rg;
-            T acc = i < k ? U[i * rg + r] : T(0);
-            const int cmax = min(i, k);
-            for (int c = 0; c < cmax; c++) acc += V[i + (size_t)c * m] * U[c * rg + r];
-            x[(size_t)i * nrhs + r0 + r] = S[idx] - acc;
-        }
+        for (int idx = tid; idx < m * rg; idx += NT) x[(size_t)(idx / rg) * nrhs + r0 + idx % rg] = S[idx];
         __syncthreads();
     }
 }
 
-// dlarft (forward, columnwise): T(i,i) = tau_i; T(0:i, i) = -tau_i T(0:i,0:i) V(:,0:i)^T v_i.
+// dlarft (forward, columnwise) for one block of at most QB reflectors [b0, b0 + nb):
+// G(a, c) = v_a^T v_c is accumulated over the block's rows in one pass, then
+// T(c, c) = tau_c, T(0:c, c) = T(0:c, 0:c) (-tau_c G(c, 0:c)^T).
 template <typename T>
 __global__ void __launch_bounds__(NT) k_larft(const LarftTask* tasks) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* tcol = reinterpret_cast<T*>(smem_raw);  // k
+    __shared__ T vs[WAVE][QB + 1];
+    __shared__ T G[QB][QB + 1];
+    __shared__ T Tl[QB][QB + 1];
+    __shared__ T tcol[QB];
     const LarftTask t = tasks[blockIdx.x];
-    const int m = t.m, k = t.k, tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
+    const int m = t.m, nb = t.k, b0 = t.b0, tid = threadIdx.x;
     const T* V = reinterpret_cast<const T*>(t.V);
-    const T* tau = reinterpret_cast<const T*>(t.tau);
+    const T* tau = reinterpret_cast<const T*>(t.tau) + b0;
+    // pairs (a > c) of the block, two per thread at most (QB * (QB - 1) / 2 <= 2 * NT)
+    int pa[2], pc[2];
+    T acc[2] = {T(0), T(0)};
+    for (int q = 0; q < 2; q++) {
+        int p = tid + q * NT, a = 1;
+        while (p >= a) {
+            p -= a;
+            a++;
+        }
+        pa[q] = a;
+        pc[q] = p;
+    }
+    for (int r0 = b0; r0 < m; r0 += WAVE) {
+        for (int idx = tid; idx < WAVE * QB; idx += NT) {
+            const int rr = idx % WAVE, c = idx / WAVE, r = r0 + rr;
+            T v = T(0);
+            if (c < nb && r < m) v = (r == b0 + c) ? T(1) : (r > b0 + c ? V[r + (size_t)(b0 + c) * m] : T(0));
+            vs[rr][c] = v;
+        }
+        __syncthreads();
+        for (int q = 0; q < 2; q++) {
+            if (pa[q] < nb) {
+                T s = acc[q];
+                for (int rr = 0; rr < WAVE; rr++) s += vs[rr][pa[q]] * vs[rr][pc[q]];
+                acc[q] = s;
+            }
+        }
+        __syncthreads();
+    }
+    for (int q = 0; q < 2; q++)
+        if (pa[q] < nb) G[pa[q]][pc[q]] = acc[q];
+    __syncthreads();
+    for (int c = 0; c < nb; c++) {
+        const T tc = tau[c];
+        if (tid < c) tcol[tid] = -tc * G[c][tid];
+        __syncthreads();
+        if (tid < c) {
+            T s = T(0);
+            for (int q = tid; q < c; q++) s += Tl[tid][q] * tcol[q];
+            Tl[tid][c] = s;
+        }
+        if (tid == 0) Tl[c][c] = tc;
+        if (tid > c && tid < QB) Tl[tid][c] = T(0);
+        __syncthreads();
+    }
     T* Tm = reinterpret_cast<T*>(t.Tm);
-    for (int i = 0; i < k; i++) {
-        const T ti = tau[i];
-        // tcol[j] = -tau_i * (V(i, j) + sum_{r > i} V(r, j) V(r, i)), j < i
-        for (int j = wv; j < i; j += nw) {
-            T acc = T(0);
-            const T* vj = V + (size_t)j * m;
-            const T* vi = V + (size_t)i * m;
-            for (int r = i + 1 + lane; r < m; r += WAVE) acc += vj[r] * vi[r];
-            acc = wave_sum(acc);
-            if (lane == 0) tcol[j] = -ti * (vj[i] + acc);
-        }
-        __syncthreads();
-        // T(0:i, i) = T(0:i, 0:i) tcol
-        for (int a = tid; a < i; a += NT) {
-            T acc = T(0);
-            for (int b = a; b < i; b++) acc += Tm[a + (size_t)b * k] * tcol[b];
-            Tm[a + (size_t)i * k] = acc;
-        }
-        if (tid == 0) Tm[i + (size_t)i * k] = ti;
-        for (int a = i + 1 + tid; a < k; a += NT) Tm[a + (size_t)i * k] = T(0);
-        __syncthreads();
+    for (int idx = tid; idx < QB * QB; idx += NT) {
+        const int a = idx % QB, c = idx / QB;
+        Tm[idx] = (a < nb && c < nb) ? Tl[a][c] : T(0);
     }
 }
 
@@ -378,29 +437,41 @@ hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld,
 }
 // LDS for one RG chunk of the largest segment in a launch is sized by the caller through
 // max_n; the launchers take the largest dimension from the task arrays' host copies.
+// Dynamic LDS of a solve launch: one chunk of min(8, nrhs) right-hand sides of the
+// launch's largest segment (plus the two QB-vectors of the Householder apply), at most
+// SOLVE_LDS_BYTES; small segments get small allocations, so many workgroups share a CU.
 template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st) {
+static size_t solve_lds(int seg_max, int nrhs) {
+    const size_t per_rhs = (size_t)std::max(seg_max, 1) * sizeof(T);
+    const size_t rg = (size_t)std::min(RG_MAX, std::max(nrhs, 1));
+    return std::min((size_t)SOLVE_LDS_BYTES, std::max(per_rhs, per_rhs * rg));
+}
+template <typename T>
+hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    k_trmv<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks, ws, nrhs, trans);
+    const size_t lds = solve_lds<T>(seg_max, nrhs);
+    k_trmv<T><<<ntasks, NT, lds, st>>>(d_tasks, ws, nrhs, trans, (int)(lds / sizeof(T)));
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, const GemvTerm* d_terms, T* ws, int nrhs,
+hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, int seg_max, const GemvTerm* d_terms, T* ws, int nrhs,
                        hipStream_t st) {
     if (ngroups == 0) return hipSuccess;
-    k_gemv<T><<<ngroups, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_groups, d_terms, ws, nrhs);
+    const size_t lds = solve_lds<T>(seg_max, nrhs);
+    k_gemv<T><<<ngroups, NT, lds, st>>>(d_groups, d_terms, ws, nrhs, (int)(lds / sizeof(T)));
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st) {
+hipError_t launch_qapply(const QTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    k_qapply<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks, ws, nrhs, trans);
+    const size_t lds = solve_lds<T>(seg_max + 2 * QB, nrhs);
+    k_qapply<T><<<ntasks, NT, lds, st>>>(d_tasks, ws, nrhs, trans, (int)(lds / sizeof(T)));
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    k_larft<T><<<ntasks, NT, SOLVE_LDS_ELEMS_BYTES, st>>>(d_tasks);
+    k_larft<T><<<ntasks, NT, 0, st>>>(d_tasks);
     return hipGetLastError();
 }
 template <typename T>
@@ -440,10 +511,10 @@ hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t
 #define SPAND_SOLVE_INST(T)                                                                                  \
     template hipError_t launch_perm_in<T>(const double*, size_t, size_t, int, const int*, int, T*, hipStream_t); \
     template hipError_t launch_perm_out<T>(const T*, const int*, int, int, double*, size_t, size_t, hipStream_t); \
-    template hipError_t launch_trmv<T>(const TrsvTask*, int, T*, int, int, hipStream_t);                    \
+    template hipError_t launch_trmv<T>(const TrsvTask*, int, int, T*, int, int, hipStream_t);               \
     template hipError_t launch_larft<T>(const LarftTask*, int, hipStream_t);                                \
-    template hipError_t launch_gemv<T>(const GemvGroup*, int, const GemvTerm*, T*, int, hipStream_t);       \
-    template hipError_t launch_qapply<T>(const QTask*, int, T*, int, int, hipStream_t);                     \
+    template hipError_t launch_gemv<T>(const GemvGroup*, int, int, const GemvTerm*, T*, int, hipStream_t);  \
+    template hipError_t launch_qapply<T>(const QTask*, int, int, T*, int, int, hipStream_t);                \
     template hipError_t launch_merge<T>(const MergeTask*, int, const int2*, T*, int, int, hipStream_t);
 SPAND_SOLVE_INST(float)
 SPAND_SOLVE_INST(double)

@@ -5,7 +5,7 @@
 namespace spand {
 
 static const char* const kFamilyNames[KF_COUNT] = {
-    "fill", "scatter", "copy", "potrf", "trsm", "schur", "rrqr", "rrqr_init", "rrqr_step",
+    "fill", "scatter", "copy", "potrf", "potrf_panel", "potrf_trail", "trsm", "schur", "rrqr", "rrqr_init", "rrqr_step",
     "rrqr_finish", "larft", "perm", "trmv", "gemv", "qapply", "merge", "spmv", "krylov_vec",
 };
 
@@ -31,18 +31,19 @@ hipEvent_t KernelProfiler::get_event() {
 }
 
 int KernelProfiler::start(int fam, hipStream_t st) {
-    if (!on_) return -1;
-    Pending p{fam, get_event(), get_event(), 0.0, 0.0};
+    if (!on(fam)) return -1;
+    Pending p{fam, get_event(), get_event(), 0.0, 0.0, 1};
     hip_check(hipEventRecord(p.a, st), "hipEventRecord");
     pending_.push_back(p);
     return (int)pending_.size() - 1;
 }
 
-void KernelProfiler::stop(int slot, hipStream_t st, double bytes, double flops) {
+void KernelProfiler::stop(int slot, hipStream_t st, double bytes, double flops, int launches) {
     if (slot < 0) return;
     Pending& p = pending_[slot];
     p.bytes = bytes;
     p.flops = flops;
+    p.launches = launches;
     hip_check(hipEventRecord(p.b, st), "hipEventRecord");
 }
 
@@ -58,7 +59,7 @@ void KernelProfiler::collect() {
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
         KernelTotals& t = acc_[p.fam];
-        t.launches++;
+        t.launches += p.launches;
         t.ms += ms;
         t.bytes += p.bytes;
         t.flops += p.flops;

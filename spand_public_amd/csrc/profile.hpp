@@ -14,6 +14,8 @@ enum KernelFamily {
     KF_SCATTER,
     KF_COPY,
     KF_POTRF,
+    KF_POTRF_PANEL,
+    KF_POTRF_TRAIL,
     KF_TRSM,
     KF_SCHUR,
     KF_RRQR,
@@ -41,11 +43,15 @@ struct KernelTotals {
 class KernelProfiler {
   public:
     ~KernelProfiler();
-    void enable(bool on) { on_ = on; }
-    bool on() const { return on_; }
-    // start() before a launch, stop() after it; both return/take a slot (-1 when off).
+    // bit f of mask enables family f (0: off)
+    void set_mask(unsigned long long mask) { mask_ = mask; }
+    bool on() const { return mask_ != 0; }
+    bool on(int fam) const { return (mask_ >> fam) & 1ull; }
+    // start() before a launch, stop() after it; both return/take a slot (-1 when the family
+    // is off). A slot may bracket `launches` consecutive launches of its family (a batch);
+    // its time then covers the batch, gaps between the launches included.
     int start(int fam, hipStream_t st);
-    void stop(int slot, hipStream_t st, double bytes, double flops);
+    void stop(int slot, hipStream_t st, double bytes, double flops, int launches = 1);
     // Work of a launch known only after it ran (sizes decided on the device).
     void set_work(int slot, double bytes, double flops);
     void collect();  // waits for pending events, accumulates, recycles the events
@@ -57,12 +63,13 @@ class KernelProfiler {
         int fam;
         hipEvent_t a, b;
         double bytes, flops;
+        int launches;
     };
     hipEvent_t get_event();
     std::vector<hipEvent_t> free_;
     std::vector<Pending> pending_;
     KernelTotals acc_[KF_COUNT];
-    bool on_ = false;
+    unsigned long long mask_ = 0;
 };
 
 // Brackets one launcher call: `hipError_t` of the launcher is returned unchanged.

@@ -119,17 +119,20 @@ struct GemvTerm {
 struct QTask {  // Orthogonal (src/operations.cpp:149-158): x := Q^T x (fwd) / Q x (bwd)
     const void* V;    // m x k Householder vectors (unit diagonal implicit), ld m
     const void* tau;
-    const void* Tm;   // k x k upper-triangular compact-WY factor, Q = I - V T V^T (dlarft)
+    const void* Tm;   // ceil(k / QB) blocks of QB x QB upper-triangular compact-WY factors:
+                      // Q = prod_b (I - V_b T_b V_b^T), block b = reflectors [QB b, QB b + QB)
     int m, k;
     int xoff;
     int pad;
 };
-// dlarft (forward, columnwise) for one Orthogonal op: T from V and tau.
+// dlarft (forward, columnwise) for one block of reflectors [b0, b0 + k) of an Orthogonal op.
 struct LarftTask {
-    const void* V;
-    const void* tau;
-    void* Tm;
+    const void* V;    // the op's m x K reflectors (ld m)
+    const void* tau;  // the op's K taus
+    void* Tm;         // this block's QB x QB factor
     int m, k;
+    int b0;
+    int pad;
 };
 struct MergeTask {  // Merge (src/operations.cpp:166-192)
     int pxoff;

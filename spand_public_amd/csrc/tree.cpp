@@ -419,7 +419,7 @@ void Tree::factorize_impl() {
     const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 16384);
 
     // the solve kernels stage one right-hand side of a segment in LDS at least
-    const int max_segment = SOLVE_LDS_BYTES / (int)sizeof(T) / 3;
+    const int max_segment = SOLVE_LDS_BYTES / (int)sizeof(T) - 2 * QB;
     for (int c = 0; c < (int)cls_.size(); c++)
         if (cls_[c].osize > max_segment)
             throw Error(ERR_UNSUPPORTED, "cluster of " + std::to_string(cls_[c].osize) +
@@ -726,6 +726,11 @@ void Tree::factorize_impl() {
         lst.nwaves = nwaves;
         // ---------------- upload + launch Elim / Scale / Sparsify ----------------
         for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
+        auto small_first = [](std::vector<PotrfTask>& v) {
+            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= POTRF_SMALL_MAX; });
+        };
+        small_first(potrfE);
+        small_first(potrfS);
         Packer pk;
         size_t o_pe = pk.add(potrfE), o_te = pk.add(trsmE), o_ti = pk.add(tiles), o_co = pk.add(contribs);
         size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
@@ -750,12 +755,43 @@ void Tree::factorize_impl() {
                             [&] { return launch_copy<T>(Packer::at<CopyTask>(dt, o_ii), (int)inv_init.size(), st); }),
                       "identity blocks");
         }
+        // pivots up to POTRF_SMALL_MAX in one launch (one workgroup each); larger ones (sorted
+        // last) one 32-column panel per launch pair
         auto run_potrf = [&](const std::vector<PotrfTask>& v, size_t off, const char* what) {
             if (v.empty()) return;
-            Work w = work_potrf(v, esz);
-            hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
-                            [&] { return launch_potrf<T>(Packer::at<PotrfTask>(dt, off), (int)v.size(), d_info_, st); }),
-                      what);
+            int nsmall = 0, nmax = 0;
+            for (auto& t : v) {
+                if (t.n <= POTRF_SMALL_MAX) nsmall++;
+                nmax = std::max(nmax, t.n);
+            }
+            const PotrfTask* d = Packer::at<PotrfTask>(dt, off);
+            if (nsmall > 0) {
+                std::vector<PotrfTask> vs(v.begin(), v.begin() + nsmall);
+                Work w = work_potrf(vs, esz);
+                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
+                                [&] { return launch_potrf<T>(d, nsmall, d_info_, st); }),
+                          what);
+            }
+            const int nbig = (int)v.size() - nsmall;
+            if (nbig == 0) return;
+            const PotrfTask* db = d + nsmall;
+            for (int kb = 0; kb < nmax; kb += 32) {
+                double pb = 0, pf = 0, tb = 0, tf = 0;
+                for (int k = nsmall; k < (int)v.size(); k++) {
+                    const double n = v[k].n, nb = std::min(32.0, n - kb), rest = n - kb - nb;
+                    if (nb <= 0) continue;
+                    pb += (nb * (nb + 1) + 2.0 * rest * nb) * esz;
+                    pf += nb * nb * nb / 3.0 + rest * nb * nb;
+                    tb += (rest * (rest + 1) + 2.0 * rest * nb) * esz;
+                    tf += rest * (rest + 1) * nb;
+                }
+                hip_check(timed(prof_, KF_POTRF_PANEL, st, pb, pf,
+                                [&] { return launch_potrf_panel<T>(db, nbig, kb, d_info_, st); }),
+                          what);
+                hip_check(timed(prof_, KF_POTRF_TRAIL, st, tb, tf,
+                                [&] { return launch_potrf_trail<T>(db, nbig, kb, nmax, st); }),
+                          what);
+            }
         };
         auto run_trsm = [&](const std::vector<TrsmTask>& v, size_t off, const char* what) {
             if (v.empty()) return;
@@ -780,7 +816,7 @@ void Tree::factorize_impl() {
         run_trsm(trsmB, o_tb, "trsm scale B");
         // stepped-path launch slots, for the post-hoc work attribution below
         std::vector<int> slot_init(nwaves, -1), slot_finish(nwaves, -1);
-        std::vector<std::vector<int>> slot_step(nwaves);
+        std::vector<int> slot_step(nwaves, -1);
         for (int w = 0; w < nwaves; w++) {
             if (!wtasks[w].empty()) {
                 Work wk;
@@ -803,15 +839,15 @@ void Tree::factorize_impl() {
                             [&] { return launch_rrqr_init<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st); },
                             &slot_init[w]),
                       "rrqr init");
-            slot_step[w].assign(wsteps[w], -1);
+            // the steps of a wave are timed as one batch (two events per wave)
+            slot_step[w] = prof_.start(KF_RRQR_STEP, st);
             for (int i = 0; i < wsteps[w]; i++)
-                hip_check(timed(prof_, KF_RRQR_STEP, st, 0, 0,
-                                [&] { return launch_rrqr_step<T>(db, ds, nsl, i, tol_, wlds_step[w], st); }, &slot_step[w][i]),
-                          "rrqr step");
+                hip_check(launch_rrqr_step<T>(db, ds, nsl, i, tol_, wlds_step[w], st), "rrqr step");
+            prof_.stop(slot_step[w], st, 0, 0, wsteps[w]);
             hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,
                             [&] {
-                                return launch_rrqr_finish<T>(db, (int)wbig[w].size(), Packer::at<RrqrSeg>(dt, o_sg),
-                                                             d_size_, tol_, wlds_fin[w], st);
+                                return launch_rrqr_finish<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
+                                                             wlds_fin[w], st);
                             },
                             &slot_finish[w]),
                       "rrqr finish");
@@ -841,16 +877,16 @@ void Tree::factorize_impl() {
                 }
                 prof_.set_work(slot_init[w], ib, 0);
                 prof_.set_work(slot_finish[w], fb, 0);
-                for (int i = 0; i < (int)slot_step[w].size(); i++) {
-                    double by = 0, fl = 0;
+                double by = 0, fl = 0;
+                for (int i = 0; i < wsteps[w]; i++) {
                     for (size_t k = 0; k < rows.size(); k++) {
                         if (i >= mns[k]) continue;
                         const double m = rows[k] - i, n = cols[k] - i;
                         by += (2.0 * m * n + m) * esz + 16.0 * n;
                         fl += 4.0 * m * n;
                     }
-                    prof_.set_work(slot_step[w][i], by, fl);
                 }
+                prof_.set_work(slot_step[w], by, fl);
             }
         }
         const double t_lvl1 = wall();
@@ -871,8 +907,10 @@ void Tree::factorize_impl() {
                 ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
                 factor_bytes_ += ((size_t)s0 * r + (size_t)r * (r + 1) / 2) * sizeof(T);
                 if (r > 0) {
-                    void* Tm = arena_.alloc((size_t)r * r * sizeof(T));
-                    larft.push_back({t->Q, t->tau, Tm, s0, r});
+                    const int nblk = (r + QB - 1) / QB;
+                    T* Tm = reinterpret_cast<T*>(arena_.alloc((size_t)nblk * QB * QB * sizeof(T)));
+                    for (int b = 0; b < nblk; b++)
+                        larft.push_back({t->Q, t->tau, Tm + (size_t)b * QB * QB, s0, std::min(QB, r - b * QB), b * QB, 0});
                     qt.push_back({t->Q, t->tau, Tm, s0, r, (int)cls_[c].xoff, 0});
                 }
                 HCluster sib;
@@ -985,8 +1023,8 @@ void Tree::factorize_impl() {
             void* dl = upload_tasks(pl.buf);
             double lb = 0, lf = 0;
             for (auto& l : larft) {
-                lb += ((double)l.m * l.k + (double)l.k * l.k) * sizeof(T);
-                lf += (double)l.m * l.k * l.k;
+                lb += ((double)(l.m - l.b0) * l.k + (double)QB * QB) * sizeof(T);
+                lf += (double)(l.m - l.b0) * l.k * l.k;
             }
             hip_check(timed(prof_, KF_LARFT, st, lb, lf,
                             [&] { return launch_larft<T>(Packer::at<LarftTask>(dl, o_l), (int)larft.size(), st); }),
@@ -1042,6 +1080,11 @@ void Tree::factorize_impl() {
                 }
                 return w;
             };
+            for (auto& t : sv_elim) lsv.nmax_elim = std::max(lsv.nmax_elim, t.n);
+            for (auto& t : sv_scale) lsv.nmax_scale = std::max(lsv.nmax_scale, t.n);
+            for (auto& g : fwd_groups) lsv.nmax_fwd = std::max(lsv.nmax_fwd, g.n);
+            for (auto& g : bwd_groups) lsv.nmax_bwd = std::max(lsv.nmax_bwd, g.n);
+            for (auto& q : qt) lsv.nmax_q = std::max(lsv.nmax_q, q.m);
             lsv.w_elim = trmv_work(sv_elim);
             lsv.w_scale = trmv_work(sv_scale);
             lsv.w_fwd = gemv_work(fwd_groups);
@@ -1240,13 +1283,16 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
     for (int l = 0; l < nlevels_; l++) {
         const LevelSolve& p = solve_plan_[l];
         if (p.n_elim_trsv)
-            run(KF_TRMV, p.w_elim, "trmv", [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 0, st); });
+            run(KF_TRMV, p.w_elim, "trmv",
+                [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, p.nmax_elim, ws, nrhs, 0, st); });
         if (p.n_fwd_groups)
             run(KF_GEMV, p.w_fwd, "gemv",
-                [&] { return launch_gemv<T>(p.fwd_groups, p.n_fwd_groups, p.terms, ws, nrhs, st); });
+                [&] { return launch_gemv<T>(p.fwd_groups, p.n_fwd_groups, p.nmax_fwd, p.terms, ws, nrhs, st); });
         if (p.n_scale_trsv)
-            run(KF_TRMV, p.w_scale, "trmv", [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 0, st); });
-        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 1, st); });
+            run(KF_TRMV, p.w_scale, "trmv",
+                [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, p.nmax_scale, ws, nrhs, 0, st); });
+        if (p.n_q)
+            run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, p.nmax_q, ws, nrhs, 1, st); });
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 1, st); });
@@ -1256,14 +1302,17 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 0, st); });
-        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 0, st); });
+        if (p.n_q)
+            run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, p.nmax_q, ws, nrhs, 0, st); });
         if (p.n_scale_trsv)
-            run(KF_TRMV, p.w_scale, "trmv", [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, ws, nrhs, 1, st); });
+            run(KF_TRMV, p.w_scale, "trmv",
+                [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, p.nmax_scale, ws, nrhs, 1, st); });
         if (p.n_bwd_groups)
             run(KF_GEMV, p.w_bwd, "gemv",
-                [&] { return launch_gemv<T>(p.bwd_groups, p.n_bwd_groups, p.terms, ws, nrhs, st); });
+                [&] { return launch_gemv<T>(p.bwd_groups, p.n_bwd_groups, p.nmax_bwd, p.terms, ws, nrhs, st); });
         if (p.n_elim_trsv)
-            run(KF_TRMV, p.w_elim, "trmv", [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, ws, nrhs, 1, st); });
+            run(KF_TRMV, p.w_elim, "trmv",
+                [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, p.nmax_elim, ws, nrhs, 1, st); });
     }
     run(KF_PERM, wp, "perm out", [&] { return launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st); });
 }
