@@ -21,6 +21,8 @@ template <typename T> hipError_t launch_potrf(const PotrfTask* d_tasks, int ntas
 // Pivots larger than POTRF_SMALL_MAX: one panel of 32 columns per call pair, kb = 0, 32, ...
 // (launch_potrf_trail's grid is sized by nmax, the largest pivot of the batch).
 constexpr int POTRF_SMALL_MAX = 128;
+// Pivots of at most 64 rows: one wavefront per pivot.
+template <typename T> hipError_t launch_potrf64(const PotrfTask* d_tasks, int ntasks, int* d_info, hipStream_t st);
 template <typename T>
 hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int* d_info, hipStream_t st);
 template <typename T>
@@ -39,6 +41,15 @@ constexpr int STEP_LDS_MAX = 40960;
 template <typename T>
 hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
                             const int* d_size, double tol, hipStream_t st);
+// All steps of a wave in one cooperative launch (clusters of at most WAVE_ROWS_MAX rows,
+// RrqrBig::vbuf holding [2][nslab][max_rows] candidate columns); lds_bytes = max_cols
+// ints + 2 max_rows elements. Returns hipErrorCooperativeLaunchTooLarge, launching
+// nothing, when the wave does not fit the device at once.
+constexpr int WAVE_ROWS_MAX = 512;
+constexpr int WAVE_LDS_MAX = 49152;
+template <typename T>
+hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
+                            double tol, size_t lds_bytes, hipStream_t st);
 template <typename T>
 hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol,
                             size_t lds_bytes, hipStream_t st);
@@ -57,22 +68,22 @@ hipError_t launch_perm_out(const T* ws, const int* perm, int N, int nrhs, double
                            hipStream_t st);
 hipError_t launch_cm2il(const double* src, int ld, int N, int nrhs, double* dst, hipStream_t st);
 hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld, hipStream_t st);
-// The solve kernels stage one right-hand side of a segment in LDS at least: a segment of m
-// rows needs m * sizeof(T) bytes (m + 2 QB for the Householder apply). seg_max is the
-// largest segment of the launch; it sizes the dynamic LDS.
-constexpr int SOLVE_LDS_BYTES = 65536;
+// Solve sweeps run in chunks of SOLVE_CHUNK outputs of one task (SolveChunk lists built by
+// the planner). trmv writes y = Linv x (or Linv^T x) to the scratch wt and copies it back.
+constexpr int SOLVE_CHUNK = 64;
 // reflectors per compact-WY block
 constexpr int QB = 32;
 // x := Linv x (trans 0) or Linv^T x (trans 1) with the explicit inverse factor.
 template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st);
+hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, T* ws, T* wt, int nrhs,
+                       int trans, hipStream_t st);
 // compact-WY factor of one block of reflectors per task
 template <typename T> hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st);
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, int seg_max, const GemvTerm* d_terms, T* ws, int nrhs,
-                       hipStream_t st);
+hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms, T* ws,
+                       int nrhs, hipStream_t st);
 template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st);
+hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
 template <typename T>
 hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, T* ws, int nrhs, int fwd,
                         hipStream_t st);

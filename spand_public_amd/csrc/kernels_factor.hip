@@ -12,6 +12,7 @@
 //             assemble_Asn and the scatter of shrink_split_scatter_phi
 //             (src/tree.cpp:1189-1224, 1292-1347, 980-1104)
 //   copy   -> merge block copies                src/tree.cpp:1133-1184
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -46,6 +47,23 @@ __global__ void __launch_bounds__(NT) k_scatter(const long long* off, const doub
 // results are bit-identical to it. The diagonal block lives in LDS; panel and trailing
 // updates read the block from global memory (L2 resident).
 constexpr int PB = 32;
+
+// Rows i >= kb + nb of panel kb, one row per thread: column k is scaled by 1/L(k,k), then
+// the later panel columns receive fma(-a_k, L(k2,k), a_k2), in k order: the element
+// recurrence of the unblocked right-looking factorization.
+template <typename T>
+__device__ inline void panel_rows(T* A, int n, int kb, int nb, const T (*D)[PB + 1], const T* rdiag) {
+#pragma clang fp contract(off)
+    for (int i = kb + nb + threadIdx.x; i < n; i += blockDim.x) {
+        for (int k = 0; k < nb; k++) {
+            const T a = A[i + (size_t)(kb + k) * n] * rdiag[k];
+            A[i + (size_t)(kb + k) * n] = a;
+            for (int k2 = k + 1; k2 < nb; k2++)
+                A[i + (size_t)(kb + k2) * n] = fma(-a, D[k2][k], A[i + (size_t)(kb + k2) * n]);
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info) {
 #pragma clang fp contract(off)
@@ -94,14 +112,7 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
             if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = D[i][k];
         }
         // panel rows i >= kb + nb: scale / update across the block's columns in order
-        for (int i = kb + nb + tid; i < n; i += NT) {
-            for (int k = 0; k < nb; k++) {
-                T a = A[i + (size_t)(kb + k) * n] * rdiag[k];
-                A[i + (size_t)(kb + k) * n] = a;
-                for (int k2 = k + 1; k2 < nb; k2++)
-                    A[i + (size_t)(kb + k2) * n] = fma(-a, D[k2][k], A[i + (size_t)(kb + k2) * n]);
-            }
-        }
+        panel_rows<T>(A, n, kb, nb, D, rdiag);
         __syncthreads();
         // trailing lower update by the block's columns, k in order
         const int mt = n - kb - nb;
@@ -115,6 +126,50 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
         __syncthreads();
     }
     if (bad && tid == 0) atomicCAS(info, 0, t.cluster + 1);
+}
+
+// Pivots of at most 64 rows: one wavefront each, the whole block in LDS, lane a owning
+// row a. Same element recurrence as k_potrf (column j scaled by 1/L(j,j), then
+// A(a,b) = fma(-L(a,j), L(b,j), A(a,b)) for a >= b > j, in j order).
+template <typename T>
+__global__ void __launch_bounds__(64) k_potrf64(const PotrfTask* tasks, int* info) {
+#pragma clang fp contract(off)
+    __shared__ T L[64][65];
+    __shared__ T rd;
+    __shared__ int bad;
+    const PotrfTask t = tasks[blockIdx.x];
+    const int n = t.n, a = threadIdx.x;
+    if (n == 0) return;
+    T* A = reinterpret_cast<T*>(t.A);
+    if (a == 0) bad = 0;
+    for (int b = 0; b < n; b++) L[a][b] = a < n ? A[a + (size_t)b * n] : T(0);
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        if (a == 0) {
+            const T d = L[j][j];
+            if (!(d > T(0))) {
+                bad = j + 1;
+            } else {
+                L[j][j] = sqrt(d);
+            }
+            rd = T(1) / L[j][j];
+        }
+        __syncthreads();
+        if (bad) break;
+        if (a > j && a < n) L[a][j] *= rd;
+        __syncthreads();
+        if (a > j && a < n) {
+            const T laj = L[a][j];
+            for (int b = j + 1; b <= a; b++) L[a][b] = fma(-laj, L[b][j], L[a][b]);
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (a == 0) atomicCAS(info, 0, t.cluster + 1);
+        return;
+    }
+    if (a < n)
+        for (int b = 0; b <= a; b++) A[a + (size_t)b * n] = L[a][b];
 }
 
 // Large pivots take the same blocked algorithm one panel per launch pair, so the trailing
@@ -170,14 +225,7 @@ __global__ void __launch_bounds__(NT) k_potrf_panel(const PotrfTask* tasks, int 
         int i = idx % nb, k = idx / nb;
         if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = D[i][k];
     }
-    for (int i = kb + nb + tid; i < n; i += NT) {
-        for (int k = 0; k < nb; k++) {
-            T a = A[i + (size_t)(kb + k) * n] * rdiag[k];
-            A[i + (size_t)(kb + k) * n] = a;
-            for (int k2 = k + 1; k2 < nb; k2++)
-                A[i + (size_t)(kb + k2) * n] = fma(-a, D[k2][k], A[i + (size_t)(kb + k2) * n]);
-        }
-    }
+    panel_rows<T>(A, n, kb, nb, D, rdiag);
 }
 
 // grid (tiles, tasks): tile index -> (bi, bj), bj <= bi, over the trailing lower triangle
@@ -462,6 +510,8 @@ __device__ inline double sumsq16_block(int n, G g, double* s_ch) {
 template <typename T>
 __device__ void larfg_block(double alpha, const T* x, int n1, double* sh, double* s_ch) {
 #pragma clang fp contract(off)
+    const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+    const double rs = 1.0 / safmin;
     const double xs0 = sumsq16_block(n1, [&](int t) { return (double)x[t]; }, s_ch);
     if (threadIdx.x == 0) {
         double tau = 0.0, scal = 1.0, beta = alpha;
@@ -469,8 +519,6 @@ __device__ void larfg_block(double alpha, const T* x, int n1, double* sh, double
         if (n1 >= 1) {
             const double xnorm = sqrt(xs0);
             if (xnorm != 0.0) {
-                const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
-                const double rs = 1.0 / safmin;
                 beta = -copysign(lapy2_d(alpha, xnorm), alpha);
                 if (fabs(beta) < safmin) {
                     do {
@@ -478,31 +526,78 @@ __device__ void larfg_block(double alpha, const T* x, int n1, double* sh, double
                         beta *= rs;
                         alpha *= rs;
                     } while (fabs(beta) < safmin && knt < 20);
-                    const double xs = sumsq16_serial(n1, [&](int t) {
-                        T xv = x[t];
-                        for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
-                        return (double)xv;
-                    });
-                    beta = -copysign(lapy2_d(alpha, sqrt(xs)), alpha);
+                } else {
+                    tau = (beta - alpha) / beta;
+                    scal = 1.0 / (alpha - beta);
                 }
-                tau = (beta - alpha) / beta;
-                scal = 1.0 / (alpha - beta);
-                for (int k = 0; k < knt; k++) beta *= safmin;
             }
         }
         sh[0] = tau;
         sh[1] = beta;
         sh[2] = scal;
         sh[3] = (double)knt;
+        s_ch[NCH] = alpha;  // rescaled alpha, for the rare path below
     }
     __syncthreads();
+    const int knt = (int)sh[3];
+    if (knt > 0) {  // |beta| below the safe minimum: norm of the rescaled vector (reference dlarfg)
+        const double xs = sumsq16_block(
+            n1,
+            [&](int t) {
+                T xv = x[t];
+                for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
+                return (double)xv;
+            },
+            s_ch);
+        if (threadIdx.x == 0) {
+            const double alpha_s = s_ch[NCH];
+            double beta = -copysign(lapy2_d(alpha_s, sqrt(xs)), alpha_s);
+            sh[0] = (beta - alpha_s) / beta;
+            sh[2] = 1.0 / (alpha_s - beta);
+            for (int k = 0; k < knt; k++) beta *= safmin;
+            sh[1] = beta;
+        }
+        __syncthreads();
+    }
 }
+
+// halving tree over 16 chain values in LDS (stride between chains)
+__device__ inline double tree16_lds(const double* c, int stride) {
+#pragma clang fp contract(off)
+    const double x0 = ((c[0] + c[8 * stride]) + (c[4 * stride] + c[12 * stride])) +
+                      ((c[2 * stride] + c[10 * stride]) + (c[6 * stride] + c[14 * stride]));
+    const double x1 = ((c[1 * stride] + c[9 * stride]) + (c[5 * stride] + c[13 * stride])) +
+                      ((c[3 * stride] + c[11 * stride]) + (c[7 * stride] + c[15 * stride]));
+    return x0 + x1;
+}
+__device__ inline float tree16_lds_f(const double* c, int stride) {
+#pragma clang fp contract(off)
+    auto v = [&](int w) { return (float)c[w * stride]; };
+    const float x0 = ((v(0) + v(8)) + (v(4) + v(12))) + ((v(2) + v(10)) + (v(6) + v(14)));
+    const float x1 = ((v(1) + v(9)) + (v(5) + v(13))) + ((v(3) + v(11)) + (v(7) + v(15)));
+    return x0 + x1;
+}
+template <typename T> __device__ inline T tree16_lds_t(const double* c, int stride);
+template <> __device__ inline double tree16_lds_t<double>(const double* c, int stride) { return tree16_lds(c, stride); }
+template <> __device__ inline float tree16_lds_t<float>(const double* c, int stride) { return tree16_lds_f(c, stride); }
 
 template <typename T>
 __device__ inline T larfg_scale(T xv, int knt, double scal) {
     const double rs = 1.0 / (DBL_MIN / (DBL_EPSILON * 0.5));
     for (int k = 0; k < knt; k++) xv = (T)((double)xv * rs);
     return xv * (T)scal;
+}
+
+// (value desc, position asc) maximum over the 64 lanes of a wavefront; every lane gets it
+__device__ inline void wave_best(double& v, int& p) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const int p2 = __shfl_xor(p, off);
+        if (v2 > v || (v2 == v && p2 < p)) {
+            v = v2;
+            p = p2;
+        }
+    }
 }
 
 // choose_rank (src/util.cpp:434-452) decided during the factorization: -1 while undecided.
@@ -519,11 +614,11 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_col0[RRQR_MAXSEG + 1];
-    __shared__ double r_val[NT];
-    __shared__ int r_idx[NT];
+    __shared__ double r_val[NT / 64];
+    __shared__ int r_idx[NT / 64];
     __shared__ double s_vn1[RRQR_LDS_COLS], s_vn2[RRQR_LDS_COLS];
     __shared__ int s_jp[RRQR_LDS_COLS];
-    __shared__ double sh[4], s_ch[NCH];
+    __shared__ double sh[4], s_ch[NCH + 1];
 
     const RrqrTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x;
@@ -582,20 +677,20 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
             double v = fabs(vn1[j]);
             if (v > bv) { bv = v; bi = j; }
         }
-        r_val[tid] = bv;
-        r_idx[tid] = bi;
-        __syncthreads();
-        for (int off = NT / 2; off > 0; off >>= 1) {
-            if (tid < off) {
-                double v2 = r_val[tid + off];
-                int i2 = r_idx[tid + off];
-                if (v2 > r_val[tid] || (v2 == r_val[tid] && i2 < r_idx[tid])) {
-                    r_val[tid] = v2;
-                    r_idx[tid] = i2;
-                }
-            }
-            __syncthreads();
+        wave_best(bv, bi);
+        if (tid % 64 == 0) {
+            r_val[tid / 64] = bv;
+            r_idx[tid / 64] = bi;
         }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < NT / 64; w++)
+                if (r_val[w] > r_val[0] || (r_val[w] == r_val[0] && r_idx[w] < r_idx[0])) {
+                    r_val[0] = r_val[w];
+                    r_idx[0] = r_idx[w];
+                }
+        }
+        __syncthreads();
         const int pvt = r_idx[0];
         if (pvt != i) {
             for (int r = tid; r < rows; r += NT) {
@@ -739,18 +834,6 @@ __device__ inline int seg_find(const int* col0, int nseg, int j) {
     return lo;
 }
 
-// (value desc, position asc) maximum over the 64 lanes of a wavefront; every lane gets it
-__device__ inline void wave_best(double& v, int& p) {
-    for (int off = SLAB / 2; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(v, off);
-        const int p2 = __shfl_xor(p, off);
-        if (v2 > v || (v2 == v && p2 < p)) {
-            v = v2;
-            p = p2;
-        }
-    }
-}
-
 template <typename T>
 __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
                                                        const int* dsize, double tol) {
@@ -778,6 +861,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         B.dims[1] = cols;
         B.dims[2] = mn;
         B.dims[3] = rank_fixed(tol, mn) == 0 ? 0 : mn;  // steps to run (lowered at the rank)
+        B.dims[4] = 0;                                   // 1 once every kept reflector is stored
     }
     T* A = reinterpret_cast<T*>(B.t.scratch);
     const int mc = B.t.max_cols;
@@ -802,10 +886,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         double best = -1.0;
         int bpos = INT32_MAX;
         if (active) {
-            double cc[NCH];
-#pragma unroll
-            for (int w = 0; w < NCH; w++) cc[w] = s_part[w][lane];
-            const double nrm = sqrt(tree16(cc));
+            const double nrm = sqrt(tree16_lds(&s_part[0][lane], SLAB));
             B.t.norms[j] = nrm;
             B.t.norms[mc + j] = nrm;
             B.map[j] = j;
@@ -826,7 +907,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double s_part[NCH][SLAB];
-    __shared__ double sh[4], s_ch[NCH];
+    __shared__ double sh[4], s_ch[NCH + 1];
     __shared__ T s_tm[SLAB];
     __shared__ int s_upd[SLAB];
     __shared__ int s_p, s_ca, s_cb, s_stop;
@@ -856,10 +937,10 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
         if (cprev / SLAB == sl.slab) write_reflector<T>(A, rows, cprev, i - 1, reinterpret_cast<T*>(B.t.tau), sh, s_ch, s_col);
     }
     if (i == stop) return;
-    if (tid == 0) {
+    if (wv == 0) {  // pivot: first maximum over the slabs' candidates
         double bv = -1.0;
         int bp = INT32_MAX;
-        for (int k = 0; k < ns; k++) {
+        for (int k = lane; k < ns; k += SLAB) {
             const double v = B.cand_v[cur * ns + k];
             const int pp = B.cand_p[cur * ns + k];
             if (v > bv || (v == bv && pp < bp)) {
@@ -867,9 +948,12 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
                 bp = pp;
             }
         }
-        s_p = bp;
-        s_ca = (bp >= i && bp < cols) ? p2c_cur[bp] : 0;
-        s_cb = p2c_cur[i];
+        wave_best(bv, bp);
+        if (lane == 0) {
+            s_p = bp;
+            s_ca = (bp >= i && bp < cols) ? p2c_cur[bp] : 0;
+            s_cb = p2c_cur[i];
+        }
     }
     __syncthreads();
     const int p = s_p, ca = s_ca, cb = s_cb;
@@ -930,10 +1014,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
         s_part[wv][lane] = (double)c;
         __syncthreads();
         if (wv == 0) {
-            T cc[NCH];
-#pragma unroll
-            for (int w = 0; w < NCH; w++) cc[w] = (T)s_part[w][lane];
-            const T w = tree16(cc);
+            const T w = tree16_lds_t<T>(&s_part[0][lane], SLAB);
             s_tm[lane] = -tau * w;
             s_upd[lane] = w != T(0);  // dger skips columns with w_j = 0
         }
@@ -998,10 +1079,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
                 const double temp2 = temp * rr * rr;
                 if (temp2 <= tol3z) {
                     if (i < rows - 1) {
-                        double cc[NCH];
-#pragma unroll
-                        for (int w = 0; w < NCH; w++) cc[w] = s_part[w][lane];
-                        vn1[j] = sqrt(tree16(cc));
+                        vn1[j] = sqrt(tree16_lds(&s_part[0][lane], SLAB));
                         vn2[j] = vn1[j];
                     } else {
                         vn1[j] = 0.0;
@@ -1022,6 +1100,242 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     }
 }
 
+// Persistent form of the stepped RRQR: one cooperative launch runs every pivot step of a
+// wave, with a grid-wide barrier between steps. Each thread keeps its column's rows
+// r = wv + 16 q (q < WAVE_RW) in registers for the whole factorization; lane l of
+// wavefront wv is column l of the slab, and chain c of every sum at step i is held by
+// wavefront (c + i) % 16, so sums run in the same 16-chain order as k_rrqr_step. Every
+// slab publishes its best column (candidate for the next pivot) in a double-buffered
+// global buffer; the pivot's dlarfg is recomputed from that copy in every workgroup, and
+// its owner stores the reflector in its registers. The registers and the final maps go
+// back to the scratch at the end, for k_rrqr_finish.
+// WAVE_RW rows per thread: clusters of up to 16 * WAVE_RW rows (16: two workgroups per CU).
+template <typename T, int WAVE_RW>
+__global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
+    k_rrqr_wave(const RrqrBig* bigs, const RrqrSlab* slabs, int nsteps, double tol) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double s_part[NCH][SLAB];
+    __shared__ double sh[4], s_ch[NCH + 1];
+    __shared__ T s_tm[SLAB], s_rowi[SLAB];
+    __shared__ int s_upd[SLAB], s_pos[SLAB];
+    __shared__ int s_p, s_ca, s_cb, s_best;
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const RrqrSlab sl = slabs[blockIdx.x];
+    const RrqrBig B = bigs[sl.big];
+    const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
+    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
+    int stop = B.dims[3];
+    const int ns = B.nslab, mc = B.t.max_cols, mr = B.t.max_rows;
+    int* p2c = reinterpret_cast<int*>(smem_raw);  // position -> column, this workgroup's copy
+    T* s_col = reinterpret_cast<T*>(smem_raw + ((size_t)mc * sizeof(int) + 15) / 16 * 16);  // pivot column
+    T* s_v = s_col + mr;                                                                     // its reflector
+    T* A = reinterpret_cast<T*>(B.t.scratch);
+    T* ccol = reinterpret_cast<T*>(B.vbuf);  // [2][nslab][max_rows] candidate columns
+    T* tau_out = reinterpret_cast<T*>(B.t.tau);
+    const int j = sl.slab * SLAB + lane;
+    const bool jin = j < cols && rows > 0;
+    T* aslab = A + (size_t)sl.slab * rows * SLAB + lane;
+    T a[WAVE_RW];
+#pragma unroll
+    for (int q = 0; q < WAVE_RW; q++) {
+        const int r = wv + NCH * q;
+        a[q] = (jin && r < rows) ? aslab[(size_t)r * SLAB] : T(0);
+    }
+    for (int k = tid; k < cols; k += STEP_NT) p2c[k] = k;
+    if (tid < SLAB) s_pos[tid] = j;
+    double vn1 = 0.0, vn2 = 0.0;
+    if (wv == 0 && jin) {
+        vn1 = B.t.norms[j];
+        vn2 = B.t.norms[mc + j];
+    }
+    const double tol3z = sqrt(DBL_EPSILON * 0.5);
+    const int fixed = rank_fixed(tol, mn);
+    double sref = 0.0;
+    __syncthreads();
+    // candidate column of step 0 (k_rrqr_init left the slab's best in cand_*[0])
+    if (rows > 0) {
+        const int bp = B.cand_p[sl.slab];
+        if (bp != INT32_MAX && lane == bp % SLAB) {
+            T* dst = ccol + (size_t)sl.slab * mr;
+#pragma unroll
+            for (int q = 0; q < WAVE_RW; q++) {
+                const int r = wv + NCH * q;
+                if (r < rows) dst[r] = a[q];
+            }
+        }
+    }
+    for (int i = 0; i < nsteps; i++) {
+        grid.sync();
+        if (rows == 0 || i >= mn || i >= stop) continue;
+        const int cur = i & 1, nxt = cur ^ 1;
+        if (wv == 0) {  // pivot: first maximum over the slabs' candidates
+            double bv = -1.0;
+            int bp = INT32_MAX;
+            for (int k = lane; k < ns; k += SLAB) {
+                const double v = B.cand_v[cur * ns + k];
+                const int pp = B.cand_p[cur * ns + k];
+                if (v > bv || (v == bv && pp < bp)) {
+                    bv = v;
+                    bp = pp;
+                }
+            }
+            wave_best(bv, bp);
+            if (lane == 0) {
+                s_p = bp;
+                s_ca = (bp >= i && bp < cols) ? p2c[bp] : 0;
+                s_cb = p2c[i];
+            }
+        }
+        __syncthreads();
+        const int p = s_p, ca = s_ca, cb = s_cb;
+        if (p < i || p >= cols) {  // cannot happen for i < mn: some column is still free
+            stop = i;
+            continue;
+        }
+        const T* pc = ccol + ((size_t)cur * ns + ca / SLAB) * mr;
+        for (int r = i + tid; r < rows; r += STEP_NT) s_col[r] = pc[r];
+        __syncthreads();
+        larfg_block<T>((double)s_col[i], s_col + i + 1, rows - i - 1, sh, s_ch);
+        // R_ii as stored (tau = 0 leaves alpha on the diagonal); the QR stops at the rank
+        const double rii = fabs((double)(T)sh[1]);
+        if (i == 0) {
+            sref = rii;
+        } else if (fixed < 0 && !(rii / sref >= tol)) {
+            stop = i;
+            continue;
+        }
+        const double taud = sh[0];
+        const T tau = (T)taud;
+        if (taud != 0.0) {
+            const int knt = (int)sh[3];
+            for (int r = i + 1 + tid; r < rows; r += STEP_NT) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
+        }
+        const bool own_pivot = ca / SLAB == sl.slab;
+        if (tid == 0) {
+            s_v[i] = T(1);
+            p2c[i] = ca;
+            p2c[p] = cb;
+            if (own_pivot) {
+                s_pos[ca % SLAB] = i;
+                tau_out[i] = tau;
+            }
+            if (cb / SLAB == sl.slab && cb != ca) s_pos[cb % SLAB] = p;
+        }
+        __syncthreads();
+        if (own_pivot && lane == ca % SLAB && taud != 0.0) {  // the reflector: beta on the diagonal, v below
+#pragma unroll
+            for (int q = 0; q < WAVE_RW; q++) {
+                const int r = wv + NCH * q;
+                if (r == i) a[q] = (T)sh[1];
+                else if (r > i && r < rows) a[q] = s_v[r];
+            }
+        }
+        const int pos = s_pos[lane];
+        const bool active = jin && pos > i;
+        const int ci = i % NCH;
+        if (taud != 0.0) {  // w_j = v^T a_j, this thread's chain
+            T c = T(0);
+            if (active) {
+#pragma unroll
+                for (int q = 0; q < WAVE_RW; q++) {
+                    const int r = wv + NCH * q;
+                    if (r >= i && r < rows) c = fma(a[q], s_v[r], c);
+                }
+            }
+            s_part[(wv - ci + NCH) % NCH][lane] = (double)c;
+            __syncthreads();
+            if (wv == 0) {
+                const T w = tree16_lds_t<T>(&s_part[0][lane], SLAB);
+                s_tm[lane] = -tau * w;
+                s_upd[lane] = w != T(0);  // dger skips columns with w_j = 0
+            }
+            __syncthreads();
+        }
+        // a_j += tm v, and the chains of the sum of squares over rows > i
+        const bool upd = active && taud != 0.0 && s_upd[lane];
+        const T tm = upd ? s_tm[lane] : T(0);
+        double c2 = 0.0;
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < WAVE_RW; q++) {
+                const int r = wv + NCH * q;
+                if (r >= i && r < rows) {
+                    if (upd) a[q] = (r == i) ? a[q] + tm : fma(s_v[r], tm, a[q]);
+                    if (r == i) {
+                        s_rowi[lane] = a[q];
+                    } else {
+                        const double d = (double)a[q];
+                        c2 = fma(d, d, c2);
+                    }
+                }
+            }
+        }
+        s_part[(wv - ci - 1 + 2 * NCH) % NCH][lane] = c2;
+        __syncthreads();
+        if (wv == 0) {
+            double best = -1.0;
+            int bpos = INT32_MAX;
+            if (active) {
+                if (vn1 != 0.0) {
+                    const double qq = fabs((double)s_rowi[lane]) / vn1;
+                    double temp = 1.0 - qq * qq;
+                    temp = fmax(temp, 0.0);
+                    const double rr = vn1 / vn2;
+                    const double temp2 = temp * rr * rr;
+                    if (temp2 <= tol3z) {
+                        if (i < rows - 1) {
+                            vn1 = sqrt(tree16_lds(&s_part[0][lane], SLAB));
+                            vn2 = vn1;
+                        } else {
+                            vn1 = 0.0;
+                            vn2 = 0.0;
+                        }
+                    } else {
+                        vn1 = vn1 * sqrt(temp);
+                    }
+                }
+                best = vn1;
+                bpos = pos;
+            }
+            wave_best(best, bpos);
+            if (lane == 0) {
+                B.cand_v[nxt * ns + sl.slab] = best;
+                B.cand_p[nxt * ns + sl.slab] = bpos;
+                s_best = bpos != INT32_MAX ? p2c[bpos] : -1;
+            }
+        }
+        __syncthreads();
+        if (s_best >= 0 && lane == s_best % SLAB) {  // the slab's candidate column for step i + 1
+            T* dst = ccol + ((size_t)nxt * ns + sl.slab) * mr;
+#pragma unroll
+            for (int q = 0; q < WAVE_RW; q++) {
+                const int r = wv + NCH * q;
+                if (r < rows) dst[r] = a[q];
+            }
+        }
+    }
+    // registers back to the slab storage; final maps after `stop` steps, as k_rrqr_finish reads them
+    if (jin) {
+#pragma unroll
+        for (int q = 0; q < WAVE_RW; q++) {
+            const int r = wv + NCH * q;
+            if (r < rows) aslab[(size_t)r * SLAB] = a[q];
+        }
+    }
+    if (sl.slab == 0 && rows > 0) {
+        const int fin = stop & 1;
+        for (int k = tid; k < cols; k += STEP_NT) {
+            B.map[fin * mc + k] = p2c[k];
+            B.map[2 * mc + fin * mc + p2c[k]] = k;
+        }
+        if (tid == 0) {
+            B.dims[3] = stop;
+            B.dims[4] = 1;
+        }
+    }
+}
+
 // One workgroup per slab: the owner of the last pivot column stores its reflector (when
 // the QR ran to min(rows, cols)), every slab copies the reflectors of its own columns into
 // Q and scatters R[:rank, j] of its columns into the shrunk edge blocks.
@@ -1031,7 +1345,7 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_col0[RRQR_MAXSEG + 1];
-    __shared__ double sh[4], s_ch[NCH];
+    __shared__ double sh[4], s_ch[NCH + 1];
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
     const RrqrSeg* sg = segs + B.t.seg0;
@@ -1053,7 +1367,7 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
     T* A = reinterpret_cast<T*>(B.t.scratch);
     __syncthreads();
     // without an early stop the last reflector has no later step to store it
-    if (stop == mn && mn >= 1 && p2c[mn - 1] / SLAB == sl.slab)
+    if (B.dims[4] == 0 && stop == mn && mn >= 1 && p2c[mn - 1] / SLAB == sl.slab)
         write_reflector<T>(A, rows, p2c[mn - 1], mn - 1, reinterpret_cast<T*>(B.t.tau), sh, s_ch,
                            step_buf<T>(B, smem_raw, sl.slab));
     const int fixed = rank_fixed(tol, mn);
@@ -1136,6 +1450,13 @@ hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib
 }
 
 template <typename T>
+hipError_t launch_potrf64(const PotrfTask* d_tasks, int ntasks, int* d_info, hipStream_t st) {
+    if (ntasks == 0) return hipSuccess;
+    k_potrf64<T><<<ntasks, 64, 0, st>>>(d_tasks, d_info);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int* d_info, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
     k_potrf_panel<T><<<ntasks, NT, 0, st>>>(d_tasks, kb, d_info);
@@ -1175,6 +1496,33 @@ hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
     return hipGetLastError();
 }
 
+// Persistent wave launch: returns hipErrorCooperativeLaunchTooLarge (and launches nothing)
+// when the wave's workgroups cannot all be resident at once.
+template <typename T, int RW>
+static hipError_t launch_wave_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol,
+                                 size_t lds_bytes, hipStream_t st) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW>);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, STEP_NT, lds_bytes);
+    if (e != hipSuccess) return e;
+    if ((long long)per_cu * ncu < nslabs) return hipErrorCooperativeLaunchTooLarge;
+    void* args[] = {(void*)&d_bigs, (void*)&d_slabs, (void*)&nsteps, (void*)&tol};
+    return hipLaunchCooperativeKernel(f, dim3(nslabs), dim3(STEP_NT), args, (unsigned)lds_bytes, st);
+}
+
+// Persistent wave launch: returns hipErrorCooperativeLaunchTooLarge (and launches nothing)
+// when the wave's workgroups cannot all be resident at once.
+template <typename T>
+hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
+                            double tol, size_t lds_bytes, hipStream_t st) {
+    if (nslabs == 0 || nsteps == 0) return hipSuccess;
+    if (max_rows <= 16 * 16) return launch_wave_rw<T, 16>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, st);
+    if (max_rows <= 16 * 32) return launch_wave_rw<T, 32>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, st);
+    return hipErrorCooperativeLaunchTooLarge;
+}
+
 template <typename T>
 hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
                               int* d_size, double tol, size_t lds_bytes, hipStream_t st) {
@@ -1194,6 +1542,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_fill<T>(T*, size_t, T, hipStream_t);                               \
     template hipError_t launch_scatter<T>(const long long*, const double*, long long, T*, hipStream_t); \
     template hipError_t launch_potrf<T>(const PotrfTask*, int, int*, hipStream_t);                \
+    template hipError_t launch_potrf64<T>(const PotrfTask*, int, int*, hipStream_t);              \
     template hipError_t launch_potrf_panel<T>(const PotrfTask*, int, int, int*, hipStream_t);     \
     template hipError_t launch_potrf_trail<T>(const PotrfTask*, int, int, int, hipStream_t);      \
     template hipError_t launch_trsm<T>(const TrsmTask*, int, hipStream_t);                        \
@@ -1202,6 +1551,8 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);                        \
     template hipError_t launch_rrqr_init<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, const int*,      \
                                             double, hipStream_t);                                                  \
+    template hipError_t launch_rrqr_wave<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t,       \
+                                            hipStream_t);                                                          \
     template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, double, size_t,            \
                                             hipStream_t);                                                          \
     template hipError_t launch_rrqr_finish<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, int*, double,  \

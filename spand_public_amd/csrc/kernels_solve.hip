@@ -78,117 +78,134 @@ __device__ inline T wave_sum(T v) {
     return v;
 }
 
-// x := Linv x (trans 0: rows of Linv, threads over output rows, coalesced column reads)
-// x := Linv^T x (trans 1: one wavefront per output j, lanes over rows i >= j).
+// Solve sweeps are split into chunks of SC = 64 outputs of one task (SolveChunk), so a large
+// cluster spreads over many workgroups. A workgroup is 64 lanes (one output each) x 4
+// partitions of the inner sum; the partitions are combined in LDS at the end.
+constexpr int SC = SOLVE_CHUNK;
+constexpr int NP = NT / SC;  // 4
+constexpr int TR = 32;       // rows of a transposed tile
+
+// y = Linv x (trans 0) or Linv^T x (trans 1) for chunk [o0, o0 + 64) of a ScalingLLT
+// segment, written to wt (the copy-back kernel returns it to ws: other chunks of the same
+// task still read the old x). Linv is zero above its diagonal, so the forward sum runs
+// over all j < min(n, o0 + 64) without a mask.
 template <typename T>
-__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, T* ws, int nrhs, int trans, int lds_elems) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* S = reinterpret_cast<T*>(smem_raw);
-    const TrsvTask t = tasks[blockIdx.x];
-    const int n = t.n, tid = threadIdx.x;
-    if (n == 0) return;
-    const T* Li = reinterpret_cast<const T*>(t.L);
-    T* x = ws + (size_t)t.xoff * nrhs;
-    const int lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
-    const int step = min(RG_MAX, lds_elems / n);
-    for (int r0 = 0; r0 < nrhs; r0 += step) {
-        const int rg = min(step, nrhs - r0);
-        for (int idx = tid; idx < n * rg; idx += NT) S[idx] = x[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
-        __syncthreads();
+__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, const SolveChunk* chunks, const T* ws, T* wt,
+                                             int nrhs, int trans) {
+    __shared__ T tile[TR][SC + 1];
+    __shared__ T red[NP][SC][RG_MAX];
+    const SolveChunk ch = chunks[blockIdx.x];
+    const TrsvTask t = tasks[ch.task];
+    const int n = t.n, o0 = ch.o0, no = min(SC, n - o0);
+    const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
+    const T* L = reinterpret_cast<const T*>(t.L);
+    const T* x = ws + (size_t)t.xoff * nrhs;
+    for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
+        const int rg = min(RG_MAX, nrhs - r0);
+        T acc[RG_MAX];
+#pragma unroll
+        for (int r = 0; r < RG_MAX; r++) acc[r] = T(0);
         if (!trans) {
-            for (int i = tid; i < n; i += NT) {
-                T acc[RG_MAX];
-                for (int r = 0; r < rg; r++) acc[r] = T(0);
-                int j = 0;
-                for (; j + 3 <= i; j += 4) {
-                    const T l0 = Li[i + (size_t)j * n], l1 = Li[i + (size_t)(j + 1) * n];
-                    const T l2 = Li[i + (size_t)(j + 2) * n], l3 = Li[i + (size_t)(j + 3) * n];
-                    for (int r = 0; r < rg; r++)
-                        acc[r] += l0 * S[j * rg + r] + l1 * S[(j + 1) * rg + r] + l2 * S[(j + 2) * rg + r] +
-                                  l3 * S[(j + 3) * rg + r];
+            const int i = o0 + lane;
+            const int jmax = min(n, o0 + SC);
+            if (lane < no) {
+                for (int j = kp; j < jmax; j += NP) {
+                    const T l = L[i + (size_t)j * n];
+                    const T* xj = x + (size_t)j * nrhs + r0;
+                    for (int r = 0; r < rg; r++) acc[r] += l * xj[r];
                 }
-                for (; j <= i; j++) {
-                    const T l = Li[i + (size_t)j * n];
-                    for (int r = 0; r < rg; r++) acc[r] += l * S[j * rg + r];
-                }
-                for (int r = 0; r < rg; r++) x[(size_t)i * nrhs + r0 + r] = acc[r];
             }
         } else {
-            for (int j = wv; j < n; j += nw) {
-                T acc[RG_MAX];
-                for (int r = 0; r < rg; r++) acc[r] = T(0);
-                const T* col = Li + (size_t)j * n;
-                for (int i = j + lane; i < n; i += WAVE) {
-                    const T l = col[i];
-                    for (int r = 0; r < rg; r++) acc[r] += l * S[i * rg + r];
+            for (int i0 = o0; i0 < n; i0 += TR) {
+                const int ni = min(TR, n - i0);
+                for (int idx = tid; idx < TR * SC; idx += NT) {
+                    const int ii = idx % TR, jj = idx / TR;
+                    tile[ii][jj] = (ii < ni && jj < no) ? L[(i0 + ii) + (size_t)(o0 + jj) * n] : T(0);
                 }
-                for (int r = 0; r < rg; r++) {
-                    T v = wave_sum(acc[r]);
-                    if (lane == 0) x[(size_t)j * nrhs + r0 + r] = v;
+                __syncthreads();
+                for (int ii = kp; ii < ni; ii += NP) {
+                    const T l = tile[ii][lane];
+                    const T* xi = x + (size_t)(i0 + ii) * nrhs + r0;
+                    for (int r = 0; r < rg; r++) acc[r] += l * xi[r];
                 }
+                __syncthreads();
             }
+        }
+        for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
+        __syncthreads();
+        if (kp == 0 && lane < no) {
+            T* y = wt + (size_t)(t.xoff + o0 + lane) * nrhs + r0;
+            for (int r = 0; r < rg; r++) y[r] = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
         }
         __syncthreads();
     }
 }
 
-// x[dst] -= sum_k op(A_k) x[src_k], terms in recorded order. The destination chunk lives in
-// LDS; op = A terms use one thread per output row, op = A^T terms one wavefront per output.
+// ws[seg] = wt[seg] for the segments of a trmv launch, one workgroup per chunk
 template <typename T>
-__global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const GemvTerm* terms, T* ws, int nrhs,
-                                             int lds_elems) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* S = reinterpret_cast<T*>(smem_raw);
-    const GemvGroup g = groups[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
-    T* xd = ws + (size_t)g.xdst * nrhs;
-    const int step = min(RG_MAX, lds_elems / max(g.n, 1));
-    for (int r0 = 0; r0 < nrhs; r0 += step) {
-        const int rg = min(step, nrhs - r0);
-        for (int idx = tid; idx < g.n * rg; idx += NT) S[idx] = xd[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
-        __syncthreads();
+__global__ void __launch_bounds__(NT) k_copyback(const TrsvTask* tasks, const SolveChunk* chunks, T* ws, const T* wt,
+                                                 int nrhs) {
+    const SolveChunk ch = chunks[blockIdx.x];
+    const TrsvTask t = tasks[ch.task];
+    const int no = min(SC, t.n - ch.o0);
+    const size_t base = (size_t)(t.xoff + ch.o0) * nrhs;
+    for (int idx = threadIdx.x; idx < no * nrhs; idx += NT) ws[base + idx] = wt[base + idx];
+}
+
+// x[dst] -= sum_k op(A_k) x[src_k] for chunk [o0, o0 + 64) of the destination segment.
+// Destinations and sources of one launch are disjoint segments, so the update is in place.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const SolveChunk* chunks, const GemvTerm* terms,
+                                             T* ws, int nrhs) {
+    __shared__ T tile[TR][SC + 1];
+    __shared__ T red[NP][SC][RG_MAX];
+    const SolveChunk ch = chunks[blockIdx.x];
+    const GemvGroup g = groups[ch.task];
+    const int o0 = ch.o0, no = min(SC, g.n - o0);
+    const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
+    for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
+        const int rg = min(RG_MAX, nrhs - r0);
+        T acc[RG_MAX];
+#pragma unroll
+        for (int r = 0; r < RG_MAX; r++) acc[r] = T(0);
         for (int k = g.k0; k < g.k1; k++) {
             const GemvTerm tm = terms[k];
             const T* A = reinterpret_cast<const T*>(tm.A);
             const T* xs = ws + (size_t)tm.xsrc * nrhs + r0;
             if (!tm.trans) {
-                // output i (rows of A), sum over j < cols
-                for (int i = tid; i < tm.rows; i += NT) {
-                    T acc[RG_MAX];
-                    for (int r = 0; r < rg; r++) acc[r] = T(0);
-                    int j = 0;
-                    for (; j + 4 <= tm.cols; j += 4) {
-                        const T a0 = A[i + (size_t)j * tm.rows], a1 = A[i + (size_t)(j + 1) * tm.rows];
-                        const T a2 = A[i + (size_t)(j + 2) * tm.rows], a3 = A[i + (size_t)(j + 3) * tm.rows];
-                        for (int r = 0; r < rg; r++)
-                            acc[r] += a0 * xs[(size_t)j * nrhs + r] + a1 * xs[(size_t)(j + 1) * nrhs + r] +
-                                      a2 * xs[(size_t)(j + 2) * nrhs + r] + a3 * xs[(size_t)(j + 3) * nrhs + r];
-                    }
-                    for (; j < tm.cols; j++) {
+                // output i = row of A (rows = destination length), sum over columns j
+                if (lane < no) {
+                    const int i = o0 + lane;
+                    for (int j = kp; j < tm.cols; j += NP) {
                         const T a = A[i + (size_t)j * tm.rows];
-                        for (int r = 0; r < rg; r++) acc[r] += a * xs[(size_t)j * nrhs + r];
+                        const T* xj = xs + (size_t)j * nrhs;
+                        for (int r = 0; r < rg; r++) acc[r] += a * xj[r];
                     }
-                    for (int r = 0; r < rg; r++) S[i * rg + r] -= acc[r];
                 }
             } else {
-                // output j (columns of A), sum over rows i
-                for (int j = wv; j < tm.cols; j += nw) {
-                    T acc[RG_MAX];
-                    for (int r = 0; r < rg; r++) acc[r] = T(0);
-                    const T* col = A + (size_t)j * tm.rows;
-                    for (int i = lane; i < tm.rows; i += WAVE) {
-                        const T a = col[i];
-                        for (int r = 0; r < rg; r++) acc[r] += a * xs[(size_t)i * nrhs + r];
+                // output j = column of A (cols = destination length), sum over rows i
+                for (int i0 = 0; i0 < tm.rows; i0 += TR) {
+                    const int ni = min(TR, tm.rows - i0);
+                    for (int idx = tid; idx < TR * SC; idx += NT) {
+                        const int ii = idx % TR, jj = idx / TR;
+                        tile[ii][jj] = (ii < ni && jj < no) ? A[(i0 + ii) + (size_t)(o0 + jj) * tm.rows] : T(0);
                     }
-                    for (int r = 0; r < rg; r++) {
-                        T v = wave_sum(acc[r]);
-                        if (lane == 0) S[j * rg + r] -= v;
+                    __syncthreads();
+                    for (int ii = kp; ii < ni; ii += NP) {
+                        const T a = tile[ii][lane];
+                        const T* xi = xs + (size_t)(i0 + ii) * nrhs;
+                        for (int r = 0; r < rg; r++) acc[r] += a * xi[r];
                     }
+                    __syncthreads();
                 }
             }
-            __syncthreads();
         }
-        for (int idx = tid; idx < g.n * rg; idx += NT) xd[(size_t)(idx / rg) * nrhs + r0 + idx % rg] = S[idx];
+        for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
+        __syncthreads();
+        if (kp == 0 && lane < no) {
+            T* y = ws + (size_t)(g.xdst + o0 + lane) * nrhs + r0;
+            for (int r = 0; r < rg; r++) y[r] -= red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
+        }
         __syncthreads();
     }
 }
@@ -196,68 +213,79 @@ __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const Gemv
 // Householder reflectors in blocked compact-WY form (the dormqr scheme): Q = Q_0 Q_1 ...
 // with Q_b = I - V_b T_b V_b^T over reflectors [32b, 32b + 32). Q^T x applies Q_0^T first,
 // Q x applies the last block first. V has a unit diagonal and zeros above it implicitly.
+// One workgroup per op (the blocks are sequential; the segment is updated in place);
+// W = V_b^T x is accumulated over 64-row tiles of V_b staged in LDS (32 reflectors x 8
+// row partitions).
 template <typename T>
-__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nrhs, int trans, int lds_elems) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nrhs, int trans) {
+    __shared__ T vt[SC][QB + 1];
+    __shared__ T wred[NT / QB][QB][RG_MAX];
+    __shared__ T W[QB][RG_MAX], U[QB][RG_MAX];
     const QTask t = tasks[blockIdx.x];
-    const int m = t.m, k = t.k, tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nw = NT / WAVE;
+    const int m = t.m, k = t.k, tid = threadIdx.x;
     if (m == 0 || k == 0) return;
-    const int step = min(RG_MAX, lds_elems / (m + 2 * QB));
-    T* S = reinterpret_cast<T*>(smem_raw);  // x chunk, m * step
-    T* W = S + (size_t)m * step;            // V_b^T x, QB * step
-    T* U = W + (size_t)QB * step;           // T_b^(T) W, QB * step
     const T* V = reinterpret_cast<const T*>(t.V);
     T* x = ws + (size_t)t.xoff * nrhs;
     const int nblk = (k + QB - 1) / QB;
-    for (int r0 = 0; r0 < nrhs; r0 += step) {
-        const int rg = min(step, nrhs - r0);
-        for (int idx = tid; idx < m * rg; idx += NT) S[idx] = x[(size_t)(idx / rg) * nrhs + r0 + idx % rg];
-        __syncthreads();
+    const int c = tid % QB, part = tid / QB, nparts = NT / QB;
+    for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
+        const int rg = min(RG_MAX, nrhs - r0);
         for (int bi = 0; bi < nblk; bi++) {
             const int b = trans ? bi : nblk - 1 - bi;
             const int b0 = b * QB, nb = min(QB, k - b0);
             const T* Tb = reinterpret_cast<const T*>(t.Tm) + (size_t)b * QB * QB;
-            // W = V_b^T x: wavefront per reflector c, w_c = x_{b0+c} + sum_{i > b0+c} V(i, b0+c) x_i
-            for (int c = wv; c < nb; c += nw) {
-                const int cc = b0 + c;
-                T acc[RG_MAX];
-                for (int r = 0; r < rg; r++) acc[r] = T(0);
-                const T* col = V + (size_t)cc * m;
-                for (int i = cc + 1 + lane; i < m; i += WAVE) {
-                    const T a = col[i];
-                    for (int r = 0; r < rg; r++) acc[r] += a * S[i * rg + r];
+            // W = V_b^T x with v_c = [1 at row b0 + c, V below]
+            T acc[RG_MAX];
+#pragma unroll
+            for (int r = 0; r < RG_MAX; r++) acc[r] = T(0);
+            for (int i0 = b0; i0 < m; i0 += SC) {
+                const int ni = min(SC, m - i0);
+                for (int idx = tid; idx < SC * QB; idx += NT) {
+                    const int ii = idx % SC, cc = idx / SC, i = i0 + ii, col = b0 + cc;
+                    T v = T(0);
+                    if (ii < ni && cc < nb) v = (i == col) ? T(1) : (i > col ? V[i + (size_t)col * m] : T(0));
+                    vt[ii][cc] = v;
                 }
-                for (int r = 0; r < rg; r++) {
-                    T v = wave_sum(acc[r]);
-                    if (lane == 0) W[c * rg + r] = S[cc * rg + r] + v;
+                __syncthreads();
+                for (int ii = part; ii < ni; ii += nparts) {
+                    const T v = vt[ii][c];
+                    const T* xi = x + (size_t)(i0 + ii) * nrhs + r0;
+                    for (int r = 0; r < rg; r++) acc[r] += v * xi[r];
                 }
+                __syncthreads();
+            }
+            for (int r = 0; r < rg; r++) wred[part][c][r] = acc[r];
+            __syncthreads();
+            if (tid < nb * rg) {
+                const int cc = tid / rg, r = tid % rg;
+                T s = T(0);
+                for (int p = 0; p < nparts; p++) s += wred[p][cc][r];
+                W[cc][r] = s;
             }
             __syncthreads();
             // U = T_b^T W (Q^T) or T_b W (Q); T_b is nb x nb upper triangular, ld QB
-            for (int idx = tid; idx < nb * rg; idx += NT) {
-                const int a = idx / rg, r = idx % rg;
-                T acc = T(0);
+            if (tid < nb * rg) {
+                const int a = tid / rg, r = tid % rg;
+                T s = T(0);
                 if (trans) {
-                    for (int q = 0; q <= a; q++) acc += Tb[q + (size_t)a * QB] * W[q * rg + r];
+                    for (int q = 0; q <= a; q++) s += Tb[q + (size_t)a * QB] * W[q][r];
                 } else {
-                    for (int q = a; q < nb; q++) acc += Tb[a + (size_t)q * QB] * W[q * rg + r];
+                    for (int q = a; q < nb; q++) s += Tb[a + (size_t)q * QB] * W[q][r];
                 }
-                U[a * rg + r] = acc;
+                U[a][r] = s;
             }
             __syncthreads();
             // x -= V_b U for rows i >= b0: U_{i-b0} on the unit diagonal + sum_{c < i-b0} V(i, b0+c) U_c
             for (int idx = tid; idx < (m - b0) * rg; idx += NT) {
                 const int i = b0 + idx / rg, r = idx % rg;
                 const int li = i - b0;
-                T acc = li < nb ? U[li * rg + r] : T(0);
+                T s = li < nb ? U[li][r] : T(0);
                 const int cmax = min(li, nb);
-                for (int c = 0; c < cmax; c++) acc += V[i + (size_t)(b0 + c) * m] * U[c * rg + r];
-                S[i * rg + r] -= acc;
+                for (int cc = 0; cc < cmax; cc++) s += V[i + (size_t)(b0 + cc) * m] * U[cc][r];
+                x[(size_t)i * nrhs + r0 + r] -= s;
             }
             __syncthreads();
         }
-        for (int idx = tid; idx < m * rg; idx += NT) x[(size_t)(idx / rg) * nrhs + r0 + idx % rg] = S[idx];
-        __syncthreads();
     }
 }
 
@@ -437,35 +465,27 @@ hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld,
 }
 // LDS for one RG chunk of the largest segment in a launch is sized by the caller through
 // max_n; the launchers take the largest dimension from the task arrays' host copies.
-// Dynamic LDS of a solve launch: one chunk of min(8, nrhs) right-hand sides of the
-// launch's largest segment (plus the two QB-vectors of the Householder apply), at most
-// SOLVE_LDS_BYTES; small segments get small allocations, so many workgroups share a CU.
 template <typename T>
-static size_t solve_lds(int seg_max, int nrhs) {
-    const size_t per_rhs = (size_t)std::max(seg_max, 1) * sizeof(T);
-    const size_t rg = (size_t)std::min(RG_MAX, std::max(nrhs, 1));
-    return std::min((size_t)SOLVE_LDS_BYTES, std::max(per_rhs, per_rhs * rg));
-}
-template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st) {
-    if (ntasks == 0) return hipSuccess;
-    const size_t lds = solve_lds<T>(seg_max, nrhs);
-    k_trmv<T><<<ntasks, NT, lds, st>>>(d_tasks, ws, nrhs, trans, (int)(lds / sizeof(T)));
+hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, T* ws, T* wt, int nrhs,
+                       int trans, hipStream_t st) {
+    if (nchunks == 0) return hipSuccess;
+    k_trmv<T><<<nchunks, NT, 0, st>>>(d_tasks, d_chunks, ws, wt, nrhs, trans);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_copyback<T><<<nchunks, NT, 0, st>>>(d_tasks, d_chunks, ws, wt, nrhs);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, int ngroups, int seg_max, const GemvTerm* d_terms, T* ws, int nrhs,
-                       hipStream_t st) {
-    if (ngroups == 0) return hipSuccess;
-    const size_t lds = solve_lds<T>(seg_max, nrhs);
-    k_gemv<T><<<ngroups, NT, lds, st>>>(d_groups, d_terms, ws, nrhs, (int)(lds / sizeof(T)));
+hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms, T* ws,
+                       int nrhs, hipStream_t st) {
+    if (nchunks == 0) return hipSuccess;
+    k_gemv<T><<<nchunks, NT, 0, st>>>(d_groups, d_chunks, d_terms, ws, nrhs);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, int seg_max, T* ws, int nrhs, int trans, hipStream_t st) {
+hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    const size_t lds = solve_lds<T>(seg_max + 2 * QB, nrhs);
-    k_qapply<T><<<ntasks, NT, lds, st>>>(d_tasks, ws, nrhs, trans, (int)(lds / sizeof(T)));
+    k_qapply<T><<<ntasks, NT, 0, st>>>(d_tasks, ws, nrhs, trans);
     return hipGetLastError();
 }
 template <typename T>
@@ -511,10 +531,11 @@ hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t
 #define SPAND_SOLVE_INST(T)                                                                                  \
     template hipError_t launch_perm_in<T>(const double*, size_t, size_t, int, const int*, int, T*, hipStream_t); \
     template hipError_t launch_perm_out<T>(const T*, const int*, int, int, double*, size_t, size_t, hipStream_t); \
-    template hipError_t launch_trmv<T>(const TrsvTask*, int, int, T*, int, int, hipStream_t);               \
+    template hipError_t launch_trmv<T>(const TrsvTask*, const SolveChunk*, int, T*, T*, int, int, hipStream_t); \
     template hipError_t launch_larft<T>(const LarftTask*, int, hipStream_t);                                \
-    template hipError_t launch_gemv<T>(const GemvGroup*, int, int, const GemvTerm*, T*, int, hipStream_t);  \
-    template hipError_t launch_qapply<T>(const QTask*, int, int, T*, int, int, hipStream_t);                \
+    template hipError_t launch_gemv<T>(const GemvGroup*, const SolveChunk*, int, const GemvTerm*, T*, int,    \
+                                       hipStream_t);                                                        \
+    template hipError_t launch_qapply<T>(const QTask*, int, T*, int, int, hipStream_t);                     \
     template hipError_t launch_merge<T>(const MergeTask*, int, const int2*, T*, int, int, hipStream_t);
 SPAND_SOLVE_INST(float)
 SPAND_SOLVE_INST(double)

@@ -5,7 +5,7 @@
 namespace spand {
 
 static const char* const kFamilyNames[KF_COUNT] = {
-    "fill", "scatter", "copy", "potrf", "potrf_panel", "potrf_trail", "trsm", "schur", "rrqr", "rrqr_init", "rrqr_step",
+    "fill", "scatter", "copy", "potrf", "potrf_panel", "potrf_trail", "trsm", "schur", "rrqr", "rrqr_init", "rrqr_step", "rrqr_wave",
     "rrqr_finish", "larft", "perm", "trmv", "gemv", "qapply", "merge", "spmv", "krylov_vec",
 };
 

@@ -21,6 +21,7 @@ enum KernelFamily {
     KF_RRQR,
     KF_RRQR_INIT,
     KF_RRQR_STEP,
+    KF_RRQR_WAVE,
     KF_RRQR_FINISH,
     KF_LARFT,
     KF_PERM,

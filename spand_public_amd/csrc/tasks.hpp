@@ -77,8 +77,11 @@ struct RrqrBig {
     int* map;        // p2c[2][max_cols] then c2p[2][max_cols]
     double* cand_v;  // [2][nslab] per-slab best partial norm for the next step, then |R_00|
     int* cand_p;     // [2][nslab] its position
-    int* dims;       // rows, cols, min(rows, cols), steps to run (the rank once it is known)
+    int* dims;       // rows, cols, min(rows, cols), steps to run (the rank once it is known),
+                     // 1 once every kept reflector is stored
     void* vbuf;      // nullptr, or [nslab][2][max_rows] pivot column / reflector buffers
+                     // (per-step launches), or [2][nslab][max_rows] candidate columns
+                     // (persistent wave launch)
     int nslab;
     int pad;
 };
@@ -133,6 +136,11 @@ struct LarftTask {
     int m, k;
     int b0;
     int pad;
+};
+// One chunk of outputs [o0, o0 + 64) of a solve task (trmv task or gemv group).
+struct SolveChunk {
+    int task;
+    int o0;
 };
 struct MergeTask {  // Merge (src/operations.cpp:166-192)
     int pxoff;

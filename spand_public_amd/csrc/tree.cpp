@@ -365,7 +365,7 @@ static double geqp3_flops(double m, double n) {
     return k > 0 ? 2.0 * m * n * k - (2.0 / 3.0) * k * k * k : 0.0;
 }
 
-// Clusters with at least this many rows and more Asn elements than this take the stepped
+// Clusters with at least 16 rows and more than 16384 Asn elements take the stepped
 // multi-workgroup RRQR; smaller ones run in one workgroup. Both paths compute the same
 // numbers; SPAND_RRQR_STEPPED_MIN_ROWS / SPAND_RRQR_STEPPED_MIN_ELEMS override the split.
 static long long env_or(const char* name, long long dflt) {
@@ -415,15 +415,9 @@ void Tree::factorize_impl() {
     for (int c : bottoms_[0]) cls_[c].xoff = cls_[c].start;
     ws_len_ = N_;
     const int lds_elems = LDS_BYTES / (int)sizeof(T);
-    const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 48);
+    const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
     const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 16384);
 
-    // the solve kernels stage one right-hand side of a segment in LDS at least
-    const int max_segment = SOLVE_LDS_BYTES / (int)sizeof(T) - 2 * QB;
-    for (int c = 0; c < (int)cls_.size(); c++)
-        if (cls_[c].osize > max_segment)
-            throw Error(ERR_UNSUPPORTED, "cluster of " + std::to_string(cls_[c].osize) +
-                                             " unknowns exceeds the solve kernels' segment limit");
     for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
         LevelStats& lst = stats_[ilvl_];
         LevelSolve& lsv = solve_plan_[ilvl_];
@@ -691,7 +685,7 @@ void Tree::factorize_impl() {
                     b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
                     b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)(2 * b.nslab + 1) * sizeof(double)));
                     b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
-                    b.dims = reinterpret_cast<int*>(arena_.alloc(4 * sizeof(int)));
+                    b.dims = reinterpret_cast<int*>(arena_.alloc(5 * sizeof(int)));
                     b.t = t;
                     where[p] = {1, (int)wbig[w].size()};
                     for (int k = 0; k < b.nslab; k++) wslab[w].push_back({(int)wbig[w].size(), k});
@@ -712,10 +706,22 @@ void Tree::factorize_impl() {
         }
         // pivot-column / reflector buffers of the stepped path: LDS unless a wave's largest
         // cluster needs more than STEP_LDS_MAX, then global buffers for the whole wave
-        std::vector<size_t> wlds_step(nwaves, 0), wlds_fin(nwaves, 0);
+        // A wave whose clusters all have at most WAVE_ROWS_MAX rows runs as one persistent
+        // launch (candidate-column buffers in vbuf); the per-step launches remain the path
+        // for larger clusters and for waves the device cannot hold at once.
+        std::vector<size_t> wlds_step(nwaves, 0), wlds_fin(nwaves, 0), wlds_wave(nwaves, 0);
+        const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 1) != 0;
         for (int w = 0; w < nwaves; w++) {
             const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
-            if (2 * vb <= (size_t)STEP_LDS_MAX) {
+            int wmaxcols = 0;
+            for (auto& b : wbig[w]) wmaxcols = std::max(wmaxcols, b.t.max_cols);
+            const size_t lw = ((size_t)wmaxcols * sizeof(int) + 15) / 16 * 16 + 2 * vb;
+            if (persistent && !wbig[w].empty() && wmaxrows[w] <= WAVE_ROWS_MAX && lw <= (size_t)WAVE_LDS_MAX) {
+                wlds_wave[w] = lw;
+                wlds_fin[w] = 0;
+                for (auto& b : wbig[w])
+                    b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
+            } else if (2 * vb <= (size_t)STEP_LDS_MAX) {
                 wlds_step[w] = 2 * vb;
                 wlds_fin[w] = vb;
             } else {
@@ -726,8 +732,9 @@ void Tree::factorize_impl() {
         lst.nwaves = nwaves;
         // ---------------- upload + launch Elim / Scale / Sparsify ----------------
         for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
-        auto small_first = [](std::vector<PotrfTask>& v) {
+        auto small_first = [](std::vector<PotrfTask>& v) {  // <= 64, <= POTRF_SMALL_MAX, larger
             std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= POTRF_SMALL_MAX; });
+            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= 64; });
         };
         small_first(potrfE);
         small_first(potrfS);
@@ -759,17 +766,25 @@ void Tree::factorize_impl() {
         // last) one 32-column panel per launch pair
         auto run_potrf = [&](const std::vector<PotrfTask>& v, size_t off, const char* what) {
             if (v.empty()) return;
-            int nsmall = 0, nmax = 0;
+            int ntiny = 0, nsmall = 0, nmax = 0;
             for (auto& t : v) {
+                if (t.n <= 64) ntiny++;
                 if (t.n <= POTRF_SMALL_MAX) nsmall++;
                 nmax = std::max(nmax, t.n);
             }
             const PotrfTask* d = Packer::at<PotrfTask>(dt, off);
-            if (nsmall > 0) {
-                std::vector<PotrfTask> vs(v.begin(), v.begin() + nsmall);
+            if (ntiny > 0) {
+                std::vector<PotrfTask> vs(v.begin(), v.begin() + ntiny);
                 Work w = work_potrf(vs, esz);
                 hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
-                                [&] { return launch_potrf<T>(d, nsmall, d_info_, st); }),
+                                [&] { return launch_potrf64<T>(d, ntiny, d_info_, st); }),
+                          what);
+            }
+            if (nsmall > ntiny) {
+                std::vector<PotrfTask> vs(v.begin() + ntiny, v.begin() + nsmall);
+                Work w = work_potrf(vs, esz);
+                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
+                                [&] { return launch_potrf<T>(d + ntiny, nsmall - ntiny, d_info_, st); }),
                           what);
             }
             const int nbig = (int)v.size() - nsmall;
@@ -840,10 +855,26 @@ void Tree::factorize_impl() {
                             &slot_init[w]),
                       "rrqr init");
             // the steps of a wave are timed as one batch (two events per wave)
-            slot_step[w] = prof_.start(KF_RRQR_STEP, st);
-            for (int i = 0; i < wsteps[w]; i++)
-                hip_check(launch_rrqr_step<T>(db, ds, nsl, i, tol_, wlds_step[w], st), "rrqr step");
-            prof_.stop(slot_step[w], st, 0, 0, wsteps[w]);
+            bool done = false;
+            if (wlds_wave[w] > 0) {
+                const int sl = prof_.start(KF_RRQR_WAVE, st);
+                const hipError_t e = launch_rrqr_wave<T>(db, ds, nsl, wsteps[w], wmaxrows[w], tol_, wlds_wave[w], st);
+                prof_.stop(sl, st, 0, 0, e == hipSuccess ? 1 : 0);
+                if (e == hipSuccess) {
+                    done = true;
+                    slot_step[w] = sl;
+                } else if (e == hipErrorCooperativeLaunchTooLarge) {
+                    (void)hipGetLastError();
+                } else {
+                    hip_check(e, "rrqr wave");
+                }
+            }
+            if (!done) {
+                slot_step[w] = prof_.start(KF_RRQR_STEP, st);
+                for (int i = 0; i < wsteps[w]; i++)
+                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, tol_, wlds_step[w], st), "rrqr step");
+                prof_.stop(slot_step[w], st, 0, 0, wsteps[w]);
+            }
             hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,
                             [&] {
                                 return launch_rrqr_finish<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
@@ -1052,10 +1083,29 @@ void Tree::factorize_impl() {
             g.k1 += bwd_base;
         }
         terms.insert(terms.end(), bwd_terms.begin(), bwd_terms.end());
+        auto chunks_of = [](const auto& v, auto len) {
+            std::vector<SolveChunk> ch;
+            for (int k = 0; k < (int)v.size(); k++)
+                for (int o0 = 0; o0 < len(v[k]); o0 += SOLVE_CHUNK) ch.push_back({k, o0});
+            return ch;
+        };
+        auto tlen = [](const TrsvTask& t) { return t.n; };
+        auto glen = [](const GemvGroup& g) { return g.n; };
+        std::vector<SolveChunk> ch_e = chunks_of(sv_elim, tlen), ch_s = chunks_of(sv_scale, tlen);
+        std::vector<SolveChunk> ch_f = chunks_of(fwd_groups, glen), ch_b = chunks_of(bwd_groups, glen);
         Packer ps;
         size_t s_et = ps.add(sv_elim), s_fg = ps.add(fwd_groups), s_bg = ps.add(bwd_groups), s_tm = ps.add(terms);
         size_t s_st = ps.add(sv_scale), s_q = ps.add(qt), s_m = ps.add(mt), s_mc = ps.add(mchildren);
+        size_t s_ce = ps.add(ch_e), s_cs = ps.add(ch_s), s_cf = ps.add(ch_f), s_cb = ps.add(ch_b);
         void* dsv = upload_tasks(ps.buf);
+        lsv.ch_elim = Packer::at<SolveChunk>(dsv, s_ce);
+        lsv.n_ch_elim = (int)ch_e.size();
+        lsv.ch_scale = Packer::at<SolveChunk>(dsv, s_cs);
+        lsv.n_ch_scale = (int)ch_s.size();
+        lsv.ch_fwd = Packer::at<SolveChunk>(dsv, s_cf);
+        lsv.n_ch_fwd = (int)ch_f.size();
+        lsv.ch_bwd = Packer::at<SolveChunk>(dsv, s_cb);
+        lsv.n_ch_bwd = (int)ch_b.size();
         {
             const double esz = sizeof(T);
             auto trmv_work = [&](const std::vector<TrsvTask>& v) {
@@ -1080,11 +1130,6 @@ void Tree::factorize_impl() {
                 }
                 return w;
             };
-            for (auto& t : sv_elim) lsv.nmax_elim = std::max(lsv.nmax_elim, t.n);
-            for (auto& t : sv_scale) lsv.nmax_scale = std::max(lsv.nmax_scale, t.n);
-            for (auto& g : fwd_groups) lsv.nmax_fwd = std::max(lsv.nmax_fwd, g.n);
-            for (auto& g : bwd_groups) lsv.nmax_bwd = std::max(lsv.nmax_bwd, g.n);
-            for (auto& q : qt) lsv.nmax_q = std::max(lsv.nmax_q, q.m);
             lsv.w_elim = trmv_work(sv_elim);
             lsv.w_scale = trmv_work(sv_scale);
             lsv.w_fwd = gemv_work(fwd_groups);
@@ -1265,7 +1310,8 @@ void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st) {
 // tail of its original cluster's segment.
 template <typename T>
 void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st) {
-    const size_t need = (size_t)ws_len_ * nrhs * sizeof(T);
+    // ws: the segments; wt: scratch of the same layout for the triangular sweeps
+    const size_t need = 2 * (size_t)ws_len_ * nrhs * sizeof(T);
     if (need > ws_cap_) {
         (void)hipFree(d_ws_);
         d_ws_ = nullptr;
@@ -1273,6 +1319,7 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
         ws_cap_ = need;
     }
     T* ws = reinterpret_cast<T*>(d_ws_);
+    T* wt = ws + (size_t)ws_len_ * nrhs;
     const double esz = sizeof(T);
     auto run = [&](int fam, const SweepWork& w, const char* what, auto&& launch) {
         hip_check(timed(prof_, fam, st, w.mat_bytes + w.vec_elems * nrhs * esz, w.flops * nrhs, launch), what);
@@ -1282,17 +1329,16 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
     run(KF_PERM, wp, "perm in", [&] { return launch_perm_in<T>(dx, sx, sr, nrhs, d_perm_, N_, ws, st); });
     for (int l = 0; l < nlevels_; l++) {
         const LevelSolve& p = solve_plan_[l];
-        if (p.n_elim_trsv)
+        if (p.n_ch_elim)
             run(KF_TRMV, p.w_elim, "trmv",
-                [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, p.nmax_elim, ws, nrhs, 0, st); });
-        if (p.n_fwd_groups)
+                [&] { return launch_trmv<T>(p.elim_trsv, p.ch_elim, p.n_ch_elim, ws, wt, nrhs, 0, st); });
+        if (p.n_ch_fwd)
             run(KF_GEMV, p.w_fwd, "gemv",
-                [&] { return launch_gemv<T>(p.fwd_groups, p.n_fwd_groups, p.nmax_fwd, p.terms, ws, nrhs, st); });
-        if (p.n_scale_trsv)
+                [&] { return launch_gemv<T>(p.fwd_groups, p.ch_fwd, p.n_ch_fwd, p.terms, ws, nrhs, st); });
+        if (p.n_ch_scale)
             run(KF_TRMV, p.w_scale, "trmv",
-                [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, p.nmax_scale, ws, nrhs, 0, st); });
-        if (p.n_q)
-            run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, p.nmax_q, ws, nrhs, 1, st); });
+                [&] { return launch_trmv<T>(p.scale_trsv, p.ch_scale, p.n_ch_scale, ws, wt, nrhs, 0, st); });
+        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 1, st); });
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 1, st); });
@@ -1302,17 +1348,16 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 0, st); });
-        if (p.n_q)
-            run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, p.nmax_q, ws, nrhs, 0, st); });
-        if (p.n_scale_trsv)
+        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 0, st); });
+        if (p.n_ch_scale)
             run(KF_TRMV, p.w_scale, "trmv",
-                [&] { return launch_trmv<T>(p.scale_trsv, p.n_scale_trsv, p.nmax_scale, ws, nrhs, 1, st); });
-        if (p.n_bwd_groups)
+                [&] { return launch_trmv<T>(p.scale_trsv, p.ch_scale, p.n_ch_scale, ws, wt, nrhs, 1, st); });
+        if (p.n_ch_bwd)
             run(KF_GEMV, p.w_bwd, "gemv",
-                [&] { return launch_gemv<T>(p.bwd_groups, p.n_bwd_groups, p.nmax_bwd, p.terms, ws, nrhs, st); });
-        if (p.n_elim_trsv)
+                [&] { return launch_gemv<T>(p.bwd_groups, p.ch_bwd, p.n_ch_bwd, p.terms, ws, nrhs, st); });
+        if (p.n_ch_elim)
             run(KF_TRMV, p.w_elim, "trmv",
-                [&] { return launch_trmv<T>(p.elim_trsv, p.n_elim_trsv, p.nmax_elim, ws, nrhs, 1, st); });
+                [&] { return launch_trmv<T>(p.elim_trsv, p.ch_elim, p.n_ch_elim, ws, wt, nrhs, 1, st); });
     }
     run(KF_PERM, wp, "perm out", [&] { return launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st); });
 }

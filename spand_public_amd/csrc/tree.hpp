@@ -109,7 +109,10 @@ struct SweepWork {
 // Per-level solve plan (device task arrays) for the preconditioner apply.
 struct LevelSolve {
     SweepWork w_elim, w_fwd, w_bwd, w_scale, w_q, w_merge;
-    int nmax_elim = 0, nmax_fwd = 0, nmax_bwd = 0, nmax_scale = 0, nmax_q = 0;  // largest segment per launch
+    SolveChunk* ch_elim = nullptr; int n_ch_elim = 0;  // 64-output chunks of each launch
+    SolveChunk* ch_scale = nullptr; int n_ch_scale = 0;
+    SolveChunk* ch_fwd = nullptr; int n_ch_fwd = 0;
+    SolveChunk* ch_bwd = nullptr; int n_ch_bwd = 0;
     TrsvTask* elim_trsv = nullptr; int n_elim_trsv = 0;
     GemvGroup* fwd_groups = nullptr; int n_fwd_groups = 0;
     GemvGroup* bwd_groups = nullptr; int n_bwd_groups = 0;

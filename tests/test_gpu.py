@@ -39,16 +39,21 @@ def gpu_tree(A, X, nl, tol, prec=S.FP64, skip=0):
 
 @pytest.fixture
 def stepped_rrqr(request, monkeypatch):
-    if request.param:
+    """Which RRQR path the clusters take: the library's own size split ("default"), every
+    cluster on the multi-workgroup path run as one persistent launch per wave ("wave"),
+    or as one launch per pivot step ("steps")."""
+    if request.param in ("wave", "steps"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
+    if request.param == "steps":
+        monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "0")
     return request.param
 
 
-@pytest.mark.parametrize("stepped_rrqr", [False, True], indirect=True, ids=["wg-rrqr", "stepped-rrqr"])
+@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps"], indirect=True)
 @pytest.mark.parametrize("n,d,nl,tol", CASES)
 def test_fp64_factor_matches_oracle(n, d, nl, tol, stepped_rrqr):
-    say(f"case fp64 {d}D n={n} levels={nl} tol={tol} stepped={stepped_rrqr}")
+    say(f"case fp64 {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")
     A, X = neglapl(n, d), linspace_nd(n, d)
     o, _ = oracle.setup_tree(A, X, nl, tol)
     o.factorize()
