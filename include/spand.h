@@ -146,6 +146,16 @@ typedef struct {
 spand_status spand_pcg(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld,
                        int maxit, double tol, int on_device, int* iters_out, spand_krylov_stats* stats);
 
+/* gmres(mat, rhs, x, precond, iters, restart, tol, verb)   include/is.h:13, src/is.cpp:123-300
+ * Left-preconditioned GMRES(restart) with Householder Arnoldi, FP64, on nrhs right-hand
+ * sides in lockstep. A column stops when |w_k| / ||M^{-1}(b - A x0)|| < tol (the
+ * reference's preconditioned-residual test, with the initial norm kept across restarts)
+ * or after maxit iterations; stats->relres_max is the true ||b - A x|| / ||b||. Same
+ * argument conventions as spand_pcg. */
+spand_status spand_gmres(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld,
+                         int maxit, int restart, double tol, int on_device, int* iters_out,
+                         spand_krylov_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

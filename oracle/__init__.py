@@ -65,6 +65,8 @@ def lib():
             "or_clusters": (None, [vp, _i32p]),
             "or_cg": (C.c_int, [vp, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int, C.c_double,
                                 C.POINTER(C.c_double)]),
+            "or_gmres": (C.c_int, [vp, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_double,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "or_potrf": (C.c_int, [C.c_int, _f64pF, C.c_int]),
             "or_trsm_llnn": (None, [C.c_int, C.c_int, _f64pF, C.c_int, _f64pF, C.c_int]),
             "or_trsm_rltn": (None, [C.c_int, C.c_int, _f64pF, C.c_int, _f64pF, C.c_int]),
@@ -229,6 +231,15 @@ class OracleTree:
         tp = C.c_double(0.0)
         it = self._L.or_cg(self.h, A.n, A.colptr, A.rowval, A.nzval, b, x, int(iters), float(tol), C.byref(tp))
         return x, it, tp.value
+
+    def gmres(self, A, b, x=None, iters=100, restart=30, tol=1e-12):
+        """Returns (x, iterations, last |w_k| / ||M^{-1} r0||)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x is None else np.ascontiguousarray(x, dtype=np.float64).copy()
+        tp, rel = C.c_double(0.0), C.c_double(0.0)
+        it = self._L.or_gmres(self.h, A.n, A.colptr, A.rowval, A.nzval, b, x, int(iters), int(restart), float(tol),
+                              C.byref(rel), C.byref(tp))
+        return x, it, rel.value
 
 
 def setup_tree(A, X, nlevels, tol, skip=0, symm="SPD"):

@@ -9,6 +9,7 @@
 // ChooseRank / swap2perm KATs, dense kernels against scipy LAPACK golden vectors, and
 // brute-force dense solves on tiny grids.
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -981,6 +982,151 @@ static int cg(Tree& t, int N, const int* cp, const int* ri, const double* v, con
     return i + 1;
 }
 
+// Eigen's HouseholderSequence primitives for real vectors, as gmres uses them:
+// makeHouseholder (Eigen/src/Householder/Householder.h) on x[0..n), and
+// applyHouseholderOnTheLeft of (essential, tau) on y[0..n).
+static void make_householder(int n, const double* x, double* essential, double& tau, double& beta) {
+    const double tail_sq = n == 1 ? 0.0 : dot(n - 1, x + 1, x + 1);
+    const double c0 = x[0];
+    if (tail_sq <= DBL_MIN) {
+        tau = 0.0;
+        beta = c0;
+        for (int i = 0; i + 1 < n; i++) essential[i] = 0.0;
+    } else {
+        beta = std::sqrt(c0 * c0 + tail_sq);
+        if (c0 >= 0.0) beta = -beta;
+        for (int i = 0; i + 1 < n; i++) essential[i] = x[i + 1] / (c0 - beta);
+        tau = (beta - c0) / beta;
+    }
+}
+static void apply_householder(int n, double* y, const double* essential, double tau) {
+    if (n == 1) {
+        y[0] *= 1.0 - tau;
+    } else if (tau != 0.0) {
+        const double tmp = dot(n - 1, essential, y + 1) + y[0];
+        y[0] -= tau * tmp;
+        for (int i = 0; i + 1 < n; i++) y[i + 1] -= tau * essential[i] * tmp;
+    }
+}
+// JacobiRotation::makeGivens (real) and applyOnTheLeft(i, i + 1, G.adjoint())
+struct Givens {
+    double c = 1.0, s = 0.0;
+};
+static Givens make_givens(double p, double q) {
+    Givens g;
+    if (q == 0.0) {
+        g.c = p < 0.0 ? -1.0 : 1.0;
+        g.s = 0.0;
+    } else if (p == 0.0) {
+        g.c = 0.0;
+        g.s = q < 0.0 ? 1.0 : -1.0;
+    } else if (std::fabs(p) > std::fabs(q)) {
+        const double t = q / p;
+        double u = std::sqrt(1.0 + t * t);
+        if (p < 0.0) u = -u;
+        g.c = 1.0 / u;
+        g.s = -t * g.c;
+    } else {
+        const double t = p / q;
+        double u = std::sqrt(1.0 + t * t);
+        if (q < 0.0) u = -u;
+        g.s = -1.0 / u;
+        g.c = -t * g.s;
+    }
+    return g;
+}
+static void apply_givens_adjoint(const Givens& g, double& x, double& y) {
+    const double x0 = x, y0 = y;
+    x = g.c * x0 - g.s * y0;
+    y = g.s * x0 + g.c * y0;
+}
+
+// gmres (src/is.cpp:123-300): left-preconditioned GMRES(restart) with Householder Arnoldi
+// and Givens rotations; stops when |w_k| / ||M^{-1}(b - A x0)|| < tol (the initial
+// preconditioned residual norm, kept across restarts). Returns the iteration count;
+// *rel_out receives the last |w_k| / ||r0||.
+static int gmres(Tree& t, int N, const int* cp, const int* ri, const double* v_, const double* rhs, double* x,
+                 int maxIters, int restart, double tol, double* rel_out, double* t_prec) {
+    const int m = N;
+    *rel_out = 0.0;
+    if (std::sqrt(dot(N, rhs, rhs)) <= DBL_MIN) {
+        std::fill(x, x + N, 0.0);
+        return 0;
+    }
+    int iters = 0;
+    std::vector<double> p0(N), r0(N), tmp(N);
+    spmv(N, cp, ri, v_, x, tmp.data());
+    for (int i = 0; i < N; i++) p0[i] = rhs[i] - tmp[i];
+    r0 = p0;
+    double tp = now();
+    t.solve(r0.data());
+    *t_prec += now() - tp;
+    const double r0Norm = std::sqrt(dot(N, r0.data(), r0.data()));
+    if (r0Norm == 0.0) return 0;
+    std::vector<double> H((size_t)m * (restart + 1), 0.0), w(restart + 1, 0.0), tau(restart + 1, 0.0);
+    std::vector<Givens> G(restart);
+    std::vector<double> v(m), x_new(m);
+    auto Hc = [&](int c) { return H.data() + (size_t)c * m; };
+    double beta;
+    make_householder(m, r0.data(), Hc(0) + 1, tau[0], beta);
+    w[0] = beta;
+    for (int k = 1; k <= restart; ++k) {
+        ++iters;
+        std::fill(v.begin(), v.end(), 0.0);
+        v[k - 1] = 1.0;
+        for (int i = k - 1; i >= 0; --i) apply_householder(m - i, v.data() + i, Hc(i) + i + 1, tau[i]);
+        spmv(N, cp, ri, v_, v.data(), tmp.data());
+        v = tmp;
+        tp = now();
+        t.solve(v.data());
+        *t_prec += now() - tp;
+        for (int i = 0; i < k; ++i) apply_householder(m - i, v.data() + i, Hc(i) + i + 1, tau[i]);
+        if (k < m && std::sqrt(dot(m - k, v.data() + k, v.data() + k)) != 0.0) {
+            make_householder(m - k, v.data() + k, Hc(k) + k + 1, tau[k], beta);
+            apply_householder(m - k, v.data() + k, Hc(k) + k + 1, tau[k]);
+        }
+        for (int i = 0; i + 1 < k; ++i) apply_givens_adjoint(G[i], v[i], v[i + 1]);
+        if (k < m && v[k] != 0.0) {
+            G[k - 1] = make_givens(v[k - 1], v[k]);
+            apply_givens_adjoint(G[k - 1], v[k - 1], v[k]);
+            apply_givens_adjoint(G[k - 1], w[k - 1], w[k]);
+        }
+        for (int i = 0; i < k; i++) Hc(k - 1)[i] = v[i];
+        const double tol_error = std::fabs(w[k]) / r0Norm;
+        *rel_out = tol_error;
+        const bool stop = (k == m || tol_error < tol || iters == maxIters);
+        if (t.verb) printf("%d: |Ax-b|/|b| = %3.2e <? %3.2e\n", iters, tol_error, tol);
+        if (stop || k == restart) {
+            std::vector<double> y(w.begin(), w.begin() + k);
+            for (int i = k - 1; i >= 0; --i) {  // upper-triangular solve
+                double s = y[i];
+                for (int j = i + 1; j < k; j++) s -= Hc(j)[i] * y[j];
+                y[i] = s / Hc(i)[i];
+            }
+            std::fill(x_new.begin(), x_new.end(), 0.0);
+            for (int i = k - 1; i >= 0; --i) {
+                x_new[i] += y[i];
+                apply_householder(m - i, x_new.data() + i, Hc(i) + i + 1, tau[i]);
+            }
+            for (int i = 0; i < N; i++) x[i] += x_new[i];
+            if (stop) return iters;
+            k = 0;
+            spmv(N, cp, ri, v_, x, tmp.data());
+            for (int i = 0; i < N; i++) p0[i] = rhs[i] - tmp[i];
+            r0 = p0;
+            tp = now();
+            t.solve(r0.data());
+            *t_prec += now() - tp;
+            std::fill(H.begin(), H.end(), 0.0);
+            std::fill(w.begin(), w.end(), 0.0);
+            std::fill(tau.begin(), tau.end(), 0.0);
+            make_householder(m, r0.data(), Hc(0) + 1, tau[0], beta);
+            w[0] = beta;
+        }
+    }
+    return iters;
+}
+
 }  // namespace oracle
 
 // =======================================================================================
@@ -1144,6 +1290,15 @@ int or_cg(void* h, int N, const int* colptr, const int* rowval, const double* va
           int iters, double tol, double* t_prec) {
     double tp = 0;
     int it = cg(*(Tree*)h, N, colptr, rowval, vals, b, x, iters, tol, &tp);
+    if (t_prec) *t_prec = tp;
+    return it;
+}
+
+int or_gmres(void* h, int N, const int* colptr, const int* rowval, const double* vals, const double* b, double* x,
+             int iters, int restart, double tol, double* rel, double* t_prec) {
+    double tp = 0, r = 0;
+    int it = gmres(*(Tree*)h, N, colptr, rowval, vals, b, x, iters, restart, tol, &r, &tp);
+    if (rel) *rel = r;
     if (t_prec) *t_prec = tp;
     return it;
 }

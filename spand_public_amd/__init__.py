@@ -20,7 +20,7 @@ from . import problems
 from ._lib import KrylovStats, SpandError, lib
 from .problems import CSC
 
-__all__ = ["Tree", "Matrix", "cg", "cg_device", "problems", "SpandError", "CSC", "FP64", "FP32"]
+__all__ = ["Tree", "Matrix", "cg", "cg_device", "gmres", "problems", "SpandError", "CSC", "FP64", "FP32"]
 
 FP64 = 0
 FP32 = 1
@@ -255,6 +255,25 @@ def cg(A, b, precond: Tree, x=None, iters: int = 100, tol: float = 1e-12, raise_
     L = lib()
     status = L.spand_pcg(precond._h, Am._h, B.ctypes.data, X.ctypes.data, nrhs, B.shape[0], int(iters), float(tol),
                          0, its.ctypes.data, C.byref(st))
+    if status not in (0, 7) or (status == 7 and raise_on_fail):
+        _check(precond._h, status)
+    info = {"iters": its, "status": status, **{f: getattr(st, f) for f, _ in KrylovStats._fields_}}
+    return X.reshape(b.shape), info
+
+
+def gmres(A, b, precond: Tree, x=None, iters: int = 100, restart: int = 30, tol: float = 1e-12,
+          raise_on_fail: bool = False):
+    """Left-preconditioned GMRES(restart) (src/is.cpp:123-300) on the GPU, one or several
+    right-hand sides. Returns (x, info) like cg; info["relres_max"] is the true residual."""
+    Am = A if isinstance(A, Matrix) else Matrix(A)
+    b = np.asarray(b, dtype=np.float64)
+    B = np.asfortranarray(b.reshape(b.shape[0], -1))
+    X = np.zeros_like(B, order="F") if x is None else np.asfortranarray(np.asarray(x, np.float64).reshape(B.shape)).copy(order="F")
+    nrhs = B.shape[1]
+    its = np.zeros(nrhs, dtype=np.int32)
+    st = KrylovStats()
+    status = lib().spand_gmres(precond._h, Am._h, B.ctypes.data, X.ctypes.data, nrhs, B.shape[0], int(iters),
+                               int(restart), float(tol), 0, its.ctypes.data, C.byref(st))
     if status not in (0, 7) or (status == 7 and raise_on_fail):
         _check(precond._h, status)
     info = {"iters": its, "status": status, **{f: getattr(st, f) for f, _ in KrylovStats._fields_}}

@@ -66,6 +66,8 @@ SIGNATURES = {
     "spand_matrix_destroy": (None, [vp]),
     "spand_pcg": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp,
                             C.POINTER(KrylovStats)]),
+    "spand_gmres": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp,
+                              C.POINTER(KrylovStats)]),
 }
 
 

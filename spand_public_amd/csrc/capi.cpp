@@ -246,16 +246,8 @@ spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, co
 }
 void spand_matrix_destroy(spand_matrix* A) { delete A; }
 
-spand_status spand_pcg(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld, int maxit,
-                       double tol, int on_device, int* iters_out, spand_krylov_stats* stats) {
-    CHECK_T(t);
-    if (!A || !A->m || !b || !x) {
-        t->err = "pcg: null argument";
-        return SPAND_ERR_ARG;
-    }
-    spand::KrylovResult r;
-    spand_status s = guarded(t, [&] { r = spand::pcg(*t->t, *A->m, b, x, nrhs, ld, maxit, tol, on_device != 0); });
-    if (s != SPAND_OK) return s;
+static spand_status krylov_out(spand_tree* t, const spand::KrylovResult& r, int nrhs, int* iters_out,
+                               spand_krylov_stats* stats, const char* name) {
     if (iters_out)
         for (int c = 0; c < nrhs; c++) iters_out[c] = r.iters[c];
     int itmax = 0;
@@ -274,10 +266,37 @@ spand_status spand_pcg(spand_tree* t, const spand_matrix* A, const double* b, do
         stats->applies = r.applies;
     }
     if (r.converged < nrhs) {
-        t->err = "pcg: iteration cap reached before the tolerance";
+        t->err = std::string(name) + ": iteration cap reached before the tolerance";
         return SPAND_ERR_NOT_CONVERGED;
     }
     return SPAND_OK;
+}
+
+spand_status spand_pcg(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld, int maxit,
+                       double tol, int on_device, int* iters_out, spand_krylov_stats* stats) {
+    CHECK_T(t);
+    if (!A || !A->m || !b || !x) {
+        t->err = "pcg: null argument";
+        return SPAND_ERR_ARG;
+    }
+    spand::KrylovResult r;
+    spand_status s = guarded(t, [&] { r = spand::pcg(*t->t, *A->m, b, x, nrhs, ld, maxit, tol, on_device != 0); });
+    if (s != SPAND_OK) return s;
+    return krylov_out(t, r, nrhs, iters_out, stats, "pcg");
+}
+
+spand_status spand_gmres(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld,
+                         int maxit, int restart, double tol, int on_device, int* iters_out, spand_krylov_stats* stats) {
+    CHECK_T(t);
+    if (!A || !A->m || !b || !x) {
+        t->err = "gmres: null argument";
+        return SPAND_ERR_ARG;
+    }
+    spand::KrylovResult r;
+    spand_status s = guarded(
+        t, [&] { r = spand::gmres(*t->t, *A->m, b, x, nrhs, ld, maxit, restart, tol, on_device != 0); });
+    if (s != SPAND_OK) return s;
+    return krylov_out(t, r, nrhs, iters_out, stats, "gmres");
 }
 
 }  // extern "C"

@@ -433,6 +433,67 @@ __global__ void __launch_bounds__(NT) k_axpby(int N, int nrhs, const double* x, 
     }
 }
 
+// GMRES kernels. V holds Householder vectors as interleaved N x nrhs blocks (block j at
+// V + j * N * nrhs). out = base + [ye; 0] - sum_{j<k} V_j z_j, per column (base may be
+// nullptr = 0; ye holds ny rows, or nullptr).
+__global__ void __launch_bounds__(NT) k_gm_comb(int N, int nrhs, const double* base, const double* ye, int ny,
+                                               const double* V, int k, const double* z, double* out) {
+    const size_t n = (size_t)N * nrhs, blk = (size_t)N * nrhs;
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.x * NT) {
+        const int i = (int)(idx / nrhs), r = (int)(idx % nrhs);
+        double acc = base ? base[idx] : 0.0;
+        if (ye && i < ny) acc += ye[idx];
+        for (int j = 0; j < k; j++) acc -= V[(size_t)j * blk + idx] * z[(size_t)j * nrhs + r];
+        out[idx] = acc;
+    }
+}
+// partial[(b * k + j) * nrhs + r] = sum over block b of V_j[i, r] * v[i, r]; grid (blocks, k)
+__global__ void __launch_bounds__(NT) k_gm_multidot_partial(int N, int nrhs, const double* V, const double* v,
+                                                            double* partial) {
+    __shared__ double red[NT];
+    int nrp = 1;
+    while (nrp < nrhs) nrp <<= 1;
+    const int groups = NT / nrp;
+    const int tid = threadIdx.x, r = tid % nrp, g = tid / nrp, j = blockIdx.y;
+    const double* Vj = V + (size_t)j * N * nrhs;
+    double s = 0.0;
+    if (r < nrhs)
+        for (size_t i = (size_t)blockIdx.x * groups + g; i < (size_t)N; i += (size_t)gridDim.x * groups)
+            s += Vj[i * nrhs + r] * v[i * nrhs + r];
+    red[tid] = s;
+    __syncthreads();
+    for (int off = groups / 2; off > 0; off >>= 1) {
+        if (g < off) red[tid] += red[tid + off * nrp];
+        __syncthreads();
+    }
+    if (g == 0 && r < nrhs) partial[((size_t)blockIdx.x * gridDim.y + j) * nrhs + r] = red[tid];
+}
+// out[j * nrhs + r] = sum over blocks; one workgroup per (j, r)
+__global__ void __launch_bounds__(NT) k_gm_multidot_final(int nblocks, int k, int nrhs, const double* partial,
+                                                          double* out) {
+    __shared__ double red[NT];
+    const int jr = blockIdx.x;  // j * nrhs + r
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += NT) s += partial[(size_t)b * k * nrhs + jr];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[jr] = red[0];
+}
+// Householder vector of step k from v: u[i] = 0 (i < k), 1 (i = k), v[i] / d[r] (i > k);
+// d[r] = 0 means a zero essential part (tau = 0).
+__global__ void __launch_bounds__(NT) k_gm_householder(int N, int nrhs, int k, const double* v, const double* d,
+                                                       double* u) {
+    const size_t n = (size_t)N * nrhs;
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.x * NT) {
+        const int i = (int)(idx / nrhs), r = (int)(idx % nrhs);
+        u[idx] = i < k ? 0.0 : (i == k ? 1.0 : (d[r] != 0.0 ? v[idx] / d[r] : 0.0));
+    }
+}
+
 __global__ void k_ratio(int nrhs, const double* num, const double* den, const int* active, double* out) {
     int r = threadIdx.x;
     if (r < nrhs) out[r] = (active == nullptr || active[r]) ? num[r] / den[r] : 0.0;
@@ -516,6 +577,25 @@ hipError_t launch_dot(int N, int nrhs, const double* a, const double* b, double*
 hipError_t launch_axpby(int N, int nrhs, const double* x, const double* ax, double sgn_ax, double* y,
                         const double* by, hipStream_t st) {
     k_axpby<<<grid_for((size_t)N * nrhs), NT, 0, st>>>(N, nrhs, x, ax, sgn_ax, y, by);
+    return hipGetLastError();
+}
+hipError_t launch_gm_comb(int N, int nrhs, const double* base, const double* ye, int ny, const double* V, int k,
+                          const double* z, double* out, hipStream_t st) {
+    k_gm_comb<<<grid_for((size_t)N * nrhs), NT, 0, st>>>(N, nrhs, base, ye, ny, V, k, z, out);
+    return hipGetLastError();
+}
+hipError_t launch_gm_multidot(int N, int nrhs, int k, const double* V, const double* v, double* partial, double* out,
+                              hipStream_t st) {
+    if (k == 0) return hipSuccess;
+    constexpr int MB = 256;
+    k_gm_multidot_partial<<<dim3(MB, k), NT, 0, st>>>(N, nrhs, V, v, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_gm_multidot_final<<<k * nrhs, NT, 0, st>>>(MB, k, nrhs, partial, out);
+    return hipGetLastError();
+}
+hipError_t launch_gm_householder(int N, int nrhs, int k, const double* v, const double* d, double* u, hipStream_t st) {
+    k_gm_householder<<<grid_for((size_t)N * nrhs), NT, 0, st>>>(N, nrhs, k, v, d, u);
     return hipGetLastError();
 }
 hipError_t launch_ratio(int nrhs, const double* num, const double* den, const int* active, double* out,

@@ -3,6 +3,7 @@
 // residual; columns run in lockstep so one preconditioner apply serves all of them.
 #include "krylov.hpp"
 
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -207,6 +208,361 @@ KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nr
         hip_check(hipMemcpy2D(x, (size_t)ld * sizeof(double), io, (size_t)N * sizeof(double),
                               (size_t)N * sizeof(double), nrhs, hipMemcpyDeviceToHost),
                   "pcg io");
+    }
+    prof.collect();
+    res.seconds = wall() - t0;
+    res.seconds_precond = t_prec;
+    res.seconds_spmv = t_spmv;
+    res.applies = applies;
+    return res;
+}
+
+// ---------------------------------------------------------------------------------------
+// gmres (src/is.cpp:123-300) on nrhs columns in lockstep, FP64 on the device.
+//
+// The reference builds the Arnoldi basis with Householder reflections H_i = I - tau_i u_i
+// u_i^T (Eigen's makeHouseholder / applyHouseholderOnTheLeft) and applies them one at a
+// time. Here the reflections of a cycle are kept as interleaved device blocks V and applied
+// together in compact-WY form, Q_k = H_0 ... H_{k-1} = I - V T V^T (T upper triangular,
+// k x k, on the host): one multi-dot and one combination kernel per application instead of
+// k dot/axpy pairs. The Hessenberg matrix, the Givens rotations and w live on the host (a
+// few hundred numbers per column). A column stops when |w_k| / ||M^{-1}(b - A x0)|| < tol,
+// with the initial preconditioned residual norm kept across restarts (src/is.cpp:161,245).
+namespace {
+struct Givens {
+    double c = 1.0, s = 0.0;
+};
+Givens make_givens(double p, double q) {  // JacobiRotation::makeGivens, real case
+    Givens g;
+    if (q == 0.0) {
+        g.c = p < 0.0 ? -1.0 : 1.0;
+    } else if (p == 0.0) {
+        g.c = 0.0;
+        g.s = q < 0.0 ? 1.0 : -1.0;
+    } else if (std::fabs(p) > std::fabs(q)) {
+        const double t = q / p;
+        double u = std::sqrt(1.0 + t * t);
+        if (p < 0.0) u = -u;
+        g.c = 1.0 / u;
+        g.s = -t * g.c;
+    } else {
+        const double t = p / q;
+        double u = std::sqrt(1.0 + t * t);
+        if (q < 0.0) u = -u;
+        g.s = -1.0 / u;
+        g.c = -t * g.s;
+    }
+    return g;
+}
+void apply_givens_adjoint(const Givens& g, double& x, double& y) {  // applyOnTheLeft(i, i+1, G.adjoint())
+    const double x0 = x, y0 = y;
+    x = g.c * x0 - g.s * y0;
+    y = g.s * x0 + g.c * y0;
+}
+}  // namespace
+
+KrylovResult gmres(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, int restart,
+                   double tol, bool on_device) {
+    if (A.N != t.N()) throw Error(ERR_ARG, "gmres: matrix and tree sizes differ");
+    if (nrhs <= 0 || nrhs > 64 || ld < A.N) throw Error(ERR_ARG, "gmres: bad nrhs / ld");
+    if (restart < 1) throw Error(ERR_ARG, "gmres: restart must be positive");
+    const double t0 = wall();
+    const int N = A.N, R = std::min(restart, N);
+    const size_t nv = (size_t)N * nrhs;
+    const hipStream_t st = 0;
+    KrylovResult res;
+    res.iters.assign(nrhs, 0);
+    res.relres.assign(nrhs, 0.0);
+    // device: b, x, v, tmp, io, V[R + 1], multidot partials, small per-column arrays
+    const size_t n_small = (size_t)(R + 2) * nrhs;
+    const size_t n_part = (size_t)256 * (R + 1) * nrhs;
+    char* wsb = static_cast<char*>(
+        A.workspace(((5 + (size_t)(R + 1)) * nv + n_part + 4 * n_small + (size_t)1024 * nrhs + 8 * nrhs) *
+                    sizeof(double)));
+    double* base = reinterpret_cast<double*>(wsb);
+    double *db = base, *dx = base + nv, *v = base + 2 * nv, *tmp = base + 3 * nv, *io = base + 4 * nv;
+    double* V = base + 5 * nv;
+    double* part = V + (size_t)(R + 1) * nv;
+    double* dz = part + n_part;
+    double* dye = dz + n_small;
+    double* dsm = dye + n_small;
+    double* dd = dsm + n_small;
+    double* dpart = dd + n_small;
+    double* dsc = dpart + (size_t)1024 * nrhs;
+    KernelProfiler& prof = t.profiler();
+    const double vbytes = (double)nv * sizeof(double);
+    auto vec = [&](double passes, const char* what, auto&& launch) {
+        hip_check(timed(prof, KF_KRYLOV_VEC, st, passes * vbytes, 0, launch), what);
+    };
+    double t_prec = 0, t_spmv = 0;
+    int applies = 0;
+    auto spmv = [&](const double* in, double* out) {
+        double a = wall();
+        hip_check(timed(prof, KF_SPMV, st, 12.0 * A.nnz + 4.0 * (N + 1) + 2.0 * vbytes, 2.0 * A.nnz * nrhs,
+                        [&] { return launch_spmv(N, A.rowptr, A.colidx, A.val, in, out, nrhs, st); }),
+                  "spmv");
+        t_spmv += wall() - a;
+    };
+    auto precond = [&](double* w) {
+        double a = wall();
+        t.solve_interleaved(w, nrhs, st);
+        applies++;
+        t_prec += wall() - a;
+    };
+    std::vector<double> hsc(nrhs);
+    auto dot_host = [&](const double* a, const double* c, int n) {  // per-column dot over n rows
+        vec(2, "dot", [&] { return launch_dot(n, nrhs, a, c, dpart, dsc, st); });
+        hip_check(hipMemcpy(hsc.data(), dsc, nrhs * sizeof(double), hipMemcpyDeviceToHost), "gmres");
+        return hsc;
+    };
+    auto to_inter = [&](const double* src, double* dst) {
+        if (on_device) {
+            vec(2, "cm2il", [&] { return launch_cm2il(src, ld, N, nrhs, dst, st); });
+        } else {
+            hip_check(hipMemcpy2D(io, (size_t)N * sizeof(double), src, (size_t)ld * sizeof(double),
+                                  (size_t)N * sizeof(double), nrhs, hipMemcpyHostToDevice),
+                      "gmres io");
+            vec(2, "cm2il", [&] { return launch_cm2il(io, N, N, nrhs, dst, st); });
+        }
+    };
+    to_inter(b, db);
+    to_inter(x, dx);
+    // per-column host state; column-major small matrices with leading dimension R + 1
+    const int L = R + 1;
+    std::vector<double> H((size_t)nrhs * L * L), T((size_t)nrhs * L * L), vtop((size_t)nrhs * L * L);
+    std::vector<double> w((size_t)nrhs * (L + 1)), tau((size_t)nrhs * L), r0norm(nrhs, 0.0), rel(nrhs, 0.0);
+    std::vector<Givens> G((size_t)nrhs * R);
+    std::vector<int> act(nrhs, 1);
+    auto h_at = [&](std::vector<double>& M, int c, int i, int j) -> double& { return M[(size_t)c * L * L + i + (size_t)j * L]; };
+    std::vector<double> hz(n_small), hye(n_small), hrows((size_t)L * nrhs), hd(nrhs);
+    // columns with b = 0 get x = 0 (src/is.cpp:140-144)
+    {
+        std::vector<double> bn2 = dot_host(db, db, N);
+        bool any = false;
+        for (int c = 0; c < nrhs; c++)
+            if (std::sqrt(bn2[c]) <= DBL_MIN) {
+                act[c] = 0;
+                any = true;
+            }
+        if (any) {
+            std::vector<double> hx(nv);
+            hip_check(hipMemcpy(hx.data(), dx, nv * sizeof(double), hipMemcpyDeviceToHost), "gmres");
+            for (int c = 0; c < nrhs; c++)
+                if (!act[c])
+                    for (int i = 0; i < N; i++) hx[(size_t)i * nrhs + c] = 0.0;
+            hip_check(hipMemcpy(dx, hx.data(), nv * sizeof(double), hipMemcpyHostToDevice), "gmres");
+        }
+    }
+    // rows 0..R of an interleaved vector
+    auto read_rows = [&](const double* dv) {
+        hip_check(hipMemcpy(hrows.data(), dv, (size_t)L * nrhs * sizeof(double), hipMemcpyDeviceToHost), "gmres");
+    };
+    // makeHouseholder of x[k:] from x[k] and ||x[k+1:]||^2; writes u_k = V_k and vtop_k
+    auto householder = [&](int k, const double* dv, const std::vector<double>& tail_sq, std::vector<double>& beta_out,
+                           const std::vector<int>& make) {
+        read_rows(dv);
+        for (int c = 0; c < nrhs; c++) {
+            const double c0 = hrows[(size_t)k * nrhs + c];
+            double tau_k = 0.0, beta = c0, d = 0.0;
+            if (make[c] && tail_sq[c] > DBL_MIN) {
+                beta = std::sqrt(c0 * c0 + tail_sq[c]);
+                if (c0 >= 0.0) beta = -beta;
+                d = c0 - beta;
+                tau_k = (beta - c0) / beta;
+            }
+            hd[c] = d;
+            tau[(size_t)c * L + k] = tau_k;
+            beta_out[c] = beta;
+            for (int i = 0; i < L; i++)
+                h_at(vtop, c, i, k) = i < k ? 0.0 : (i == k ? 1.0 : (d != 0.0 ? hrows[(size_t)i * nrhs + c] / d : 0.0));
+        }
+        hip_check(hipMemcpy(dd, hd.data(), nrhs * sizeof(double), hipMemcpyHostToDevice), "gmres");
+        vec(2, "householder", [&] { return launch_gm_householder(N, nrhs, k, dv, dd, V + (size_t)k * nv, st); });
+    };
+    std::vector<double> beta(nrhs), zero_tail(nrhs, 0.0);
+    // a new cycle: r0 = M^{-1}(b - A x), u_0 from r0, w = beta e_0, T = tau_0
+    auto start_cycle = [&](bool first) {
+        spmv(dx, tmp);
+        vec(2, "copy", [&] { return launch_copy_f64(v, db, nv, st); });
+        vec(3, "axpby", [&] { return launch_axpby(N, nrhs, tmp, nullptr, -1.0, v, nullptr, st); });
+        precond(v);
+        if (first) {
+            std::vector<double> n2 = dot_host(v, v, N);
+            for (int c = 0; c < nrhs; c++) {
+                r0norm[c] = std::sqrt(n2[c]);
+                if (act[c] && r0norm[c] == 0.0) act[c] = 0;  // initial guess already exact
+            }
+        }
+        std::vector<double> tail = N > 1 ? dot_host(v + nrhs, v + nrhs, N - 1) : zero_tail;
+        std::vector<int> make(nrhs, 1);
+        householder(0, v, tail, beta, make);
+        for (int c = 0; c < nrhs; c++) {
+            std::fill(H.begin() + (size_t)c * L * L, H.begin() + (size_t)(c + 1) * L * L, 0.0);
+            std::fill(T.begin() + (size_t)c * L * L, T.begin() + (size_t)(c + 1) * L * L, 0.0);
+            std::fill(w.begin() + (size_t)c * (L + 1), w.begin() + (size_t)(c + 1) * (L + 1), 0.0);
+            w[(size_t)c * (L + 1)] = beta[c];
+            h_at(T, c, 0, 0) = tau[(size_t)c * L];
+        }
+    };
+    // device v := op applied with z (k x nrhs, host) and ye (ny rows, host)
+    auto comb = [&](const double* basev, bool with_ye, int ny, int k, double* out) {
+        hip_check(hipMemcpy(dz, hz.data(), (size_t)std::max(k, 1) * nrhs * sizeof(double), hipMemcpyHostToDevice), "gmres");
+        if (with_ye)
+            hip_check(hipMemcpy(dye, hye.data(), (size_t)std::max(ny, 1) * nrhs * sizeof(double), hipMemcpyHostToDevice),
+                      "gmres");
+        vec(1.0 + k, "gmres comb", [&] {
+            return launch_gm_comb(N, nrhs, basev, with_ye ? dye : nullptr, ny, V, k, dz, out, st);
+        });
+    };
+    auto multidot = [&](int k, const double* vv, std::vector<double>& out) {  // out[j * nrhs + c] = V_j . vv
+        vec(1.0 + k, "gmres multidot", [&] { return launch_gm_multidot(N, nrhs, k, V, vv, part, dsm, st); });
+        out.resize((size_t)k * nrhs);
+        hip_check(hipMemcpy(out.data(), dsm, (size_t)k * nrhs * sizeof(double), hipMemcpyDeviceToHost), "gmres");
+    };
+    int nact = 0;
+    for (int c = 0; c < nrhs; c++) nact += act[c];
+    if (nact > 0) start_cycle(true);
+    nact = 0;
+    for (int c = 0; c < nrhs; c++) nact += act[c];
+    std::vector<double> W;
+    int k = 1;
+    while (nact > 0) {
+        for (int c = 0; c < nrhs; c++)
+            if (act[c]) res.iters[c]++;
+        // v = Q_k e_{k-1} = e_{k-1} - V T (V^T e_{k-1}); (V^T e_{k-1})_j = u_j[k-1]
+        for (int c = 0; c < nrhs; c++) {
+            for (int a = 0; a < k; a++) {
+                double s = 0.0;
+                for (int bb = a; bb < k; bb++) s += h_at(T, c, a, bb) * h_at(vtop, c, k - 1, bb);
+                hz[(size_t)a * nrhs + c] = s;
+            }
+            for (int i = 0; i < k; i++) hye[(size_t)i * nrhs + c] = i == k - 1 ? 1.0 : 0.0;
+        }
+        comb(nullptr, true, k, k, v);
+        spmv(v, tmp);
+        vec(2, "copy", [&] { return launch_copy_f64(v, tmp, nv, st); });
+        precond(v);
+        // v = Q_k^T v = v - V T^T (V^T v)
+        multidot(k, v, W);
+        for (int c = 0; c < nrhs; c++)
+            for (int a = 0; a < k; a++) {
+                double s = 0.0;
+                for (int bb = 0; bb <= a; bb++) s += h_at(T, c, bb, a) * W[(size_t)bb * nrhs + c];
+                hz[(size_t)a * nrhs + c] = s;
+            }
+        comb(v, false, 0, k, v);
+        // Householder reflection of v[k:] (src/is.cpp:209-217)
+        std::vector<double> tail = (k + 1 < N) ? dot_host(v + (size_t)(k + 1) * nrhs, v + (size_t)(k + 1) * nrhs, N - k - 1)
+                                               : zero_tail;
+        read_rows(v);
+        std::vector<double> vh((size_t)(k + 1) * nrhs);
+        for (size_t q = 0; q < vh.size(); q++) vh[q] = hrows[q];
+        std::vector<int> make(nrhs, 0);
+        for (int c = 0; c < nrhs; c++) {
+            const double c0 = k < N ? vh[(size_t)k * nrhs + c] : 0.0;
+            make[c] = k < N && std::sqrt(c0 * c0 + tail[c]) != 0.0;
+        }
+        if (k < N && k <= R) {
+            householder(k, v, tail, beta, make);
+            for (int c = 0; c < nrhs; c++)
+                if (make[c]) vh[(size_t)k * nrhs + c] = beta[c];  // H_k maps v[k:] to beta e_0
+        }
+        // T gains the column of u_k: T(0:k, k) = -tau_k T(0:k, 0:k) (V^T u_k), T(k, k) = tau_k
+        if (k < R) {
+            multidot(k, V + (size_t)k * nv, W);
+            for (int c = 0; c < nrhs; c++) {
+                const double tk = tau[(size_t)c * L + k];
+                for (int a = 0; a < k; a++) {
+                    double s = 0.0;
+                    for (int bb = a; bb < k; bb++) s += h_at(T, c, a, bb) * W[(size_t)bb * nrhs + c];
+                    h_at(T, c, a, k) = -tk * s;
+                }
+                h_at(T, c, k, k) = tk;
+            }
+        }
+        // Givens rotations, Hessenberg column, residual estimate
+        std::vector<int> finish(nrhs, 0);
+        bool any_finish = false;
+        for (int c = 0; c < nrhs; c++) {
+            if (!act[c]) continue;
+            auto vv = [&](int i) -> double& { return vh[(size_t)i * nrhs + c]; };
+            Givens* Gc = G.data() + (size_t)c * R;
+            double* wc = w.data() + (size_t)c * (L + 1);
+            for (int i = 0; i + 1 < k; i++) apply_givens_adjoint(Gc[i], vv(i), vv(i + 1));
+            if (k < N && vv(k) != 0.0) {
+                Gc[k - 1] = make_givens(vv(k - 1), vv(k));
+                apply_givens_adjoint(Gc[k - 1], vv(k - 1), vv(k));
+                apply_givens_adjoint(Gc[k - 1], wc[k - 1], wc[k]);
+            }
+            for (int i = 0; i < k; i++) h_at(H, c, i, k - 1) = vv(i);
+            rel[c] = std::fabs(wc[k]) / r0norm[c];
+            const bool stop = (k == N || rel[c] < tol || res.iters[c] == maxit);
+            if (stop || k == R) {
+                finish[c] = stop ? 2 : 1;
+                any_finish = true;
+            }
+        }
+        if (!any_finish) {
+            k++;
+            continue;
+        }
+        // x += Q_k [y; 0] for the columns ending their cycle: y = H^{-1} w (upper triangular)
+        for (int c = 0; c < nrhs; c++) {
+            std::vector<double> y(k, 0.0);
+            if (finish[c]) {
+                const double* wc = w.data() + (size_t)c * (L + 1);
+                for (int i = k - 1; i >= 0; i--) {
+                    double s = wc[i];
+                    for (int j = i + 1; j < k; j++) s -= h_at(H, c, i, j) * y[j];
+                    y[i] = s / h_at(H, c, i, i);
+                }
+            }
+            // e_j = (V^T [y; 0])_j = sum_{i >= j} u_j[i] y_i; z = T e
+            std::vector<double> e(k, 0.0);
+            for (int j = 0; j < k; j++)
+                for (int i = j; i < k; i++) e[j] += h_at(vtop, c, i, j) * y[i];
+            for (int a = 0; a < k; a++) {
+                double s = 0.0;
+                for (int bb = a; bb < k; bb++) s += h_at(T, c, a, bb) * e[bb];
+                hz[(size_t)a * nrhs + c] = finish[c] ? s : 0.0;
+            }
+            for (int i = 0; i < k; i++) hye[(size_t)i * nrhs + c] = finish[c] ? y[i] : 0.0;
+        }
+        comb(dx, true, k, k, dx);
+        bool restart_cycle = false;
+        for (int c = 0; c < nrhs; c++) {
+            if (finish[c] == 2) {
+                act[c] = 0;
+                nact--;
+            } else if (finish[c] == 1) {
+                restart_cycle = true;
+            }
+        }
+        if (nact > 0 && restart_cycle) {
+            start_cycle(false);
+            k = 1;
+        } else {
+            k++;
+        }
+    }
+    // true residual
+    spmv(dx, tmp);
+    vec(2, "copy", [&] { return launch_copy_f64(v, db, nv, st); });
+    vec(3, "axpby", [&] { return launch_axpby(N, nrhs, tmp, nullptr, -1.0, v, nullptr, st); });
+    std::vector<double> rn2 = dot_host(v, v, N), bn2 = dot_host(db, db, N);
+    res.converged = 0;
+    for (int c = 0; c < nrhs; c++) {
+        res.relres[c] = bn2[c] > 0 ? std::sqrt(rn2[c] / bn2[c]) : 0.0;
+        if (rel[c] < tol || res.iters[c] == 0) res.converged++;
+    }
+    if (on_device) {
+        vec(2, "il2cm", [&] { return launch_il2cm(dx, N, nrhs, x, ld, st); });
+        hip_check(hipStreamSynchronize(st), "gmres");
+    } else {
+        vec(2, "il2cm", [&] { return launch_il2cm(dx, N, nrhs, io, N, st); });
+        hip_check(hipMemcpy2D(x, (size_t)ld * sizeof(double), io, (size_t)N * sizeof(double),
+                              (size_t)N * sizeof(double), nrhs, hipMemcpyDeviceToHost),
+                  "gmres io");
     }
     prof.collect();
     res.seconds = wall() - t0;

@@ -38,5 +38,8 @@ struct KrylovResult {
 // cg (src/is.cpp:39-121) on nrhs columns; b, x column-major with leading dimension ld.
 KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, double tol,
                  bool on_device);
+// gmres (src/is.cpp:123-300): left-preconditioned GMRES(restart), columns in lockstep.
+KrylovResult gmres(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, int restart,
+                   double tol, bool on_device);
 
 }  // namespace spand

@@ -105,6 +105,7 @@ Tree::~Tree() {
     (void)hipFree(d_info_);
     (void)hipFree(d_perm_);
     (void)hipFree(d_ws_);
+    (void)hipHostFree(h_stage_);
 }
 
 void Tree::set_symm_kind(int k) {
@@ -164,10 +165,34 @@ void Tree::set_eliminated(int c) {
     s.elim = true;
 }
 
+// Host -> device uploads of the factorization go through a pinned staging buffer and are
+// asynchronous on the factorization stream, so the host plans the next batch while the
+// device works. The staging area is reused from the start at every factorization (the
+// previous one ended with a device synchronization); when it runs out, the device is
+// synchronized and the buffer grown.
+void Tree::stage_upload(void* d, const void* h, size_t bytes) {
+    if (bytes == 0) return;
+    const size_t off = (h_stage_used_ + 255) & ~(size_t)255;
+    if (off + bytes > h_stage_cap_) {
+        hip_check(hipDeviceSynchronize(), "staging");
+        const size_t cap = std::max<size_t>(2 * h_stage_cap_, std::max<size_t>(bytes, (size_t)8 << 20));
+        (void)hipHostFree(h_stage_);
+        h_stage_ = nullptr;
+        h_stage_cap_ = 0;
+        hip_check(hipHostMalloc((void**)&h_stage_, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
+        h_stage_cap_ = cap;
+        h_stage_used_ = 0;
+        return stage_upload(d, h, bytes);
+    }
+    std::memcpy(h_stage_ + off, h, bytes);
+    h_stage_used_ = off + bytes;
+    hip_check(hipMemcpyAsync(d, h_stage_ + off, bytes, hipMemcpyHostToDevice, 0), "upload");
+}
+
 void* Tree::upload_tasks(const std::vector<char>& host) {
     if (host.empty()) return nullptr;
     void* d = task_arena_.alloc(host.size());
-    hip_check(hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice), "upload tasks");
+    stage_upload(d, host.data(), host.size());
     return d;
 }
 
@@ -313,11 +338,13 @@ void Tree::factorize() {
 static inline double potrf_flops(double n) { return n * n * n / 3.0; }
 
 // Splits a triangular solve into independent chunks of at most 64 columns (left solve,
-// block n x m, ld n) or 64 rows (right solve, block m x n, ld m).
+// block n x m, ld n) or 64 rows (right solve, block m x n, ld m); 32 / 16 for n >= 128 / 256.
 static void push_trsm(std::vector<TrsmTask>& v, const void* L, void* B, int n, int m, int side, size_t esz) {
     if (n == 0 || m == 0) return;
-    for (int c0 = 0; c0 < m; c0 += 64) {
-        const int w = std::min(64, m - c0);
+    // long triangles walk many row blocks per chunk: narrower chunks give more workgroups
+    const int cw = n >= 256 ? 16 : (n >= 128 ? 32 : 64);
+    for (int c0 = 0; c0 < m; c0 += cw) {
+        const int w = std::min(cw, m - c0);
         char* b = reinterpret_cast<char*>(B) + (side == 0 ? (size_t)c0 * n : (size_t)c0) * esz;
         v.push_back({L, b, n, w, side, side == 0 ? n : m});
     }
@@ -379,6 +406,8 @@ void Tree::factorize_impl() {
     const hipStream_t st = 0;
     arena_.reset();
     task_arena_.reset();
+    hip_check(hipDeviceSynchronize(), "factorize");
+    h_stage_used_ = 0;
     ops_.clear();
     cls_ = cls0_;
     edges_ = edges0_;
@@ -750,7 +779,7 @@ void Tree::factorize_impl() {
         }
         const double t_plan1 = wall();
         void* dt = upload_tasks(pk.buf);
-        hip_check(hipMemcpy(d_size_, hsize.data(), cls_.size() * sizeof(int), hipMemcpyHostToDevice), "upload sizes");
+        stage_upload(d_size_, hsize.data(), cls_.size() * sizeof(int));
         const size_t esz = sizeof(T);
         if (fill_elems > 0)
             hip_check(timed(prof_, KF_FILL, st, (double)fill_elems * esz, 0,

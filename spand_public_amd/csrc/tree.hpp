@@ -184,6 +184,9 @@ class Tree {
     int new_edge(int n1, int n2, void* A, bool original);
     void set_eliminated(int c);
     void* upload_tasks(const std::vector<char>& host);
+    void stage_upload(void* d, const void* h, size_t bytes);
+    char* h_stage_ = nullptr;  // pinned staging of host -> device uploads
+    size_t h_stage_cap_ = 0, h_stage_used_ = 0;
 
     // options
     int nlevels_;

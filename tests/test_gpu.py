@@ -214,3 +214,24 @@ def test_kernel_profile_and_device_pcg():
     torch.cuda.synchronize()
     assert np.array_equal(its, info["iters"])
     assert np.array_equal(d_x.cpu().numpy().T, xh)
+
+
+@pytest.mark.parametrize("restart", [30, 4])
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32], ids=["fp64", "fp32"])
+def test_gmres_iterations_and_residual(restart, prec):
+    """Device GMRES(restart) (src/is.cpp:123-300) against the oracle's: iteration counts
+    within 10 %, preconditioned residual below tol, true residual small."""
+    say(f"case gmres restart={restart} prec={prec}")
+    A, X = neglapl(24, 3), linspace_nd(24, 3)
+    B = np.random.default_rng(31).standard_normal((A.n, 3))
+    o, _ = oracle.setup_tree(A, X, 8, 1e-1)
+    o.factorize()
+    its_o = [o.gmres(A, B[:, j], iters=300, restart=restart, tol=1e-10)[1] for j in range(3)]
+    g = gpu_tree(A, X, 8, 1e-1, prec=prec)
+    g.factorize()
+    x, info = S.gmres(A, B, g, iters=300, restart=restart, tol=1e-10)
+    assert info["status"] == 0, info
+    res = np.linalg.norm(A.matvec(x) - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert res.max() <= 1e-8, res
+    for io, ig in zip(its_o, info["iters"]):
+        assert abs(int(ig) - io) <= max(1, 0.1 * io), (its_o, info["iters"])

@@ -74,3 +74,31 @@ def test_not_spd_raises():
     t, _ = oracle.setup_tree(A, X, 3, 1e-2)
     with pytest.raises(RuntimeError, match="Non-SPD"):
         t.factorize()
+
+
+def test_gmres_properties():
+    """gmres (src/is.cpp:123-300): with the exact factorization (tol 0) the preconditioned
+    operator is the identity and GMRES stops after one iteration with x = A^{-1} b; with an
+    approximate factorization it converges, restarted or not, to the same solution; b = 0
+    returns x = 0 without iterating."""
+    A, X = neglapl(8, 2), linspace_nd(8, 2)
+    b = np.random.default_rng(11).standard_normal(A.n)
+    xs = np.linalg.solve(A.to_scipy().toarray(), b)
+    t, _ = oracle.setup_tree(A, X, 3, 0.0)
+    t.factorize()
+    x, it, rel = t.gmres(A, b, iters=50, restart=30, tol=1e-12)
+    assert it == 1 and np.linalg.norm(x - xs) <= 1e-12 * np.linalg.norm(xs)
+    A, X = neglapl(20, 3), linspace_nd(20, 3)
+    t, _ = oracle.setup_tree(A, X, 7, 1e-1)
+    t.factorize()
+    b = np.random.default_rng(12).standard_normal(A.n)
+    sols = []
+    for restart in (30, 5, 2):
+        x, it, rel = t.gmres(A, b, iters=300, restart=restart, tol=1e-10)
+        assert rel < 1e-10, (restart, it, rel)
+        assert np.linalg.norm(A.matvec(x) - b) / np.linalg.norm(b) <= 1e-8
+        sols.append((restart, it, x))
+    # more restarts never help: GMRES(2) needs at least as many iterations as GMRES(30)
+    assert sols[2][1] >= sols[0][1]
+    x, it, _ = t.gmres(A, np.zeros(A.n), iters=50)
+    assert it == 0 and not np.any(x)
