@@ -113,12 +113,17 @@ spand_status spand_kernel_profile(spand_tree* t, int family, long long* launches
                                   double* flops);
 
 /* Recorded operations (the Operation list, include/operations.h), for op-by-op comparison.
- * 8 ints per op: type (0 ScalingLLT, 2 GemmSymmOut, 3 GemmSymmIn, 6 Orthogonal, 7 Merge,
- * 8 Split), order(a), order(b), size(a), size(b), rows, cols, level. */
+ * 8 ints per op: type (0 ScalingLLT, 1 ScalingPLUQ, 2 GemmSymmOut, 3 GemmSymmIn, 4 GemmOut,
+ * 5 GemmIn, 6 Orthogonal, 7 Merge, 8 Split), order(a), order(b), size(a), size(b), rows,
+ * cols, level. */
 spand_status spand_num_ops(const spand_tree* t, int* nops);
 spand_status spand_ops_info(const spand_tree* t, int* info);
 /* Copies op i's matrix (rows x cols, column-major, as FP64) and its tau (Orthogonal). */
 spand_status spand_op_data(const spand_tree* t, int i, double* M, double* tau);
+/* ScalingPLUQ op i (include/operations.h, ScalingPLUQ): U (size x size, column-major, as
+ * FP64) and the row permutation p (P^T A = L U, row i of P^T A is row p[i] of A); its L is
+ * the op's matrix of spand_op_data. */
+spand_status spand_op_plu(const spand_tree* t, int i, double* U, int* perm);
 
 /* Device copy of a square CSC matrix, used as the Krylov operator (the SpMat argument of
  * cg / gmres, include/is.h:12-14). */

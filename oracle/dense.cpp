@@ -87,16 +87,17 @@ void trsm_right_lower_trans(int m, int n, const double* L, int ldl, double* B, i
     }
 }
 
+// B := B U^{-1}: column j receives fma(-U(k,j), b_k, b_j) for k < j in increasing k, then
+// the scaling by 1 / U(j,j) -- the element recurrence of trsm_right_lower_trans with
+// L = U^T, which is how the GPU applies it.
 void trsm_right_upper_notrans(int m, int n, const double* U, int ldu, double* B, int ldb) {
     if (m == 0 || n == 0) return;
     for (int j = 0; j < n; j++) {
         double* bj = B + (size_t)j * ldb;
         for (int k = 0; k < j; k++) {
-            double a = IDX(U, k, j, ldu);
-            if (a != 0.0) {
-                const double* bk = B + (size_t)k * ldb;
-                for (int i = 0; i < m; i++) bj[i] -= a * bk[i];
-            }
+            const double a = IDX(U, k, j, ldu);
+            const double* bk = B + (size_t)k * ldb;
+            for (int i = 0; i < m; i++) bj[i] = std::fma(-a, bk[i], bj[i]);
         }
         double t = 1.0 / IDX(U, j, j, ldu);
         for (int i = 0; i < m; i++) bj[i] *= t;

@@ -204,6 +204,12 @@ class Tree:
         out = {"M": M[: r * c].reshape(c, r).T.copy()}
         if tau is not None:
             out["tau"] = tau[:c]
+        if t == 1:  # ScalingPLUQ: U and the row permutation
+            U = np.zeros(max(sa * sa, 1))
+            p = np.zeros(max(sa, 1), dtype=np.int32)
+            _check(self._h, self._L.spand_op_plu(self._h, int(i), U.ctypes.data, p.ctypes.data))
+            out["U"] = U[: sa * sa].reshape(sa, sa).T.copy()
+            out["p"] = p[:sa]
         return out
 
 

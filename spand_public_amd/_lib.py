@@ -62,6 +62,7 @@ SIGNATURES = {
     "spand_num_ops": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "spand_ops_info": (C.c_int, [vp, i32p]),
     "spand_op_data": (C.c_int, [vp, C.c_int, vp, vp]),
+    "spand_op_plu": (C.c_int, [vp, C.c_int, vp, vp]),
     "spand_matrix_create": (C.c_int, [C.c_int, i32p, i32p, f64p, C.POINTER(vp)]),
     "spand_matrix_destroy": (None, [vp]),
     "spand_pcg": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp,

@@ -231,6 +231,10 @@ spand_status spand_op_data(const spand_tree* t, int i, double* M, double* tau) {
     CHECK_T(t);
     return guarded(const_cast<spand_tree*>(t), [&] { t->t->op_data(i, M, tau); });
 }
+spand_status spand_op_plu(const spand_tree* t, int i, double* U, int* perm) {
+    CHECK_T(t);
+    return guarded(const_cast<spand_tree*>(t), [&] { t->t->op_plu(i, U, perm); });
+}
 
 spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, const double* vals, spand_matrix** out) {
     if (!out || N <= 0 || !colptr || !rowval || !vals) {

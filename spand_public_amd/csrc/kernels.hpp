@@ -28,6 +28,11 @@ hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int*
 template <typename T>
 hipError_t launch_potrf_trail(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, hipStream_t st);
 template <typename T> hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st);
+// LU pivots (GEN / PLU): in_lds = 1 runs the batch in LDS (every task must satisfy
+// n (n + 1) sizeof(T) <= GETRF_LDS_BYTES, nmax the largest n), in_lds = 0 in global memory.
+constexpr int GETRF_LDS_BYTES = 65536;
+template <typename T>
+hipError_t launch_getrf(const GetrfTask* d_tasks, int ntasks, int in_lds, int nmax, int* d_info, hipStream_t st);
 template <typename T>
 hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib* d_contribs, hipStream_t st);
 template <typename T>
@@ -60,11 +65,12 @@ template <typename T> hipError_t launch_copy(const CopyTask* d_tasks, int ntasks
 
 // Preconditioner sweeps. ws holds the per-cluster segments, interleaved over nrhs.
 // Element (i, r) of the caller's block is at b[i * sx + r * sr].
+// The segments are FP64 for either factor type T (T selects the factor blocks' type).
 template <typename T>
-hipError_t launch_perm_in(const double* b, size_t sx, size_t sr, int nrhs, const int* perm, int N, T* ws,
+hipError_t launch_perm_in(const double* b, size_t sx, size_t sr, int nrhs, const int* perm, int N, double* ws,
                           hipStream_t st);
 template <typename T>
-hipError_t launch_perm_out(const T* ws, const int* perm, int N, int nrhs, double* x, size_t sx, size_t sr,
+hipError_t launch_perm_out(const double* ws, const int* perm, int N, int nrhs, double* x, size_t sx, size_t sr,
                            hipStream_t st);
 hipError_t launch_cm2il(const double* src, int ld, int N, int nrhs, double* dst, hipStream_t st);
 hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld, hipStream_t st);
@@ -73,19 +79,19 @@ hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld,
 constexpr int SOLVE_CHUNK = 64;
 // reflectors per compact-WY block
 constexpr int QB = 32;
-// x := Linv x (trans 0) or Linv^T x (trans 1) with the explicit inverse factor.
+// x := Linv x[perm] (trans 0) or Lb^T x (trans 1) with the explicit inverse factors.
 template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, T* ws, T* wt, int nrhs,
-                       int trans, hipStream_t st);
+hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, double* ws, double* wt,
+                       int nrhs, int trans, hipStream_t st);
 // compact-WY factor of one block of reflectors per task
 template <typename T> hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st);
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms, T* ws,
-                       int nrhs, hipStream_t st);
+hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms,
+                       double* ws, int nrhs, hipStream_t st);
 template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st);
+hipError_t launch_qapply(const QTask* d_tasks, int ntasks, double* ws, int nrhs, int trans, hipStream_t st);
 template <typename T>
-hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, T* ws, int nrhs, int fwd,
+hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, double* ws, int nrhs, int fwd,
                         hipStream_t st);
 
 // FP64 Krylov vector kernels (interleaved N x nrhs).

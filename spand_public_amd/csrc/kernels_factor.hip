@@ -18,6 +18,7 @@
 #include <cfloat>
 #include <algorithm>
 #include <cstdint>
+#include <limits>
 
 #include "kernels.hpp"
 #include "tasks.hpp"
@@ -287,12 +288,19 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
     const bool left = t.side == 0;
     // element (triangular index i, independent index c)
     auto at = [&](int i, int c) -> T& { return left ? G[i + (size_t)c * ld] : G[c + (size_t)i * ld]; };
+    // a left solve from another block reads row perm[i] of it at the first touch of row i
+    // (every row is first read in the kb = 0 pass), and its own block afterwards
+    const T* S = reinterpret_cast<const T*>(t.src);
+    const int* P = t.perm;
+    auto first = [&](int i, int c) -> T {
+        return S ? S[(P ? P[i] : i) + (size_t)c * ld] : at(i, c);
+    };
     for (int kb = 0; kb < n; kb += PB) {
         const int nb = min(PB, n - kb);
         for (int idx = tid; idx < nb * m; idx += NT) {
             int k, c;
             if (left) { k = idx % nb; c = idx / nb; } else { c = idx % m; k = idx / m; }
-            Bk[k][c] = at(kb + k, c);
+            Bk[k][c] = kb == 0 ? first(kb + k, c) : at(kb + k, c);
         }
         for (int idx = tid; idx < nb * nb; idx += NT) {
             int i = idx % nb, k = idx / nb;
@@ -330,13 +338,128 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
             for (int idx = tid; idx < ni * m; idx += NT) {
                 int ii, c;
                 if (left) { ii = idx % ni; c = idx / ni; } else { c = idx % m; ii = idx / m; }
-                T acc = at(i0 + ii, c);
+                T acc = kb == 0 ? first(i0 + ii, c) : at(i0 + ii, c);
                 for (int k = 0; k < nb; k++) acc = fma(-Bk[k][c], Lr[k][ii], acc);
                 at(i0 + ii, c) = acc;
             }
             __syncthreads();
         }
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// getrf + split_LU, one workgroup per pivot (getf_cluster, src/tree.cpp:614-635; getf,
+// src/util.cpp:184-199; split_LU, src/util.cpp:213-227). The steps follow dgetf2: the pivot
+// row is the first one of largest |a_ij| (idamax), rows j and jp are interchanged across
+// all columns, the column is scaled by 1/a_jj (divided by a_jj when |a_jj| is below the
+// smallest normal number), then a_ic = a_ic + a_ij * (-a_jc) for every trailing element with
+// a_jc != 0 (dger), in j order -- the element recurrence of the oracle. The row
+// permutation is accumulated as swap2perm does (src/util.cpp:76-88). Blocks of at most
+// GETRF_LDS_BYTES are factored in LDS (ld n + 1), larger ones in place in global memory.
+template <typename T, int NTG>
+__global__ void __launch_bounds__(NTG) k_getrf(const GetrfTask* tasks, int in_lds, int* info) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char getrf_lds[];
+    constexpr int NW = NTG / 64;
+    __shared__ T red_v[NW];
+    __shared__ int red_i[NW];
+    __shared__ int s_jp, s_nz, bad;
+    const GetrfTask t = tasks[blockIdx.x];
+    const int n = t.n;
+    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    if (n == 0) return;
+    T* G = reinterpret_cast<T*>(t.A);
+    T* Ut = reinterpret_cast<T*>(t.Ut);
+    int* perm = t.perm;
+    T* A = in_lds ? reinterpret_cast<T*>(getrf_lds) : G;
+    const int ld = in_lds ? n + 1 : n;
+    if (in_lds)
+        for (int c = wv; c < n; c += NW)
+            for (int i = lane; i < n; i += 64) A[i + (size_t)c * ld] = G[i + (size_t)c * n];
+    for (int i = tid; i < n; i += NTG) perm[i] = i;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        // idamax over rows j..n-1 of column j: first index of the largest magnitude
+        T bv = T(-1);
+        int bi = n;
+        for (int i = j + tid; i < n; i += NTG) {
+            const T v = fabs(A[i + (size_t)j * ld]);
+            if (v > bv) { bv = v; bi = i; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(bv, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if (lane == 0) { red_v[wv] = bv; red_i[wv] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            T v = red_v[0];
+            int jp = red_i[0];
+            for (int w = 1; w < NW; w++)
+                if (red_v[w] > v || (red_v[w] == v && red_i[w] < jp)) { v = red_v[w]; jp = red_i[w]; }
+            // idamax returns the first index when a_jj is NaN (every comparison fails), and
+            // an all-NaN column has no candidate: keep row j in both cases, so that jp never
+            // leaves [j, n)
+            const T ajj = A[j + (size_t)j * ld];
+            if (jp < j || jp >= n || ajj != ajj) jp = j;
+            s_jp = jp;
+            s_nz = A[jp + (size_t)j * ld] != T(0);
+            if (!s_nz && !bad) bad = j + 1;
+            const int tmp = perm[j];
+            perm[j] = perm[jp];
+            perm[jp] = tmp;
+        }
+        __syncthreads();
+        const int jp = s_jp;
+        if (s_nz) {
+            if (jp != j)
+                for (int c = tid; c < n; c += NTG) {
+                    const T a = A[j + (size_t)c * ld];
+                    A[j + (size_t)c * ld] = A[jp + (size_t)c * ld];
+                    A[jp + (size_t)c * ld] = a;
+                }
+            __syncthreads();
+            const T piv = A[j + (size_t)j * ld];
+            if (fabs(piv) >= std::numeric_limits<T>::min()) {
+                const T r = T(1) / piv;
+                for (int i = j + 1 + tid; i < n; i += NTG) A[i + (size_t)j * ld] *= r;
+            } else {
+                for (int i = j + 1 + tid; i < n; i += NTG) A[i + (size_t)j * ld] /= piv;
+            }
+        }
+        __syncthreads();
+        // dger: trailing columns by wavefront, rows by lane
+        for (int c = j + 1 + wv; c < n; c += NW) {
+            const T y = A[j + (size_t)c * ld];
+            if (y != T(0)) {
+                const T mt = -y;
+                for (int i = j + 1 + lane; i < n; i += 64)
+                    A[i + (size_t)c * ld] = A[i + (size_t)c * ld] + A[i + (size_t)j * ld] * mt;
+            }
+        }
+        __syncthreads();
+    }
+    // split_LU: U(i,c) = (sign(d_i) sqrt|d_i|) * ((1 / d_i) * a_ic) above the diagonal,
+    // sign(d_i) sqrt|d_i| on it; stored transposed (Ut column i = row i of U), zero above
+    for (int i = wv; i < n; i += NW) {
+        const T d = A[i + (size_t)i * ld];
+        const T f = T((d > T(0)) - (d < T(0))) * sqrt(fabs(d));
+        const T inv = T(1) / d;
+        for (int c = lane; c < n; c += 64)
+            Ut[c + (size_t)i * n] = c < i ? T(0) : (c == i ? f * T(1) : f * (inv * A[i + (size_t)c * ld]));
+    }
+    __syncthreads();
+    // L(r,j) = a_rj * sqrt|d_j| below the diagonal, sqrt|d_j| on it, zero above
+    for (int j = wv; j < n; j += NW) {
+        const T sj = sqrt(fabs(A[j + (size_t)j * ld]));
+        for (int r = lane; r < n; r += 64)
+            if (r != j) G[r + (size_t)j * n] = r > j ? A[r + (size_t)j * ld] * sj : T(0);
+    }
+    __syncthreads();
+    for (int j = tid; j < n; j += NTG) G[j + (size_t)j * n] = sqrt(fabs(A[j + (size_t)j * ld])) * T(1);
+    if (bad && tid == 0) atomicCAS(info, 0, t.cluster + 1);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -691,7 +814,9 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
                 }
         }
         __syncthreads();
-        const int pvt = r_idx[0];
+        // no candidate (every remaining norm NaN, after a failed pivot upstream): keep column i
+        // in place, so that no index leaves [i, cols)
+        const int pvt = (r_idx[0] >= i && r_idx[0] < cols) ? r_idx[0] : i;
         if (pvt != i) {
             for (int r = tid; r < rows; r += NT) {
                 T tmp = A[r + (size_t)pvt * rows];
@@ -764,7 +889,8 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
     for (long long idx = tid; idx < (long long)rows * rank; idx += NT) Q[idx] = A[idx];
     // AsnP[:, jpvt[j]] = triu(R)[:rank, j], scattered into the shrunk edge blocks
     for (int j = tid; j < cols; j += NT) {
-        int p = jp[j];
+        const int p = jp[j];
+        if (p < 0 || p >= cols) continue;  // jp is a permutation of [0, cols)
         int k = seg_of(p);
         int jj = p - s_col0[k], w = s_col0[k + 1] - s_col0[k];
         T* E = reinterpret_cast<T*>(sg[k].E);
@@ -1443,6 +1569,19 @@ hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st) {
 }
 
 template <typename T>
+hipError_t launch_getrf(const GetrfTask* d_tasks, int ntasks, int in_lds, int nmax, int* d_info, hipStream_t st) {
+    if (ntasks == 0) return hipSuccess;
+    if (in_lds) {
+        const size_t bytes = (size_t)nmax * (nmax + 1) * sizeof(T);
+        if (bytes > (size_t)GETRF_LDS_BYTES) return hipErrorInvalidValue;
+        k_getrf<T, NT><<<ntasks, NT, bytes, st>>>(d_tasks, 1, d_info);
+    } else {
+        k_getrf<T, 1024><<<ntasks, 1024, 0, st>>>(d_tasks, 0, d_info);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib* d_contribs, hipStream_t st) {
     if (ntiles == 0) return hipSuccess;
     k_schur<T><<<ntiles, NT, 0, st>>>(d_tiles, d_contribs);
@@ -1546,6 +1685,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_potrf_panel<T>(const PotrfTask*, int, int, int*, hipStream_t);     \
     template hipError_t launch_potrf_trail<T>(const PotrfTask*, int, int, int, hipStream_t);      \
     template hipError_t launch_trsm<T>(const TrsmTask*, int, hipStream_t);                        \
+    template hipError_t launch_getrf<T>(const GetrfTask*, int, int, int, int*, hipStream_t);      \
     template hipError_t launch_schur<T>(const SchurTile*, int, const SchurContrib*, hipStream_t); \
     template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, hipStream_t); \
     template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);                        \

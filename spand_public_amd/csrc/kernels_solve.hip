@@ -19,6 +19,9 @@
 namespace spand {
 
 constexpr int NT = 256;
+// Segments are FP64 whatever the factor's element type T: an FP32 factor is read as FP32 and
+// applied in FP64 arithmetic, so the preconditioner stays one fixed linear operator.
+using VT = double;
 // Dynamic LDS of the solve kernels: a chunk of up to 8 right-hand sides of the segment (and
 // for the Householder apply two k-vectors per right-hand side); the chunk width is derived
 // from the segment size, and the host rejects segments that would not fit one column.
@@ -33,20 +36,20 @@ static inline int grid_for(size_t n) {
 // sx = 1, sr = ld; interleaved: sx = nrhs, sr = 1).
 template <typename T>
 __global__ void __launch_bounds__(NT) k_perm_in(const double* b, size_t sx, size_t sr, int nrhs, const int* perm,
-                                                int N, T* ws) {
+                                                int N, VT* ws) {
     size_t n = (size_t)N * nrhs;
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.x * NT) {
         int i = (int)(idx / nrhs), r = (int)(idx % nrhs);
-        ws[idx] = (T)b[(size_t)perm[i] * sx + (size_t)r * sr];
+        ws[idx] = b[(size_t)perm[i] * sx + (size_t)r * sr];
     }
 }
 template <typename T>
-__global__ void __launch_bounds__(NT) k_perm_out(const T* ws, const int* perm, int N, int nrhs, double* x, size_t sx,
+__global__ void __launch_bounds__(NT) k_perm_out(const VT* ws, const int* perm, int N, int nrhs, double* x, size_t sx,
                                                  size_t sr) {
     size_t n = (size_t)N * nrhs;
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.x * NT) {
         int i = (int)(idx / nrhs), r = (int)(idx % nrhs);
-        x[(size_t)perm[i] * sx + (size_t)r * sr] = (double)ws[idx];
+        x[(size_t)perm[i] * sx + (size_t)r * sr] = ws[idx];
     }
 }
 
@@ -85,33 +88,34 @@ constexpr int SC = SOLVE_CHUNK;
 constexpr int NP = NT / SC;  // 4
 constexpr int TR = 32;       // rows of a transposed tile
 
-// y = Linv x (trans 0) or Linv^T x (trans 1) for chunk [o0, o0 + 64) of a ScalingLLT
+// y = Linv x[perm] (trans 0) or Lb^T x (trans 1) for chunk [o0, o0 + 64) of a scaling
 // segment, written to wt (the copy-back kernel returns it to ws: other chunks of the same
-// task still read the old x). Linv is zero above its diagonal, so the forward sum runs
-// over all j < min(n, o0 + 64) without a mask.
+// task still read the old x). Linv and Lb are zero above their diagonals, so the forward
+// sum runs over all j < min(n, o0 + 64) without a mask.
 template <typename T>
-__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, const SolveChunk* chunks, const T* ws, T* wt,
+__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, const SolveChunk* chunks, const VT* ws, VT* wt,
                                              int nrhs, int trans) {
     __shared__ T tile[TR][SC + 1];
-    __shared__ T red[NP][SC][RG_MAX];
+    __shared__ VT red[NP][SC][RG_MAX];
     const SolveChunk ch = chunks[blockIdx.x];
     const TrsvTask t = tasks[ch.task];
     const int n = t.n, o0 = ch.o0, no = min(SC, n - o0);
     const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
-    const T* L = reinterpret_cast<const T*>(t.L);
-    const T* x = ws + (size_t)t.xoff * nrhs;
+    const T* L = reinterpret_cast<const T*>(trans ? t.Lb : t.L);
+    const int* P = t.perm;
+    const VT* x = ws + (size_t)t.xoff * nrhs;
     for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
         const int rg = min(RG_MAX, nrhs - r0);
-        T acc[RG_MAX];
+        VT acc[RG_MAX];
 #pragma unroll
-        for (int r = 0; r < RG_MAX; r++) acc[r] = T(0);
+        for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
         if (!trans) {
             const int i = o0 + lane;
             const int jmax = min(n, o0 + SC);
             if (lane < no) {
                 for (int j = kp; j < jmax; j += NP) {
-                    const T l = L[i + (size_t)j * n];
-                    const T* xj = x + (size_t)j * nrhs + r0;
+                    const VT l = L[i + (size_t)j * n];
+                    const VT* xj = x + (size_t)(P ? P[j] : j) * nrhs + r0;
                     for (int r = 0; r < rg; r++) acc[r] += l * xj[r];
                 }
             }
@@ -124,8 +128,8 @@ __global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, const SolveC
                 }
                 __syncthreads();
                 for (int ii = kp; ii < ni; ii += NP) {
-                    const T l = tile[ii][lane];
-                    const T* xi = x + (size_t)(i0 + ii) * nrhs + r0;
+                    const VT l = tile[ii][lane];
+                    const VT* xi = x + (size_t)(i0 + ii) * nrhs + r0;
                     for (int r = 0; r < rg; r++) acc[r] += l * xi[r];
                 }
                 __syncthreads();
@@ -134,7 +138,7 @@ __global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, const SolveC
         for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
         __syncthreads();
         if (kp == 0 && lane < no) {
-            T* y = wt + (size_t)(t.xoff + o0 + lane) * nrhs + r0;
+            VT* y = wt + (size_t)(t.xoff + o0 + lane) * nrhs + r0;
             for (int r = 0; r < rg; r++) y[r] = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
         }
         __syncthreads();
@@ -143,7 +147,7 @@ __global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, const SolveC
 
 // ws[seg] = wt[seg] for the segments of a trmv launch, one workgroup per chunk
 template <typename T>
-__global__ void __launch_bounds__(NT) k_copyback(const TrsvTask* tasks, const SolveChunk* chunks, T* ws, const T* wt,
+__global__ void __launch_bounds__(NT) k_copyback(const TrsvTask* tasks, const SolveChunk* chunks, VT* ws, const VT* wt,
                                                  int nrhs) {
     const SolveChunk ch = chunks[blockIdx.x];
     const TrsvTask t = tasks[ch.task];
@@ -156,29 +160,29 @@ __global__ void __launch_bounds__(NT) k_copyback(const TrsvTask* tasks, const So
 // Destinations and sources of one launch are disjoint segments, so the update is in place.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const SolveChunk* chunks, const GemvTerm* terms,
-                                             T* ws, int nrhs) {
+                                             VT* ws, int nrhs) {
     __shared__ T tile[TR][SC + 1];
-    __shared__ T red[NP][SC][RG_MAX];
+    __shared__ VT red[NP][SC][RG_MAX];
     const SolveChunk ch = chunks[blockIdx.x];
     const GemvGroup g = groups[ch.task];
     const int o0 = ch.o0, no = min(SC, g.n - o0);
     const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
     for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
         const int rg = min(RG_MAX, nrhs - r0);
-        T acc[RG_MAX];
+        VT acc[RG_MAX];
 #pragma unroll
-        for (int r = 0; r < RG_MAX; r++) acc[r] = T(0);
+        for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
         for (int k = g.k0; k < g.k1; k++) {
             const GemvTerm tm = terms[k];
             const T* A = reinterpret_cast<const T*>(tm.A);
-            const T* xs = ws + (size_t)tm.xsrc * nrhs + r0;
+            const VT* xs = ws + (size_t)tm.xsrc * nrhs + r0;
             if (!tm.trans) {
                 // output i = row of A (rows = destination length), sum over columns j
                 if (lane < no) {
                     const int i = o0 + lane;
                     for (int j = kp; j < tm.cols; j += NP) {
-                        const T a = A[i + (size_t)j * tm.rows];
-                        const T* xj = xs + (size_t)j * nrhs;
+                        const VT a = A[i + (size_t)j * tm.rows];
+                        const VT* xj = xs + (size_t)j * nrhs;
                         for (int r = 0; r < rg; r++) acc[r] += a * xj[r];
                     }
                 }
@@ -192,8 +196,8 @@ __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const Solv
                     }
                     __syncthreads();
                     for (int ii = kp; ii < ni; ii += NP) {
-                        const T a = tile[ii][lane];
-                        const T* xi = xs + (size_t)(i0 + ii) * nrhs;
+                        const VT a = tile[ii][lane];
+                        const VT* xi = xs + (size_t)(i0 + ii) * nrhs;
                         for (int r = 0; r < rg; r++) acc[r] += a * xi[r];
                     }
                     __syncthreads();
@@ -203,7 +207,7 @@ __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const Solv
         for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
         __syncthreads();
         if (kp == 0 && lane < no) {
-            T* y = ws + (size_t)(g.xdst + o0 + lane) * nrhs + r0;
+            VT* y = ws + (size_t)(g.xdst + o0 + lane) * nrhs + r0;
             for (int r = 0; r < rg; r++) y[r] -= red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
         }
         __syncthreads();
@@ -217,15 +221,15 @@ __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const Solv
 // W = V_b^T x is accumulated over 64-row tiles of V_b staged in LDS (32 reflectors x 8
 // row partitions).
 template <typename T>
-__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nrhs, int trans) {
+__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, VT* ws, int nrhs, int trans) {
     __shared__ T vt[SC][QB + 1];
-    __shared__ T wred[NT / QB][QB][RG_MAX];
-    __shared__ T W[QB][RG_MAX], U[QB][RG_MAX];
+    __shared__ VT wred[NT / QB][QB][RG_MAX];
+    __shared__ VT W[QB][RG_MAX], U[QB][RG_MAX];
     const QTask t = tasks[blockIdx.x];
     const int m = t.m, k = t.k, tid = threadIdx.x;
     if (m == 0 || k == 0) return;
     const T* V = reinterpret_cast<const T*>(t.V);
-    T* x = ws + (size_t)t.xoff * nrhs;
+    VT* x = ws + (size_t)t.xoff * nrhs;
     const int nblk = (k + QB - 1) / QB;
     const int c = tid % QB, part = tid / QB, nparts = NT / QB;
     for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
@@ -235,9 +239,9 @@ __global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nr
             const int b0 = b * QB, nb = min(QB, k - b0);
             const T* Tb = reinterpret_cast<const T*>(t.Tm) + (size_t)b * QB * QB;
             // W = V_b^T x with v_c = [1 at row b0 + c, V below]
-            T acc[RG_MAX];
+            VT acc[RG_MAX];
 #pragma unroll
-            for (int r = 0; r < RG_MAX; r++) acc[r] = T(0);
+            for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
             for (int i0 = b0; i0 < m; i0 += SC) {
                 const int ni = min(SC, m - i0);
                 for (int idx = tid; idx < SC * QB; idx += NT) {
@@ -248,8 +252,8 @@ __global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nr
                 }
                 __syncthreads();
                 for (int ii = part; ii < ni; ii += nparts) {
-                    const T v = vt[ii][c];
-                    const T* xi = x + (size_t)(i0 + ii) * nrhs + r0;
+                    const VT v = vt[ii][c];
+                    const VT* xi = x + (size_t)(i0 + ii) * nrhs + r0;
                     for (int r = 0; r < rg; r++) acc[r] += v * xi[r];
                 }
                 __syncthreads();
@@ -258,7 +262,7 @@ __global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nr
             __syncthreads();
             if (tid < nb * rg) {
                 const int cc = tid / rg, r = tid % rg;
-                T s = T(0);
+                VT s = VT(0);
                 for (int p = 0; p < nparts; p++) s += wred[p][cc][r];
                 W[cc][r] = s;
             }
@@ -266,11 +270,11 @@ __global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nr
             // U = T_b^T W (Q^T) or T_b W (Q); T_b is nb x nb upper triangular, ld QB
             if (tid < nb * rg) {
                 const int a = tid / rg, r = tid % rg;
-                T s = T(0);
+                VT s = VT(0);
                 if (trans) {
-                    for (int q = 0; q <= a; q++) s += Tb[q + (size_t)a * QB] * W[q][r];
+                    for (int q = 0; q <= a; q++) s += (VT)Tb[q + (size_t)a * QB] * W[q][r];
                 } else {
-                    for (int q = a; q < nb; q++) s += Tb[a + (size_t)q * QB] * W[q][r];
+                    for (int q = a; q < nb; q++) s += (VT)Tb[a + (size_t)q * QB] * W[q][r];
                 }
                 U[a][r] = s;
             }
@@ -279,9 +283,9 @@ __global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, T* ws, int nr
             for (int idx = tid; idx < (m - b0) * rg; idx += NT) {
                 const int i = b0 + idx / rg, r = idx % rg;
                 const int li = i - b0;
-                T s = li < nb ? U[li][r] : T(0);
+                VT s = li < nb ? U[li][r] : VT(0);
                 const int cmax = min(li, nb);
-                for (int cc = 0; cc < cmax; cc++) s += V[i + (size_t)(b0 + cc) * m] * U[cc][r];
+                for (int cc = 0; cc < cmax; cc++) s += (VT)V[i + (size_t)(b0 + cc) * m] * U[cc][r];
                 x[(size_t)i * nrhs + r0 + r] -= s;
             }
             __syncthreads();
@@ -356,14 +360,14 @@ __global__ void __launch_bounds__(NT) k_larft(const LarftTask* tasks) {
 
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(NT) k_merge(const MergeTask* tasks, const int2* children, T* ws, int nrhs, int fwd) {
+__global__ void __launch_bounds__(NT) k_merge(const MergeTask* tasks, const int2* children, VT* ws, int nrhs, int fwd) {
     const MergeTask t = tasks[blockIdx.x];
     int k = 0;
     for (int c = t.c0; c < t.c1; c++) {
         const int2 ch = children[c];  // (xoff, size)
         const int n = ch.y * nrhs;
-        T* P = ws + (size_t)(t.pxoff + k) * nrhs;
-        T* Cx = ws + (size_t)ch.x * nrhs;
+        VT* P = ws + (size_t)(t.pxoff + k) * nrhs;
+        VT* Cx = ws + (size_t)ch.x * nrhs;
         if (fwd)
             for (int i = threadIdx.x; i < n; i += NT) P[i] = Cx[i];
         else
@@ -505,13 +509,13 @@ __global__ void __launch_bounds__(NT) k_copy_f64(double* dst, const double* src,
 
 // ---------------------------------------------------------------------------------------
 template <typename T>
-hipError_t launch_perm_in(const double* b, size_t sx, size_t sr, int nrhs, const int* perm, int N, T* ws,
+hipError_t launch_perm_in(const double* b, size_t sx, size_t sr, int nrhs, const int* perm, int N, double* ws,
                           hipStream_t st) {
     k_perm_in<T><<<grid_for((size_t)N * nrhs), NT, 0, st>>>(b, sx, sr, nrhs, perm, N, ws);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_perm_out(const T* ws, const int* perm, int N, int nrhs, double* x, size_t sx, size_t sr,
+hipError_t launch_perm_out(const double* ws, const int* perm, int N, int nrhs, double* x, size_t sx, size_t sr,
                            hipStream_t st) {
     k_perm_out<T><<<grid_for((size_t)N * nrhs), NT, 0, st>>>(ws, perm, N, nrhs, x, sx, sr);
     return hipGetLastError();
@@ -527,7 +531,7 @@ hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld,
 // LDS for one RG chunk of the largest segment in a launch is sized by the caller through
 // max_n; the launchers take the largest dimension from the task arrays' host copies.
 template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, T* ws, T* wt, int nrhs,
+hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, double* ws, double* wt, int nrhs,
                        int trans, hipStream_t st) {
     if (nchunks == 0) return hipSuccess;
     k_trmv<T><<<nchunks, NT, 0, st>>>(d_tasks, d_chunks, ws, wt, nrhs, trans);
@@ -537,14 +541,14 @@ hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int 
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms, T* ws,
+hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms, double* ws,
                        int nrhs, hipStream_t st) {
     if (nchunks == 0) return hipSuccess;
     k_gemv<T><<<nchunks, NT, 0, st>>>(d_groups, d_chunks, d_terms, ws, nrhs);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, T* ws, int nrhs, int trans, hipStream_t st) {
+hipError_t launch_qapply(const QTask* d_tasks, int ntasks, double* ws, int nrhs, int trans, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
     k_qapply<T><<<ntasks, NT, 0, st>>>(d_tasks, ws, nrhs, trans);
     return hipGetLastError();
@@ -556,7 +560,7 @@ hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st) {
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, T* ws, int nrhs, int fwd,
+hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, double* ws, int nrhs, int fwd,
                         hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
     k_merge<T><<<ntasks, NT, 0, st>>>(d_tasks, d_children, ws, nrhs, fwd);
@@ -609,14 +613,14 @@ hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t
 }
 
 #define SPAND_SOLVE_INST(T)                                                                                  \
-    template hipError_t launch_perm_in<T>(const double*, size_t, size_t, int, const int*, int, T*, hipStream_t); \
-    template hipError_t launch_perm_out<T>(const T*, const int*, int, int, double*, size_t, size_t, hipStream_t); \
-    template hipError_t launch_trmv<T>(const TrsvTask*, const SolveChunk*, int, T*, T*, int, int, hipStream_t); \
+    template hipError_t launch_perm_in<T>(const double*, size_t, size_t, int, const int*, int, double*, hipStream_t); \
+    template hipError_t launch_perm_out<T>(const double*, const int*, int, int, double*, size_t, size_t, hipStream_t); \
+    template hipError_t launch_trmv<T>(const TrsvTask*, const SolveChunk*, int, double*, double*, int, int, hipStream_t); \
     template hipError_t launch_larft<T>(const LarftTask*, int, hipStream_t);                                \
-    template hipError_t launch_gemv<T>(const GemvGroup*, const SolveChunk*, int, const GemvTerm*, T*, int,    \
+    template hipError_t launch_gemv<T>(const GemvGroup*, const SolveChunk*, int, const GemvTerm*, double*, int, \
                                        hipStream_t);                                                        \
-    template hipError_t launch_qapply<T>(const QTask*, int, T*, int, int, hipStream_t);                     \
-    template hipError_t launch_merge<T>(const MergeTask*, int, const int2*, T*, int, int, hipStream_t);
+    template hipError_t launch_qapply<T>(const QTask*, int, double*, int, int, hipStream_t);                \
+    template hipError_t launch_merge<T>(const MergeTask*, int, const int2*, double*, int, int, hipStream_t);
 SPAND_SOLVE_INST(float)
 SPAND_SOLVE_INST(double)
 

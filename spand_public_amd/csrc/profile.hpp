@@ -31,6 +31,7 @@ enum KernelFamily {
     KF_MERGE,
     KF_SPMV,
     KF_KRYLOV_VEC,
+    KF_GETRF,
     KF_COUNT
 };
 

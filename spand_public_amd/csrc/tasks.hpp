@@ -20,6 +20,9 @@ struct PotrfTask {
 //   side 0: B (n x m, ld ldb) := L^{-1} B      (trsm_left,  src/util.cpp:266-274)
 //   side 1: B (m x n, ld ldb) := B L^{-T}      (trsm_right, src/util.cpp:256-264)
 // m <= 64: the host splits wide blocks into independent column (left) / row (right) chunks.
+// A left solve may read its right-hand side from another block: B := L^{-1} P^T src, row i
+// of P^T src being row perm[i] of src (perm nullptr: the identity), the row interchanges of
+// an LU pivot applied to an in-edge (trsm_getf_edgeIn, src/tree.cpp:668-677).
 struct TrsmTask {
     const void* L;
     void* B;
@@ -27,6 +30,20 @@ struct TrsmTask {
     int m;
     int side;
     int ldb;
+    const void* src;  // nullptr: in place
+    const int* perm;
+};
+
+// LU with partial pivoting of an n x n pivot block (getf, src/util.cpp:184-199, in dgetf2
+// order) followed by split_LU (src/util.cpp:213-227). On return A holds L (lower, diagonal
+// sqrt|d_j|, zero above), Ut holds U^T (lower, zero above) and perm the row permutation p
+// with P^T A = L U (row i of P^T A is row p[i] of A).
+struct GetrfTask {
+    void* A;
+    void* Ut;
+    int* perm;
+    int n;
+    int cluster;  // reported through the info array on a zero pivot
 };
 
 // One Schur-complement target tile: C[r0:r0+tm, c0:c0+tn] of a block C (ldc rows) receives
@@ -102,8 +119,13 @@ struct CopyTask {
 // ---------------------------------------------------------------------------------------
 // Preconditioner apply (Tree::solve, src/tree.cpp:1610-1635). Vectors are stored
 // interleaved: element i of right-hand side r lives at x[(xoff + i) * nrhs + r].
-struct TrsvTask {  // ScalingLLT (src/operations.cpp:95-103): x := L^{-1} x (fwd) / L^{-T} x (bwd)
-    const void* L;  // the explicit inverse L^{-1} (n x n, lower), computed at factorization
+// ScalingLLT (src/operations.cpp:95-103): x := L^{-1} x (fwd) / L^{-T} x (bwd).
+// ScalingPLUQ with Q = I (src/operations.cpp:114-121): x := L^{-1} P^T x (fwd) / U^{-1} x
+// (bwd), with Lb = (U^T)^{-1} so that the backward sweep is again Lb^T x.
+struct TrsvTask {
+    const void* L;     // the explicit inverse L^{-1} (n x n, lower), computed at factorization
+    const void* Lb;    // backward factor (lower): L^{-1} again for LLT, U^{-T} for PLU
+    const int* perm;   // forward gather x_j := x[perm[j]] (nullptr: none)
     int n;
     int xoff;
 };

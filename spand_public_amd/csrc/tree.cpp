@@ -108,8 +108,10 @@ Tree::~Tree() {
     (void)hipHostFree(h_stage_);
 }
 
+// SymmKind (include/spaND.h): SPD (0) with ScalingKind::LLT, GEN (2) with ScalingKind::PLU.
+// SYM (1, LDLT) is not built.
 void Tree::set_symm_kind(int k) {
-    if (k != 0) throw Error(ERR_UNSUPPORTED, "only the SPD / LLT path is available");
+    if (k != 0 && k != 2) throw Error(ERR_UNSUPPORTED, "only SPD (LLT) and GEN (PLU) are available");
     symm_ = k;
 }
 void Tree::set_Xcoo(int dim, int N, const double* X) {
@@ -336,17 +338,22 @@ void Tree::factorize() {
 }
 
 static inline double potrf_flops(double n) { return n * n * n / 3.0; }
+static inline double getrf_flops(double n) { return 2.0 * n * n * n / 3.0; }
 
 // Splits a triangular solve into independent chunks of at most 64 columns (left solve,
 // block n x m, ld n) or 64 rows (right solve, block m x n, ld m); 32 / 16 for n >= 128 / 256.
-static void push_trsm(std::vector<TrsmTask>& v, const void* L, void* B, int n, int m, int side, size_t esz) {
+// A left solve may read its right-hand side rows permuted from another block (src, perm).
+static void push_trsm(std::vector<TrsmTask>& v, const void* L, void* B, int n, int m, int side, size_t esz,
+                      const void* src = nullptr, const int* perm = nullptr) {
     if (n == 0 || m == 0) return;
     // long triangles walk many row blocks per chunk: narrower chunks give more workgroups
     const int cw = n >= 256 ? 16 : (n >= 128 ? 32 : 64);
     for (int c0 = 0; c0 < m; c0 += cw) {
         const int w = std::min(cw, m - c0);
-        char* b = reinterpret_cast<char*>(B) + (side == 0 ? (size_t)c0 * n : (size_t)c0) * esz;
-        v.push_back({L, b, n, w, side, side == 0 ? n : m});
+        const size_t off = (side == 0 ? (size_t)c0 * n : (size_t)c0) * esz;
+        char* b = reinterpret_cast<char*>(B) + off;
+        const char* sb = src ? reinterpret_cast<const char*>(src) + off : nullptr;
+        v.push_back({L, b, n, w, side, side == 0 ? n : m, sb, perm});
     }
 }
 
@@ -446,6 +453,7 @@ void Tree::factorize_impl() {
     const int lds_elems = LDS_BYTES / (int)sizeof(T);
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
     const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 16384);
+    const bool gen = symm_ == 2;
 
     for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
         LevelStats& lst = stats_[ilvl_];
@@ -454,6 +462,7 @@ void Tree::factorize_impl() {
         const std::vector<int>& bot = bottoms_[current_bottom_];
         // ---------------- Elim (src/tree.cpp:1494-1506, 895-967) ----------------
         std::vector<PotrfTask> potrfE, potrfS;
+        std::vector<GetrfTask> getrfE, getrfS;
         std::vector<TrsmTask> trsmE, trsmA, trsmB;
         struct PC { int target; SchurContrib c; };
         std::vector<PC> pcs;
@@ -483,6 +492,69 @@ void Tree::factorize_impl() {
             // edges_ may grow (fill-in) below: keep the pivot block pointer, not a reference
             const HEdge piv = edges_[s.out[0]];
             if (piv.ident) throw Error(ERR_STATE, "internal: identity pivot on an eliminated cluster");
+            if (gen) {
+                // getf_cluster + panel_getf + gemm_edges, PLU branch (src/tree.cpp:929-956)
+                void* Ut = arena_.alloc((size_t)std::max(1, n * n) * sizeof(T));
+                int* pp = reinterpret_cast<int*>(arena_.alloc((size_t)std::max(1, n) * sizeof(int)));
+                getrfE.push_back({piv.A, Ut, pp, n, c});
+                lst.flops_potrf += getrf_flops(n);
+                std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
+                for (int e : in) {  // A_sn := L^{-1} P^T A_sn, into a fresh block
+                    const int m = size(edges_[e].n1);
+                    void* nb = arena_.alloc((size_t)std::max(1, n * m) * sizeof(T));
+                    push_trsm(trsmE, piv.A, nb, n, m, 0, sizeof(T), edges_[e].A, pp);
+                    edges_[e].A = nb;
+                    lst.flops_trsm += (double)n * n * m;
+                }
+                for (int e : out) {  // A_ns := A_ns U^{-1} = A_ns (U^T)^{-T}
+                    push_trsm(trsmE, Ut, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
+                    lst.flops_trsm += (double)n * n * size(edges_[e].n2);
+                }
+                for (int e1 : out)
+                    for (int e2 : in) {
+                        int n1 = edges_[e1].n2, n2 = edges_[e2].n1;
+                        int tgt = target_edge(n1, n2);
+                        SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
+                        pcs.push_back({tgt, sc});
+                        lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
+                    }
+                // record ScalingPLUQ + GemmOut / GemmIn (src/tree.cpp:945-951)
+                ops_.push_back({OP_SCALING_PLUQ, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr, Ut, pp});
+                if (n > 0) {
+                    void* Li = arena_.alloc((size_t)n * n * sizeof(T));
+                    void* Wi = arena_.alloc((size_t)n * n * sizeof(T));
+                    inv_init.push_back({nullptr, Li, n, n, n, 0});
+                    inv_init.push_back({nullptr, Wi, n, n, n, 0});
+                    push_trsm(trsmE, piv.A, Li, n, n, 0, sizeof(T));
+                    push_trsm(trsmE, Ut, Wi, n, n, 0, sizeof(T));
+                    lst.flops_trsm += 2.0 * n * n * n;
+                    sv_elim.push_back({Li, Wi, pp, n, (int)s.xoff});
+                }
+                factor_bytes_ += (size_t)n * (n + 1) * sizeof(T) + (size_t)n * sizeof(int);
+                std::vector<GemvTerm> mine;
+                for (int e : out) {
+                    int nb = edges_[e].n2;
+                    ops_.push_back({OP_GEMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
+                    factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+                    if (n > 0 && size(nb) > 0) {
+                        if (!fwd_terms.count(nb)) fwd_order.push_back(nb);
+                        fwd_terms[nb].push_back({edges_[e].A, size(nb), n, (int)s.xoff, 0});
+                    }
+                }
+                for (int e : in) {
+                    int nb = edges_[e].n1;
+                    ops_.push_back({OP_GEMM_IN, c, nb, n, size(nb), n, size(nb), ilvl_, edges_[e].A, nullptr});
+                    factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+                    if (n > 0 && size(nb) > 0) mine.push_back({edges_[e].A, n, size(nb), (int)cls_[nb].xoff, 0});
+                }
+                if (!mine.empty()) {
+                    int k0 = (int)bwd_terms.size();
+                    for (auto it = mine.rbegin(); it != mine.rend(); ++it) bwd_terms.push_back(*it);
+                    bwd_groups.push_back({(int)s.xoff, n, k0, (int)bwd_terms.size()});
+                }
+                set_eliminated(c);
+                continue;
+            }
             potrfE.push_back({piv.A, n, c});
             lst.flops_potrf += potrf_flops(n);
             std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
@@ -529,7 +601,7 @@ void Tree::factorize_impl() {
                 inv_init.push_back({nullptr, Li, n, n, n, 0});
                 push_trsm(trsmE, piv.A, Li, n, n, 0, sizeof(T));
                 lst.flops_trsm += (double)n * n * n;
-                sv_elim.push_back({Li, n, (int)s.xoff});
+                sv_elim.push_back({Li, Li, nullptr, n, (int)s.xoff});
             }
             factor_bytes_ += (size_t)n * (n + 1) / 2 * sizeof(T);
             std::vector<GemvTerm> mine;
@@ -588,7 +660,7 @@ void Tree::factorize_impl() {
                 const int k0 = (int)contribs.size();
                 for (int k : by_target[tg]) contribs.push_back(pcs[k].c);
                 const int k1 = (int)contribs.size();
-                const int lower = e.n1 == e.n2 ? 1 : 0;
+                const int lower = !gen && e.n1 == e.n2 ? 1 : 0;  // syrk on symmetric diagonal targets
                 for (int c0 = 0; c0 < nn; c0 += 32)
                     for (int r0 = 0; r0 < m; r0 += 32) {
                         int tm = std::min(32, m - r0), tn = std::min(32, nn - c0);
@@ -606,7 +678,58 @@ void Tree::factorize_impl() {
         std::vector<int> live;
         for (int c : bot)
             if (cls_[c].level > ilvl_) live.push_back(c);
-        if (do_sparsify_level) {
+        if (do_sparsify_level && gen) {
+            // scale_cluster, PLU branch (src/tree.cpp:831-849): an edge a -> b (block
+            // size(b) x size(a)) receives A U_a^{-1} from a and L_b^{-1} P_b^T A from b, in
+            // the order of a and b in the bottom list, into a fresh block at b's step
+            std::unordered_map<int, std::pair<void*, int*>> lu;  // cluster -> (U^T, perm)
+            for (int c : live) {
+                HCluster& s = cls_[c];
+                const int n = s.size;
+                const HEdge& piv = edges_[s.out[0]];
+                void* Ut = arena_.alloc((size_t)std::max(1, n * n) * sizeof(T));
+                int* pp = reinterpret_cast<int*>(arena_.alloc((size_t)std::max(1, n) * sizeof(int)));
+                lu[c] = {Ut, pp};
+                getrfS.push_back({piv.A, Ut, pp, n, c});
+                lst.flops_potrf += getrf_flops(n);
+                ops_.push_back({OP_SCALING_PLUQ, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr, Ut, pp});
+                factor_bytes_ += (size_t)n * (n + 1) * sizeof(T) + (size_t)n * sizeof(int);
+                if (n > 0) {
+                    void* Li = arena_.alloc((size_t)n * n * sizeof(T));
+                    void* Wi = arena_.alloc((size_t)n * n * sizeof(T));
+                    inv_init.push_back({nullptr, Li, n, n, n, 0});
+                    inv_init.push_back({nullptr, Wi, n, n, n, 0});
+                    push_trsm(trsmA, piv.A, Li, n, n, 0, sizeof(T));
+                    push_trsm(trsmA, Ut, Wi, n, n, 0, sizeof(T));
+                    lst.flops_trsm += 2.0 * n * n * n;
+                    sv_scale.push_back({Li, Wi, pp, n, (int)s.xoff});
+                }
+            }
+            for (int c : live) {
+                for (size_t k = 1; k < cls_[c].out.size(); k++) {
+                    HEdge& e = edges_[cls_[c].out[k]];
+                    const int a = e.n1, b = e.n2;  // block is size(b) x size(a)
+                    const void* Lb = edges_[cls_[b].out[0]].A;
+                    lst.flops_trsm += (double)size(a) * size(b) * (size(a) + size(b));
+                    const bool a_first = cls_[a].order < cls_[b].order;
+                    void* nb = arena_.alloc((size_t)std::max(1, size(a) * size(b)) * sizeof(T));
+                    if (a_first) {
+                        push_trsm(trsmA, lu[a].first, e.A, size(a), size(b), 1, sizeof(T));
+                        push_trsm(trsmB, Lb, nb, size(b), size(a), 0, sizeof(T), e.A, lu[b].second);
+                    } else {
+                        push_trsm(trsmA, Lb, nb, size(b), size(a), 0, sizeof(T), e.A, lu[b].second);
+                        push_trsm(trsmB, lu[a].first, nb, size(a), size(b), 1, sizeof(T));
+                    }
+                    // a zero-sized block keeps its (unread) pointer
+                    if (size(a) > 0 && size(b) > 0) e.A = nb;
+                }
+            }
+            for (int c : live) {
+                HEdge& piv = edges_[cls_[c].out[0]];
+                piv.A = nullptr;
+                piv.ident = true;
+            }
+        } else if (do_sparsify_level) {
             for (int c : live) {
                 HCluster& s = cls_[c];
                 const HEdge& piv = edges_[s.out[0]];
@@ -619,7 +742,7 @@ void Tree::factorize_impl() {
                     inv_init.push_back({nullptr, Li, s.size, s.size, s.size, 0});
                     push_trsm(trsmA, piv.A, Li, s.size, s.size, 0, sizeof(T));
                     lst.flops_trsm += (double)s.size * s.size * s.size;
-                    sv_scale.push_back({Li, s.size, (int)s.xoff});
+                    sv_scale.push_back({Li, Li, nullptr, s.size, (int)s.xoff});
                 }
             }
             for (int c : live) {
@@ -771,6 +894,15 @@ void Tree::factorize_impl() {
         size_t o_pe = pk.add(potrfE), o_te = pk.add(trsmE), o_ti = pk.add(tiles), o_co = pk.add(contribs);
         size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
         size_t o_ii = pk.add(inv_init);
+        // LU pivots that fit LDS first (one launch), the larger ones after (another)
+        auto lds_first = [&](std::vector<GetrfTask>& v) {
+            std::stable_partition(v.begin(), v.end(), [&](const GetrfTask& t) {
+                return (size_t)t.n * (t.n + 1) * sizeof(T) <= (size_t)GETRF_LDS_BYTES;
+            });
+        };
+        lds_first(getrfE);
+        lds_first(getrfS);
+        size_t o_ge = pk.add(getrfE), o_gs = pk.add(getrfS);
         std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves);
         for (int w = 0; w < nwaves; w++) {
             o_w[w] = pk.add(wtasks[w]);
@@ -844,7 +976,35 @@ void Tree::factorize_impl() {
                             [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), st); }),
                       what);
         };
+        auto run_getrf = [&](const std::vector<GetrfTask>& v, size_t off, const char* what) {
+            if (v.empty()) return;
+            int nl = 0, nmax_l = 0;
+            double lb = 0, lf = 0, gb = 0, gf = 0;
+            for (auto& t : v) {
+                const double n = t.n;
+                const double by = (3.0 * n * n) * esz + 4.0 * n, fl = getrf_flops(n);
+                if ((size_t)t.n * (t.n + 1) * esz <= (size_t)GETRF_LDS_BYTES) {
+                    nl++;
+                    nmax_l = std::max(nmax_l, t.n);
+                    lb += by;
+                    lf += fl;
+                } else {
+                    gb += by;
+                    gf += fl;
+                }
+            }
+            const GetrfTask* d = Packer::at<GetrfTask>(dt, off);
+            if (nl > 0)
+                hip_check(timed(prof_, KF_GETRF, st, lb, lf,
+                                [&] { return launch_getrf<T>(d, nl, 1, nmax_l, d_info_, st); }),
+                          what);
+            if (nl < (int)v.size())
+                hip_check(timed(prof_, KF_GETRF, st, gb, gf,
+                                [&] { return launch_getrf<T>(d + nl, (int)v.size() - nl, 0, 0, d_info_, st); }),
+                          what);
+        };
         run_potrf(potrfE, o_pe, "potrf elim");
+        run_getrf(getrfE, o_ge, "getrf elim");
         run_trsm(trsmE, o_te, "trsm elim");
         if (!tiles.empty()) {
             Work w = work_schur(tiles, contribs, esz);
@@ -856,6 +1016,7 @@ void Tree::factorize_impl() {
                       "schur");
         }
         run_potrf(potrfS, o_ps, "potrf scale");
+        run_getrf(getrfS, o_gs, "getrf scale");
         run_trsm(trsmA, o_ta, "trsm scale A");
         run_trsm(trsmB, o_tb, "trsm scale B");
         // stepped-path launch slots, for the post-hoc work attribution below
@@ -914,8 +1075,10 @@ void Tree::factorize_impl() {
         }
         int info = 0;
         hip_check(hipMemcpy(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost), "read info");
-        if (info != 0)
+        if (info != 0) {
+            if (gen) throw Error(ERR_SINGULAR, "Error: Singular Pivot (cluster order " + std::to_string(info - 1) + ")");
             throw Error(ERR_NOT_SPD, "Error: Non-SPD Pivot (cluster order " + std::to_string(info - 1) + ")");
+        }
         if (!plist.empty())
             hip_check(hipMemcpy(hsize.data(), d_size_, cls_.size() * sizeof(int), hipMemcpyDeviceToHost), "read ranks");
         if (prof_.on()) {
@@ -986,7 +1149,8 @@ void Tree::factorize_impl() {
                 others_.push_back(sid);
                 cls_[c].size = r;
                 ops_.push_back({OP_SPLIT, c, sid, r, s0 - r, 0, 0, ilvl_, nullptr, nullptr});
-                ops_.push_back({OP_SCALING_LLT, sid, -1, s0 - r, 0, s0 - r, s0 - r, ilvl_, nullptr, nullptr});
+                ops_.push_back({gen ? OP_SCALING_PLUQ : OP_SCALING_LLT, sid, -1, s0 - r, 0, s0 - r, s0 - r, ilvl_,
+                                nullptr, nullptr});
                 // the sibling's pivot is the identity and is never stored (src/tree.cpp:1076-1099)
             }
             lst.rank_after += cls_[c].size;
@@ -1209,7 +1373,10 @@ long long Tree::nnz() const {
     for (auto& o : ops_) {
         switch (o.type) {
             case OP_SCALING_LLT: n += (long long)o.sa * (o.sa + 1) / 2; break;
-            case OP_GEMM_SYMM_OUT: case OP_GEMM_SYMM_IN: n += (long long)o.sa * o.sb; break;
+            case OP_SCALING_PLUQ: n += (long long)o.sa * o.sa + 2LL * o.sa; break;
+            case OP_GEMM_SYMM_OUT: case OP_GEMM_SYMM_IN: case OP_GEMM_OUT: case OP_GEMM_IN:
+                n += (long long)o.sa * o.sb;
+                break;
             case OP_ORTHOGONAL: n += (long long)o.sa * o.sa; break;
             default: break;
         }
@@ -1241,6 +1408,27 @@ void Tree::op_data(int i, double* M, double* tau) const {
     if (tau && o.tau && o.cols > 0) {
         if (prec_ == 0) d2h_as_double<double>(o.tau, o.cols, tau);
         else d2h_as_double<float>(o.tau, o.cols, tau);
+    }
+}
+
+void Tree::op_plu(int i, double* U, int* perm) const {
+    if (i < 0 || i >= (int)ops_.size()) throw Error(ERR_ARG, "op index out of range");
+    const OpRec& o = ops_[i];
+    if (o.type != OP_SCALING_PLUQ) throw Error(ERR_ARG, "op is not a ScalingPLUQ");
+    const int n = o.sa;
+    std::vector<double> ut((size_t)n * n, 0.0);
+    if (o.Ut) {
+        if (prec_ == 0) d2h_as_double<double>(o.Ut, ut.size(), ut.data());
+        else d2h_as_double<float>(o.Ut, ut.size(), ut.data());
+    } else {
+        for (int k = 0; k < n; k++) ut[k + (size_t)k * n] = 1.0;
+    }
+    if (U)
+        for (int c = 0; c < n; c++)
+            for (int r = 0; r < n; r++) U[r + (size_t)c * n] = ut[c + (size_t)r * n];
+    if (perm) {
+        if (o.perm && n > 0) hip_check(hipMemcpy(perm, o.perm, (size_t)n * sizeof(int), hipMemcpyDeviceToHost), "read perm");
+        else for (int k = 0; k < n; k++) perm[k] = k;
     }
 }
 
@@ -1340,21 +1528,22 @@ void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st) {
 template <typename T>
 void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st) {
     // ws: the segments; wt: scratch of the same layout for the triangular sweeps
-    const size_t need = 2 * (size_t)ws_len_ * nrhs * sizeof(T);
+    const size_t need = 2 * (size_t)ws_len_ * nrhs * sizeof(double);
     if (need > ws_cap_) {
         (void)hipFree(d_ws_);
         d_ws_ = nullptr;
         hip_check(hipMalloc(&d_ws_, need), "hipMalloc(solve workspace)");
         ws_cap_ = need;
     }
-    T* ws = reinterpret_cast<T*>(d_ws_);
-    T* wt = ws + (size_t)ws_len_ * nrhs;
-    const double esz = sizeof(T);
+    // the segments are FP64 whatever the factor's element type T
+    double* ws = reinterpret_cast<double*>(d_ws_);
+    double* wt = ws + (size_t)ws_len_ * nrhs;
+    const double vsz = sizeof(double);
     auto run = [&](int fam, const SweepWork& w, const char* what, auto&& launch) {
-        hip_check(timed(prof_, fam, st, w.mat_bytes + w.vec_elems * nrhs * esz, w.flops * nrhs, launch), what);
+        hip_check(timed(prof_, fam, st, w.mat_bytes + w.vec_elems * nrhs * vsz, w.flops * nrhs, launch), what);
     };
     SweepWork wp;
-    wp.vec_elems = (double)N_ * (1.0 + 8.0 / esz) + (double)N_ * 4.0 / esz / nrhs;
+    wp.vec_elems = (double)N_ * 2.0 + (double)N_ * 4.0 / vsz / nrhs;
     run(KF_PERM, wp, "perm in", [&] { return launch_perm_in<T>(dx, sx, sr, nrhs, d_perm_, N_, ws, st); });
     for (int l = 0; l < nlevels_; l++) {
         const LevelSolve& p = solve_plan_[l];

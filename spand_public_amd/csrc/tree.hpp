@@ -98,6 +98,8 @@ struct OpRec {
     int type, a, b, sa, sb, rows, cols, level;
     const void* M;
     const void* tau;
+    const void* Ut = nullptr;    // ScalingPLUQ: U^T (rows x rows, lower)
+    const int* perm = nullptr;   // ScalingPLUQ: row permutation p
 };
 
 // Algorithmic work of one solve launch: matrix bytes read once per apply, vector elements
@@ -170,6 +172,8 @@ class Tree {
     size_t arena_bytes() const { return arena_.bytes_in_use(); }
     // Copies op i's block (rows x cols, column-major) and tau (if any) to host, as FP64.
     void op_data(int i, double* M, double* tau) const;
+    // ScalingPLUQ op i: U (rows x rows, column-major, FP64) and the row permutation p.
+    void op_plu(int i, double* U, int* perm) const;
     // Trailing (current) matrix as COO triplets; returns the count (arrays may be null).
     long long trailing_coo(int* I, int* J, double* V) const;
     // Per-launch kernel timing (HIP events on the launch stream); off by default.
@@ -193,7 +197,7 @@ class Tree {
     double tol_ = 10.0;  // src/tree.cpp:94
     int skip_ = 0;
     bool verb_ = false;
-    int symm_ = 0;       // 0 SPD (LLT); 2 GEN (PLU) is not built yet
+    int symm_ = 0;       // 0 SPD (LLT), 2 GEN (PLU)
     bool geo_ = false;
     int dim_ = 0;
     std::vector<double> X_;

@@ -11,6 +11,12 @@ These produce the matrices spaND is run on, as CSC arrays (``colptr``, ``rowval`
   (src/util.cpp:488-517). Dof ``id`` of ``neglapl`` sits at ``linspace_nd[:, id]``; the
   Laplacian is axis-symmetric, so the reference's pairing of ``mats/*.mm`` (coordinate 0
   fastest) with ``linspace_nd`` describes the same matrix.
+* ``neglapl_unsym(n, d, seed)``: ``neglapl`` with every stored entry perturbed by
+  U(-0.1, 0.1) draws, the unsymmetric test matrix of the reference's tests
+  (tests/tests.cpp:98-109).
+* ``convdiff(n, d, pe, velocity)``: Dirichlet convection-diffusion, 7-point (5-point in
+  2D) diffusion plus first-order upwind convection, scaled by h^2 like ``neglapl``. No
+  reference counterpart (a nonsymmetric workload for the GEN / PLU path).
 * ``symmetric_graph(A)``: |A| + |A^T| + I pattern (src/util.cpp:47-63).
 * ``mmread(path)``: MatrixMarket coordinate reader with the reference's symmetric /
   hermitian expansion (include/mmio.hpp:160-219).
@@ -98,6 +104,71 @@ def neglapl(n: int, d: int) -> CSC:
             rows.append(idx[ok])
             cols.append(idx[ok] + sgn * stride)
             vals.append(np.full(int(ok.sum()), -1.0))
+        stride *= n
+    return CSC.from_coo(N, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+
+
+class MinStd0:
+    """std::minstd_rand0 (Lehmer, a = 16807, m = 2^31 - 1), the libstdc++
+    ``std::default_random_engine``, with libstdc++'s ``generate_canonical<double, 53>``
+    (two draws: (g1 - 1) + (g2 - 1) * r over r^2, r = 2^31 - 2, in double arithmetic)."""
+
+    M = 2147483647
+
+    def __init__(self, seed: int):
+        s = seed % self.M
+        self.x = s if s != 0 else 1
+
+    def __call__(self) -> int:
+        self.x = (16807 * self.x) % self.M
+        return self.x
+
+    def canonical(self) -> float:
+        r = 2147483646.0
+        s, t = 0.0, 1.0
+        for _ in range(2):
+            s += float(self() - 1) * t
+            t *= r
+        v = s / t
+        return v if v < 1.0 else float(np.nextafter(1.0, 0.0))
+
+    def uniform(self, a: float, b: float) -> float:
+        return self.canonical() * (b - a) + a
+
+
+def neglapl_unsym(n: int, d: int, seed: int) -> CSC:
+    """neglapl(n, d) + U(-0.1, 0.1) on every stored entry, drawn in column-major storage
+    order (tests/tests.cpp:98-109: default_random_engine seeded with ``seed``)."""
+    A = neglapl(n, d)
+    g = MinStd0(seed)
+    v = A.nzval.copy()
+    for k in range(len(v)):
+        v[k] += g.uniform(-0.1, 0.1)
+    return CSC(A.n, A.colptr, A.rowval, v)
+
+
+def convdiff(n: int, d: int, pe: float = 10.0, velocity=(1.0, 0.5, 0.25)) -> CSC:
+    """-Laplace(u) + w . grad(u) on an n^d grid, Dirichlet, times h^2 (h the grid step):
+    diagonal 2d + pe * sum(v), -1 per grid neighbour, and -pe * v_k more on the upwind
+    neighbour along axis k (w = pe * v / h; axis 0 is the slowest index, as in
+    ``linspace_nd``). Nonsymmetric for pe != 0, an M-matrix for velocities >= 0."""
+    v = np.asarray(velocity[:d], dtype=np.float64)
+    N = n ** d
+    idx = np.arange(N, dtype=np.int64)
+    coords = np.stack(np.unravel_index(idx, (n,) * d), axis=0)
+    rows = [idx]
+    cols = [idx]
+    vals = [np.full(N, 2.0 * d + pe * float(np.abs(v).sum()))]
+    stride = 1
+    for axis in range(d - 1, -1, -1):
+        c = coords[axis]
+        for sgn in (-1, 1):
+            ok = (c + sgn >= 0) & (c + sgn < n)
+            # upwind: the neighbour the flow comes from (behind for v > 0)
+            up = (v[axis] > 0 and sgn == -1) or (v[axis] < 0 and sgn == 1)
+            rows.append(idx[ok])
+            cols.append(idx[ok] + sgn * stride)
+            vals.append(np.full(int(ok.sum()), -1.0 - (pe * abs(v[axis]) if up else 0.0)))
         stride *= n
     return CSC.from_coo(N, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
 

@@ -1,0 +1,140 @@
+"""GPU parity tests of the GEN / PLU path (SymmKind::GEN, ScalingKind::PLU) against the CPU
+oracle, through the C ABI (include/spand.h).
+
+FP64: identical op sequence (ScalingPLUQ / GemmOut / GemmIn / Orthogonal / Merge / Split,
+cluster orders and sizes, i.e. identical kept ranks), every stored block within 1e-12 of the
+oracle's (L, U and tau; the kernels follow the oracle's element recurrences, so the
+observed difference is zero), identical row permutations, solves within 1e-12. FP32: solves
+within 1e-4 of the FP64 oracle. GMRES (src/is.cpp:123-300): iteration counts within 10 % of
+the oracle's and true ||b - Ax|| / ||b|| <= 1e-9 at tol 1e-10, for the FP64 and the FP32
+factorization (the FP32 factor is applied in FP64 arithmetic)."""
+import numpy as np
+import pytest
+
+import oracle
+import spand_public_amd as S
+from spand_public_amd.problems import convdiff, linspace_nd, neglapl_unsym, symmetric_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def matrix(kind, n, d):
+    return neglapl_unsym(n, d, 2019) if kind == "unsym" else convdiff(n, d)
+
+
+def gen_gpu_tree(A, X, nl, tol, prec=S.FP64):
+    t = S.Tree(nl)
+    t.set_use_geo(True)
+    t.set_Xcoo(X)
+    t.set_tol(tol)
+    t.set_precision(prec)
+    t.set_symm_kind("GEN")
+    t.partition(symmetric_graph(A))
+    t.assemble(A)
+    return t
+
+
+@pytest.fixture
+def stepped_rrqr(request, monkeypatch):
+    """RRQR path: the library's own size split, or every cluster on the multi-workgroup path
+    (one persistent launch per wave, or one launch per pivot step)."""
+    if request.param in ("wave", "steps"):
+        monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
+        monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
+    if request.param == "steps":
+        monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "0")
+    return request.param
+
+
+# (40, 2, 3): level-0 pivots of about 100 rows, factored by the global-memory getrf
+GEN_CASES = [("unsym", 5, 2, 3, 0.0), ("unsym", 20, 2, 5, 1e-2), ("unsym", 10, 3, 5, 1e-2),
+             ("unsym", 15, 3, 6, 0.0), ("cd", 16, 3, 7, 1e-2), ("unsym", 40, 2, 3, 1e-2)]
+
+
+@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps"], indirect=True)
+@pytest.mark.parametrize("kind,n,d,nl,tol", GEN_CASES)
+def test_gen_fp64_factor_matches_oracle(kind, n, d, nl, tol, stepped_rrqr):
+    say(f"case gen fp64 {kind} {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")
+    A, X = matrix(kind, n, d), linspace_nd(n, d)
+    o, _ = oracle.setup_tree(A, X, nl, tol, symm="GEN")
+    o.factorize()
+    g = gen_gpu_tree(A, X, nl, tol)
+    g.factorize()
+    oi, gi = o.ops(), g.ops()
+    assert np.array_equal(oi, gi), "op sequence / kept ranks differ from the oracle"
+    worst, nplu = 0.0, 0
+    for i in range(len(oi)):
+        t, r, c = oi[i, 0], oi[i, 5], oi[i, 6]
+        if t in (7, 8) or r * c == 0:
+            continue
+        od, gd = o.op_data(i, oi[i]), g.op_data(i, gi[i])
+        rel = np.linalg.norm(od["M"] - gd["M"]) / max(np.linalg.norm(od["M"]), 1e-300)
+        if t == 6:
+            rel = max(rel, np.abs(od["tau"] - gd["tau"]).max())
+        if t == 1:
+            rel = max(rel, np.linalg.norm(od["U"] - gd["U"]) / max(np.linalg.norm(od["U"]), 1e-300))
+            assert np.array_equal(od["p"], gd["p"]), f"row permutation of op {i}"
+            nplu += 1
+        worst = max(worst, rel)
+    assert nplu > 0
+    assert worst <= 1e-12, worst
+    b = np.random.default_rng(1).standard_normal(A.n)
+    xo, xg = o.solve(b), g.solve(b)
+    assert np.linalg.norm(xo - xg) <= 1e-12 * np.linalg.norm(xo)
+    assert g.nnz() == o.nnz() and g.get_stop() == o.get_stop()
+
+
+@pytest.mark.parametrize("kind,n,d,nl,tol", [("unsym", 20, 2, 5, 1e-2), ("cd", 16, 3, 7, 1e-2),
+                                             ("unsym", 15, 3, 6, 0.0)])
+def test_gen_fp32_solve_close_to_oracle(kind, n, d, nl, tol):
+    say(f"case gen fp32 {kind} {d}D n={n} levels={nl} tol={tol}")
+    A, X = matrix(kind, n, d), linspace_nd(n, d)
+    o, _ = oracle.setup_tree(A, X, nl, tol, symm="GEN")
+    o.factorize()
+    g = gen_gpu_tree(A, X, nl, tol, prec=S.FP32)
+    g.factorize()
+    b = np.random.default_rng(2).standard_normal(A.n)
+    xo, xg = o.solve(b), g.solve(b)
+    assert np.linalg.norm(xo - xg) <= 1e-4 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32], ids=["fp64", "fp32"])
+def test_gen_gmres_convdiff(prec):
+    say(f"case gen gmres convdiff prec={prec}")
+    n = 20
+    A, X = convdiff(n, 3), linspace_nd(n, 3)
+    B = np.random.default_rng(41).standard_normal((A.n, 4))
+    o, _ = oracle.setup_tree(A, X, 7, 1e-2, symm="GEN")
+    o.factorize()
+    its_o = [o.gmres(A, B[:, j], iters=300, restart=30, tol=1e-10)[1] for j in range(4)]
+    g = gen_gpu_tree(A, X, 7, 1e-2, prec=prec)
+    g.factorize()
+    x, info = S.gmres(A, B, g, iters=300, restart=30, tol=1e-10)
+    assert info["status"] == 0, info
+    res = np.linalg.norm(A.matvec(x) - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert res.max() <= 1e-9, res
+    for io, ig in zip(its_o, info["iters"]):
+        assert abs(int(ig) - io) <= max(1, 0.1 * io), (its_o, info["iters"])
+
+
+def test_gen_singular_pivot_raises():
+    """A zero row inside a level-0 interior makes its pivot exactly singular: the oracle and
+    the GPU both stop with "Singular Pivot" (src/tree.cpp:625-628)."""
+    n, d = 10, 2
+    A, X = neglapl_unsym(n, d, 5), linspace_nd(n, d)
+    o, part = oracle.setup_tree(A, X, 3, 1e-2, symm="GEN")
+    # a dof of the first level-0 interior (part: 6 ints per dof, first two = (lvl, sep))
+    dof = int(np.nonzero((part[:, 0] == 0))[0][0])
+    rows = A.rowval
+    A.nzval[rows == dof] = 0.0
+    o, _ = oracle.setup_tree(A, X, 3, 1e-2, symm="GEN")
+    with pytest.raises(RuntimeError, match="Singular Pivot"):
+        o.factorize()
+    g = gen_gpu_tree(A, X, 3, 1e-2)
+    with pytest.raises(S.SpandError, match="Singular Pivot") as ei:
+        g.factorize()
+    assert ei.value.status == 3  # SPAND_ERR_SINGULAR

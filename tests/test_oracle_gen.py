@@ -1,0 +1,144 @@
+"""The oracle's GEN / PLU path (SymmKind::GEN with ScalingKind::PLU), pinned by the
+reference's own tolerance tests on its unsymmetric test matrix and by closed forms:
+  ApproxTest.Exact  (tests/tests.cpp:562-609): tol 0 / 1e-14 -> ||Ax-b||/||b|| <= 1e-10
+  ApproxTest.Approx (tests/tests.cpp:799-856): tol > 0 -> error <= tol * 2e2
+  neglapl_unsym     (tests/tests.cpp:98-109): libstdc++ default_random_engine draws,
+                    pinned here by the C++ standard's minstd_rand0 known answer
+  getf + split_LU   (src/tree.cpp:614-635, src/util.cpp:213-227): L U = P^T A_ss on every
+                    level-0 pivot, L lower with L(j,j) = sqrt|d_j|, U upper with
+                    |U(i,i)| = sqrt|d_i|
+  GMRES on convection-diffusion (src/is.cpp:123-300): converges to 1e-10."""
+import numpy as np
+import pytest
+
+import oracle
+from spand_public_amd.problems import MinStd0, convdiff, linspace_nd, neglapl, neglapl_unsym
+
+
+def test_minstd0_known_answers():
+    # C++ standard [rand.predef]: the 10000th draw of a default-constructed minstd_rand0
+    g = MinStd0(1)
+    for _ in range(9999):
+        g()
+    assert g() == 1043618065
+    # std::uniform_real_distribution<double>(-0.1, 0.1) over std::default_random_engine
+    # seeded with 0 and 2019 (libstdc++, g++ 11)
+    want = {0: [-0.073692442452247875, -0.0082699735953560422, -0.056208162757504211],
+            2019: [0.014958852172399853, -0.097076729350129315, -0.084280560741736871]}
+    for seed, vals in want.items():
+        g = MinStd0(seed)
+        assert [g.uniform(-0.1, 0.1) for _ in vals] == vals
+
+
+def test_neglapl_unsym_pattern():
+    A, B = neglapl(6, 2), neglapl_unsym(6, 2, 3)
+    assert np.array_equal(A.colptr, B.colptr) and np.array_equal(A.rowval, B.rowval)
+    d = B.nzval - A.nzval
+    assert np.all(np.abs(d) <= 0.1) and np.abs(d).min() > 0
+    M = B.to_scipy().toarray()
+    assert np.abs(M - M.T).max() > 1e-3
+
+
+def test_convdiff_closed_form():
+    A = convdiff(4, 3, pe=10.0)
+    M = A.to_scipy().toarray()
+    assert np.allclose(np.diag(M), 6 + 17.5)
+    # zero row sums away from the boundary (consistent discretization of a constant)
+    i = np.ravel_multi_index((1, 2, 1), (4, 4, 4))
+    assert abs(M[i].sum()) < 1e-13
+    # upwind entries along axis 0 (slowest), 1, 2: -1 - pe * v_k behind, -1 ahead
+    for stride, v in ((16, 1.0), (4, 0.5), (1, 0.25)):
+        assert M[i, i - stride] == -1 - 10 * v and M[i, i + stride] == -1
+    assert np.allclose(convdiff(5, 2, pe=0).to_scipy().toarray(), neglapl(5, 2).to_scipy().toarray())
+
+
+def gen_tree(A, n, d, nl, tol, skip=0):
+    t, _ = oracle.setup_tree(A, linspace_nd(n, d), nl, tol, skip, symm="GEN")
+    t.factorize()
+    return t
+
+
+@pytest.mark.parametrize("test,n,d", [(0, 5, 2), (1, 10, 2), (2, 20, 2), (3, 5, 3), (4, 15, 3)])
+def test_gen_exact(test, n, d):
+    A = neglapl_unsym(n, d, test)
+    lmin = 1 if n ** d < 1000 else 8
+    for nl in range(lmin, lmin + 5, 2):
+        for tol, skip in ((1e-14, 0), (1e-14, 4), (0.0, 1000)):
+            t = gen_tree(A, n, d, nl, tol, skip)
+            b = np.random.default_rng(test + nl).uniform(-1, 1, A.n)
+            x = t.solve(b)
+            assert np.linalg.norm(A.matvec(x) - b) / np.linalg.norm(b) <= 1e-10, (nl, tol, skip)
+
+
+@pytest.mark.parametrize("test,n,d", [(0, 5, 2), (1, 10, 2), (2, 20, 2), (4, 5, 3), (5, 15, 3)])
+def test_gen_approx(test, n, d):
+    A = neglapl_unsym(n, d, test + 2019)
+    lmin = 1 if n ** d < 1000 else 8
+    for nl in (lmin, lmin + 2):
+        for tol in (0.0, 1e-10, 1e-6, 1e-2):
+            for skip in (0, 1):
+                t = gen_tree(A, n, d, nl, tol, skip)
+                b = np.random.default_rng(nl + skip).uniform(-1, 1, A.n)
+                x = t.solve(b)
+                err = np.linalg.norm(A.matvec(x) - b) / np.linalg.norm(b)
+                assert err <= (5e-12 if tol == 0 else tol * 2e2), (nl, tol, skip, err)
+
+
+def test_gen_brute_force_and_lu_closed_form():
+    n, d, nl = 7, 2, 3
+    A = neglapl_unsym(n, d, 7)
+    t, _ = oracle.setup_tree(A, linspace_nd(n, d), nl, 0.0, symm="GEN")
+    T = t.get_trailing_mat().toarray()  # P^T A P before the factorization
+    t.factorize()
+    b = np.random.default_rng(3).standard_normal(A.n)
+    xd = np.linalg.solve(A.to_scipy().toarray(), b)
+    assert np.allclose(t.solve(b), xd, rtol=0, atol=1e-12 * np.abs(xd).max())
+    cl = {int(r[0]): r for r in t.clusters()}
+    info = t.ops()
+    checked = 0
+    for i, row in enumerate(info):
+        if row[0] != 1 or row[7] != 0:
+            continue
+        c = cl[int(row[1])]
+        if c[4] != 0:  # level-0 interiors: their pivots are untouched assembled blocks
+            continue
+        s0, sz = int(c[1]), int(c[3])
+        D = t.op_data(i, row)
+        L, U, p = D["M"], D["U"], D["p"]
+        Ass = T[s0:s0 + sz, s0:s0 + sz]
+        assert np.allclose(np.triu(L, 1), 0) and np.allclose(np.tril(U, -1), 0)
+        assert np.allclose(np.diag(L), np.sqrt(np.abs(np.diag(U) ** 2)))
+        assert sorted(p.tolist()) == list(range(sz))
+        assert np.abs(L @ U - Ass[p, :]).max() <= 1e-13 * np.abs(Ass).max()
+        checked += 1
+    assert checked > 0
+
+
+def test_gen_repro_bitwise():
+    A = neglapl_unsym(20, 2, 1)
+    ref = None
+    for _ in range(3):
+        t = gen_tree(A, 20, 2, 5, 1e-5, 1)
+        x = t.solve(np.linspace(-1, 1, A.n))
+        if ref is None:
+            ref = x
+        assert np.array_equal(ref, x)
+
+
+def test_gen_on_spd_matches_direct():
+    """GEN / PLU on an SPD matrix is still an exact direct solver at tol 0."""
+    A = neglapl(10, 2)
+    t = gen_tree(A, 10, 2, 4, 0.0)
+    b = np.random.default_rng(9).standard_normal(A.n)
+    xd = np.linalg.solve(A.to_scipy().toarray(), b)
+    assert np.linalg.norm(t.solve(b) - xd) <= 1e-12 * np.linalg.norm(xd)
+
+
+def test_convdiff_gmres():
+    n = 16
+    A = convdiff(n, 3)
+    t = gen_tree(A, n, 3, 6, 1e-2)
+    b = np.random.default_rng(4).standard_normal(A.n)
+    x, it, rel = t.gmres(A, b, iters=200, restart=30, tol=1e-10)
+    assert rel < 1e-10 and it <= 30
+    assert np.linalg.norm(A.matvec(x) - b) / np.linalg.norm(b) <= 1e-9
