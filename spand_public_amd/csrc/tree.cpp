@@ -862,7 +862,8 @@ void Tree::factorize_impl() {
         // launch (candidate-column buffers in vbuf); the per-step launches remain the path
         // for larger clusters and for waves the device cannot hold at once.
         std::vector<size_t> wlds_step(nwaves, 0), wlds_fin(nwaves, 0), wlds_wave(nwaves, 0);
-        const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 1) != 0;
+        // off by default: on gfx950 the per-step launches measure faster than the grid barrier
+    const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
         for (int w = 0; w < nwaves; w++) {
             const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
             int wmaxcols = 0;
