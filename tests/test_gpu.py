@@ -45,8 +45,7 @@ def stepped_rrqr(request, monkeypatch):
     if request.param in ("wave", "steps"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
-    if request.param == "steps":
-        monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "0")
+        monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "1" if request.param == "wave" else "0")
     return request.param
 
 
