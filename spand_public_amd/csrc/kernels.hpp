@@ -74,22 +74,19 @@ hipError_t launch_perm_out(const double* ws, const int* perm, int N, int nrhs, d
                            hipStream_t st);
 hipError_t launch_cm2il(const double* src, int ld, int N, int nrhs, double* dst, hipStream_t st);
 hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld, hipStream_t st);
-// Solve sweeps run in chunks of SOLVE_CHUNK outputs of one task (SolveChunk lists built by
-// the planner). trmv writes y = Linv x (or Linv^T x) to the scratch wt and copies it back.
+// mv phases of the preconditioner apply: SOLVE_CHUNK outputs per chunk, pieces of at most
+// MV_PIECE_WORK inner elements; partial holds npieces x SOLVE_CHUNK x nrhs doubles.
 constexpr int SOLVE_CHUNK = 64;
-// reflectors per compact-WY block
-constexpr int QB = 32;
-// x := Linv x[perm] (trans 0) or Lb^T x (trans 1) with the explicit inverse factors.
+constexpr int MV_PIECE_WORK = 128;
 template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, double* ws, double* wt,
-                       int nrhs, int trans, hipStream_t st);
-// compact-WY factor of one block of reflectors per task
-template <typename T> hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st);
+hipError_t launch_mv(const MvGroup* d_groups, const MvChunk* d_chunks, const MvPiece* d_pieces, int npieces,
+                     const MvTerm* d_terms, const int* d_rchunks, int nreduce, double* ws, double* partial, int nrhs,
+                     hipStream_t st);
+// Explicit V, S and the identity X of the Orthogonal ops of one level (WyTask); tiles are
+// the (task, a0, c0) 32 x 32 tiles of the Gram matrices (a0 <= c0). X = S^{-1} is left to
+// launch_trsm.
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms,
-                       double* ws, int nrhs, hipStream_t st);
-template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, double* ws, int nrhs, int trans, hipStream_t st);
+hipError_t launch_wy(const WyTask* d_tasks, int ntasks, const int4* d_tiles, int ntiles, hipStream_t st);
 template <typename T>
 hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_children, double* ws, int nrhs, int fwd,
                         hipStream_t st);

@@ -70,126 +70,71 @@ __global__ void __launch_bounds__(NT) k_il2cm(const double* src, int N, int nrhs
 }
 
 // ---------------------------------------------------------------------------------------
-// Block-of-right-hand-sides helpers. A segment's nrhs columns are processed in chunks of
-// RG <= 8 right-hand sides, staged in LDS as S[i * RG + r].
+// mv phases (task layouts in tasks.hpp). A workgroup is 64 lanes (one output each) x 4
+// partitions of the inner sum, over chunks of up to 8 right-hand sides; the partitions are
+// combined in LDS in a fixed order.
 constexpr int RG_MAX = 8;
 constexpr int WAVE = 64;
-
-template <typename T>
-__device__ inline T wave_sum(T v) {
-    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
-}
-
-// Solve sweeps are split into chunks of SC = 64 outputs of one task (SolveChunk), so a large
-// cluster spreads over many workgroups. A workgroup is 64 lanes (one output each) x 4
-// partitions of the inner sum; the partitions are combined in LDS at the end.
 constexpr int SC = SOLVE_CHUNK;
 constexpr int NP = NT / SC;  // 4
-constexpr int TR = 32;       // rows of a transposed tile
+constexpr int TR = 64;       // rows of a transposed tile
+constexpr int UNR = 8;       // independent matrix loads in flight per thread (non-transposed)
 
-// y = Linv x[perm] (trans 0) or Lb^T x (trans 1) for chunk [o0, o0 + 64) of a scaling
-// segment, written to wt (the copy-back kernel returns it to ws: other chunks of the same
-// task still read the old x). Linv and Lb are zero above their diagonals, so the forward
-// sum runs over all j < min(n, o0 + 64) without a mask.
 template <typename T>
-__global__ void __launch_bounds__(NT) k_trmv(const TrsvTask* tasks, const SolveChunk* chunks, const VT* ws, VT* wt,
-                                             int nrhs, int trans) {
+__global__ void __launch_bounds__(NT) k_mv_piece(const MvGroup* groups, const MvChunk* chunks, const MvPiece* pieces,
+                                                 const MvTerm* terms, VT* ws, VT* partial, int nrhs) {
     __shared__ T tile[TR][SC + 1];
     __shared__ VT red[NP][SC][RG_MAX];
-    const SolveChunk ch = chunks[blockIdx.x];
-    const TrsvTask t = tasks[ch.task];
-    const int n = t.n, o0 = ch.o0, no = min(SC, n - o0);
-    const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
-    const T* L = reinterpret_cast<const T*>(trans ? t.Lb : t.L);
-    const int* P = t.perm;
-    const VT* x = ws + (size_t)t.xoff * nrhs;
-    for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
-        const int rg = min(RG_MAX, nrhs - r0);
-        VT acc[RG_MAX];
-#pragma unroll
-        for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
-        if (!trans) {
-            const int i = o0 + lane;
-            const int jmax = min(n, o0 + SC);
-            if (lane < no) {
-                for (int j = kp; j < jmax; j += NP) {
-                    const VT l = L[i + (size_t)j * n];
-                    const VT* xj = x + (size_t)(P ? P[j] : j) * nrhs + r0;
-                    for (int r = 0; r < rg; r++) acc[r] += l * xj[r];
-                }
-            }
-        } else {
-            for (int i0 = o0; i0 < n; i0 += TR) {
-                const int ni = min(TR, n - i0);
-                for (int idx = tid; idx < TR * SC; idx += NT) {
-                    const int ii = idx % TR, jj = idx / TR;
-                    tile[ii][jj] = (ii < ni && jj < no) ? L[(i0 + ii) + (size_t)(o0 + jj) * n] : T(0);
-                }
-                __syncthreads();
-                for (int ii = kp; ii < ni; ii += NP) {
-                    const VT l = tile[ii][lane];
-                    const VT* xi = x + (size_t)(i0 + ii) * nrhs + r0;
-                    for (int r = 0; r < rg; r++) acc[r] += l * xi[r];
-                }
-                __syncthreads();
-            }
-        }
-        for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
-        __syncthreads();
-        if (kp == 0 && lane < no) {
-            VT* y = wt + (size_t)(t.xoff + o0 + lane) * nrhs + r0;
-            for (int r = 0; r < rg; r++) y[r] = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
-        }
-        __syncthreads();
-    }
-}
-
-// ws[seg] = wt[seg] for the segments of a trmv launch, one workgroup per chunk
-template <typename T>
-__global__ void __launch_bounds__(NT) k_copyback(const TrsvTask* tasks, const SolveChunk* chunks, VT* ws, const VT* wt,
-                                                 int nrhs) {
-    const SolveChunk ch = chunks[blockIdx.x];
-    const TrsvTask t = tasks[ch.task];
-    const int no = min(SC, t.n - ch.o0);
-    const size_t base = (size_t)(t.xoff + ch.o0) * nrhs;
-    for (int idx = threadIdx.x; idx < no * nrhs; idx += NT) ws[base + idx] = wt[base + idx];
-}
-
-// x[dst] -= sum_k op(A_k) x[src_k] for chunk [o0, o0 + 64) of the destination segment.
-// Destinations and sources of one launch are disjoint segments, so the update is in place.
-template <typename T>
-__global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const SolveChunk* chunks, const GemvTerm* terms,
-                                             VT* ws, int nrhs) {
-    __shared__ T tile[TR][SC + 1];
-    __shared__ VT red[NP][SC][RG_MAX];
-    const SolveChunk ch = chunks[blockIdx.x];
-    const GemvGroup g = groups[ch.task];
+    const MvPiece pc = pieces[blockIdx.x];
+    const MvChunk ch = chunks[pc.chunk];
+    const MvGroup g = groups[ch.group];
     const int o0 = ch.o0, no = min(SC, g.n - o0);
     const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
+    const bool one = pc.k1 == pc.k0 + 1;
     for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
         const int rg = min(RG_MAX, nrhs - r0);
         VT acc[RG_MAX];
 #pragma unroll
         for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
-        for (int k = g.k0; k < g.k1; k++) {
-            const GemvTerm tm = terms[k];
+        for (int k = pc.k0; k < pc.k1; k++) {
+            const MvTerm tm = terms[k];
             const T* A = reinterpret_cast<const T*>(tm.A);
             const VT* xs = ws + (size_t)tm.xsrc * nrhs + r0;
+            int lo = 0, hi = tm.trans ? tm.rows : tm.cols;
+            if (one) { lo = pc.j0; hi = pc.j1; }
+            if (tm.tri == 1) hi = min(hi, o0 + SC);
+            if (tm.tri == 2) lo = max(lo, o0);
             if (!tm.trans) {
-                // output i = row of A (rows = destination length), sum over columns j
+                // output i = row of A, inner j = column; UNR columns per round, their loads
+                // issued before any use
                 if (lane < no) {
-                    const int i = o0 + lane;
-                    for (int j = kp; j < tm.cols; j += NP) {
-                        const VT a = A[i + (size_t)j * tm.rows];
-                        const VT* xj = xs + (size_t)j * nrhs;
+                    const T* Ai = A + o0 + lane;
+                    int j = lo + kp;
+                    for (; j + (UNR - 1) * NP < hi; j += UNR * NP) {
+                        VT a[UNR];
+                        int src[UNR];
+#pragma unroll
+                        for (int u = 0; u < UNR; u++) {
+                            const int jj = j + u * NP;
+                            a[u] = Ai[(size_t)jj * tm.rows];
+                            src[u] = tm.perm ? tm.perm[jj] : jj;
+                        }
+#pragma unroll
+                        for (int u = 0; u < UNR; u++) {
+                            const VT* xj = xs + (size_t)src[u] * nrhs;
+                            for (int r = 0; r < rg; r++) acc[r] += a[u] * xj[r];
+                        }
+                    }
+                    for (; j < hi; j += NP) {
+                        const VT a = Ai[(size_t)j * tm.rows];
+                        const VT* xj = xs + (size_t)(tm.perm ? tm.perm[j] : j) * nrhs;
                         for (int r = 0; r < rg; r++) acc[r] += a * xj[r];
                     }
                 }
             } else {
-                // output j = column of A (cols = destination length), sum over rows i
-                for (int i0 = 0; i0 < tm.rows; i0 += TR) {
-                    const int ni = min(TR, tm.rows - i0);
+                // output j = column of A, inner i = row: A^T tiles staged through LDS
+                for (int i0 = lo; i0 < hi; i0 += TR) {
+                    const int ni = min(TR, hi - i0);
                     for (int idx = tid; idx < TR * SC; idx += NT) {
                         const int ii = idx % TR, jj = idx / TR;
                         tile[ii][jj] = (ii < ni && jj < no) ? A[(i0 + ii) + (size_t)(o0 + jj) * tm.rows] : T(0);
@@ -207,158 +152,99 @@ __global__ void __launch_bounds__(NT) k_gemv(const GemvGroup* groups, const Solv
         for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
         __syncthreads();
         if (kp == 0 && lane < no) {
-            VT* y = ws + (size_t)(g.xdst + o0 + lane) * nrhs + r0;
-            for (int r = 0; r < rg; r++) y[r] -= red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
+            if (pc.direct) {
+                VT* y = ws + (size_t)(g.xdst + o0 + lane) * nrhs + r0;
+                for (int r = 0; r < rg; r++) {
+                    const VT sum = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
+                    y[r] = g.mode == MV_SUB ? y[r] - sum : (g.mode == MV_SETNEG ? -sum : sum);
+                }
+            } else {
+                VT* y = partial + ((size_t)blockIdx.x * SC + lane) * nrhs + r0;
+                for (int r = 0; r < rg; r++) y[r] = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
+            }
         }
         __syncthreads();
     }
 }
 
-// Householder reflectors in blocked compact-WY form (the dormqr scheme): Q = Q_0 Q_1 ...
-// with Q_b = I - V_b T_b V_b^T over reflectors [32b, 32b + 32). Q^T x applies Q_0^T first,
-// Q x applies the last block first. V has a unit diagonal and zeros above it implicitly.
-// One workgroup per op (the blocks are sequential; the segment is updated in place);
-// W = V_b^T x is accumulated over 64-row tiles of V_b staged in LDS (32 reflectors x 8
-// row partitions).
-template <typename T>
-__global__ void __launch_bounds__(NT) k_qapply(const QTask* tasks, VT* ws, int nrhs, int trans) {
-    __shared__ T vt[SC][QB + 1];
-    __shared__ VT wred[NT / QB][QB][RG_MAX];
-    __shared__ VT W[QB][RG_MAX], U[QB][RG_MAX];
-    const QTask t = tasks[blockIdx.x];
-    const int m = t.m, k = t.k, tid = threadIdx.x;
-    if (m == 0 || k == 0) return;
-    const T* V = reinterpret_cast<const T*>(t.V);
-    VT* x = ws + (size_t)t.xoff * nrhs;
-    const int nblk = (k + QB - 1) / QB;
-    const int c = tid % QB, part = tid / QB, nparts = NT / QB;
-    for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
-        const int rg = min(RG_MAX, nrhs - r0);
-        for (int bi = 0; bi < nblk; bi++) {
-            const int b = trans ? bi : nblk - 1 - bi;
-            const int b0 = b * QB, nb = min(QB, k - b0);
-            const T* Tb = reinterpret_cast<const T*>(t.Tm) + (size_t)b * QB * QB;
-            // W = V_b^T x with v_c = [1 at row b0 + c, V below]
-            VT acc[RG_MAX];
-#pragma unroll
-            for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
-            for (int i0 = b0; i0 < m; i0 += SC) {
-                const int ni = min(SC, m - i0);
-                for (int idx = tid; idx < SC * QB; idx += NT) {
-                    const int ii = idx % SC, cc = idx / SC, i = i0 + ii, col = b0 + cc;
-                    T v = T(0);
-                    if (ii < ni && cc < nb) v = (i == col) ? T(1) : (i > col ? V[i + (size_t)col * m] : T(0));
-                    vt[ii][cc] = v;
-                }
-                __syncthreads();
-                for (int ii = part; ii < ni; ii += nparts) {
-                    const VT v = vt[ii][c];
-                    const VT* xi = x + (size_t)(i0 + ii) * nrhs + r0;
-                    for (int r = 0; r < rg; r++) acc[r] += v * xi[r];
-                }
-                __syncthreads();
-            }
-            for (int r = 0; r < rg; r++) wred[part][c][r] = acc[r];
-            __syncthreads();
-            if (tid < nb * rg) {
-                const int cc = tid / rg, r = tid % rg;
-                VT s = VT(0);
-                for (int p = 0; p < nparts; p++) s += wred[p][cc][r];
-                W[cc][r] = s;
-            }
-            __syncthreads();
-            // U = T_b^T W (Q^T) or T_b W (Q); T_b is nb x nb upper triangular, ld QB
-            if (tid < nb * rg) {
-                const int a = tid / rg, r = tid % rg;
-                VT s = VT(0);
-                if (trans) {
-                    for (int q = 0; q <= a; q++) s += (VT)Tb[q + (size_t)a * QB] * W[q][r];
-                } else {
-                    for (int q = a; q < nb; q++) s += (VT)Tb[a + (size_t)q * QB] * W[q][r];
-                }
-                U[a][r] = s;
-            }
-            __syncthreads();
-            // x -= V_b U for rows i >= b0: U_{i-b0} on the unit diagonal + sum_{c < i-b0} V(i, b0+c) U_c
-            for (int idx = tid; idx < (m - b0) * rg; idx += NT) {
-                const int i = b0 + idx / rg, r = idx % rg;
-                const int li = i - b0;
-                VT s = li < nb ? U[li][r] : VT(0);
-                const int cmax = min(li, nb);
-                for (int cc = 0; cc < cmax; cc++) s += (VT)V[i + (size_t)(b0 + cc) * m] * U[cc][r];
-                x[(size_t)i * nrhs + r0 + r] -= s;
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// dlarft (forward, columnwise) for one block of at most QB reflectors [b0, b0 + nb):
-// G(a, c) = v_a^T v_c is accumulated over the block's rows in one pass, then
-// T(c, c) = tau_c, T(0:c, c) = T(0:c, 0:c) (-tau_c G(c, 0:c)^T).
-template <typename T>
-__global__ void __launch_bounds__(NT) k_larft(const LarftTask* tasks) {
-    __shared__ T vs[WAVE][QB + 1];
-    __shared__ T G[QB][QB + 1];
-    __shared__ T Tl[QB][QB + 1];
-    __shared__ T tcol[QB];
-    const LarftTask t = tasks[blockIdx.x];
-    const int m = t.m, nb = t.k, b0 = t.b0, tid = threadIdx.x;
-    const T* V = reinterpret_cast<const T*>(t.V);
-    const T* tau = reinterpret_cast<const T*>(t.tau) + b0;
-    // pairs (a > c) of the block, two per thread at most (QB * (QB - 1) / 2 <= 2 * NT)
-    int pa[2], pc[2];
-    T acc[2] = {T(0), T(0)};
-    for (int q = 0; q < 2; q++) {
-        int p = tid + q * NT, a = 1;
-        while (p >= a) {
-            p -= a;
-            a++;
-        }
-        pa[q] = a;
-        pc[q] = p;
-    }
-    for (int r0 = b0; r0 < m; r0 += WAVE) {
-        for (int idx = tid; idx < WAVE * QB; idx += NT) {
-            const int rr = idx % WAVE, c = idx / WAVE, r = r0 + rr;
-            T v = T(0);
-            if (c < nb && r < m) v = (r == b0 + c) ? T(1) : (r > b0 + c ? V[r + (size_t)(b0 + c) * m] : T(0));
-            vs[rr][c] = v;
-        }
-        __syncthreads();
-        for (int q = 0; q < 2; q++) {
-            if (pa[q] < nb) {
-                T s = acc[q];
-                for (int rr = 0; rr < WAVE; rr++) s += vs[rr][pa[q]] * vs[rr][pc[q]];
-                acc[q] = s;
-            }
-        }
-        __syncthreads();
-    }
-    for (int q = 0; q < 2; q++)
-        if (pa[q] < nb) G[pa[q]][pc[q]] = acc[q];
-    __syncthreads();
-    for (int c = 0; c < nb; c++) {
-        const T tc = tau[c];
-        if (tid < c) tcol[tid] = -tc * G[c][tid];
-        __syncthreads();
-        if (tid < c) {
-            T s = T(0);
-            for (int q = tid; q < c; q++) s += Tl[tid][q] * tcol[q];
-            Tl[tid][c] = s;
-        }
-        if (tid == 0) Tl[c][c] = tc;
-        if (tid > c && tid < QB) Tl[tid][c] = T(0);
-        __syncthreads();
-    }
-    T* Tm = reinterpret_cast<T*>(t.Tm);
-    for (int idx = tid; idx < QB * QB; idx += NT) {
-        const int a = idx % QB, c = idx / QB;
-        Tm[idx] = (a < nb && c < nb) ? Tl[a][c] : T(0);
+// Chunks with several pieces: add the pieces' partial sums in piece order, then apply.
+__global__ void __launch_bounds__(NT) k_mv_reduce(const MvGroup* groups, const MvChunk* chunks, const int* rchunks,
+                                                  const VT* partial, VT* ws, int nrhs) {
+    const MvChunk ch = chunks[rchunks[blockIdx.x]];
+    const MvGroup g = groups[ch.group];
+    const int no = min(SC, g.n - ch.o0);
+    for (int idx = threadIdx.x; idx < no * nrhs; idx += NT) {
+        const int lane = idx / nrhs, r = idx % nrhs;
+        VT sum = VT(0);
+        for (int p = ch.p0; p < ch.p1; p++) sum += partial[((size_t)p * SC + lane) * nrhs + r];
+        VT* y = ws + (size_t)(g.xdst + ch.o0 + lane) * nrhs + r;
+        *y = g.mode == MV_SUB ? *y - sum : (g.mode == MV_SETNEG ? -sum : sum);
     }
 }
 
 // ---------------------------------------------------------------------------------------
+// Compact-WY factors of the Orthogonal ops (task layout in tasks.hpp): Ve = V with its unit
+// diagonal and zeros above; S = diag(1 / tau) + stril(Ve^T Ve); X = identity (the caller
+// then solves S X = I with the batched trsm, so that X = T^T).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_wy_explicit(const WyTask* tasks) {
+    const WyTask t = tasks[blockIdx.x];
+    const T* V = reinterpret_cast<const T*>(t.V);
+    T* Ve = reinterpret_cast<T*>(t.Ve);
+    const size_t n = (size_t)t.m * t.k;
+    for (size_t idx = (size_t)blockIdx.y * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.y * NT) {
+        const int i = (int)(idx % t.m), c = (int)(idx / t.m);
+        Ve[idx] = i == c ? T(1) : (i > c ? V[idx] : T(0));
+    }
+    // S on and above the diagonal, X
+    const T* tau = reinterpret_cast<const T*>(t.tau);
+    T* S = reinterpret_cast<T*>(t.S);
+    T* X = reinterpret_cast<T*>(t.X);
+    const size_t kk = (size_t)t.k * t.k;
+    for (size_t idx = (size_t)blockIdx.y * NT + threadIdx.x; idx < kk; idx += (size_t)gridDim.y * NT) {
+        const int a = (int)(idx % t.k), c = (int)(idx / t.k);
+        if (a <= c) S[idx] = a == c ? T(1) / tau[a] : T(0);
+        X[idx] = a == c ? T(1) : T(0);
+    }
+}
+
+// one workgroup per 32 x 32 tile (a0, c0), a0 <= c0, of one op's Gram matrix; entry
+// (a, c), a < c, goes to S(c, a)
+template <typename T>
+__global__ void __launch_bounds__(NT) k_wy_gram(const WyTask* tasks, const int4* tiles) {
+    constexpr int GT = 32;  // tile edge (the host plans 32 x 32 tiles)
+    __shared__ T As[GT][GT + 1], Cs[GT][GT + 1];
+    const int4 tl = tiles[blockIdx.x];  // (task, a0, c0, unused)
+    const WyTask t = tasks[tl.x];
+    const int a0 = tl.y, c0 = tl.z, m = t.m, k = t.k;
+    const T* Ve = reinterpret_cast<const T*>(t.Ve);
+    const int tid = threadIdx.x;
+    // thread -> 4 outputs: a = a0 + tid % 32, c = c0 + (tid / 32) * 4 + q
+    const int ta = tid % GT, tc0 = (tid / GT) * 4;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i0 = c0; i0 < m; i0 += GT) {  // Ve(i, c) = 0 for i < c
+        const int ni = min(GT, m - i0);
+        for (int idx = tid; idx < GT * GT; idx += NT) {
+            const int ii = idx % GT, cc = idx / GT;
+            As[ii][cc] = (ii < ni && a0 + cc < k) ? Ve[(i0 + ii) + (size_t)(a0 + cc) * m] : T(0);
+            Cs[ii][cc] = (ii < ni && c0 + cc < k) ? Ve[(i0 + ii) + (size_t)(c0 + cc) * m] : T(0);
+        }
+        __syncthreads();
+        for (int ii = 0; ii < ni; ii++) {
+            const double a = As[ii][ta];
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] += a * (double)Cs[ii][tc0 + q];
+        }
+        __syncthreads();
+    }
+    T* S = reinterpret_cast<T*>(t.S);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int a = a0 + ta, c = c0 + tc0 + q;
+        if (a < k && c < k && a < c) S[c + (size_t)a * k] = (T)acc[q];
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(NT) k_merge(const MergeTask* tasks, const int2* children, VT* ws, int nrhs, int fwd) {
     const MergeTask t = tasks[blockIdx.x];
@@ -528,35 +414,24 @@ hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld,
     k_il2cm<<<grid_for((size_t)N * nrhs), NT, 0, st>>>(src, N, nrhs, dst, ld);
     return hipGetLastError();
 }
-// LDS for one RG chunk of the largest segment in a launch is sized by the caller through
-// max_n; the launchers take the largest dimension from the task arrays' host copies.
 template <typename T>
-hipError_t launch_trmv(const TrsvTask* d_tasks, const SolveChunk* d_chunks, int nchunks, double* ws, double* wt, int nrhs,
-                       int trans, hipStream_t st) {
-    if (nchunks == 0) return hipSuccess;
-    k_trmv<T><<<nchunks, NT, 0, st>>>(d_tasks, d_chunks, ws, wt, nrhs, trans);
+hipError_t launch_mv(const MvGroup* d_groups, const MvChunk* d_chunks, const MvPiece* d_pieces, int npieces,
+                     const MvTerm* d_terms, const int* d_rchunks, int nreduce, double* ws, double* partial, int nrhs,
+                     hipStream_t st) {
+    if (npieces == 0) return hipSuccess;
+    k_mv_piece<T><<<npieces, NT, 0, st>>>(d_groups, d_chunks, d_pieces, d_terms, ws, partial, nrhs);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_copyback<T><<<nchunks, NT, 0, st>>>(d_tasks, d_chunks, ws, wt, nrhs);
+    if (e != hipSuccess || nreduce == 0) return e;
+    k_mv_reduce<<<nreduce, NT, 0, st>>>(d_groups, d_chunks, d_rchunks, partial, ws, nrhs);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_gemv(const GemvGroup* d_groups, const SolveChunk* d_chunks, int nchunks, const GemvTerm* d_terms, double* ws,
-                       int nrhs, hipStream_t st) {
-    if (nchunks == 0) return hipSuccess;
-    k_gemv<T><<<nchunks, NT, 0, st>>>(d_groups, d_chunks, d_terms, ws, nrhs);
-    return hipGetLastError();
-}
-template <typename T>
-hipError_t launch_qapply(const QTask* d_tasks, int ntasks, double* ws, int nrhs, int trans, hipStream_t st) {
+hipError_t launch_wy(const WyTask* d_tasks, int ntasks, const int4* d_tiles, int ntiles, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    k_qapply<T><<<ntasks, NT, 0, st>>>(d_tasks, ws, nrhs, trans);
-    return hipGetLastError();
-}
-template <typename T>
-hipError_t launch_larft(const LarftTask* d_tasks, int ntasks, hipStream_t st) {
-    if (ntasks == 0) return hipSuccess;
-    k_larft<T><<<ntasks, NT, 0, st>>>(d_tasks);
+    k_wy_explicit<T><<<dim3(ntasks, 16), NT, 0, st>>>(d_tasks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || ntiles == 0) return e;
+    k_wy_gram<T><<<ntiles, NT, 0, st>>>(d_tasks, d_tiles);
     return hipGetLastError();
 }
 template <typename T>
@@ -615,11 +490,9 @@ hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t
 #define SPAND_SOLVE_INST(T)                                                                                  \
     template hipError_t launch_perm_in<T>(const double*, size_t, size_t, int, const int*, int, double*, hipStream_t); \
     template hipError_t launch_perm_out<T>(const double*, const int*, int, int, double*, size_t, size_t, hipStream_t); \
-    template hipError_t launch_trmv<T>(const TrsvTask*, const SolveChunk*, int, double*, double*, int, int, hipStream_t); \
-    template hipError_t launch_larft<T>(const LarftTask*, int, hipStream_t);                                \
-    template hipError_t launch_gemv<T>(const GemvGroup*, const SolveChunk*, int, const GemvTerm*, double*, int, \
-                                       hipStream_t);                                                        \
-    template hipError_t launch_qapply<T>(const QTask*, int, double*, int, int, hipStream_t);                \
+    template hipError_t launch_mv<T>(const MvGroup*, const MvChunk*, const MvPiece*, int, const MvTerm*, const int*, \
+                                     int, double*, double*, int, hipStream_t);                             \
+    template hipError_t launch_wy<T>(const WyTask*, int, const int4*, int, hipStream_t);                    \
     template hipError_t launch_merge<T>(const MergeTask*, int, const int2*, double*, int, int, hipStream_t);
 SPAND_SOLVE_INST(float)
 SPAND_SOLVE_INST(double)

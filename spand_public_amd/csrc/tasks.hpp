@@ -119,50 +119,56 @@ struct CopyTask {
 // ---------------------------------------------------------------------------------------
 // Preconditioner apply (Tree::solve, src/tree.cpp:1610-1635). Vectors are stored
 // interleaved: element i of right-hand side r lives at x[(xoff + i) * nrhs + r].
-// ScalingLLT (src/operations.cpp:95-103): x := L^{-1} x (fwd) / L^{-T} x (bwd).
-// ScalingPLUQ with Q = I (src/operations.cpp:114-121): x := L^{-1} P^T x (fwd) / U^{-1} x
-// (bwd), with Lb = (U^T)^{-1} so that the backward sweep is again Lb^T x.
-struct TrsvTask {
-    const void* L;     // the explicit inverse L^{-1} (n x n, lower), computed at factorization
-    const void* Lb;    // backward factor (lower): L^{-1} again for LLT, U^{-T} for PLU
-    const int* perm;   // forward gather x_j := x[perm[j]] (nullptr: none)
-    int n;
-    int xoff;
+//
+// Every recorded op is a matrix-vector product on segments, applied per ND level and op
+// kind in one "mv phase": a set of groups x[dst] (op)= sum_k op(A_k) x[src_k], split into
+// chunks of 64 outputs and, along the inner dimension, into pieces of bounded work, one
+// workgroup each. A chunk with one piece writes its destination directly; otherwise the
+// pieces write partial sums that a second launch adds in piece order (deterministic).
+//   ScalingLLT   (src/operations.cpp:95-103):  x := Linv x,        x := Linv^T x
+//   ScalingPLUQ  (src/operations.cpp:114-121): x := Linv x[perm],  x := (U^{-T})^T x
+//   Gemm*        (src/operations.cpp:31-93):   x_n -= A x_s / A^T x_s
+//   Orthogonal   (src/operations.cpp:149-158): Q = I - V T V^T (compact WY over all kept
+//                reflectors), X = T^T: y := -V^T x, z := -X y (fwd) or -X^T y (bwd), x -= V z
+struct MvTerm {
+    const void* A;     // rows x cols, ld rows, the factor's element type
+    const int* perm;   // not transposed: inner index j reads source element perm[j]
+    int rows, cols;
+    int xsrc;          // source segment
+    int trans;         // 0: output = row of A, inner = column; 1: output = column, inner = row
+    int tri;           // inner-range clip of a triangular A for outputs [o0, o0 + 64):
+                       // 1 inner < o0 + 64, 2 inner >= o0, 0 none
+    int pad;
 };
-// Gemm ops (src/operations.cpp:31-93) grouped by the segment they modify:
-// x[xdst] -= sum_k op(A_k) x[xsrc_k]   with op = A (trans 0) or A^T (trans 1).
-struct GemvGroup {
+enum MvMode { MV_SUB = 0, MV_SETNEG = 1, MV_SET = 2 };  // x -= s, x = -s, x = s
+struct MvGroup {
     int xdst, n;  // destination segment and length
     int k0, k1;   // term range
+    int mode;
+    int pad;
 };
-struct GemvTerm {
-    const void* A;
-    int rows, cols;  // A is rows x cols (ld rows)
-    int xsrc;
-    int trans;
+struct MvChunk {
+    int group, o0;  // outputs [o0, o0 + 64) of the group
+    int p0, p1;     // its pieces (indices into the phase's piece array)
 };
-struct QTask {  // Orthogonal (src/operations.cpp:149-158): x := Q^T x (fwd) / Q x (bwd)
-    const void* V;    // m x k Householder vectors (unit diagonal implicit), ld m
+struct MvPiece {
+    int chunk;
+    int k0, k1;    // terms; when k1 == k0 + 1 only inner indices [j0, j1) of term k0
+    int j0, j1;
+    int direct;    // the chunk's only piece: writes the destination itself
+};
+// Explicit Householder vectors V (unit diagonal, zeros above) and the compact-WY factor of
+// one Orthogonal op, built at factorization. Q = H_0 H_1 ... H_{k-1} = I - V T V^T with T
+// upper triangular and T^{-1} = diag(1 / tau) + striu(V^T V) (the UT transform); the solve
+// stores X = T^T = S^{-1}, S = T^{-T} = diag(1 / tau) + stril(V^T V), computed by a
+// triangular solve against the identity.
+struct WyTask {
+    const void* V;    // m x k reflectors as stored by the QR (R on and above the diagonal)
     const void* tau;
-    const void* Tm;   // ceil(k / QB) blocks of QB x QB upper-triangular compact-WY factors:
-                      // Q = prod_b (I - V_b T_b V_b^T), block b = reflectors [QB b, QB b + QB)
+    void* Ve;         // m x k explicit V
+    void* S;          // k x k lower: diag(1 / tau) + stril(V^T V)
+    void* X;          // k x k, set to the identity here (then S^{-1} by the trsm)
     int m, k;
-    int xoff;
-    int pad;
-};
-// dlarft (forward, columnwise) for one block of reflectors [b0, b0 + k) of an Orthogonal op.
-struct LarftTask {
-    const void* V;    // the op's m x K reflectors (ld m)
-    const void* tau;  // the op's K taus
-    void* Tm;         // this block's QB x QB factor
-    int m, k;
-    int b0;
-    int pad;
-};
-// One chunk of outputs [o0, o0 + 64) of a solve task (trmv task or gemv group).
-struct SolveChunk {
-    int task;
-    int o0;
 };
 struct MergeTask {  // Merge (src/operations.cpp:166-192)
     int pxoff;

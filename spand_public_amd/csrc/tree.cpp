@@ -15,6 +15,7 @@
 #include "tree.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -337,6 +338,76 @@ void Tree::factorize() {
     state_ = State::Factorized;
 }
 
+// Host-side records of the solve ops of one level, turned into mv phases at its end.
+struct TrsvTask {  // ScalingLLT / ScalingPLUQ of one cluster
+    const void* L;   // forward factor (explicit inverse, lower)
+    const void* Lb;  // backward factor (lower, applied transposed)
+    const int* perm;
+    int n, xoff;
+};
+struct GemvGroup { int xdst, n, k0, k1; };
+struct GemvTerm {
+    const void* A;
+    int rows, cols, xsrc, trans;
+};
+
+// One mv phase on the host: groups and terms, then split into chunks and pieces.
+struct MvPlan {
+    std::vector<MvGroup> groups;
+    std::vector<MvTerm> terms;
+    std::vector<MvChunk> chunks;
+    std::vector<MvPiece> pieces;
+    std::vector<int> rchunks;
+    int family = KF_GEMV;
+    bool inplace = false;  // destinations are also sources: every chunk goes through partials
+    SweepWork work;
+
+    void split(size_t esz) {
+        const int W = MV_PIECE_WORK;
+        for (int gi = 0; gi < (int)groups.size(); gi++) {
+            const MvGroup& g = groups[gi];
+            for (int o0 = 0; o0 < g.n; o0 += SOLVE_CHUNK) {
+                const int no = std::min(SOLVE_CHUNK, g.n - o0);
+                const int ci = (int)chunks.size(), p0 = (int)pieces.size();
+                int pk0 = -1, pwork = 0;
+                auto flush = [&](int kend) {
+                    if (pk0 >= 0) {
+                        const MvTerm& t = terms[pk0];
+                        pieces.push_back({ci, pk0, kend, 0, t.trans ? t.rows : t.cols, 0});
+                    }
+                    pk0 = -1;
+                    pwork = 0;
+                };
+                for (int k = g.k0; k < g.k1; k++) {
+                    const MvTerm& t = terms[k];
+                    int lo = 0, hi = t.trans ? t.rows : t.cols;
+                    if (t.tri == 1) hi = std::min(hi, o0 + SOLVE_CHUNK);
+                    if (t.tri == 2) lo = std::max(lo, o0);
+                    const int len = std::max(0, hi - lo);
+                    work.mat_bytes += (double)no * len * esz;
+                    work.flops += 2.0 * no * len;
+                    work.vec_elems += len;
+                    if (len > W) {  // a long term alone: pieces of W inner indices
+                        flush(k);
+                        for (int j = lo; j < hi; j += W) pieces.push_back({ci, k, k + 1, j, std::min(hi, j + W), 0});
+                    } else {
+                        if (pk0 >= 0 && pwork + len > W) flush(k);
+                        if (pk0 < 0) pk0 = k;
+                        pwork += len;
+                    }
+                }
+                flush(g.k1);
+                const int p1 = (int)pieces.size();
+                if (p1 == p0) continue;
+                chunks.push_back({gi, o0, p0, p1});
+                work.vec_elems += 2.0 * no;
+                if (p1 - p0 == 1 && !inplace) pieces[p0].direct = 1;
+                else rchunks.push_back(ci);
+            }
+        }
+    }
+};
+
 static inline double potrf_flops(double n) { return n * n * n / 3.0; }
 static inline double getrf_flops(double n) { return 2.0 * n * n * n / 3.0; }
 
@@ -428,6 +499,7 @@ void Tree::factorize_impl() {
         s.dofs_left_nd = nl;
     }
     factor_bytes_ = 0;
+    max_pieces_ = 0;
     // level-0 blocks
     T* base0 = reinterpret_cast<T*>(arena_.alloc((size_t)std::max<long long>(level0_elems_, 1) * sizeof(T)));
     hip_check(timed(prof_, KF_FILL, st, (double)level0_elems_ * sizeof(T), 0,
@@ -1117,8 +1189,11 @@ void Tree::factorize_impl() {
         lst.t_plan = t_plan1 - t_lvl0;
         lst.t_elim_scale = t_lvl1 - t_plan1;
         // ---------------- apply ranks: Orthogonal, Split, sibling (src/tree.cpp:1317-1342) ----------------
-        std::vector<QTask> qt;
-        std::vector<LarftTask> larft;
+        struct QOp { const void* Ve; const void* X; int m, k, xoff, yoff; };
+        std::vector<QOp> qops;
+        std::vector<WyTask> wy;
+        std::vector<int4> wy_tiles;
+        std::vector<TrsmTask> wy_trsm;
         for (size_t p = 0; p < plist.size(); p++) {
             const int c = plist[p];
             const int r = hsize[c];
@@ -1131,11 +1206,17 @@ void Tree::factorize_impl() {
                 ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
                 factor_bytes_ += ((size_t)s0 * r + (size_t)r * (r + 1) / 2) * sizeof(T);
                 if (r > 0) {
-                    const int nblk = (r + QB - 1) / QB;
-                    T* Tm = reinterpret_cast<T*>(arena_.alloc((size_t)nblk * QB * QB * sizeof(T)));
-                    for (int b = 0; b < nblk; b++)
-                        larft.push_back({t->Q, t->tau, Tm + (size_t)b * QB * QB, s0, std::min(QB, r - b * QB), b * QB, 0});
-                    qt.push_back({t->Q, t->tau, Tm, s0, r, (int)cls_[c].xoff, 0});
+                    // compact WY over all r kept reflectors, for the solve: Q = I - Ve T Ve^T
+                    void* Ve = arena_.alloc((size_t)s0 * r * sizeof(T));
+                    void* Sm = arena_.alloc((size_t)r * r * sizeof(T));
+                    void* X = arena_.alloc((size_t)r * r * sizeof(T));
+                    const int tw = (int)wy.size();
+                    wy.push_back({t->Q, t->tau, Ve, Sm, X, s0, r});
+                    for (int c0 = 0; c0 < r; c0 += 32)
+                        for (int a0 = 0; a0 <= c0; a0 += 32) wy_tiles.push_back(make_int4(tw, a0, c0, 0));
+                    push_trsm(wy_trsm, Sm, X, r, r, 0, sizeof(T));
+                    qops.push_back({Ve, X, s0, r, (int)cls_[c].xoff, (int)ws_len_});
+                    ws_len_ += 2 * r;  // the y and z segments of the apply
                 }
                 HCluster sib;
                 sib.start = cls_[c].start + r;
@@ -1242,18 +1323,25 @@ void Tree::factorize_impl() {
                             [&] { return launch_copy<T>(Packer::at<CopyTask>(dm, o_c), (int)copies.size(), st); }),
                       "merge copy");
         }
-        if (!larft.empty()) {
+        if (!wy.empty()) {
             Packer pl;
-            size_t o_l = pl.add(larft);
+            size_t o_w = pl.add(wy), o_t = pl.add(wy_tiles), o_s = pl.add(wy_trsm);
             void* dl = upload_tasks(pl.buf);
             double lb = 0, lf = 0;
-            for (auto& l : larft) {
-                lb += ((double)(l.m - l.b0) * l.k + (double)QB * QB) * sizeof(T);
-                lf += (double)(l.m - l.b0) * l.k * l.k;
+            for (auto& w : wy) {
+                lb += (3.0 * w.m * w.k + 3.0 * w.k * w.k) * sizeof(T);
+                lf += (double)w.m * w.k * w.k;
             }
             hip_check(timed(prof_, KF_LARFT, st, lb, lf,
-                            [&] { return launch_larft<T>(Packer::at<LarftTask>(dl, o_l), (int)larft.size(), st); }),
-                      "larft");
+                            [&] {
+                                return launch_wy<T>(Packer::at<WyTask>(dl, o_w), (int)wy.size(),
+                                                    Packer::at<int4>(dl, o_t), (int)wy_tiles.size(), st);
+                            }),
+                      "compact WY");
+            Work w = work_trsm(wy_trsm, sizeof(T));
+            hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
+                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dl, o_s), (int)wy_trsm.size(), st); }),
+                      "compact WY inverse");
         }
         lst.t_merge = wall() - t_lvl2;
         int left = 0;
@@ -1262,93 +1350,107 @@ void Tree::factorize_impl() {
         lst.dofs_left_spars = left;
         lst.fact_nnz = nnz();
         // ---------------- solve plan for this level ----------------
-        std::vector<GemvGroup> fwd_groups;
-        std::vector<GemvTerm> terms;
-        for (int nbc : fwd_order) {
-            auto& v = fwd_terms[nbc];
-            int k0 = (int)terms.size();
-            terms.insert(terms.end(), v.begin(), v.end());
-            int n = v[0].trans ? v[0].cols : v[0].rows;
-            fwd_groups.push_back({(int)cls_[nbc].xoff, n, k0, (int)terms.size()});
-        }
-        const int bwd_base = (int)terms.size();
-        for (auto& g : bwd_groups) {
-            g.k0 += bwd_base;
-            g.k1 += bwd_base;
-        }
-        terms.insert(terms.end(), bwd_terms.begin(), bwd_terms.end());
-        auto chunks_of = [](const auto& v, auto len) {
-            std::vector<SolveChunk> ch;
-            for (int k = 0; k < (int)v.size(); k++)
-                for (int o0 = 0; o0 < len(v[k]); o0 += SOLVE_CHUNK) ch.push_back({k, o0});
-            return ch;
-        };
-        auto tlen = [](const TrsvTask& t) { return t.n; };
-        auto glen = [](const GemvGroup& g) { return g.n; };
-        std::vector<SolveChunk> ch_e = chunks_of(sv_elim, tlen), ch_s = chunks_of(sv_scale, tlen);
-        std::vector<SolveChunk> ch_f = chunks_of(fwd_groups, glen), ch_b = chunks_of(bwd_groups, glen);
-        Packer ps;
-        size_t s_et = ps.add(sv_elim), s_fg = ps.add(fwd_groups), s_bg = ps.add(bwd_groups), s_tm = ps.add(terms);
-        size_t s_st = ps.add(sv_scale), s_q = ps.add(qt), s_m = ps.add(mt), s_mc = ps.add(mchildren);
-        size_t s_ce = ps.add(ch_e), s_cs = ps.add(ch_s), s_cf = ps.add(ch_f), s_cb = ps.add(ch_b);
-        void* dsv = upload_tasks(ps.buf);
-        lsv.ch_elim = Packer::at<SolveChunk>(dsv, s_ce);
-        lsv.n_ch_elim = (int)ch_e.size();
-        lsv.ch_scale = Packer::at<SolveChunk>(dsv, s_cs);
-        lsv.n_ch_scale = (int)ch_s.size();
-        lsv.ch_fwd = Packer::at<SolveChunk>(dsv, s_cf);
-        lsv.n_ch_fwd = (int)ch_f.size();
-        lsv.ch_bwd = Packer::at<SolveChunk>(dsv, s_cb);
-        lsv.n_ch_bwd = (int)ch_b.size();
         {
-            const double esz = sizeof(T);
-            auto trmv_work = [&](const std::vector<TrsvTask>& v) {
-                SweepWork w;
+            std::vector<MvPlan> fwd, bwd;
+            auto trsv_phase = [&](const std::vector<TrsvTask>& v, bool backward) {
+                MvPlan m;
+                m.family = KF_TRMV;
+                m.inplace = true;
                 for (auto& t : v) {
-                    w.mat_bytes += (double)t.n * (t.n + 1) / 2 * esz;
-                    w.vec_elems += 2.0 * t.n;
-                    w.flops += (double)t.n * (t.n + 1);
+                    const int k0 = (int)m.terms.size();
+                    if (!backward) m.terms.push_back({t.L, t.perm, t.n, t.n, t.xoff, 0, 1, 0});
+                    else m.terms.push_back({t.Lb, nullptr, t.n, t.n, t.xoff, 1, 2, 0});
+                    m.groups.push_back({t.xoff, t.n, k0, k0 + 1, MV_SET, 0});
                 }
-                return w;
+                return m;
             };
-            auto gemv_work = [&](const std::vector<GemvGroup>& gs) {
-                SweepWork w;
+            auto gemv_phase = [&](const std::vector<GemvGroup>& gs, const std::vector<GemvTerm>& ts) {
+                MvPlan m;
+                m.family = KF_GEMV;
                 for (auto& g : gs) {
-                    w.vec_elems += 2.0 * g.n;
-                    for (int k = g.k0; k < g.k1; k++) {
-                        const GemvTerm& t = terms[k];
-                        w.mat_bytes += (double)t.rows * t.cols * esz;
-                        w.vec_elems += t.trans ? t.rows : t.cols;
-                        w.flops += 2.0 * t.rows * t.cols;
+                    const int k0 = (int)m.terms.size();
+                    for (int k = g.k0; k < g.k1; k++)
+                        m.terms.push_back({ts[k].A, nullptr, ts[k].rows, ts[k].cols, ts[k].xsrc, ts[k].trans, 0, 0});
+                    m.groups.push_back({g.xdst, g.n, k0, (int)m.terms.size(), MV_SUB, 0});
+                }
+                return m;
+            };
+            std::vector<GemvGroup> fwd_groups;
+            std::vector<GemvTerm> fwd_t;
+            for (int nbc : fwd_order) {
+                auto& v = fwd_terms[nbc];
+                const int k0 = (int)fwd_t.size();
+                fwd_t.insert(fwd_t.end(), v.begin(), v.end());
+                const int n = v[0].trans ? v[0].cols : v[0].rows;
+                fwd_groups.push_back({(int)cls_[nbc].xoff, n, k0, (int)fwd_t.size()});
+            }
+            // Orthogonal: y = -V^T x, z = -T^T y (fwd) or -T y (bwd), x -= V z, with X = T^T
+            auto q_phase = [&](int step, bool backward) {
+                MvPlan m;
+                m.family = KF_QAPPLY;
+                for (auto& q : qops) {
+                    const int k0 = (int)m.terms.size();
+                    if (step == 0) {
+                        m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.xoff, 1, 2, 0});
+                        m.groups.push_back({q.yoff, q.k, k0, k0 + 1, MV_SETNEG, 0});
+                    } else if (step == 1) {  // X = T^T lower: z = -X y (fwd), -X^T y (bwd)
+                        m.terms.push_back({q.X, nullptr, q.k, q.k, q.yoff, backward ? 1 : 0, backward ? 2 : 1, 0});
+                        m.groups.push_back({q.yoff + q.k, q.k, k0, k0 + 1, MV_SETNEG, 0});
+                    } else {
+                        m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.yoff + q.k, 0, 1, 0});
+                        m.groups.push_back({q.xoff, q.m, k0, k0 + 1, MV_SUB, 0});
                     }
                 }
-                return w;
+                return m;
             };
-            lsv.w_elim = trmv_work(sv_elim);
-            lsv.w_scale = trmv_work(sv_scale);
-            lsv.w_fwd = gemv_work(fwd_groups);
-            lsv.w_bwd = gemv_work(bwd_groups);
-            for (auto& q : qt) {
-                lsv.w_q.mat_bytes += ((double)q.m * q.k + (double)q.k * (q.k + 1) / 2) * esz;
-                lsv.w_q.vec_elems += 2.0 * q.m;
-                lsv.w_q.flops += 4.0 * q.m * q.k + (double)q.k * q.k;
-            }
+            fwd.push_back(trsv_phase(sv_elim, false));
+            fwd.push_back(gemv_phase(fwd_groups, fwd_t));
+            fwd.push_back(trsv_phase(sv_scale, false));
+            for (int q = 0; q < 3; q++) fwd.push_back(q_phase(q, false));
+            for (int q = 0; q < 3; q++) bwd.push_back(q_phase(q, true));
+            bwd.push_back(trsv_phase(sv_scale, true));
+            bwd.push_back(gemv_phase(bwd_groups, bwd_terms));
+            bwd.push_back(trsv_phase(sv_elim, true));
+            Packer ps;
+            std::vector<std::array<size_t, 5>> offs;
+            auto add_all = [&](std::vector<MvPlan>& v) {
+                for (auto& m : v) {
+                    m.split(sizeof(T));
+                    max_pieces_ = std::max(max_pieces_, (long long)m.pieces.size());
+                    offs.push_back({ps.add(m.groups), ps.add(m.terms), ps.add(m.chunks), ps.add(m.pieces),
+                                    ps.add(m.rchunks)});
+                }
+            };
+            add_all(fwd);
+            add_all(bwd);
+            size_t s_m = ps.add(mt), s_mc = ps.add(mchildren);
+            void* dsv = upload_tasks(ps.buf);
+            size_t oi = 0;
+            auto to_dev = [&](std::vector<MvPlan>& v, std::vector<MvPhase>& out) {
+                out.clear();
+                for (auto& m : v) {
+                    const auto& o = offs[oi++];
+                    if (m.pieces.empty()) continue;
+                    MvPhase ph;
+                    ph.groups = Packer::at<MvGroup>(dsv, o[0]);
+                    ph.terms = Packer::at<MvTerm>(dsv, o[1]);
+                    ph.chunks = Packer::at<MvChunk>(dsv, o[2]);
+                    ph.pieces = Packer::at<MvPiece>(dsv, o[3]);
+                    ph.rchunks = Packer::at<int>(dsv, o[4]);
+                    ph.npieces = (int)m.pieces.size();
+                    ph.nreduce = (int)m.rchunks.size();
+                    ph.family = m.family;
+                    ph.w = m.work;
+                    out.push_back(ph);
+                }
+            };
+            to_dev(fwd, lsv.fwd);
+            to_dev(bwd, lsv.bwd);
             for (auto& c : mchildren) lsv.w_merge.vec_elems += 2.0 * c.y;
+            lsv.merge = Packer::at<MergeTask>(dsv, s_m);
+            lsv.n_merge = (int)mt.size();
+            lsv.children = Packer::at<int2>(dsv, s_mc);
         }
-        lsv.elim_trsv = Packer::at<TrsvTask>(dsv, s_et);
-        lsv.n_elim_trsv = (int)sv_elim.size();
-        lsv.fwd_groups = Packer::at<GemvGroup>(dsv, s_fg);
-        lsv.n_fwd_groups = (int)fwd_groups.size();
-        lsv.bwd_groups = Packer::at<GemvGroup>(dsv, s_bg);
-        lsv.n_bwd_groups = (int)bwd_groups.size();
-        lsv.terms = Packer::at<GemvTerm>(dsv, s_tm);
-        lsv.scale_trsv = Packer::at<TrsvTask>(dsv, s_st);
-        lsv.n_scale_trsv = (int)sv_scale.size();
-        lsv.q = Packer::at<QTask>(dsv, s_q);
-        lsv.n_q = (int)qt.size();
-        lsv.merge = Packer::at<MergeTask>(dsv, s_m);
-        lsv.n_merge = (int)mt.size();
-        lsv.children = Packer::at<int2>(dsv, s_mc);
         if (verb_)
             printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, potrfE.size(),
                    potrfS.size(), plist.size(), nwaves, left);
@@ -1528,36 +1630,34 @@ void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st) {
 // tail of its original cluster's segment.
 template <typename T>
 void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st) {
-    // ws: the segments; wt: scratch of the same layout for the triangular sweeps
-    const size_t need = 2 * (size_t)ws_len_ * nrhs * sizeof(double);
+    // ws: the segments (FP64 whatever the factor's element type T), then the partial sums
+    // of the largest mv phase
+    const size_t seg = (size_t)ws_len_ * nrhs, part = (size_t)max_pieces_ * SOLVE_CHUNK * nrhs;
+    const size_t need = (seg + part) * sizeof(double);
     if (need > ws_cap_) {
         (void)hipFree(d_ws_);
         d_ws_ = nullptr;
         hip_check(hipMalloc(&d_ws_, need), "hipMalloc(solve workspace)");
         ws_cap_ = need;
     }
-    // the segments are FP64 whatever the factor's element type T
     double* ws = reinterpret_cast<double*>(d_ws_);
-    double* wt = ws + (size_t)ws_len_ * nrhs;
+    double* partial = ws + seg;
     const double vsz = sizeof(double);
     auto run = [&](int fam, const SweepWork& w, const char* what, auto&& launch) {
         hip_check(timed(prof_, fam, st, w.mat_bytes + w.vec_elems * nrhs * vsz, w.flops * nrhs, launch), what);
+    };
+    auto run_phase = [&](const MvPhase& ph) {
+        run(ph.family, ph.w, "mv", [&] {
+            return launch_mv<T>(ph.groups, ph.chunks, ph.pieces, ph.npieces, ph.terms, ph.rchunks, ph.nreduce, ws,
+                                partial, nrhs, st);
+        });
     };
     SweepWork wp;
     wp.vec_elems = (double)N_ * 2.0 + (double)N_ * 4.0 / vsz / nrhs;
     run(KF_PERM, wp, "perm in", [&] { return launch_perm_in<T>(dx, sx, sr, nrhs, d_perm_, N_, ws, st); });
     for (int l = 0; l < nlevels_; l++) {
         const LevelSolve& p = solve_plan_[l];
-        if (p.n_ch_elim)
-            run(KF_TRMV, p.w_elim, "trmv",
-                [&] { return launch_trmv<T>(p.elim_trsv, p.ch_elim, p.n_ch_elim, ws, wt, nrhs, 0, st); });
-        if (p.n_ch_fwd)
-            run(KF_GEMV, p.w_fwd, "gemv",
-                [&] { return launch_gemv<T>(p.fwd_groups, p.ch_fwd, p.n_ch_fwd, p.terms, ws, nrhs, st); });
-        if (p.n_ch_scale)
-            run(KF_TRMV, p.w_scale, "trmv",
-                [&] { return launch_trmv<T>(p.scale_trsv, p.ch_scale, p.n_ch_scale, ws, wt, nrhs, 0, st); });
-        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 1, st); });
+        for (const MvPhase& ph : p.fwd) run_phase(ph);
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 1, st); });
@@ -1567,16 +1667,7 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 0, st); });
-        if (p.n_q) run(KF_QAPPLY, p.w_q, "qapply", [&] { return launch_qapply<T>(p.q, p.n_q, ws, nrhs, 0, st); });
-        if (p.n_ch_scale)
-            run(KF_TRMV, p.w_scale, "trmv",
-                [&] { return launch_trmv<T>(p.scale_trsv, p.ch_scale, p.n_ch_scale, ws, wt, nrhs, 1, st); });
-        if (p.n_ch_bwd)
-            run(KF_GEMV, p.w_bwd, "gemv",
-                [&] { return launch_gemv<T>(p.bwd_groups, p.ch_bwd, p.n_ch_bwd, p.terms, ws, nrhs, st); });
-        if (p.n_ch_elim)
-            run(KF_TRMV, p.w_elim, "trmv",
-                [&] { return launch_trmv<T>(p.elim_trsv, p.ch_elim, p.n_ch_elim, ws, wt, nrhs, 1, st); });
+        for (const MvPhase& ph : p.bwd) run_phase(ph);
     }
     run(KF_PERM, wp, "perm out", [&] { return launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st); });
 }

@@ -108,19 +108,23 @@ struct SweepWork {
     double mat_bytes = 0, vec_elems = 0, flops = 0;
 };
 
-// Per-level solve plan (device task arrays) for the preconditioner apply.
+// One mv phase of the preconditioner apply (device task arrays, see MvPiece).
+struct MvPhase {
+    const MvGroup* groups = nullptr;
+    const MvTerm* terms = nullptr;
+    const MvChunk* chunks = nullptr;
+    const MvPiece* pieces = nullptr;
+    const int* rchunks = nullptr;
+    int npieces = 0, nreduce = 0;
+    int family = 0;  // kernel family for the timer
+    SweepWork w;
+};
+
+// Per-level solve plan: the forward phases (before the merge), the backward phases (after
+// the split back to the children), and the merge itself.
 struct LevelSolve {
-    SweepWork w_elim, w_fwd, w_bwd, w_scale, w_q, w_merge;
-    SolveChunk* ch_elim = nullptr; int n_ch_elim = 0;  // 64-output chunks of each launch
-    SolveChunk* ch_scale = nullptr; int n_ch_scale = 0;
-    SolveChunk* ch_fwd = nullptr; int n_ch_fwd = 0;
-    SolveChunk* ch_bwd = nullptr; int n_ch_bwd = 0;
-    TrsvTask* elim_trsv = nullptr; int n_elim_trsv = 0;
-    GemvGroup* fwd_groups = nullptr; int n_fwd_groups = 0;
-    GemvGroup* bwd_groups = nullptr; int n_bwd_groups = 0;
-    GemvTerm* terms = nullptr;
-    TrsvTask* scale_trsv = nullptr; int n_scale_trsv = 0;
-    QTask* q = nullptr; int n_q = 0;
+    std::vector<MvPhase> fwd, bwd;
+    SweepWork w_merge;
     MergeTask* merge = nullptr; int n_merge = 0;
     int2* children = nullptr;
 };
@@ -241,6 +245,7 @@ class Tree {
     void* d_ws_ = nullptr;
     size_t ws_cap_ = 0;
     long long ws_len_ = 0;   // segments in the workspace (per right-hand side)
+    long long max_pieces_ = 0;  // largest mv phase, in pieces (sizes the partial-sum buffer)
 
     KernelProfiler prof_;
     std::vector<LevelSolve> solve_plan_;
