@@ -83,7 +83,8 @@ constexpr int UNR = 8;       // independent matrix loads in flight per thread (n
 template <typename T>
 __global__ void __launch_bounds__(NT) k_mv_piece(const MvGroup* groups, const MvChunk* chunks, const MvPiece* pieces,
                                                  const MvTerm* terms, VT* ws, VT* partial, int nrhs) {
-    __shared__ T tile[TR][SC + 1];
+    __shared__ T tile[TR][SC + 1];      // A^T tile: inner rows x outputs
+    __shared__ VT xl[TR][RG_MAX];       // the source entries of the current inner tile
     __shared__ VT red[NP][SC][RG_MAX];
     const MvPiece pc = pieces[blockIdx.x];
     const MvChunk ch = chunks[pc.chunk];
@@ -104,49 +105,45 @@ __global__ void __launch_bounds__(NT) k_mv_piece(const MvGroup* groups, const Mv
             if (one) { lo = pc.j0; hi = pc.j1; }
             if (tm.tri == 1) hi = min(hi, o0 + SC);
             if (tm.tri == 2) lo = max(lo, o0);
-            if (!tm.trans) {
-                // output i = row of A, inner j = column; UNR columns per round, their loads
-                // issued before any use
-                if (lane < no) {
-                    const T* Ai = A + o0 + lane;
-                    int j = lo + kp;
-                    for (; j + (UNR - 1) * NP < hi; j += UNR * NP) {
-                        VT a[UNR];
-                        int src[UNR];
-#pragma unroll
-                        for (int u = 0; u < UNR; u++) {
-                            const int jj = j + u * NP;
-                            a[u] = Ai[(size_t)jj * tm.rows];
-                            src[u] = tm.perm ? tm.perm[jj] : jj;
-                        }
-#pragma unroll
-                        for (int u = 0; u < UNR; u++) {
-                            const VT* xj = xs + (size_t)src[u] * nrhs;
-                            for (int r = 0; r < rg; r++) acc[r] += a[u] * xj[r];
-                        }
-                    }
-                    for (; j < hi; j += NP) {
-                        const VT a = Ai[(size_t)j * tm.rows];
-                        const VT* xj = xs + (size_t)(tm.perm ? tm.perm[j] : j) * nrhs;
-                        for (int r = 0; r < rg; r++) acc[r] += a * xj[r];
-                    }
+            // inner tiles of TR: the sources (and, transposed, the A^T tile) staged in LDS
+            for (int i0 = lo; i0 < hi; i0 += TR) {
+                const int ni = min(TR, hi - i0);
+                for (int idx = tid; idx < ni * rg; idx += NT) {
+                    const int jj = idx / rg, r = idx % rg;
+                    const int src = (!tm.trans && tm.perm) ? tm.perm[i0 + jj] : i0 + jj;
+                    xl[jj][r] = xs[(size_t)src * nrhs + r];
                 }
-            } else {
-                // output j = column of A, inner i = row: A^T tiles staged through LDS
-                for (int i0 = lo; i0 < hi; i0 += TR) {
-                    const int ni = min(TR, hi - i0);
+                if (tm.trans)
                     for (int idx = tid; idx < TR * SC; idx += NT) {
                         const int ii = idx % TR, jj = idx / TR;
                         tile[ii][jj] = (ii < ni && jj < no) ? A[(i0 + ii) + (size_t)(o0 + jj) * tm.rows] : T(0);
                     }
-                    __syncthreads();
+                __syncthreads();
+                if (!tm.trans) {
+                    // output i = row of A, inner j = column; UNR column loads in flight
+                    if (lane < no) {
+                        const T* Ai = A + o0 + lane + (size_t)i0 * tm.rows;
+                        int jj = kp;
+                        for (; jj + (UNR - 1) * NP < ni; jj += UNR * NP) {
+                            VT a[UNR];
+#pragma unroll
+                            for (int u = 0; u < UNR; u++) a[u] = Ai[(size_t)(jj + u * NP) * tm.rows];
+#pragma unroll
+                            for (int u = 0; u < UNR; u++)
+                                for (int r = 0; r < rg; r++) acc[r] += a[u] * xl[jj + u * NP][r];
+                        }
+                        for (; jj < ni; jj += NP) {
+                            const VT a = Ai[(size_t)jj * tm.rows];
+                            for (int r = 0; r < rg; r++) acc[r] += a * xl[jj][r];
+                        }
+                    }
+                } else {
                     for (int ii = kp; ii < ni; ii += NP) {
                         const VT a = tile[ii][lane];
-                        const VT* xi = xs + (size_t)(i0 + ii) * nrhs;
-                        for (int r = 0; r < rg; r++) acc[r] += a * xi[r];
+                        for (int r = 0; r < rg; r++) acc[r] += a * xl[ii][r];
                     }
-                    __syncthreads();
                 }
+                __syncthreads();
             }
         }
         for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
