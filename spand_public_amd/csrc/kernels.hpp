@@ -56,8 +56,8 @@ template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
                             double tol, size_t lds_bytes, hipStream_t st);
 template <typename T>
-hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol,
-                            size_t lds_bytes, hipStream_t st);
+hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows,
+                            double tol, size_t lds_bytes, hipStream_t st);
 template <typename T>
 hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
                               int* d_size, double tol, size_t lds_bytes, hipStream_t st);

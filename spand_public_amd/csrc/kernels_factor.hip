@@ -930,27 +930,6 @@ __device__ inline T* step_buf(const RrqrBig& B, unsigned char* smem, int slab) {
     return B.vbuf ? reinterpret_cast<T*>(B.vbuf) + (size_t)slab * 2 * B.t.max_rows : reinterpret_cast<T*>(smem);
 }
 
-// Writes the reflector of step k (column `col`, rows k..rows-1) in place: beta on the
-// diagonal, the scaled vector below, tau[k]. All threads of the block take part; `col_buf`
-// holds at least rows elements.
-template <typename T>
-__device__ void write_reflector(T* A, int rows, int col, int k, T* tau_out, double* sh, double* s_ch, T* col_buf) {
-    for (int r = k + threadIdx.x; r < rows; r += blockDim.x) col_buf[r] = A[slab_idx(rows, r, col)];
-    __syncthreads();
-    larfg_block<T>((double)col_buf[k], col_buf + k + 1, rows - k - 1, sh, s_ch);
-    const double tau = sh[0];
-    if (tau != 0.0) {
-        const int knt = (int)sh[3];
-        for (int r = k + 1 + threadIdx.x; r < rows; r += blockDim.x)
-            A[slab_idx(rows, r, col)] = larfg_scale<T>(col_buf[r], knt, sh[2]);
-    }
-    if (threadIdx.x == 0) {
-        if (tau != 0.0) A[slab_idx(rows, k, col)] = (T)sh[1];
-        tau_out[k] = (T)tau;
-    }
-    __syncthreads();
-}
-
 __device__ inline int seg_find(const int* col0, int nseg, int j) {
     int lo = 0, hi = nseg - 1;
     while (lo < hi) {
@@ -987,7 +966,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         B.dims[1] = cols;
         B.dims[2] = mn;
         B.dims[3] = rank_fixed(tol, mn) == 0 ? 0 : mn;  // steps to run (lowered at the rank)
-        B.dims[4] = 0;                                   // 1 once every kept reflector is stored
+        B.dims[4] = 2;  // 2: the steps store every kept reflector in Q; 1: the wave kernel left them in A
     }
     T* A = reinterpret_cast<T*>(B.t.scratch);
     const int mc = B.t.max_cols;
@@ -1028,7 +1007,11 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
     }
 }
 
-template <typename T>
+// One pivot step of the stepped RRQR. RW > 0: the thread's chain rows of its column
+// (r = i + wv + 16 q, q < RW; rows <= 16 RW) are loaded into registers at the start, before
+// the pivot is known, and serve both the w_j sums and the update; RW = 0 re-reads them.
+// The pivot column's owner stores the step's reflector (R above, beta, v below) in Q.
+template <typename T, int RW>
 __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step, double tol) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1042,27 +1025,34 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
     // dims[3] may be lowered by slab 0 during this launch: read it once per workgroup
     if (tid == 0) s_stop = B.dims[3];
-    __syncthreads();
-    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2], stop = s_stop;
+    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
     const int i = step;
-    if (rows == 0 || i >= mn || i > stop) return;
+    const int j = sl.slab * SLAB + lane;
+    T* A = reinterpret_cast<T*>(B.t.scratch);
+    T* cj = A + (size_t)sl.slab * rows * SLAB + lane;  // element r of column j at cj[r * SLAB]
+    __syncthreads();
+    const int stop = s_stop;
+    if (rows == 0 || i >= mn || i >= stop) return;
+    // this thread's rows, loaded while wavefront 0 picks the pivot
+    constexpr int NR = RW > 0 ? RW : 1;
+    T a[NR];
+    if (RW > 0 && j < cols) {
+#pragma unroll
+        for (int q = 0; q < NR; q++) {
+            const int r = i + wv + NCH * q;
+            a[q] = r < rows ? cj[(size_t)r * SLAB] : T(0);
+        }
+    }
     const int cur = i & 1, nxt = cur ^ 1;
     const int mc = B.t.max_cols, ns = B.nslab;
     const int* p2c_cur = B.map + cur * mc;
     int* p2c_nxt = B.map + nxt * mc;
     const int* c2p_cur = B.map + 2 * mc + cur * mc;
     int* c2p_nxt = B.map + 2 * mc + nxt * mc;
-    T* A = reinterpret_cast<T*>(B.t.scratch);
     double* vn1 = B.t.norms;
     double* vn2 = B.t.norms + mc;
     T* s_col = step_buf<T>(B, smem_raw, sl.slab);  // the pivot column
     T* s_v = s_col + B.t.max_rows;                 // its reflector, v[i] = 1
-    // deferred store of the previous step's reflector by the owner of its column
-    if (i >= 1) {
-        const int cprev = p2c_cur[i - 1];
-        if (cprev / SLAB == sl.slab) write_reflector<T>(A, rows, cprev, i - 1, reinterpret_cast<T*>(B.t.tau), sh, s_ch, s_col);
-    }
-    if (i == stop) return;
     if (wv == 0) {  // pivot: first maximum over the slabs' candidates
         double bv = -1.0;
         int bp = INT32_MAX;
@@ -1085,9 +1075,9 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     const int p = s_p, ca = s_ca, cb = s_cb;
     if (p < i || p >= cols) return;  // cannot happen for i < mn: some column is still free
     if (sl.slab == 0) {
-        for (int j = tid; j < cols; j += STEP_NT) {
-            p2c_nxt[j] = p2c_cur[j];
-            c2p_nxt[j] = c2p_cur[j];
+        for (int jj = tid; jj < cols; jj += STEP_NT) {
+            p2c_nxt[jj] = p2c_cur[jj];
+            c2p_nxt[jj] = c2p_cur[jj];
         }
         __syncthreads();
         if (tid == 0) {
@@ -1117,25 +1107,42 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     }
     if (tid == 0) s_v[i] = T(1);
     __syncthreads();
-    const int j = sl.slab * SLAB + lane;
+    if (ca / SLAB == sl.slab) {  // the owner stores the reflector: Q[:, i] = (R(0:i, ca), beta, v)
+        T* Qi = reinterpret_cast<T*>(B.t.Q) + (size_t)i * rows;
+        for (int r = tid; r < rows; r += STEP_NT) {
+            T v;
+            if (r < i) v = A[slab_idx(rows, r, ca)];
+            else if (taud == 0.0) v = s_col[r];
+            else v = r == i ? (T)sh[1] : s_v[r];
+            Qi[r] = v;
+        }
+        if (tid == 0) reinterpret_cast<T*>(B.t.tau)[i] = tau;
+    }
     const int pos = (j == cb) ? p : (j < cols ? c2p_cur[j] : 0);
     const bool active = j < cols && j != ca && pos > i;
-    T* cj = A + (size_t)sl.slab * rows * SLAB + lane;  // element r of column j at cj[r * SLAB]
     // w_j = v^T a_j: chain wv of the sum
     if (taud != 0.0) {
         T c = T(0);
         if (active) {
-            // four rows of the chain loaded ahead of the (ordered) accumulation
-            int r = i + wv;
-            for (; r + 3 * NCH < rows; r += 4 * NCH) {
-                const T a0 = cj[(size_t)r * SLAB], a1 = cj[(size_t)(r + NCH) * SLAB];
-                const T a2 = cj[(size_t)(r + 2 * NCH) * SLAB], a3 = cj[(size_t)(r + 3 * NCH) * SLAB];
-                c = fma(a0, s_v[r], c);
-                c = fma(a1, s_v[r + NCH], c);
-                c = fma(a2, s_v[r + 2 * NCH], c);
-                c = fma(a3, s_v[r + 3 * NCH], c);
+            if (RW > 0) {
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    const int r = i + wv + NCH * q;
+                    if (r < rows) c = fma(a[q], s_v[r], c);
+                }
+            } else {
+                // four rows of the chain loaded ahead of the (ordered) accumulation
+                int r = i + wv;
+                for (; r + 3 * NCH < rows; r += 4 * NCH) {
+                    const T a0 = cj[(size_t)r * SLAB], a1 = cj[(size_t)(r + NCH) * SLAB];
+                    const T a2 = cj[(size_t)(r + 2 * NCH) * SLAB], a3 = cj[(size_t)(r + 3 * NCH) * SLAB];
+                    c = fma(a0, s_v[r], c);
+                    c = fma(a1, s_v[r + NCH], c);
+                    c = fma(a2, s_v[r + 2 * NCH], c);
+                    c = fma(a3, s_v[r + 3 * NCH], c);
+                }
+                for (; r < rows; r += NCH) c = fma(cj[(size_t)r * SLAB], s_v[r], c);
             }
-            for (; r < rows; r += NCH) c = fma(cj[(size_t)r * SLAB], s_v[r], c);
         }
         s_part[wv][lane] = (double)c;
         __syncthreads();
@@ -1153,41 +1160,61 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     if (active) {
         const bool upd = taud != 0.0 && s_upd[lane];
         const T tm = upd ? s_tm[lane] : T(0);
-        int r = i + wv;
-        if (r == i && r < rows) {  // row i, chain 0's first row
-            T a = cj[(size_t)r * SLAB];
-            if (upd) {
-                a = a + tm;
-                cj[(size_t)r * SLAB] = a;
-            }
-            ai = a;
-            r += NCH;
-        }
-        for (; r + 3 * NCH < rows; r += 4 * NCH) {
-            T a[4];
+        if (RW > 0) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) a[q] = cj[(size_t)(r + q * NCH) * SLAB];
-            if (upd) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    a[q] = fma(s_v[r + q * NCH], tm, a[q]);
-                    cj[(size_t)(r + q * NCH) * SLAB] = a[q];
+            for (int q = 0; q < NR; q++) {
+                const int r = i + wv + NCH * q;
+                if (r < rows) {
+                    T x = a[q];
+                    if (upd) {
+                        x = (r == i) ? x + tm : fma(s_v[r], tm, x);
+                        cj[(size_t)r * SLAB] = x;
+                    }
+                    if (r == i) {
+                        ai = x;
+                    } else {
+                        const double d = (double)x;
+                        c2 = fma(d, d, c2);
+                    }
                 }
             }
+        } else {
+            int r = i + wv;
+            if (r == i && r < rows) {  // row i, chain 0's first row
+                T x = cj[(size_t)r * SLAB];
+                if (upd) {
+                    x = x + tm;
+                    cj[(size_t)r * SLAB] = x;
+                }
+                ai = x;
+                r += NCH;
+            }
+            for (; r + 3 * NCH < rows; r += 4 * NCH) {
+                T x[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const double d = (double)a[q];
+                for (int q = 0; q < 4; q++) x[q] = cj[(size_t)(r + q * NCH) * SLAB];
+                if (upd) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        x[q] = fma(s_v[r + q * NCH], tm, x[q]);
+                        cj[(size_t)(r + q * NCH) * SLAB] = x[q];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double d = (double)x[q];
+                    c2 = fma(d, d, c2);
+                }
+            }
+            for (; r < rows; r += NCH) {
+                T x = cj[(size_t)r * SLAB];
+                if (upd) {
+                    x = fma(s_v[r], tm, x);
+                    cj[(size_t)r * SLAB] = x;
+                }
+                const double d = (double)x;
                 c2 = fma(d, d, c2);
             }
-        }
-        for (; r < rows; r += NCH) {
-            T a = cj[(size_t)r * SLAB];
-            if (upd) {
-                a = fma(s_v[r], tm, a);
-                cj[(size_t)r * SLAB] = a;
-            }
-            const double d = (double)a;
-            c2 = fma(d, d, c2);
         }
     }
     s_part[(wv + NCH - 1) % NCH][lane] = c2;
@@ -1468,10 +1495,7 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
 template <typename T>
 __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
                                                     int* dsize, double tol) {
-#pragma clang fp contract(off)
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_col0[RRQR_MAXSEG + 1];
-    __shared__ double sh[4], s_ch[NCH + 1];
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
     const RrqrSeg* sg = segs + B.t.seg0;
@@ -1492,19 +1516,19 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
     const int* c2p = B.map + 2 * mc + fin * mc;
     T* A = reinterpret_cast<T*>(B.t.scratch);
     __syncthreads();
-    // without an early stop the last reflector has no later step to store it
-    if (B.dims[4] == 0 && stop == mn && mn >= 1 && p2c[mn - 1] / SLAB == sl.slab)
-        write_reflector<T>(A, rows, p2c[mn - 1], mn - 1, reinterpret_cast<T*>(B.t.tau), sh, s_ch,
-                           step_buf<T>(B, smem_raw, sl.slab));
     const int fixed = rank_fixed(tol, mn);
     const int rank = fixed >= 0 ? fixed : stop;
     if (rank >= rows) return;
     T* Q = reinterpret_cast<T*>(B.t.Q);
-    for (int k = 0; k < rank; k++) {
-        const int col = p2c[k];
-        if (col / SLAB != sl.slab) continue;
-        for (int r = tid; r < rows; r += NT) Q[r + (size_t)k * rows] = A[slab_idx(rows, r, col)];
-    }
+    // the wave kernel leaves the reflectors in the pivot columns of A; the per-step path has
+    // stored them in Q already, and the pivot columns' R entries come from there
+    const bool in_a = B.dims[4] == 1;
+    if (in_a)
+        for (int k = 0; k < rank; k++) {
+            const int col = p2c[k];
+            if (col / SLAB != sl.slab) continue;
+            for (int r = tid; r < rows; r += NT) Q[r + (size_t)k * rows] = A[slab_idx(rows, r, col)];
+        }
     const int j0 = sl.slab * SLAB;
     for (int idx = tid; idx < rank * SLAB; idx += NT) {
         const int j = j0 + idx % SLAB, r = idx / SLAB;
@@ -1513,7 +1537,7 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
         const int k = seg_find(s_col0, B.t.nseg, j);
         const int jj = j - s_col0[k], w = s_col0[k + 1] - s_col0[k];
         T* E = reinterpret_cast<T*>(sg[k].E);
-        const T v = (r <= pos) ? A[slab_idx(rows, r, j)] : T(0);
+        const T v = (r > pos) ? T(0) : ((!in_a && pos < rank) ? Q[r + (size_t)pos * rows] : A[slab_idx(rows, r, j)]);
         if (sg[k].out) E[jj + (size_t)r * w] = v;
         else E[r + (size_t)jj * rank] = v;
     }
@@ -1628,10 +1652,14 @@ hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
 }
 
 template <typename T>
-hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol,
-                            size_t lds_bytes, hipStream_t st) {
+hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows,
+                            double tol, size_t lds_bytes, hipStream_t st) {
     if (nslabs == 0) return hipSuccess;
-    k_rrqr_step<T><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    if (max_rows <= 16 * 4) k_rrqr_step<T, 4><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    else if (max_rows <= 16 * 8) k_rrqr_step<T, 8><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    else if (max_rows <= 16 * 16) k_rrqr_step<T, 16><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    else if (max_rows <= 16 * 32) k_rrqr_step<T, 32><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    else k_rrqr_step<T, 0><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
     return hipGetLastError();
 }
 
@@ -1693,7 +1721,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
                                             double, hipStream_t);                                                  \
     template hipError_t launch_rrqr_wave<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t,       \
                                             hipStream_t);                                                          \
-    template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, double, size_t,            \
+    template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t,       \
                                             hipStream_t);                                                          \
     template hipError_t launch_rrqr_finish<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, int*, double,  \
                                               size_t, hipStream_t);

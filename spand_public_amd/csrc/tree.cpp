@@ -933,9 +933,9 @@ void Tree::factorize_impl() {
         // A wave whose clusters all have at most WAVE_ROWS_MAX rows runs as one persistent
         // launch (candidate-column buffers in vbuf); the per-step launches remain the path
         // for larger clusters and for waves the device cannot hold at once.
-        std::vector<size_t> wlds_step(nwaves, 0), wlds_fin(nwaves, 0), wlds_wave(nwaves, 0);
+        std::vector<size_t> wlds_step(nwaves, 0), wlds_wave(nwaves, 0);
         // off by default: on gfx950 the per-step launches measure faster than the grid barrier
-    const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
+        const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
         for (int w = 0; w < nwaves; w++) {
             const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
             int wmaxcols = 0;
@@ -943,12 +943,10 @@ void Tree::factorize_impl() {
             const size_t lw = ((size_t)wmaxcols * sizeof(int) + 15) / 16 * 16 + 2 * vb;
             if (persistent && !wbig[w].empty() && wmaxrows[w] <= WAVE_ROWS_MAX && lw <= (size_t)WAVE_LDS_MAX) {
                 wlds_wave[w] = lw;
-                wlds_fin[w] = 0;
                 for (auto& b : wbig[w])
                     b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
             } else if (2 * vb <= (size_t)STEP_LDS_MAX) {
                 wlds_step[w] = 2 * vb;
-                wlds_fin[w] = vb;
             } else {
                 for (auto& b : wbig[w])
                     b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
@@ -1135,13 +1133,13 @@ void Tree::factorize_impl() {
             if (!done) {
                 slot_step[w] = prof_.start(KF_RRQR_STEP, st);
                 for (int i = 0; i < wsteps[w]; i++)
-                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, tol_, wlds_step[w], st), "rrqr step");
+                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, wmaxrows[w], tol_, wlds_step[w], st), "rrqr step");
                 prof_.stop(slot_step[w], st, 0, 0, wsteps[w]);
             }
             hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,
                             [&] {
                                 return launch_rrqr_finish<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
-                                                             wlds_fin[w], st);
+                                                             0, st);
                             },
                             &slot_finish[w]),
                       "rrqr finish");
