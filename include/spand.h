@@ -50,6 +50,7 @@ const char* spand_last_error(const spand_tree* t);
 spand_status spand_set_tol(spand_tree* t, double tol);
 spand_status spand_set_skip(spand_tree* t, int skip);
 spand_status spand_set_verb(spand_tree* t, int verb);
+/* symm_kind: 0 SPD (ScalingKind::LLT), 2 GEN (ScalingKind::PLU); 1 (SYM) is not built. */
 spand_status spand_set_symm_kind(spand_tree* t, int symm_kind);
 spand_status spand_set_use_geo(spand_tree* t, int use_geo);
 /* X is dim x N, column-major (the reference's Eigen::MatrixXd Xcoo). Copied. */
