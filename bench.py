@@ -43,6 +43,14 @@ HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 # kernel families bound by memory traffic vs by arithmetic (all of them are reported)
 FLOP_BOUND = {"potrf", "trsm", "schur"}
+# HBM traffic per dispatch of the dominant kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE on this workload and scaled by the same counters on a plain
+# 256 MiB device copy (FETCH_SIZE reads 0.5x of the copied bytes, WRITE_SIZE 1.0x).
+# Keyed by (family, factor dtype, grid edge n, levels); other runs report traffic null.
+TRAFFIC_MEASURED = {
+    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 2609.97, "write_kib": 3496.41, "fetch_factor": 2.0,
+                                   "write_factor": 1.0},
+}
 
 
 def log(*a):
@@ -99,7 +107,7 @@ def run_steps(S, torch, t, Am, d_b, d_x, N, steps, warmup, profile):
     return dt, its.tolist(), wprof, tprof
 
 
-def roofline(tprof, wprof, steps, dtype):
+def roofline(tprof, wprof, steps, dtype, n=64, nlevels=12):
     if not tprof:
         return None
     fam, k = max(tprof.items(), key=lambda kv: kv[1]["ms"])
@@ -110,9 +118,13 @@ def roofline(tprof, wprof, steps, dtype):
     else:
         achieved = k["bytes"] / k["launches"] / avg_s / 1e9
         peak, unit, bound = HBM_PEAK_GBS, "GB/s", "hbm"
+    tm = TRAFFIC_MEASURED.get((fam, dtype, n, nlevels))
+    traffic = None
+    if tm:
+        traffic = round((tm["fetch_kib"] * tm["fetch_factor"] + tm["write_kib"] * tm["write_factor"]) * 1024.0)
     out = {
         "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-        "frac": round(achieved / peak, 6), "traffic": None,
+        "frac": round(achieved / peak, 6), "traffic": traffic,
         "kernel": fam, "launches_per_step": round(k["launches"] / steps, 1),
         "avg_launch_us": round(avg_s * 1e6, 3),
         "algorithmic_bytes_per_launch": round(k["bytes"] / k["launches"], 1),
@@ -158,7 +170,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-factor comparison")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed steps")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device copy of MIB MiB (the calibration kernel for HBM-traffic counters)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+
+        n = args.copy_probe << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
@@ -224,12 +249,12 @@ def main():
         },
         "pcg_iters": head["iters"],
         "relres": head["relres"],
-        "roofline": roofline(head["tprof"], head["wprof"], args.steps, "f32"),
+        "roofline": roofline(head["tprof"], head["wprof"], args.steps, "f32", n, nl),
     }
     if "f64" in results:
         r64 = results["f64"]
         out["fp64_gpu"] = {"value": round(r64["s"], 6), "unit": "s", "pcg_iters": r64["iters"], "relres": r64["relres"],
-                           "roofline": roofline(r64["tprof"], r64["wprof"], args.steps, "f64")}
+                           "roofline": roofline(r64["tprof"], r64["wprof"], args.steps, "f64", n, nl)}
         out["speedup_f32_vs_f64"] = round(r64["s"] / head["s"], 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, X, B, nl)
