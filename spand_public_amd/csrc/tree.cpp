@@ -173,10 +173,10 @@ void Tree::set_eliminated(int c) {
 // device works. The staging area is reused from the start at every factorization (the
 // previous one ended with a device synchronization); when it runs out, the device is
 // synchronized and the buffer grown.
-void Tree::stage_upload(void* d, const void* h, size_t bytes) {
-    if (bytes == 0) return;
+char* Tree::stage_reserve(size_t bytes) {
     const size_t off = (h_stage_used_ + 255) & ~(size_t)255;
     if (off + bytes > h_stage_cap_) {
+        // earlier uploads may still read the old buffer: drain before replacing it
         hip_check(hipDeviceSynchronize(), "staging");
         const size_t cap = std::max<size_t>(2 * h_stage_cap_, std::max<size_t>(bytes, (size_t)8 << 20));
         (void)hipHostFree(h_stage_);
@@ -185,28 +185,34 @@ void Tree::stage_upload(void* d, const void* h, size_t bytes) {
         hip_check(hipHostMalloc((void**)&h_stage_, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
         h_stage_cap_ = cap;
         h_stage_used_ = 0;
-        return stage_upload(d, h, bytes);
+        return stage_reserve(bytes);
     }
-    std::memcpy(h_stage_ + off, h, bytes);
     h_stage_used_ = off + bytes;
-    hip_check(hipMemcpyAsync(d, h_stage_ + off, bytes, hipMemcpyHostToDevice, 0), "upload");
+    return h_stage_ + off;
 }
 
-void* Tree::upload_tasks(const std::vector<char>& host) {
-    if (host.empty()) return nullptr;
-    void* d = task_arena_.alloc(host.size());
-    stage_upload(d, host.data(), host.size());
-    return d;
+void Tree::stage_upload(void* d, const void* h, size_t bytes) {
+    if (bytes == 0) return;
+    char* p = stage_reserve(bytes);
+    std::memcpy(p, h, bytes);
+    hip_check(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, 0), "upload");
 }
 
-// Packs several task arrays into one host buffer (one upload per batch of launches).
+// Packs several task arrays into one upload. add() only records the array (it must stay
+// unchanged until upload_tasks); upload_tasks copies each array once, straight into the
+// pinned staging buffer, and issues one asynchronous copy.
 struct Packer {
-    std::vector<char> buf;
+    struct Item {
+        const void* p;
+        size_t off, bytes;
+    };
+    std::vector<Item> items;
+    size_t size = 0;
     template <class X>
     size_t add(const std::vector<X>& v) {
-        size_t off = (buf.size() + 255) & ~(size_t)255;
-        buf.resize(off + v.size() * sizeof(X));
-        if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(X));
+        const size_t off = (size + 255) & ~(size_t)255;
+        items.push_back({v.data(), off, v.size() * sizeof(X)});
+        size = off + v.size() * sizeof(X);
         return off;
     }
     template <class X>
@@ -214,6 +220,16 @@ struct Packer {
         return reinterpret_cast<X*>(reinterpret_cast<char*>(base) + off);
     }
 };
+
+void* Tree::upload_tasks(const Packer& pk) {
+    if (pk.size == 0) return nullptr;
+    void* d = task_arena_.alloc(pk.size);
+    char* h = stage_reserve(pk.size);
+    for (const auto& it : pk.items)
+        if (it.bytes) std::memcpy(h + it.off, it.p, it.bytes);
+    hip_check(hipMemcpyAsync(d, h, pk.size, hipMemcpyHostToDevice, 0), "upload");
+    return d;
+}
 
 // ---------------------------------------------------------------------------------------
 // Tree::assemble (src/tree.cpp:505-575): P^T A P cut into dense blocks, one edge per
@@ -408,6 +424,29 @@ struct MvPlan {
     }
 };
 
+// Stable grouping of n items by key (keys in [0, nkeys)): groups in order of first
+// appearance, items in their original order inside a group. Returns the item order and
+// fills start (group g is order[start[g] .. start[g + 1])) and the group keys.
+template <typename KeyOf>
+static std::vector<int> group_by_first(int n, int nkeys, KeyOf key, std::vector<int>& start, std::vector<int>& keys) {
+    std::vector<int> slot(nkeys, -1), cnt;
+    keys.clear();
+    for (int k = 0; k < n; k++) {
+        int& sl = slot[key(k)];
+        if (sl < 0) {
+            sl = (int)keys.size();
+            keys.push_back(key(k));
+            cnt.push_back(0);
+        }
+        cnt[sl]++;
+    }
+    start.assign(keys.size() + 1, 0);
+    for (size_t g = 0; g < keys.size(); g++) start[g + 1] = start[g] + cnt[g];
+    std::vector<int> pos(start.begin(), start.end() - 1), order(n);
+    for (int k = 0; k < n; k++) order[pos[slot[key(k)]]++] = k;
+    return order;
+}
+
 static inline double potrf_flops(double n) { return n * n * n / 3.0; }
 static inline double getrf_flops(double n) { return 2.0 * n * n * n / 3.0; }
 
@@ -543,8 +582,7 @@ void Tree::factorize_impl() {
         std::vector<long long> fill_off;  // by position in fill_edges
         std::vector<TrsvTask> sv_elim, sv_scale;
         std::vector<CopyTask> inv_init;  // identity blocks that become explicit inverses L^{-1}
-        std::unordered_map<int, std::vector<GemvTerm>> fwd_terms;  // by destination cluster
-        std::vector<int> fwd_order;
+        std::vector<std::pair<int, GemvTerm>> fwd_list;  // (destination cluster, term), in op order
         std::vector<GemvGroup> bwd_groups;
         std::vector<GemvTerm> bwd_terms;
         auto target_edge = [&](int n1, int n2) {
@@ -609,8 +647,7 @@ void Tree::factorize_impl() {
                     ops_.push_back({OP_GEMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
                     factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
                     if (n > 0 && size(nb) > 0) {
-                        if (!fwd_terms.count(nb)) fwd_order.push_back(nb);
-                        fwd_terms[nb].push_back({edges_[e].A, size(nb), n, (int)s.xoff, 0});
+                        fwd_list.push_back({nb, GemvTerm{edges_[e].A, size(nb), n, (int)s.xoff, 0}});
                     }
                 }
                 for (int e : in) {
@@ -682,8 +719,7 @@ void Tree::factorize_impl() {
                 ops_.push_back({OP_GEMM_SYMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
                 factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
                 if (n > 0 && size(nb) > 0) {
-                    if (!fwd_terms.count(nb)) fwd_order.push_back(nb);
-                    fwd_terms[nb].push_back({edges_[e].A, size(nb), n, (int)s.xoff, 0});
+                    fwd_list.push_back({nb, GemvTerm{edges_[e].A, size(nb), n, (int)s.xoff, 0}});
                     mine.push_back({edges_[e].A, size(nb), n, (int)cls_[nb].xoff, 1});
                 }
             }
@@ -692,8 +728,7 @@ void Tree::factorize_impl() {
                 ops_.push_back({OP_GEMM_SYMM_IN, c, nb, n, size(nb), n, size(nb), ilvl_, edges_[e].A, nullptr});
                 factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
                 if (n > 0 && size(nb) > 0) {
-                    if (!fwd_terms.count(nb)) fwd_order.push_back(nb);
-                    fwd_terms[nb].push_back({edges_[e].A, n, size(nb), (int)s.xoff, 1});
+                    fwd_list.push_back({nb, GemvTerm{edges_[e].A, n, size(nb), (int)s.xoff, 1}});
                     mine.push_back({edges_[e].A, n, size(nb), (int)cls_[nb].xoff, 0});
                 }
             }
@@ -715,22 +750,15 @@ void Tree::factorize_impl() {
         std::vector<SchurTile> tiles;
         std::vector<SchurContrib> contribs;
         {
-            std::unordered_map<int, std::vector<int>> by_target;
-            std::vector<int> torder;
-            for (size_t k = 0; k < pcs.size(); k++) {
-                auto it = by_target.find(pcs[k].target);
-                if (it == by_target.end()) {
-                    torder.push_back(pcs[k].target);
-                    by_target[pcs[k].target].push_back((int)k);
-                } else {
-                    it->second.push_back((int)k);
-                }
-            }
-            for (int tg : torder) {
+            std::vector<int> tstart, torder;
+            const std::vector<int> korder = group_by_first(
+                (int)pcs.size(), (int)edges_.size(), [&](int k) { return pcs[k].target; }, tstart, torder);
+            for (size_t gi = 0; gi < torder.size(); gi++) {
+                const int tg = torder[gi];
                 const HEdge& e = edges_[tg];
                 const int m = size(e.n2), nn = size(e.n1);
                 const int k0 = (int)contribs.size();
-                for (int k : by_target[tg]) contribs.push_back(pcs[k].c);
+                for (int q = tstart[gi]; q < tstart[gi + 1]; q++) contribs.push_back(pcs[korder[q]].c);
                 const int k1 = (int)contribs.size();
                 const int lower = !gen && e.n1 == e.n2 ? 1 : 0;  // syrk on symmetric diagonal targets
                 for (int c0 = 0; c0 < nn; c0 += 32)
@@ -981,7 +1009,7 @@ void Tree::factorize_impl() {
             o_s[w] = pk.add(wslab[w]);
         }
         const double t_plan1 = wall();
-        void* dt = upload_tasks(pk.buf);
+        void* dt = upload_tasks(pk);
         stage_upload(d_size_, hsize.data(), cls_.size() * sizeof(int));
         const size_t esz = sizeof(T);
         if (fill_elems > 0)
@@ -1311,7 +1339,7 @@ void Tree::factorize_impl() {
             }
             Packer pm;
             size_t o_c = pm.add(copies);
-            void* dm = upload_tasks(pm.buf);
+            void* dm = upload_tasks(pm);
             if (melems > 0)
                 hip_check(timed(prof_, KF_FILL, st, (double)melems * sizeof(T), 0,
                                 [&] { return launch_fill<T>(mbase, (size_t)melems, T(0), st); }),
@@ -1324,7 +1352,7 @@ void Tree::factorize_impl() {
         if (!wy.empty()) {
             Packer pl;
             size_t o_w = pl.add(wy), o_t = pl.add(wy_tiles), o_s = pl.add(wy_trsm);
-            void* dl = upload_tasks(pl.buf);
+            void* dl = upload_tasks(pl);
             double lb = 0, lf = 0;
             for (auto& w : wy) {
                 lb += (3.0 * w.m * w.k + 3.0 * w.k * w.k) * sizeof(T);
@@ -1375,12 +1403,17 @@ void Tree::factorize_impl() {
             };
             std::vector<GemvGroup> fwd_groups;
             std::vector<GemvTerm> fwd_t;
-            for (int nbc : fwd_order) {
-                auto& v = fwd_terms[nbc];
-                const int k0 = (int)fwd_t.size();
-                fwd_t.insert(fwd_t.end(), v.begin(), v.end());
-                const int n = v[0].trans ? v[0].cols : v[0].rows;
-                fwd_groups.push_back({(int)cls_[nbc].xoff, n, k0, (int)fwd_t.size()});
+            {
+                std::vector<int> gstart, gkeys;
+                const std::vector<int> forder = group_by_first(
+                    (int)fwd_list.size(), (int)cls_.size(), [&](int k) { return fwd_list[k].first; }, gstart, gkeys);
+                for (size_t gi = 0; gi < gkeys.size(); gi++) {
+                    const int k0 = (int)fwd_t.size();
+                    for (int q = gstart[gi]; q < gstart[gi + 1]; q++) fwd_t.push_back(fwd_list[forder[q]].second);
+                    const GemvTerm& t0 = fwd_t[k0];
+                    const int n = t0.trans ? t0.cols : t0.rows;
+                    fwd_groups.push_back({(int)cls_[gkeys[gi]].xoff, n, k0, (int)fwd_t.size()});
+                }
             }
             // Orthogonal: y = -V^T x, z = -T^T y (fwd) or -T y (bwd), x -= V z, with X = T^T
             auto q_phase = [&](int step, bool backward) {
@@ -1422,7 +1455,7 @@ void Tree::factorize_impl() {
             add_all(fwd);
             add_all(bwd);
             size_t s_m = ps.add(mt), s_mc = ps.add(mchildren);
-            void* dsv = upload_tasks(ps.buf);
+            void* dsv = upload_tasks(ps);
             size_t oi = 0;
             auto to_dev = [&](std::vector<MvPlan>& v, std::vector<MvPhase>& out) {
                 out.clear();

@@ -191,7 +191,8 @@ class Tree {
     int find_out_edge(int from, int to) const;
     int new_edge(int n1, int n2, void* A, bool original);
     void set_eliminated(int c);
-    void* upload_tasks(const std::vector<char>& host);
+    void* upload_tasks(const struct Packer& pk);
+    char* stage_reserve(size_t bytes);
     void stage_upload(void* d, const void* h, size_t bytes);
     char* h_stage_ = nullptr;  // pinned staging of host -> device uploads
     size_t h_stage_cap_ = 0, h_stage_used_ = 0;
