@@ -35,9 +35,12 @@ template <typename T>
 hipError_t launch_getrf(const GetrfTask* d_tasks, int ntasks, int in_lds, int nmax, int* d_info, hipStream_t st);
 template <typename T>
 hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib* d_contribs, hipStream_t st);
+// Small clusters, one workgroup each. lds_bytes (<= RRQR_LDS_MAX) holds a task's Asn when
+// rows * cols fits; larger tasks carry a global scratch.
+constexpr int RRQR_LDS_MAX = 98304;
 template <typename T>
 hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
-                       hipStream_t st);
+                       int lds_bytes, hipStream_t st);
 // Stepped RRQR of the large clusters of one wave: init, then steps 0..nsteps-1, then finish.
 // lds_bytes is the dynamic LDS of the pivot-column / reflector buffers (2 x max_rows
 // elements for a step, max_rows for finish), or 0 when the tasks carry global buffers

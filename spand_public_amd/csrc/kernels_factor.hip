@@ -1636,10 +1636,17 @@ hipError_t launch_potrf_trail(const PotrfTask* d_tasks, int ntasks, int kb, int 
 
 template <typename T>
 hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
-                       hipStream_t st) {
+                       int lds_bytes, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    const int bytes = LDS_BYTES;
-    k_rrqr<T><<<ntasks, NT, bytes, st>>>(d_tasks, d_segs, d_size, tol, bytes / (int)sizeof(T));
+    if (lds_bytes <= 0 || lds_bytes > RRQR_LDS_MAX) return hipErrorInvalidValue;
+    static bool opted = false;  // dynamic LDS above 64 KiB needs the opt-in
+    if (!opted) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_rrqr<T>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, RRQR_LDS_MAX);
+        if (e != hipSuccess) return e;
+        opted = true;
+    }
+    k_rrqr<T><<<ntasks, NT, lds_bytes, st>>>(d_tasks, d_segs, d_size, tol, lds_bytes / (int)sizeof(T));
     return hipGetLastError();
 }
 
@@ -1715,7 +1722,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_trsm<T>(const TrsmTask*, int, hipStream_t);                        \
     template hipError_t launch_getrf<T>(const GetrfTask*, int, int, int, int*, hipStream_t);      \
     template hipError_t launch_schur<T>(const SchurTile*, int, const SchurContrib*, hipStream_t); \
-    template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, hipStream_t); \
+    template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, int, hipStream_t); \
     template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);                        \
     template hipError_t launch_rrqr_init<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, const int*,      \
                                             double, hipStream_t);                                                  \

@@ -561,7 +561,9 @@ void Tree::factorize_impl() {
     std::vector<int> hsize(cap, 0);
     for (int c : bottoms_[0]) cls_[c].xoff = cls_[c].start;
     ws_len_ = N_;
-    const int lds_elems = LDS_BYTES / (int)sizeof(T);
+    // small-path RRQR: Asn in LDS up to RRQR_LDS_MAX; each wave's launch gets the LDS of
+    // its largest resident task
+    const long long lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
     const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 16384);
     const bool gen = symm_ == 2;
@@ -869,6 +871,7 @@ void Tree::factorize_impl() {
         std::vector<int> wave, widx;
         int nwaves = 0;
         std::vector<std::vector<RrqrTask>> wtasks;
+        std::vector<long long> wlds_small;  // dynamic LDS of each wave's small-path launch
         std::vector<std::vector<RrqrBig>> wbig;       // stepped path, per wave
         std::vector<std::vector<RrqrSlab>> wslab;
         std::vector<int> wsteps, wmaxrows;
@@ -897,6 +900,7 @@ void Tree::factorize_impl() {
                 nwaves = std::max(nwaves, w + 1);
             }
             wtasks.assign(nwaves, {});
+            wlds_small.assign(nwaves, 0);
             wbig.assign(nwaves, {});
             wslab.assign(nwaves, {});
             wsteps.assign(nwaves, 0);
@@ -947,6 +951,8 @@ void Tree::factorize_impl() {
                 } else {
                     if ((long long)s.size * cols > lds_elems)
                         t.scratch = arena_.alloc((size_t)s.size * cols * sizeof(T));
+                    else
+                        wlds_small[w] = std::max(wlds_small[w], (long long)s.size * cols * (long long)sizeof(T));
                     if (cols > 1024) {
                         t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * cols * sizeof(double)));
                         t.jpvt = reinterpret_cast<int*>(arena_.alloc((size_t)cols * sizeof(int)));
@@ -1131,7 +1137,8 @@ void Tree::factorize_impl() {
                 hip_check(timed(prof_, KF_RRQR, st, wk.bytes, wk.flops,
                                 [&] {
                                     return launch_rrqr<T>(Packer::at<RrqrTask>(dt, o_w[w]), (int)wtasks[w].size(),
-                                                          Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st);
+                                                          Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
+                                                          (int)std::max(wlds_small[w], (long long)LDS_BYTES), st);
                                 }),
                           "rrqr");
             }
