@@ -1007,13 +1007,18 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
     }
 }
 
-// One pivot step of the stepped RRQR. RW > 0: the thread's chain rows of its column
-// (r = i + wv + 16 q, q < RW; rows <= 16 RW) are loaded into registers at the start, before
+// One pivot step of the stepped RRQR. RW > 0: each chain's rows of the thread's column
+// (r = i + c + 16 q, q < RW; rows <= 16 RW) are loaded into registers at the start, before
 // the pivot is known, and serve both the w_j sums and the update; RW = 0 re-reads them.
+// NW wavefronts per workgroup: wavefront wv holds chains c = wv + NW k (k < 16 / NW), each
+// with its own accumulator, so every sum runs in the same 16-chain order for any NW; fewer
+// wavefronts per slab keep a wave with many slabs resident in one round on the chip.
 // The pivot column's owner stores the step's reflector (R above, beta, v below) in Q.
-template <typename T, int RW>
-__global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step, double tol) {
+template <typename T, int RW, int NW>
+__global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step, double tol) {
 #pragma clang fp contract(off)
+    static_assert(NCH % NW == 0 && (RW > 0 || NW == NCH), "chains split evenly; RW = 0 only with 16 wavefronts");
+    constexpr int NTH = SLAB * NW, CPW = NCH / NW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double s_part[NCH][SLAB];
     __shared__ double sh[4], s_ch[NCH + 1];
@@ -1035,13 +1040,15 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     if (rows == 0 || i >= mn || i >= stop) return;
     // this thread's rows, loaded while wavefront 0 picks the pivot
     constexpr int NR = RW > 0 ? RW : 1;
-    T a[NR];
+    T a[CPW][NR];
     if (RW > 0 && j < cols) {
 #pragma unroll
-        for (int q = 0; q < NR; q++) {
-            const int r = i + wv + NCH * q;
-            a[q] = r < rows ? cj[(size_t)r * SLAB] : T(0);
-        }
+        for (int k = 0; k < CPW; k++)
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+                const int r = i + wv + NW * k + NCH * q;
+                a[k][q] = r < rows ? cj[(size_t)r * SLAB] : T(0);
+            }
     }
     const int cur = i & 1, nxt = cur ^ 1;
     const int mc = B.t.max_cols, ns = B.nslab;
@@ -1075,7 +1082,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     const int p = s_p, ca = s_ca, cb = s_cb;
     if (p < i || p >= cols) return;  // cannot happen for i < mn: some column is still free
     if (sl.slab == 0) {
-        for (int jj = tid; jj < cols; jj += STEP_NT) {
+        for (int jj = tid; jj < cols; jj += NTH) {
             p2c_nxt[jj] = p2c_cur[jj];
             c2p_nxt[jj] = c2p_cur[jj];
         }
@@ -1088,7 +1095,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
         }
     }
     // this step's reflector, recomputed identically in every slab
-    for (int r = i + tid; r < rows; r += STEP_NT) s_col[r] = A[slab_idx(rows, r, ca)];
+    for (int r = i + tid; r < rows; r += NTH) s_col[r] = A[slab_idx(rows, r, ca)];
     __syncthreads();
     larfg_block<T>((double)s_col[i], s_col + i + 1, rows - i - 1, sh, s_ch);
     // R_ii as stored (tau = 0 leaves alpha on the diagonal)
@@ -1103,13 +1110,13 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     const T tau = (T)taud;
     if (taud != 0.0) {
         const int knt = (int)sh[3];
-        for (int r = i + 1 + tid; r < rows; r += STEP_NT) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
+        for (int r = i + 1 + tid; r < rows; r += NTH) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
     }
     if (tid == 0) s_v[i] = T(1);
     __syncthreads();
     if (ca / SLAB == sl.slab) {  // the owner stores the reflector: Q[:, i] = (R(0:i, ca), beta, v)
         T* Qi = reinterpret_cast<T*>(B.t.Q) + (size_t)i * rows;
-        for (int r = tid; r < rows; r += STEP_NT) {
+        for (int r = tid; r < rows; r += NTH) {
             T v;
             if (r < i) v = A[slab_idx(rows, r, ca)];
             else if (taud == 0.0) v = s_col[r];
@@ -1120,31 +1127,38 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
     }
     const int pos = (j == cb) ? p : (j < cols ? c2p_cur[j] : 0);
     const bool active = j < cols && j != ca && pos > i;
-    // w_j = v^T a_j: chain wv of the sum
+    // w_j = v^T a_j: chain c = wv + NW k of the sum
     if (taud != 0.0) {
-        T c = T(0);
+        T c[CPW];
+#pragma unroll
+        for (int k = 0; k < CPW; k++) c[k] = T(0);
         if (active) {
             if (RW > 0) {
 #pragma unroll
-                for (int q = 0; q < NR; q++) {
-                    const int r = i + wv + NCH * q;
-                    if (r < rows) c = fma(a[q], s_v[r], c);
-                }
+                for (int k = 0; k < CPW; k++)
+#pragma unroll
+                    for (int q = 0; q < NR; q++) {
+                        const int r = i + wv + NW * k + NCH * q;
+                        if (r < rows) c[k] = fma(a[k][q], s_v[r], c[k]);
+                    }
             } else {
                 // four rows of the chain loaded ahead of the (ordered) accumulation
                 int r = i + wv;
+                T c0 = T(0);
                 for (; r + 3 * NCH < rows; r += 4 * NCH) {
                     const T a0 = cj[(size_t)r * SLAB], a1 = cj[(size_t)(r + NCH) * SLAB];
                     const T a2 = cj[(size_t)(r + 2 * NCH) * SLAB], a3 = cj[(size_t)(r + 3 * NCH) * SLAB];
-                    c = fma(a0, s_v[r], c);
-                    c = fma(a1, s_v[r + NCH], c);
-                    c = fma(a2, s_v[r + 2 * NCH], c);
-                    c = fma(a3, s_v[r + 3 * NCH], c);
+                    c0 = fma(a0, s_v[r], c0);
+                    c0 = fma(a1, s_v[r + NCH], c0);
+                    c0 = fma(a2, s_v[r + 2 * NCH], c0);
+                    c0 = fma(a3, s_v[r + 3 * NCH], c0);
                 }
-                for (; r < rows; r += NCH) c = fma(cj[(size_t)r * SLAB], s_v[r], c);
+                for (; r < rows; r += NCH) c0 = fma(cj[(size_t)r * SLAB], s_v[r], c0);
+                c[0] = c0;
             }
         }
-        s_part[wv][lane] = (double)c;
+#pragma unroll
+        for (int k = 0; k < CPW; k++) s_part[wv + NW * k][lane] = (double)c[k];
         __syncthreads();
         if (wv == 0) {
             const T w = tree16_lds_t<T>(&s_part[0][lane], SLAB);
@@ -1154,31 +1168,36 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
         __syncthreads();
     }
     // update a_j := a_j + tm v and the chains of sum over rows > i of a_j^2 (the norm
-    // recomputation): row r > i belongs to chain (r - i - 1) % 16 = (wv + 15) % 16
+    // recomputation): row r > i of chain c belongs to chain (r - i - 1) % 16 = (c + 15) % 16
     T ai = T(0);
-    double c2 = 0.0;
+    double c2[CPW];
+#pragma unroll
+    for (int k = 0; k < CPW; k++) c2[k] = 0.0;
     if (active) {
         const bool upd = taud != 0.0 && s_upd[lane];
         const T tm = upd ? s_tm[lane] : T(0);
         if (RW > 0) {
 #pragma unroll
-            for (int q = 0; q < NR; q++) {
-                const int r = i + wv + NCH * q;
-                if (r < rows) {
-                    T x = a[q];
-                    if (upd) {
-                        x = (r == i) ? x + tm : fma(s_v[r], tm, x);
-                        cj[(size_t)r * SLAB] = x;
-                    }
-                    if (r == i) {
-                        ai = x;
-                    } else {
-                        const double d = (double)x;
-                        c2 = fma(d, d, c2);
+            for (int k = 0; k < CPW; k++)
+#pragma unroll
+                for (int q = 0; q < NR; q++) {
+                    const int r = i + wv + NW * k + NCH * q;
+                    if (r < rows) {
+                        T x = a[k][q];
+                        if (upd) {
+                            x = (r == i) ? x + tm : fma(s_v[r], tm, x);
+                            cj[(size_t)r * SLAB] = x;
+                        }
+                        if (r == i) {
+                            ai = x;
+                        } else {
+                            const double d = (double)x;
+                            c2[k] = fma(d, d, c2[k]);
+                        }
                     }
                 }
-            }
         } else {
+            double c20 = 0.0;
             int r = i + wv;
             if (r == i && r < rows) {  // row i, chain 0's first row
                 T x = cj[(size_t)r * SLAB];
@@ -1203,7 +1222,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const double d = (double)x[q];
-                    c2 = fma(d, d, c2);
+                    c20 = fma(d, d, c20);
                 }
             }
             for (; r < rows; r += NCH) {
@@ -1213,11 +1232,13 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_step(const RrqrBig* bigs, cons
                     cj[(size_t)r * SLAB] = x;
                 }
                 const double d = (double)x;
-                c2 = fma(d, d, c2);
+                c20 = fma(d, d, c20);
             }
+            c2[0] = c20;
         }
     }
-    s_part[(wv + NCH - 1) % NCH][lane] = c2;
+#pragma unroll
+    for (int k = 0; k < CPW; k++) s_part[(wv + NW * k + NCH - 1) % NCH][lane] = c2[k];
     __syncthreads();
     if (wv == 0) {
         const double tol3z = sqrt(DBL_EPSILON * 0.5);
@@ -1658,15 +1679,42 @@ hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
     return hipGetLastError();
 }
 
+template <typename T, int RW, int NW>
+static void step_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol, size_t lds_bytes,
+                    hipStream_t st) {
+    k_rrqr_step<T, RW, NW><<<nslabs, SLAB * NW, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+}
+
+template <typename T, int NW>
+static void step_nw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows, double tol,
+                    size_t lds_bytes, hipStream_t st) {
+    if (max_rows <= NCH * 4) step_rw<T, 4, NW>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
+    else if (max_rows <= NCH * 8) step_rw<T, 8, NW>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
+    else if (max_rows <= NCH * 16) step_rw<T, 16, NW>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
+    else step_rw<T, 32, (NW < 8 ? 8 : NW)>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
+}
+
+// Wavefronts per slab workgroup: 16 while the wave's slabs fit the chip in one round at
+// 16 wavefronts each, fewer for waves with many slabs (each thread then holds 16 / NW
+// chains, at most 64 cached rows). waves = 0 picks by the slab count.
+int rrqr_step_waves(int nslabs, int max_rows, int waves) {
+    if (max_rows > NCH * 32) return NCH;  // uncached rows: one chain per wavefront
+    const int rw = max_rows <= NCH * 4 ? 4 : max_rows <= NCH * 8 ? 8 : max_rows <= NCH * 16 ? 16 : 32;
+    int nw = waves;
+    if (nw != 4 && nw != 8 && nw != 16) nw = nslabs <= RRQR_STEP_SLABS16 ? 16 : nslabs <= RRQR_STEP_SLABS8 ? 8 : 4;
+    while (nw < NCH && (NCH / nw) * rw > 64) nw *= 2;
+    return nw;
+}
+
 template <typename T>
 hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows,
-                            double tol, size_t lds_bytes, hipStream_t st) {
+                            double tol, size_t lds_bytes, int waves, hipStream_t st) {
     if (nslabs == 0) return hipSuccess;
-    if (max_rows <= 16 * 4) k_rrqr_step<T, 4><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
-    else if (max_rows <= 16 * 8) k_rrqr_step<T, 8><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
-    else if (max_rows <= 16 * 16) k_rrqr_step<T, 16><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
-    else if (max_rows <= 16 * 32) k_rrqr_step<T, 32><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
-    else k_rrqr_step<T, 0><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    const int nw = rrqr_step_waves(nslabs, max_rows, waves);
+    if (max_rows > NCH * 32) k_rrqr_step<T, 0, NCH><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+    else if (nw == 4) step_nw<T, 4>(d_bigs, d_slabs, nslabs, step, max_rows, tol, lds_bytes, st);
+    else if (nw == 8) step_nw<T, 8>(d_bigs, d_slabs, nslabs, step, max_rows, tol, lds_bytes, st);
+    else step_nw<T, 16>(d_bigs, d_slabs, nslabs, step, max_rows, tol, lds_bytes, st);
     return hipGetLastError();
 }
 
@@ -1728,7 +1776,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
                                             double, hipStream_t);                                                  \
     template hipError_t launch_rrqr_wave<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t,       \
                                             hipStream_t);                                                          \
-    template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t,       \
+    template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t, int,  \
                                             hipStream_t);                                                          \
     template hipError_t launch_rrqr_finish<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, int*, double,  \
                                               size_t, hipStream_t);

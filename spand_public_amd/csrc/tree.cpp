@@ -970,6 +970,7 @@ void Tree::factorize_impl() {
         std::vector<size_t> wlds_step(nwaves, 0), wlds_wave(nwaves, 0);
         // off by default: on gfx950 the per-step launches measure faster than the grid barrier
         const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
+        const int step_waves = (int)env_or("SPAND_RRQR_STEP_WAVES", 0);  // 4, 8, 16; 0: by slab count
         for (int w = 0; w < nwaves; w++) {
             const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
             int wmaxcols = 0;
@@ -1168,7 +1169,8 @@ void Tree::factorize_impl() {
             if (!done) {
                 slot_step[w] = prof_.start(KF_RRQR_STEP, st);
                 for (int i = 0; i < wsteps[w]; i++)
-                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, wmaxrows[w], tol_, wlds_step[w], st), "rrqr step");
+                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, wmaxrows[w], tol_, wlds_step[w], step_waves, st),
+                              "rrqr step");
                 prof_.stop(slot_step[w], st, 0, 0, wsteps[w]);
             }
             hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,

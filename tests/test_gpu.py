@@ -41,15 +41,18 @@ def gpu_tree(A, X, nl, tol, prec=S.FP64, skip=0):
 def stepped_rrqr(request, monkeypatch):
     """Which RRQR path the clusters take: the library's own size split ("default"), every
     cluster on the multi-workgroup path run as one persistent launch per wave ("wave"),
-    or as one launch per pivot step ("steps")."""
-    if request.param in ("wave", "steps"):
+    or as one launch per pivot step ("steps"; "steps8" / "steps4": 8 or 4 wavefronts per
+    64-column slab)."""
+    if request.param in ("wave", "steps", "steps8", "steps4"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
         monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "1" if request.param == "wave" else "0")
+        # per-step launches with 8 or 4 wavefronts per slab (each holding 2 or 4 chains)
+        monkeypatch.setenv("SPAND_RRQR_STEP_WAVES", request.param[5:] if request.param[5:] else "0")
     return request.param
 
 
-@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps"], indirect=True)
+@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps", "steps8", "steps4"], indirect=True)
 @pytest.mark.parametrize("n,d,nl,tol", CASES)
 def test_fp64_factor_matches_oracle(n, d, nl, tol, stepped_rrqr):
     say(f"case fp64 {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")

@@ -41,11 +41,14 @@ def gen_gpu_tree(A, X, nl, tol, prec=S.FP64):
 @pytest.fixture
 def stepped_rrqr(request, monkeypatch):
     """RRQR path: the library's own size split, or every cluster on the multi-workgroup path
-    (one persistent launch per wave, or one launch per pivot step)."""
-    if request.param in ("wave", "steps"):
+    (one persistent launch per wave, or one launch per pivot step with 16, 8 or 4 wavefronts
+    per 64-column slab)."""
+    if request.param in ("wave", "steps", "steps8", "steps4"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
         monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "1" if request.param == "wave" else "0")
+        # per-step launches with 8 or 4 wavefronts per slab (each holding 2 or 4 chains)
+        monkeypatch.setenv("SPAND_RRQR_STEP_WAVES", request.param[5:] if request.param[5:] else "0")
     return request.param
 
 
@@ -54,7 +57,7 @@ GEN_CASES = [("unsym", 5, 2, 3, 0.0), ("unsym", 20, 2, 5, 1e-2), ("unsym", 10, 3
              ("unsym", 15, 3, 6, 0.0), ("cd", 16, 3, 7, 1e-2), ("unsym", 40, 2, 3, 1e-2)]
 
 
-@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps"], indirect=True)
+@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps", "steps8", "steps4"], indirect=True)
 @pytest.mark.parametrize("kind,n,d,nl,tol", GEN_CASES)
 def test_gen_fp64_factor_matches_oracle(kind, n, d, nl, tol, stepped_rrqr):
     say(f"case gen fp64 {kind} {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")
