@@ -59,13 +59,15 @@ template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
                             double tol, size_t lds_bytes, hipStream_t st);
 // One pivot step per launch. waves: wavefronts per slab workgroup (4, 8 or 16; 0 picks by
-// the slab count, with at most 64 cached rows per thread).
+// the slab count, with at most 64 cached rows per thread). prog (host-visible, or nullptr)
+// receives seq when the launch starts; each cluster's RrqrBig::done is set by its last
+// step that does work.
 constexpr int RRQR_STEP_SLABS16 = 384;  // up to this many slabs: 16 wavefronts each
 constexpr int RRQR_STEP_SLABS8 = 768;   // up to this many: 8, above: 4
 int rrqr_step_waves(int nslabs, int max_rows, int waves);
 template <typename T>
 hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows,
-                            double tol, size_t lds_bytes, int waves, hipStream_t st);
+                            double tol, size_t lds_bytes, int waves, long long* prog, long long seq, hipStream_t st);
 template <typename T>
 hipError_t launch_rrqr_finish(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
                               int* d_size, double tol, size_t lds_bytes, hipStream_t st);

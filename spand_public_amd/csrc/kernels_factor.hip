@@ -939,6 +939,12 @@ __device__ inline int seg_find(const int* col0, int nseg, int j) {
     return lo;
 }
 
+// Host-visible signals of the per-step launches (fine-grained pinned host memory): plain
+// system-scope stores that the host polls to stop launching steps that would do nothing.
+__device__ inline void signal_store(long long* p, long long v) {
+    if (p) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
                                                        const int* dsize, double tol) {
@@ -967,6 +973,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         B.dims[2] = mn;
         B.dims[3] = rank_fixed(tol, mn) == 0 ? 0 : mn;  // steps to run (lowered at the rank)
         B.dims[4] = 2;  // 2: the steps store every kept reflector in Q; 1: the wave kernel left them in A
+        if (B.dims[3] == 0) signal_store(B.done, 1);
     }
     T* A = reinterpret_cast<T*>(B.t.scratch);
     const int mc = B.t.max_cols;
@@ -1015,7 +1022,8 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
 // wavefronts per slab keep a wave with many slabs resident in one round on the chip.
 // The pivot column's owner stores the step's reflector (R above, beta, v below) in Q.
 template <typename T, int RW, int NW>
-__global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step, double tol) {
+__global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, const RrqrSlab* slabs, int step, double tol,
+                                                         long long* prog, long long seq) {
 #pragma clang fp contract(off)
     static_assert(NCH % NW == 0 && (RW > 0 || NW == NCH), "chains split evenly; RW = 0 only with 16 wavefronts");
     constexpr int NTH = SLAB * NW, CPW = NCH / NW;
@@ -1028,6 +1036,7 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
     const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
+    if (blockIdx.x == 0 && tid == 0) signal_store(prog, seq);  // this launch has started
     // dims[3] may be lowered by slab 0 during this launch: read it once per workgroup
     if (tid == 0) s_stop = B.dims[3];
     const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
@@ -1037,7 +1046,10 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     T* cj = A + (size_t)sl.slab * rows * SLAB + lane;  // element r of column j at cj[r * SLAB]
     __syncthreads();
     const int stop = s_stop;
-    if (rows == 0 || i >= mn || i >= stop) return;
+    if (rows == 0 || i >= mn || i >= stop) {
+        if (sl.slab == 0 && tid == 0) signal_store(B.done, 1);
+        return;
+    }
     // this thread's rows, loaded while wavefront 0 picks the pivot
     constexpr int NR = RW > 0 ? RW : 1;
     T a[CPW][NR];
@@ -1103,9 +1115,13 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     if (i == 0) {
         if (sl.slab == 0 && tid == 0) B.cand_v[2 * ns] = rii;
     } else if (rank_fixed(tol, mn) < 0 && !(rii / B.cand_v[2 * ns] >= tol)) {
-        if (sl.slab == 0 && tid == 0) B.dims[3] = i;  // rank = i: step i and later are not kept
+        if (sl.slab == 0 && tid == 0) {
+            B.dims[3] = i;  // rank = i: step i and later are not kept
+            signal_store(B.done, 1);
+        }
         return;
     }
+    if (sl.slab == 0 && tid == 0 && i + 1 >= mn) signal_store(B.done, 1);  // the last step
     const double taud = sh[0];
     const T tau = (T)taud;
     if (taud != 0.0) {
@@ -1679,19 +1695,28 @@ hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
     return hipGetLastError();
 }
 
+struct StepArgs {
+    const RrqrBig* bigs;
+    const RrqrSlab* slabs;
+    int nslabs, step;
+    double tol;
+    size_t lds;
+    long long* prog;
+    long long seq;
+    hipStream_t st;
+};
+
 template <typename T, int RW, int NW>
-static void step_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, double tol, size_t lds_bytes,
-                    hipStream_t st) {
-    k_rrqr_step<T, RW, NW><<<nslabs, SLAB * NW, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
+static void step_rw(const StepArgs& a) {
+    k_rrqr_step<T, RW, NW><<<a.nslabs, SLAB * NW, a.lds, a.st>>>(a.bigs, a.slabs, a.step, a.tol, a.prog, a.seq);
 }
 
 template <typename T, int NW>
-static void step_nw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows, double tol,
-                    size_t lds_bytes, hipStream_t st) {
-    if (max_rows <= NCH * 4) step_rw<T, 4, NW>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
-    else if (max_rows <= NCH * 8) step_rw<T, 8, NW>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
-    else if (max_rows <= NCH * 16) step_rw<T, 16, NW>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
-    else step_rw<T, 32, (NW < 8 ? 8 : NW)>(d_bigs, d_slabs, nslabs, step, tol, lds_bytes, st);
+static void step_nw(const StepArgs& a, int max_rows) {
+    if (max_rows <= NCH * 4) step_rw<T, 4, NW>(a);
+    else if (max_rows <= NCH * 8) step_rw<T, 8, NW>(a);
+    else if (max_rows <= NCH * 16) step_rw<T, 16, NW>(a);
+    else step_rw<T, 32, (NW < 8 ? 8 : NW)>(a);
 }
 
 // Wavefronts per slab workgroup: 16 while the wave's slabs fit the chip in one round at
@@ -1708,13 +1733,14 @@ int rrqr_step_waves(int nslabs, int max_rows, int waves) {
 
 template <typename T>
 hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows,
-                            double tol, size_t lds_bytes, int waves, hipStream_t st) {
+                            double tol, size_t lds_bytes, int waves, long long* prog, long long seq, hipStream_t st) {
     if (nslabs == 0) return hipSuccess;
+    const StepArgs a{d_bigs, d_slabs, nslabs, step, tol, lds_bytes, prog, seq, st};
     const int nw = rrqr_step_waves(nslabs, max_rows, waves);
-    if (max_rows > NCH * 32) k_rrqr_step<T, 0, NCH><<<nslabs, STEP_NT, lds_bytes, st>>>(d_bigs, d_slabs, step, tol);
-    else if (nw == 4) step_nw<T, 4>(d_bigs, d_slabs, nslabs, step, max_rows, tol, lds_bytes, st);
-    else if (nw == 8) step_nw<T, 8>(d_bigs, d_slabs, nslabs, step, max_rows, tol, lds_bytes, st);
-    else step_nw<T, 16>(d_bigs, d_slabs, nslabs, step, max_rows, tol, lds_bytes, st);
+    if (max_rows > NCH * 32) step_rw<T, 0, NCH>(a);
+    else if (nw == 4) step_nw<T, 4>(a, max_rows);
+    else if (nw == 8) step_nw<T, 8>(a, max_rows);
+    else step_nw<T, 16>(a, max_rows);
     return hipGetLastError();
 }
 
@@ -1777,7 +1803,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_rrqr_wave<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t,       \
                                             hipStream_t);                                                          \
     template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t, int,  \
-                                            hipStream_t);                                                          \
+                                            long long*, long long, hipStream_t);                                                       \
     template hipError_t launch_rrqr_finish<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, int*, double,  \
                                               size_t, hipStream_t);
 SPAND_INST(float)

@@ -99,6 +99,8 @@ struct RrqrBig {
     void* vbuf;      // nullptr, or [nslab][2][max_rows] pivot column / reflector buffers
                      // (per-step launches), or [2][nslab][max_rows] candidate columns
                      // (persistent wave launch)
+    long long* done; // host-visible flag, or nullptr: set to 1 by the last step that does
+                     // work (rank reached or all min(rows, cols) steps taken)
     int nslab;
     int pad;
 };

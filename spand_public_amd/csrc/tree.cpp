@@ -107,6 +107,7 @@ Tree::~Tree() {
     (void)hipFree(d_perm_);
     (void)hipFree(d_ws_);
     (void)hipHostFree(h_stage_);
+    (void)hipHostFree(h_sig_);
 }
 
 // SymmKind (include/spaND.h): SPD (0) with ScalingKind::LLT, GEN (2) with ScalingKind::PLU.
@@ -189,6 +190,46 @@ char* Tree::stage_reserve(size_t bytes) {
     }
     h_stage_used_ = off + bytes;
     return h_stage_ + off;
+}
+
+// Grows the signal buffer (between levels: no launch that writes it is in flight).
+void Tree::signal_reserve(size_t n) {
+    if (n <= h_sig_cap_) return;
+    hip_check(hipDeviceSynchronize(), "signals");
+    (void)hipHostFree(h_sig_);
+    h_sig_ = nullptr;
+    d_sig_ = nullptr;
+    h_sig_cap_ = 0;
+    const size_t cap = std::max<size_t>(n, 4096);
+    hip_check(hipHostMalloc((void**)&h_sig_, cap * sizeof(long long), hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc(signals)");
+    hip_check(hipHostGetDevicePointer((void**)&d_sig_, h_sig_, 0), "signals device address");
+    std::memset(h_sig_, 0, cap * sizeof(long long));
+    h_sig_cap_ = cap;
+    sig_seq_ = 0;
+}
+
+static inline void cpu_relax() {
+#if defined(__x86_64__) || defined(__i386__)
+    __builtin_ia32_pause();
+#endif
+}
+
+// Waits until the step launch numbered target has started on the device. A wait longer
+// than two seconds turns the throttle off for the life of the tree (every step is then
+// launched up front, as without signals).
+void Tree::wait_signal(long long target) {
+    if (!sig_throttle_) return;
+    const volatile long long* p = h_sig_;
+    if (*p >= target) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned n = 1; *p < target; n++) {
+        cpu_relax();
+        if ((n & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+            sig_throttle_ = false;
+            return;
+        }
+    }
 }
 
 void Tree::stage_upload(void* d, const void* h, size_t bytes) {
@@ -508,6 +549,10 @@ static double geqp3_flops(double m, double n) {
     const double k = std::min(m, n);
     return k > 0 ? 2.0 * m * n * k - (2.0 / 3.0) * k * k * k : 0.0;
 }
+
+// Stepped RRQR launches queued ahead of the device: enough to keep it busy while the host
+// launches the next one, few enough that launches past every cluster's rank stay rare.
+constexpr int RRQR_STEPS_AHEAD = 12;
 
 // Clusters with at least 16 rows and more than 16384 Asn elements take the stepped
 // multi-workgroup RRQR; smaller ones run in one workgroup. Both paths compute the same
@@ -987,6 +1032,24 @@ void Tree::factorize_impl() {
                     b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
             }
         }
+        // done flags of the stepped clusters, reset before any of this level's launches
+        std::vector<size_t> wflag0(nwaves, 0);
+        {
+            size_t nbig = 0;
+            for (int w = 0; w < nwaves; w++) nbig += wbig[w].size();
+            if (nbig > 0) {
+                signal_reserve(1 + nbig);
+                size_t f = 1;
+                for (int w = 0; w < nwaves; w++) {
+                    wflag0[w] = f;
+                    for (auto& b : wbig[w]) {
+                        h_sig_[f] = 0;
+                        b.done = d_sig_ + f;
+                        f++;
+                    }
+                }
+            }
+        }
         lst.nwaves = nwaves;
         // ---------------- upload + launch Elim / Scale / Sparsify ----------------
         for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
@@ -1167,11 +1230,31 @@ void Tree::factorize_impl() {
                 }
             }
             if (!done) {
+                // Steps are launched at most RRQR_STEPS_AHEAD ahead of the device, and none
+                // once every cluster of the wave has flagged its last working step: the
+                // ranks (decided on the device) end most clusters well before min(rows, cols).
                 slot_step[w] = prof_.start(KF_RRQR_STEP, st);
-                for (int i = 0; i < wsteps[w]; i++)
-                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, wmaxrows[w], tol_, wlds_step[w], step_waves, st),
+                std::vector<int> live(wbig[w].size());
+                std::iota(live.begin(), live.end(), 0);
+                std::vector<long long> seq(wsteps[w], 0);
+                int launched = 0;
+                for (int i = 0; i < wsteps[w]; i++) {
+                    if (i >= RRQR_STEPS_AHEAD) wait_signal(seq[i - RRQR_STEPS_AHEAD]);
+                    size_t k = 0;
+                    for (int x : live) {
+                        const RrqrBig& b = wbig[w][x];
+                        const volatile long long* f = h_sig_ + wflag0[w] + x;
+                        if (*f == 0 && i < std::min(b.t.max_rows, b.t.max_cols)) live[k++] = x;
+                    }
+                    live.resize(k);
+                    if (live.empty()) break;
+                    seq[i] = ++sig_seq_;
+                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, wmaxrows[w], tol_, wlds_step[w], step_waves, d_sig_,
+                                                  seq[i], st),
                               "rrqr step");
-                prof_.stop(slot_step[w], st, 0, 0, wsteps[w]);
+                    launched++;
+                }
+                prof_.stop(slot_step[w], st, 0, 0, launched);
             }
             hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,
                             [&] {

@@ -196,6 +196,15 @@ class Tree {
     void stage_upload(void* d, const void* h, size_t bytes);
     char* h_stage_ = nullptr;  // pinned staging of host -> device uploads
     size_t h_stage_cap_ = 0, h_stage_used_ = 0;
+    // Fine-grained pinned signals of the stepped RRQR: [0] the sequence number of the latest
+    // step launch to start, [1..] one done flag per stepped cluster of the current level.
+    void signal_reserve(size_t n);
+    void wait_signal(long long target);
+    long long* h_sig_ = nullptr;
+    long long* d_sig_ = nullptr;  // device address of h_sig_
+    size_t h_sig_cap_ = 0;
+    long long sig_seq_ = 0;
+    bool sig_throttle_ = true;  // off after a wait that timed out
 
     // options
     int nlevels_;
