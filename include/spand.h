@@ -92,7 +92,8 @@ spand_status spand_get_trailing_coo(const spand_tree* t, long long* nnz, int* I,
  *  0 dofs_nd, 1 dofs_left_nd, 2 dofs_left_elim, 3 dofs_left_spars, 4 fact_nnz,
  *  5 rank_before (sum), 6 rank_after (sum), 7 #sparsified, 8 #ignored, 9 #dependency waves,
  * 10 t_plan, 11 t_elim+scale+rrqr (device), 12 t_rank_apply, 13 t_merge (seconds),
- * 14 potrf flops, 15 trsm flops, 16 schur flops, 17 rrqr flops, 18 reserved, 19 reserved */
+ * 14 potrf flops, 15 trsm flops, 16 schur flops, 17 rrqr flops, 18 t_solve_plan (seconds,
+ * host build of the level's preconditioner-apply plan), 19 reserved */
 spand_status spand_get_level_stats(const spand_tree* t, double* out);
 /* Wall seconds of the last spand_factorize, device bytes of the stored factor, of the arena. */
 spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long long* factor_bytes,

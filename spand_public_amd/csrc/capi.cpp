@@ -172,7 +172,7 @@ spand_status spand_get_level_stats(const spand_tree* t, double* out) {
         o[4] = (double)s.fact_nnz; o[5] = s.rank_before; o[6] = s.rank_after; o[7] = s.nsparsified;
         o[8] = s.nignored; o[9] = s.nwaves; o[10] = s.t_plan; o[11] = s.t_elim_scale; o[12] = s.t_spars;
         o[13] = s.t_merge; o[14] = s.flops_potrf; o[15] = s.flops_trsm; o[16] = s.flops_schur;
-        o[17] = s.flops_rrqr; o[18] = 0; o[19] = 0;
+        o[17] = s.flops_rrqr; o[18] = s.t_solve_plan; o[19] = 0;
     }
     return SPAND_OK;
 }

@@ -232,6 +232,37 @@ void Tree::wait_signal(long long target) {
     }
 }
 
+// Algorithmic bytes / flops of the profiled stepped-RRQR waves (init, steps, finish), from
+// the rows, columns and steps run that the device decided.
+void Tree::price_waves(size_t esz) {
+    std::vector<std::vector<int>> dims(level_dims_.size());
+    for (const WaveWork& ww : wave_work_) {
+        std::vector<int>& d = dims[ww.level];
+        if (d.empty()) {
+            d.resize(5 * level_dims_[ww.level].second);
+            hip_check(hipMemcpy(d.data(), level_dims_[ww.level].first, d.size() * sizeof(int), hipMemcpyDeviceToHost),
+                      "read dims");
+        }
+        double ib = 0, fb = 0, by = 0, fl = 0;
+        for (size_t k = ww.first; k < ww.first + ww.n; k++) {
+            const double rows = d[5 * k], cols = d[5 * k + 1];
+            const int steps = d[5 * k + 3];  // the QR stops at the rank
+            ib += rows * cols * 2 * esz + 16.0 * cols;
+            fb += (rows * cols * 2 + rows * steps) * esz;
+            for (int i = 0; i < std::min(steps, ww.steps); i++) {
+                const double m = rows - i, n = cols - i;
+                by += (2.0 * m * n + m) * esz + 16.0 * n;
+                fl += 4.0 * m * n;
+            }
+        }
+        prof_.set_work(ww.slot_init, ib, 0);
+        prof_.set_work(ww.slot_finish, fb, 0);
+        prof_.set_work(ww.slot_step, by, fl);
+    }
+    wave_work_.clear();
+    level_dims_.clear();
+}
+
 void Tree::stage_upload(void* d, const void* h, size_t bytes) {
     if (bytes == 0) return;
     char* p = stage_reserve(bytes);
@@ -568,6 +599,8 @@ void Tree::factorize_impl() {
     const hipStream_t st = 0;
     arena_.reset();
     task_arena_.reset();
+    wave_work_.clear();
+    level_dims_.clear();
     hip_check(hipDeviceSynchronize(), "factorize");
     h_stage_used_ = 0;
     ops_.clear();
@@ -986,7 +1019,6 @@ void Tree::factorize_impl() {
                     b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
                     b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)(2 * b.nslab + 1) * sizeof(double)));
                     b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
-                    b.dims = reinterpret_cast<int*>(arena_.alloc(5 * sizeof(int)));
                     b.t = t;
                     where[p] = {1, (int)wbig[w].size()};
                     for (int k = 0; k < b.nslab; k++) wslab[w].push_back({(int)wbig[w].size(), k});
@@ -1032,23 +1064,28 @@ void Tree::factorize_impl() {
                     b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
             }
         }
-        // done flags of the stepped clusters, reset before any of this level's launches
+        // dims (one block per level) and done flags of the stepped clusters; the flags are
+        // reset before any of this level's launches
         std::vector<size_t> wflag0(nwaves, 0);
         {
             size_t nbig = 0;
             for (int w = 0; w < nwaves; w++) nbig += wbig[w].size();
+            int* dblock = nullptr;
             if (nbig > 0) {
                 signal_reserve(1 + nbig);
+                dblock = reinterpret_cast<int*>(arena_.alloc(nbig * 5 * sizeof(int)));
                 size_t f = 1;
                 for (int w = 0; w < nwaves; w++) {
                     wflag0[w] = f;
                     for (auto& b : wbig[w]) {
                         h_sig_[f] = 0;
                         b.done = d_sig_ + f;
+                        b.dims = dblock + 5 * (f - 1);
                         f++;
                     }
                 }
             }
+            level_dims_.push_back({dblock, nbig});
         }
         lst.nwaves = nwaves;
         // ---------------- upload + launch Elim / Scale / Sparsify ----------------
@@ -1272,37 +1309,11 @@ void Tree::factorize_impl() {
         }
         if (!plist.empty())
             hip_check(hipMemcpy(hsize.data(), d_size_, cls_.size() * sizeof(int), hipMemcpyDeviceToHost), "read ranks");
-        if (prof_.on()) {
-            // the stepped path's sizes are decided on the device (ranks of earlier waves)
-            for (int w = 0; w < nwaves; w++) {
-                if (wbig[w].empty()) continue;
-                std::vector<int> rows, cols, mns;
-                for (auto& b : wbig[w]) {
-                    int dims[4];
-                    hip_check(hipMemcpy(dims, b.dims, 4 * sizeof(int), hipMemcpyDeviceToHost), "read dims");
-                    rows.push_back(dims[0]);
-                    cols.push_back(dims[1]);
-                    mns.push_back(dims[3]);  // steps run: the QR stops at the rank
-                }
-                double ib = 0, fb = 0;
-                for (size_t k = 0; k < rows.size(); k++) {
-                    ib += ((double)rows[k] * cols[k] * 2 * esz) + 16.0 * cols[k];
-                    fb += ((double)rows[k] * cols[k] * 2 + (double)rows[k] * mns[k]) * esz;
-                }
-                prof_.set_work(slot_init[w], ib, 0);
-                prof_.set_work(slot_finish[w], fb, 0);
-                double by = 0, fl = 0;
-                for (int i = 0; i < wsteps[w]; i++) {
-                    for (size_t k = 0; k < rows.size(); k++) {
-                        if (i >= mns[k]) continue;
-                        const double m = rows[k] - i, n = cols[k] - i;
-                        by += (2.0 * m * n + m) * esz + 16.0 * n;
-                        fl += 4.0 * m * n;
-                    }
-                }
-                prof_.set_work(slot_step[w], by, fl);
-            }
-        }
+        if (prof_.on())  // the stepped path's sizes are decided on the device: priced at the end
+            for (int w = 0; w < nwaves; w++)
+                if (!wbig[w].empty())
+                    wave_work_.push_back({slot_init[w], slot_finish[w], slot_step[w], wsteps[w], wflag0[w] - 1,
+                                          wbig[w].size(), ilvl_});
         const double t_lvl1 = wall();
         lst.t_plan = t_plan1 - t_lvl0;
         lst.t_elim_scale = t_lvl1 - t_plan1;
@@ -1468,6 +1479,7 @@ void Tree::factorize_impl() {
         lst.dofs_left_spars = left;
         lst.fact_nnz = nnz();
         // ---------------- solve plan for this level ----------------
+        const double t_sp0 = wall();
         {
             std::vector<MvPlan> fwd, bwd;
             auto trsv_phase = [&](const std::vector<TrsvTask>& v, bool backward) {
@@ -1574,12 +1586,14 @@ void Tree::factorize_impl() {
             lsv.n_merge = (int)mt.size();
             lsv.children = Packer::at<int2>(dsv, s_mc);
         }
+        lst.t_solve_plan = wall() - t_sp0;
         if (verb_)
             printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, potrfE.size(),
                    potrfS.size(), plist.size(), nwaves, left);
     }
     hip_check(hipDeviceSynchronize(), "factorize");
     t_factor_ = wall() - t_start;
+    price_waves(sizeof(T));
     prof_.collect();
 }
 

@@ -133,7 +133,7 @@ struct LevelStats {
     int dofs_nd = 0, dofs_left_nd = 0, dofs_left_elim = 0, dofs_left_spars = 0;
     long long fact_nnz = 0;
     int rank_before = 0, rank_after = 0, nsparsified = 0, nignored = 0, nwaves = 0;
-    double t_plan = 0, t_elim_scale = 0, t_spars = 0, t_merge = 0;
+    double t_plan = 0, t_elim_scale = 0, t_spars = 0, t_merge = 0, t_solve_plan = 0;
     double flops_potrf = 0, flops_trsm = 0, flops_schur = 0, flops_rrqr = 0;
 };
 
@@ -199,7 +199,17 @@ class Tree {
     // Fine-grained pinned signals of the stepped RRQR: [0] the sequence number of the latest
     // step launch to start, [1..] one done flag per stepped cluster of the current level.
     void signal_reserve(size_t n);
+    void price_waves(size_t esz);
     void wait_signal(long long target);
+    // Profiled stepped-RRQR waves, priced after the factorization from the clusters' final
+    // dims (5 ints each, one block per level), so that no copy waits inside the level loop.
+    struct WaveWork {
+        int slot_init, slot_finish, slot_step, steps;
+        size_t first, n;  // the wave's clusters in its level's dims block
+        int level;
+    };
+    std::vector<WaveWork> wave_work_;
+    std::vector<std::pair<const int*, size_t>> level_dims_;  // per level: dims block, clusters
     long long* h_sig_ = nullptr;
     long long* d_sig_ = nullptr;  // device address of h_sig_
     size_t h_sig_cap_ = 0;

@@ -192,6 +192,28 @@ def test_not_spd_reports_status():
         g.factorize()
 
 
+def test_kernel_profile_stepped_rrqr(monkeypatch):
+    """With every cluster on the per-step RRQR path, the stepped families are timed and priced
+    from the sizes and ranks the device decided (read once the factorization has finished)."""
+    say("case profile stepped rrqr")
+    monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
+    monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
+    A, X = neglapl(12, 3), linspace_nd(12, 3)
+    g = gpu_tree(A, X, 6, 1e-2)
+    g.set_profiling(True)
+    g.factorize()
+    prof = g.kernel_profile()
+    for fam in ("rrqr_init", "rrqr_step", "rrqr_finish"):
+        assert fam in prof and prof[fam]["launches"] > 0 and prof[fam]["ms"] > 0, (fam, prof.get(fam))
+        assert prof[fam]["bytes"] > 0, (fam, prof[fam])
+    assert prof["rrqr_step"]["flops"] > 0
+    # a second factorization prices its own waves: the totals double
+    g.factorize()
+    prof2 = g.kernel_profile()
+    for fam in ("rrqr_init", "rrqr_step", "rrqr_finish"):
+        assert prof2[fam]["bytes"] == pytest.approx(2 * prof[fam]["bytes"]), fam
+
+
 def test_kernel_profile_and_device_pcg():
     """Kernel timer totals are consistent with the launches, and PCG on device-resident
     right-hand sides gives the same iterates as the host-array entry point."""
