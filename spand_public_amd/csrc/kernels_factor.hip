@@ -307,19 +307,30 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
             Lk[i][k] = L[(kb + i) + (size_t)(kb + k) * n];
         }
         __syncthreads();
-        if (tid < m) {
+        if (tid < m) {  // column c of the diagonal block, held in registers
             const int c = tid;
-            for (int k = 0; k < nb; k++) {
-                T b;
-                if (left) {
-                    b = Bk[k][c] / Lk[k][k];
-                } else {
-                    const T r = T(1) / Lk[k][k];
-                    b = Bk[k][c] * r;
+            T x[PB];
+#pragma unroll
+            for (int k = 0; k < PB; k++) x[k] = k < nb ? Bk[k][c] : T(0);
+#pragma unroll
+            for (int k = 0; k < PB; k++) {
+                if (k < nb) {
+                    T b;
+                    if (left) {
+                        b = x[k] / Lk[k][k];
+                    } else {
+                        const T r = T(1) / Lk[k][k];
+                        b = x[k] * r;
+                    }
+                    x[k] = b;
+#pragma unroll
+                    for (int i = k + 1; i < PB; i++)
+                        if (i < nb) x[i] = fma(-b, Lk[i][k], x[i]);
                 }
-                Bk[k][c] = b;
-                for (int i = k + 1; i < nb; i++) Bk[i][c] = fma(-b, Lk[i][k], Bk[i][c]);
             }
+#pragma unroll
+            for (int k = 0; k < PB; k++)
+                if (k < nb) Bk[k][c] = x[k];
         }
         __syncthreads();
         for (int idx = tid; idx < nb * m; idx += NT) {
