@@ -19,12 +19,13 @@ template <typename T>
 hipError_t launch_scatter(const long long* off, const double* val, long long n, T* base, hipStream_t st);
 template <typename T> hipError_t launch_potrf(const PotrfTask* d_tasks, int ntasks, int* d_info, hipStream_t st);
 // Pivots larger than POTRF_SMALL_MAX: one panel of 32 columns per call pair, kb = 0, 32, ...
-// (launch_potrf_trail's grid is sized by nmax, the largest pivot of the batch).
+// (both grids are sized by nmax, the largest pivot of the batch).
 constexpr int POTRF_SMALL_MAX = 128;
 // Pivots of at most 64 rows: one wavefront per pivot.
 template <typename T> hipError_t launch_potrf64(const PotrfTask* d_tasks, int ntasks, int* d_info, hipStream_t st);
 template <typename T>
-hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int* d_info, hipStream_t st);
+hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, int* d_info,
+                              hipStream_t st);
 template <typename T>
 hipError_t launch_potrf_trail(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, hipStream_t st);
 template <typename T> hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st);

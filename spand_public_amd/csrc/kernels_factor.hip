@@ -53,21 +53,90 @@ constexpr int PB = 32;
 // the later panel columns receive fma(-a_k, L(k2,k), a_k2), in k order: the element
 // recurrence of the unblocked right-looking factorization.
 template <typename T>
-__device__ inline void panel_rows(T* A, int n, int kb, int nb, const T (*D)[PB + 1], const T* rdiag) {
+__device__ inline void panel_rows(T* A, int n, int kb, int nb, const T (*D)[PB + 1], const T* rdiag, int i0, int i1) {
 #pragma clang fp contract(off)
-    for (int i = kb + nb + threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        T* Ai = A + i + (size_t)kb * n;  // row i of the panel, held in registers
+        T x[PB];
+#pragma unroll
+        for (int k = 0; k < PB; k++) x[k] = k < nb ? Ai[(size_t)k * n] : T(0);
+        // a rolled loop over the columns: x is indexed by the unrolled q / k2 only (unrolled
+        // over k, the block's reads would all be scheduled ahead and spill)
+#pragma clang loop unroll(disable)
         for (int k = 0; k < nb; k++) {
-            const T a = A[i + (size_t)(kb + k) * n] * rdiag[k];
-            A[i + (size_t)(kb + k) * n] = a;
-            for (int k2 = k + 1; k2 < nb; k2++)
-                A[i + (size_t)(kb + k2) * n] = fma(-a, D[k2][k], A[i + (size_t)(kb + k2) * n]);
+            const T r = rdiag[k];
+            T a = T(0);
+#pragma unroll
+            for (int q = 0; q < PB; q++) {
+                const T xs = x[q] * r;
+                a = q == k ? xs : a;
+                x[q] = q == k ? xs : x[q];
+            }
+            // every block entry of column k is read (in bounds) and selected, no branches:
+            // the reads then issue together instead of one round trip per row
+#pragma unroll
+            for (int k2 = 1; k2 < PB; k2++) {
+                const T d = D[k2][k];
+                const T u = fma(-a, d, x[k2]);
+                x[k2] = (k2 > k && k2 < nb) ? u : x[k2];
+            }
         }
+#pragma unroll
+        for (int k = 0; k < PB; k++)
+            if (k < nb) Ai[(size_t)k * n] = x[k];
     }
+}
+
+// Unblocked right-looking Cholesky of the nb x nb diagonal block D (nb <= 32) by wavefront
+// 0, lane a holding row a in registers and column j broadcast by shuffles: L(j,j) =
+// sqrt(d), rdiag[j] = 1 / L(j,j), L(a,j) = A(a,j) * rdiag[j], then A(a,b) = fma(-L(a,j),
+// L(b,j), A(a,b)) for a >= b > j, in j order. On a non-positive pivot at column j, *bad
+// becomes kb + j + 1 and the block is left as it stands. Every thread calls; ends with a
+// barrier.
+template <typename T>
+__device__ inline void chol_diag_wave(T (*D)[PB + 1], T* rdiag, int nb, int kb, int* bad) {
+#pragma clang fp contract(off)
+    if (threadIdx.x < 64) {
+        const int a = threadIdx.x;
+        const bool row = a < nb;
+        T x[PB];
+#pragma unroll
+        for (int b = 0; b < PB; b++) x[b] = (row && b < nb) ? D[a][b] : T(0);
+        int fail = 0;
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            if (j < nb) {  // uniform
+                const T d = __shfl(x[j], j);
+                const bool ok = d > T(0);
+                if (!ok && !fail) fail = kb + j + 1;  // the block is not used after a failure
+                const T ljj = ok ? T(sqrt(d)) : T(1);
+                const T rd = T(1) / ljj;
+                if (a == j) rdiag[j] = rd;
+                x[j] = a == j ? ljj : (a > j ? x[j] * rd : x[j]);
+                const T laj = x[j];
+                // all shuffles issue together (no branch per b); rows a >= b < nb update
+#pragma unroll
+                for (int b = j + 1; b < PB; b++) {
+                    const T lbj = __shfl(laj, b);
+                    const T u = fma(-laj, lbj, x[b]);
+                    x[b] = (row && a >= b && b < nb) ? u : x[b];
+                }
+            }
+        }
+        if (row)
+#pragma unroll
+            for (int b = 0; b < PB; b++)
+                if (b < nb) D[a][b] = x[b];
+        if (a == 0 && fail) *bad = fail;
+    }
+    __syncthreads();
 }
 
 template <typename T>
 __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info) {
 #pragma clang fp contract(off)
+    static_assert(POTRF_SMALL_MAX <= 128 + PB, "panel staging holds n - 32 rows");
+    __shared__ T P[128][PB + 1];  // rows below the diagonal block of the current panel
     __shared__ T D[PB][PB + 1];
     __shared__ T rdiag[PB];
     __shared__ int bad;
@@ -85,44 +154,29 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
         }
         __syncthreads();
         // diagonal block: unblocked right-looking
-        for (int j = 0; j < nb; j++) {
-            if (tid == 0) {
-                T d = D[j][j];
-                if (!(d > T(0))) {
-                    if (!bad) bad = kb + j + 1;
-                } else {
-                    D[j][j] = sqrt(d);
-                }
-                rdiag[j] = T(1) / D[j][j];
-            }
-            __syncthreads();
-            if (bad) break;
-            const T r = rdiag[j];
-            for (int i = j + 1 + tid; i < nb; i += NT) D[i][j] *= r;
-            __syncthreads();
-            const int mm = nb - j - 1;
-            for (int idx = tid; idx < mm * mm; idx += NT) {
-                int a = j + 1 + idx % mm, b = j + 1 + idx / mm;
-                if (a >= b) D[a][b] = fma(-D[a][j], D[b][j], D[a][b]);
-            }
-            __syncthreads();
-        }
+        chol_diag_wave<T>(D, rdiag, nb, kb, &bad);
         if (bad) break;
         for (int idx = tid; idx < nb * nb; idx += NT) {
             int i = idx % nb, k = idx / nb;
             if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = D[i][k];
         }
         // panel rows i >= kb + nb: scale / update across the block's columns in order
-        panel_rows<T>(A, n, kb, nb, D, rdiag);
+        panel_rows<T>(A, n, kb, nb, D, rdiag, kb + nb, n);
         __syncthreads();
-        // trailing lower update by the block's columns, k in order
+        // trailing lower update by the block's columns, k in order, the panel staged in LDS
         const int mt = n - kb - nb;
-        for (long long idx = tid; idx < (long long)mt * mt; idx += NT) {
-            int a = kb + nb + (int)(idx % mt), b = kb + nb + (int)(idx / mt);
-            if (a < b) continue;
-            T acc = A[a + (size_t)b * n];
-            for (int k = 0; k < nb; k++) acc = fma(-A[a + (size_t)(kb + k) * n], A[b + (size_t)(kb + k) * n], acc);
-            A[a + (size_t)b * n] = acc;
+        for (int idx = tid; idx < mt * nb; idx += NT) {
+            const int i = idx % mt, k = idx / mt;
+            P[i][k] = A[(kb + nb + i) + (size_t)(kb + k) * n];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < mt * mt; idx += NT) {
+            const int i = idx % mt, j = idx / mt;
+            if (i < j) continue;
+            T* a = A + (kb + nb + i) + (size_t)(kb + nb + j) * n;
+            T acc = *a;
+            for (int k = 0; k < nb; k++) acc = fma(-P[i][k], P[j][k], acc);
+            *a = acc;
         }
         __syncthreads();
     }
@@ -173,60 +227,45 @@ __global__ void __launch_bounds__(64) k_potrf64(const PotrfTask* tasks, int* inf
         for (int b = 0; b <= a; b++) A[a + (size_t)b * n] = L[a][b];
 }
 
-// Large pivots take the same blocked algorithm one panel per launch pair, so the trailing
-// update of every panel spreads over many workgroups: k_potrf_panel factors the diagonal
-// block kb and scales / updates the panel rows below it (one workgroup per pivot), then
-// k_potrf_trail applies the panel to the trailing lower triangle, one 32 x 32 tile per
-// workgroup. Every element sees the same operations in the same order as in k_potrf.
+// Large pivots take the same blocked algorithm one panel per launch pair, so the work of
+// every panel spreads over many workgroups: k_potrf_panel (grid: blocks of NT panel rows x
+// pivots) factors the diagonal block kb in every workgroup and scales / updates its NT
+// rows below it, then k_potrf_trail stores the factored diagonal block (from dbuf) and applies the
+// panel to the trailing lower triangle, one 32 x 32 tile per workgroup. Every element sees
+// the same operations in the same order as in k_potrf.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_potrf_panel(const PotrfTask* tasks, int kb, int* info) {
 #pragma clang fp contract(off)
     __shared__ T D[PB][PB + 1];
     __shared__ T rdiag[PB];
     __shared__ int bad;
-    const PotrfTask t = tasks[blockIdx.x];
+    const PotrfTask t = tasks[blockIdx.y];
     const int n = t.n;
     const int tid = threadIdx.x;
     if (kb >= n) return;
+    const int nb = min(PB, n - kb);
+    const int r0 = kb + nb + blockIdx.x * NT;  // this workgroup's panel rows [r0, r0 + NT)
+    if (blockIdx.x > 0 && r0 >= n) return;
     T* A = reinterpret_cast<T*>(t.A);
     if (tid == 0) bad = 0;
-    const int nb = min(PB, n - kb);
     for (int idx = tid; idx < nb * nb; idx += NT) {
         int i = idx % nb, k = idx / nb;
         D[i][k] = A[(kb + i) + (size_t)(kb + k) * n];
     }
     __syncthreads();
-    for (int j = 0; j < nb; j++) {
-        if (tid == 0) {
-            T d = D[j][j];
-            if (!(d > T(0))) {
-                if (!bad) bad = kb + j + 1;
-            } else {
-                D[j][j] = sqrt(d);
-            }
-            rdiag[j] = T(1) / D[j][j];
-        }
-        __syncthreads();
-        if (bad) break;
-        const T r = rdiag[j];
-        for (int i = j + 1 + tid; i < nb; i += NT) D[i][j] *= r;
-        __syncthreads();
-        const int mm = nb - j - 1;
-        for (int idx = tid; idx < mm * mm; idx += NT) {
-            int a = j + 1 + idx % mm, b = j + 1 + idx / mm;
-            if (a >= b) D[a][b] = fma(-D[a][j], D[b][j], D[a][b]);
-        }
-        __syncthreads();
-    }
+    // every workgroup of the task factors the block itself; workgroup 0 leaves it in dbuf
+    // for k_potrf_trail to store (a store into A here could reach another workgroup of this
+    // launch before it reads the block)
+    chol_diag_wave<T>(D, rdiag, nb, kb, &bad);
     if (bad) {
-        if (tid == 0) atomicCAS(info, 0, t.cluster + 1);
+        if (blockIdx.x == 0 && tid == 0) atomicCAS(info, 0, t.cluster + 1);
         return;
     }
-    for (int idx = tid; idx < nb * nb; idx += NT) {
-        int i = idx % nb, k = idx / nb;
-        if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = D[i][k];
+    if (blockIdx.x == 0) {
+        T* db = reinterpret_cast<T*>(t.dbuf);
+        for (int idx = tid; idx < nb * nb; idx += NT) db[idx] = D[idx % nb][idx / nb];
     }
-    panel_rows<T>(A, n, kb, nb, D, rdiag);
+    panel_rows<T>(A, n, kb, nb, D, rdiag, r0, min(n, r0 + NT));
 }
 
 // grid (tiles, tasks): tile index -> (bi, bj), bj <= bi, over the trailing lower triangle
@@ -237,6 +276,16 @@ __global__ void __launch_bounds__(NT) k_potrf_trail(const PotrfTask* tasks, int 
     __shared__ T Bs[PB][PB + 1];
     const PotrfTask t = tasks[blockIdx.y];
     const int n = t.n, tid = threadIdx.x;
+    if (kb >= n) return;
+    if (blockIdx.x == 0) {  // the panel's factored diagonal block, left in dbuf by k_potrf_panel
+        const int nb = min(PB, n - kb);
+        T* A = reinterpret_cast<T*>(t.A);
+        const T* db = reinterpret_cast<const T*>(t.dbuf);
+        for (int idx = tid; idx < nb * nb; idx += NT) {
+            int i = idx % nb, k = idx / nb;
+            if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = db[idx];
+        }
+    }
     const int r0 = kb + PB;  // a full panel precedes any trailing rows
     if (r0 >= n) return;
     const int nt = (n - r0 + PB - 1) / PB;
@@ -1668,17 +1717,20 @@ hipError_t launch_potrf64(const PotrfTask* d_tasks, int ntasks, int* d_info, hip
 }
 
 template <typename T>
-hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int* d_info, hipStream_t st) {
-    if (ntasks == 0) return hipSuccess;
-    k_potrf_panel<T><<<ntasks, NT, 0, st>>>(d_tasks, kb, d_info);
+hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, int* d_info, hipStream_t st) {
+    if (ntasks == 0 || kb >= nmax) return hipSuccess;
+    const int rows = nmax - kb - PB;  // panel rows of the largest pivot, NT per workgroup
+    const int nblk = rows > 0 ? (rows + NT - 1) / NT : 1;
+    k_potrf_panel<T><<<dim3(nblk, ntasks), NT, 0, st>>>(d_tasks, kb, d_info);
     return hipGetLastError();
 }
 
+// Always at least one workgroup per task: workgroup 0 stores the panel's diagonal block.
 template <typename T>
 hipError_t launch_potrf_trail(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, hipStream_t st) {
-    const int nt = (nmax - kb - PB + PB - 1) / PB;
-    if (ntasks == 0 || nt <= 0) return hipSuccess;
-    k_potrf_trail<T><<<dim3(nt * (nt + 1) / 2, ntasks), NT, 0, st>>>(d_tasks, kb);
+    if (ntasks == 0 || kb >= nmax) return hipSuccess;
+    const int nt = std::max((nmax - kb - PB + PB - 1) / PB, 0);
+    k_potrf_trail<T><<<dim3(std::max(nt * (nt + 1) / 2, 1), ntasks), NT, 0, st>>>(d_tasks, kb);
     return hipGetLastError();
 }
 
@@ -1802,7 +1854,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_scatter<T>(const long long*, const double*, long long, T*, hipStream_t); \
     template hipError_t launch_potrf<T>(const PotrfTask*, int, int*, hipStream_t);                \
     template hipError_t launch_potrf64<T>(const PotrfTask*, int, int*, hipStream_t);              \
-    template hipError_t launch_potrf_panel<T>(const PotrfTask*, int, int, int*, hipStream_t);     \
+    template hipError_t launch_potrf_panel<T>(const PotrfTask*, int, int, int, int*, hipStream_t);     \
     template hipError_t launch_potrf_trail<T>(const PotrfTask*, int, int, int, hipStream_t);      \
     template hipError_t launch_trsm<T>(const TrsmTask*, int, hipStream_t);                        \
     template hipError_t launch_getrf<T>(const GetrfTask*, int, int, int, int*, hipStream_t);      \

@@ -13,6 +13,8 @@ struct PotrfTask {
     void* A;
     int n;
     int cluster;  // reported through the info array on a non-SPD pivot
+    void* dbuf;   // pivots above POTRF_SMALL_MAX: the factored 32 x 32 diagonal block of the
+                  // current panel, handed from k_potrf_panel to k_potrf_trail
 };
 
 // Triangular solve of (a 64-wide chunk of) an edge block against a pivot factor L

@@ -744,7 +744,7 @@ void Tree::factorize_impl() {
                 set_eliminated(c);
                 continue;
             }
-            potrfE.push_back({piv.A, n, c});
+            potrfE.push_back({piv.A, n, c, nullptr});
             lst.flops_potrf += potrf_flops(n);
             std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
             for (int e : in) {
@@ -913,7 +913,7 @@ void Tree::factorize_impl() {
             for (int c : live) {
                 HCluster& s = cls_[c];
                 const HEdge& piv = edges_[s.out[0]];
-                potrfS.push_back({piv.A, s.size, c});
+                potrfS.push_back({piv.A, s.size, c, nullptr});
                 lst.flops_potrf += potrf_flops(s.size);
                 ops_.push_back({OP_SCALING_LLT, c, -1, s.size, 0, s.size, s.size, ilvl_, piv.A, nullptr});
                 factor_bytes_ += (size_t)s.size * (s.size + 1) / 2 * sizeof(T);
@@ -1096,6 +1096,9 @@ void Tree::factorize_impl() {
         };
         small_first(potrfE);
         small_first(potrfS);
+        for (auto* v : {&potrfE, &potrfS})
+            for (PotrfTask& t : *v)
+                if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
         Packer pk;
         size_t o_pe = pk.add(potrfE), o_te = pk.add(trsmE), o_ti = pk.add(tiles), o_co = pk.add(contribs);
         size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
@@ -1168,7 +1171,7 @@ void Tree::factorize_impl() {
                     tf += rest * (rest + 1) * nb;
                 }
                 hip_check(timed(prof_, KF_POTRF_PANEL, st, pb, pf,
-                                [&] { return launch_potrf_panel<T>(db, nbig, kb, d_info_, st); }),
+                                [&] { return launch_potrf_panel<T>(db, nbig, kb, nmax, d_info_, st); }),
                           what);
                 hip_check(timed(prof_, KF_POTRF_TRAIL, st, tb, tf,
                                 [&] { return launch_potrf_trail<T>(db, nbig, kb, nmax, st); }),
