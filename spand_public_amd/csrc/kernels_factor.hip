@@ -819,7 +819,11 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
     const int cols = s_col0[t.nseg];
     if (rows == 0) return;
     const int mn = min(rows, cols);
-    T* A = ((long long)rows * cols <= lds_elems) ? reinterpret_cast<T*>(smem_raw) : reinterpret_cast<T*>(t.scratch);
+    // column stride: odd in LDS, so that the threads of a wavefront (one column each) fall
+    // in different banks; rows in the global scratch
+    const bool in_lds = (long long)(rows | 1) * cols <= lds_elems;
+    const int ld = in_lds ? (rows | 1) : rows;
+    T* A = in_lds ? reinterpret_cast<T*>(smem_raw) : reinterpret_cast<T*>(t.scratch);
     double* vn1 = cols <= RRQR_LDS_COLS ? s_vn1 : t.norms;
     double* vn2 = cols <= RRQR_LDS_COLS ? s_vn2 : t.norms + t.max_cols;
     int* jp = cols <= RRQR_LDS_COLS ? s_jp : t.jpvt;
@@ -838,11 +842,11 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
         int k = seg_of(j);
         int jj = j - s_col0[k], w = s_col0[k + 1] - s_col0[k];
         const T* E = reinterpret_cast<const T*>(sg[k].E);
-        A[idx] = sg[k].out ? E[jj + (size_t)i * w] : E[i + (size_t)jj * rows];
+        A[i + (size_t)j * ld] = sg[k].out ? E[jj + (size_t)i * w] : E[i + (size_t)jj * rows];
     }
     __syncthreads();
     for (int j = tid; j < cols; j += NT) {
-        const T* a = A + (size_t)j * rows;
+        const T* a = A + (size_t)j * ld;
         jp[j] = j;
         vn1[j] = sqrt(sumsq16_serial(rows, [&](int r) { return (double)a[r]; }));
         vn2[j] = vn1[j];
@@ -879,9 +883,9 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
         const int pvt = (r_idx[0] >= i && r_idx[0] < cols) ? r_idx[0] : i;
         if (pvt != i) {
             for (int r = tid; r < rows; r += NT) {
-                T tmp = A[r + (size_t)pvt * rows];
-                A[r + (size_t)pvt * rows] = A[r + (size_t)i * rows];
-                A[r + (size_t)i * rows] = tmp;
+                T tmp = A[r + (size_t)pvt * ld];
+                A[r + (size_t)pvt * ld] = A[r + (size_t)i * ld];
+                A[r + (size_t)i * ld] = tmp;
             }
             if (tid == 0) {
                 int tj = jp[pvt]; jp[pvt] = jp[i]; jp[i] = tj;
@@ -891,7 +895,7 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
         }
         __syncthreads();
         // dlarfg on A(i:rows, i)
-        T* ci = A + (size_t)i * rows;
+        T* ci = A + (size_t)i * ld;
         larfg_block<T>((double)ci[i], ci + i + 1, rows - i - 1, sh, s_ch);
         const double taud = sh[0];
         if (taud != 0.0) {
@@ -911,7 +915,7 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
         // apply H(i) to A(i:rows, i+1:cols) (dlarf: w = v^T a_j, skip w = 0 as dger does)
         // and downdate the partial norms
         for (int j = i + 1 + tid; j < cols; j += NT) {
-            T* cj = A + (size_t)j * rows;
+            T* cj = A + (size_t)j * ld;
             if (tau != T(0)) {
                 const T w = dot16_serial<T>(
                     rows - i, [&](int q) { return cj[i + q]; }, [&](int q) { return q == 0 ? T(1) : ci[i + q]; });
@@ -946,7 +950,8 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
     if (rank >= rows) return;
     // Householder vectors v = A[:, :rank] (rows x rank, ld rows)
     T* Q = reinterpret_cast<T*>(t.Q);
-    for (long long idx = tid; idx < (long long)rows * rank; idx += NT) Q[idx] = A[idx];
+    for (long long idx = tid; idx < (long long)rows * rank; idx += NT)
+        Q[idx] = A[(idx % rows) + (size_t)(idx / rows) * ld];
     // AsnP[:, jpvt[j]] = triu(R)[:rank, j], scattered into the shrunk edge blocks
     for (int j = tid; j < cols; j += NT) {
         const int p = jp[j];
@@ -954,7 +959,7 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
         int k = seg_of(p);
         int jj = p - s_col0[k], w = s_col0[k + 1] - s_col0[k];
         T* E = reinterpret_cast<T*>(sg[k].E);
-        const T* rj = A + (size_t)j * rows;
+        const T* rj = A + (size_t)j * ld;
         if (sg[k].out) {
             for (int r = 0; r < rank; r++) E[jj + (size_t)r * w] = (r <= j) ? rj[r] : T(0);
         } else {

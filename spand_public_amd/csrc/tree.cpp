@@ -1026,10 +1026,12 @@ void Tree::factorize_impl() {
                     wsteps[w] = std::max(wsteps[w], mn);
                     wmaxrows[w] = std::max(wmaxrows[w], s.size);
                 } else {
-                    if ((long long)s.size * cols > lds_elems)
+                    // LDS-resident with an odd column stride (rows | 1), else a global scratch
+                    if ((long long)(s.size | 1) * cols > lds_elems)
                         t.scratch = arena_.alloc((size_t)s.size * cols * sizeof(T));
                     else
-                        wlds_small[w] = std::max(wlds_small[w], (long long)s.size * cols * (long long)sizeof(T));
+                        wlds_small[w] =
+                            std::max(wlds_small[w], (long long)(s.size | 1) * cols * (long long)sizeof(T));
                     if (cols > 1024) {
                         t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * cols * sizeof(double)));
                         t.jpvt = reinterpret_cast<int*>(arena_.alloc((size_t)cols * sizeof(int)));
