@@ -48,7 +48,7 @@ FLOP_BOUND = {"potrf", "trsm", "schur"}
 # 256 MiB device copy (FETCH_SIZE reads 0.5x of the copied bytes, WRITE_SIZE 1.0x).
 # Keyed by (family, factor dtype, grid edge n, levels); other runs report traffic null.
 TRAFFIC_MEASURED = {
-    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 2609.97, "write_kib": 3496.41, "fetch_factor": 2.0,
+    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 3442.37, "write_kib": 4629.85, "fetch_factor": 2.0,
                                    "write_factor": 1.0},
 }
 
