@@ -221,9 +221,10 @@ def main():
                 f"{k['bytes'] / max(k['ms'], 1e-9) / 1e6:9.1f} GB/s {k['flops'] / max(k['ms'], 1e-9) / 1e9:9.3f} TFLOP/s"
                 "   (warmup, every launch timed)")
         ls = t.level_stats()
-        log("   level  plan_ms  device_ms  ranks_ms  merge_ms   (last factorization; host planning vs device)")
+        log("   level  plan_ms  device_ms  ranks_ms  merge_ms  solveplan_ms   (last factorization; host vs device)")
         for l in range(nl):
-            log(f"   {l:5d} {ls[l, 10] * 1e3:8.2f} {ls[l, 11] * 1e3:10.2f} {ls[l, 12] * 1e3:9.2f} {ls[l, 13] * 1e3:9.2f}")
+            log(f"   {l:5d} {ls[l, 10] * 1e3:8.2f} {ls[l, 11] * 1e3:10.2f} {ls[l, 12] * 1e3:9.2f} {ls[l, 13] * 1e3:9.2f}"
+                f" {ls[l, 18] * 1e3:13.2f}")
         if not res <= KRYLOV_TOL:
             raise RuntimeError(f"{name}: true residual {res:.2e} above {KRYLOV_TOL}")
         del t

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <numeric>
 #include <unordered_map>
 
@@ -593,6 +594,128 @@ static long long env_or(const char* name, long long dflt) {
     return (e && *e) ? std::strtoll(e, nullptr, 10) : dflt;
 }
 
+// One Orthogonal op of the solve: y = -V^T x, z = -T^T y (forward) or -T y (backward),
+// x -= V z, with X = T^T stored (compact WY over its kept reflectors).
+struct QOp {
+    const void* Ve;
+    const void* X;
+    int m, k, xoff, yoff;
+};
+
+// Host side of one level's preconditioner-apply plan, before upload.
+struct SolvePlanHost {
+    std::vector<MvPlan> fwd, bwd;
+    std::vector<MergeTask> mt;
+    std::vector<int2> mchildren;
+    long long max_pieces = 0;
+};
+
+// The level's mv phases: forward {trsv Elim, gemv, trsv Scale, Q 0..2}, backward
+// {Q 0..2, trsv Scale', gemv, trsv Elim'}, each split into chunks and pieces.
+static SolvePlanHost build_solve_plan(size_t esz, const std::vector<TrsvTask>& sv_elim,
+                                      const std::vector<TrsvTask>& sv_scale, const std::vector<GemvGroup>& fwd_groups,
+                                      const std::vector<GemvTerm>& fwd_t, const std::vector<GemvGroup>& bwd_groups,
+                                      const std::vector<GemvTerm>& bwd_terms, const std::vector<QOp>& qops,
+                                      std::vector<MergeTask> mt, std::vector<int2> mchildren) {
+    SolvePlanHost h;
+    auto trsv_phase = [&](const std::vector<TrsvTask>& v, bool backward) {
+        MvPlan m;
+        m.family = KF_TRMV;
+        m.inplace = true;
+        for (auto& t : v) {
+            const int k0 = (int)m.terms.size();
+            if (!backward) m.terms.push_back({t.L, t.perm, t.n, t.n, t.xoff, 0, 1, 0});
+            else m.terms.push_back({t.Lb, nullptr, t.n, t.n, t.xoff, 1, 2, 0});
+            m.groups.push_back({t.xoff, t.n, k0, k0 + 1, MV_SET, 0});
+        }
+        return m;
+    };
+    auto gemv_phase = [&](const std::vector<GemvGroup>& gs, const std::vector<GemvTerm>& ts) {
+        MvPlan m;
+        m.family = KF_GEMV;
+        for (auto& g : gs) {
+            const int k0 = (int)m.terms.size();
+            for (int k = g.k0; k < g.k1; k++)
+                m.terms.push_back({ts[k].A, nullptr, ts[k].rows, ts[k].cols, ts[k].xsrc, ts[k].trans, 0, 0});
+            m.groups.push_back({g.xdst, g.n, k0, (int)m.terms.size(), MV_SUB, 0});
+        }
+        return m;
+    };
+    auto q_phase = [&](int step, bool backward) {
+        MvPlan m;
+        m.family = KF_QAPPLY;
+        for (auto& q : qops) {
+            const int k0 = (int)m.terms.size();
+            if (step == 0) {
+                m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.xoff, 1, 2, 0});
+                m.groups.push_back({q.yoff, q.k, k0, k0 + 1, MV_SETNEG, 0});
+            } else if (step == 1) {  // X = T^T lower: z = -X y (fwd), -X^T y (bwd)
+                m.terms.push_back({q.X, nullptr, q.k, q.k, q.yoff, backward ? 1 : 0, backward ? 2 : 1, 0});
+                m.groups.push_back({q.yoff + q.k, q.k, k0, k0 + 1, MV_SETNEG, 0});
+            } else {
+                m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.yoff + q.k, 0, 1, 0});
+                m.groups.push_back({q.xoff, q.m, k0, k0 + 1, MV_SUB, 0});
+            }
+        }
+        return m;
+    };
+    h.fwd.push_back(trsv_phase(sv_elim, false));
+    h.fwd.push_back(gemv_phase(fwd_groups, fwd_t));
+    h.fwd.push_back(trsv_phase(sv_scale, false));
+    for (int q = 0; q < 3; q++) h.fwd.push_back(q_phase(q, false));
+    for (int q = 0; q < 3; q++) h.bwd.push_back(q_phase(q, true));
+    h.bwd.push_back(trsv_phase(sv_scale, true));
+    h.bwd.push_back(gemv_phase(bwd_groups, bwd_terms));
+    h.bwd.push_back(trsv_phase(sv_elim, true));
+    for (auto* v : {&h.fwd, &h.bwd})
+        for (auto& m : *v) {
+            m.split(esz);
+            h.max_pieces = std::max(h.max_pieces, (long long)m.pieces.size());
+        }
+    h.mt = std::move(mt);
+    h.mchildren = std::move(mchildren);
+    return h;
+}
+
+// Uploads one level's solve plan and points the level's phases at the device copy.
+void Tree::upload_solve_plan(int level, void* host_plan) {
+    SolvePlanHost& h = *static_cast<SolvePlanHost*>(host_plan);
+    LevelSolve& lsv = solve_plan_[level];
+    Packer ps;
+    std::vector<std::array<size_t, 5>> offs;
+    for (auto* v : {&h.fwd, &h.bwd})
+        for (auto& m : *v)
+            offs.push_back({ps.add(m.groups), ps.add(m.terms), ps.add(m.chunks), ps.add(m.pieces), ps.add(m.rchunks)});
+    const size_t s_m = ps.add(h.mt), s_mc = ps.add(h.mchildren);
+    void* dsv = upload_tasks(ps);
+    size_t oi = 0;
+    auto to_dev = [&](std::vector<MvPlan>& v, std::vector<MvPhase>& out) {
+        out.clear();
+        for (auto& m : v) {
+            const auto& o = offs[oi++];
+            if (m.pieces.empty()) continue;
+            MvPhase ph;
+            ph.groups = Packer::at<MvGroup>(dsv, o[0]);
+            ph.terms = Packer::at<MvTerm>(dsv, o[1]);
+            ph.chunks = Packer::at<MvChunk>(dsv, o[2]);
+            ph.pieces = Packer::at<MvPiece>(dsv, o[3]);
+            ph.rchunks = Packer::at<int>(dsv, o[4]);
+            ph.npieces = (int)m.pieces.size();
+            ph.nreduce = (int)m.rchunks.size();
+            ph.family = m.family;
+            ph.w = m.work;
+            out.push_back(ph);
+        }
+    };
+    to_dev(h.fwd, lsv.fwd);
+    to_dev(h.bwd, lsv.bwd);
+    for (auto& c : h.mchildren) lsv.w_merge.vec_elems += 2.0 * c.y;
+    lsv.merge = Packer::at<MergeTask>(dsv, s_m);
+    lsv.n_merge = (int)h.mt.size();
+    lsv.children = Packer::at<int2>(dsv, s_mc);
+    max_pieces_ = std::max(max_pieces_, h.max_pieces);
+}
+
 template <typename T>
 void Tree::factorize_impl() {
     const double t_start = wall();
@@ -601,6 +724,7 @@ void Tree::factorize_impl() {
     task_arena_.reset();
     wave_work_.clear();
     level_dims_.clear();
+    std::vector<std::future<SolvePlanHost>> pending_solve(nlevels_);  // joined on every exit
     hip_check(hipDeviceSynchronize(), "factorize");
     h_stage_used_ = 0;
     ops_.clear();
@@ -1323,7 +1447,6 @@ void Tree::factorize_impl() {
         lst.t_plan = t_plan1 - t_lvl0;
         lst.t_elim_scale = t_lvl1 - t_plan1;
         // ---------------- apply ranks: Orthogonal, Split, sibling (src/tree.cpp:1317-1342) ----------------
-        struct QOp { const void* Ve; const void* X; int m, k, xoff, yoff; };
         std::vector<QOp> qops;
         std::vector<WyTask> wy;
         std::vector<int4> wy_tiles;
@@ -1484,32 +1607,11 @@ void Tree::factorize_impl() {
         lst.dofs_left_spars = left;
         lst.fact_nnz = nnz();
         // ---------------- solve plan for this level ----------------
+        // The forward terms are grouped here (the grouping reads cluster offsets, and the
+        // cluster array grows at the next merge); the rest is built on a worker thread from
+        // moved copies of this level's lists and uploaded once the factorization is done.
         const double t_sp0 = wall();
         {
-            std::vector<MvPlan> fwd, bwd;
-            auto trsv_phase = [&](const std::vector<TrsvTask>& v, bool backward) {
-                MvPlan m;
-                m.family = KF_TRMV;
-                m.inplace = true;
-                for (auto& t : v) {
-                    const int k0 = (int)m.terms.size();
-                    if (!backward) m.terms.push_back({t.L, t.perm, t.n, t.n, t.xoff, 0, 1, 0});
-                    else m.terms.push_back({t.Lb, nullptr, t.n, t.n, t.xoff, 1, 2, 0});
-                    m.groups.push_back({t.xoff, t.n, k0, k0 + 1, MV_SET, 0});
-                }
-                return m;
-            };
-            auto gemv_phase = [&](const std::vector<GemvGroup>& gs, const std::vector<GemvTerm>& ts) {
-                MvPlan m;
-                m.family = KF_GEMV;
-                for (auto& g : gs) {
-                    const int k0 = (int)m.terms.size();
-                    for (int k = g.k0; k < g.k1; k++)
-                        m.terms.push_back({ts[k].A, nullptr, ts[k].rows, ts[k].cols, ts[k].xsrc, ts[k].trans, 0, 0});
-                    m.groups.push_back({g.xdst, g.n, k0, (int)m.terms.size(), MV_SUB, 0});
-                }
-                return m;
-            };
             std::vector<GemvGroup> fwd_groups;
             std::vector<GemvTerm> fwd_t;
             {
@@ -1524,78 +1626,26 @@ void Tree::factorize_impl() {
                     fwd_groups.push_back({(int)cls_[gkeys[gi]].xoff, n, k0, (int)fwd_t.size()});
                 }
             }
-            // Orthogonal: y = -V^T x, z = -T^T y (fwd) or -T y (bwd), x -= V z, with X = T^T
-            auto q_phase = [&](int step, bool backward) {
-                MvPlan m;
-                m.family = KF_QAPPLY;
-                for (auto& q : qops) {
-                    const int k0 = (int)m.terms.size();
-                    if (step == 0) {
-                        m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.xoff, 1, 2, 0});
-                        m.groups.push_back({q.yoff, q.k, k0, k0 + 1, MV_SETNEG, 0});
-                    } else if (step == 1) {  // X = T^T lower: z = -X y (fwd), -X^T y (bwd)
-                        m.terms.push_back({q.X, nullptr, q.k, q.k, q.yoff, backward ? 1 : 0, backward ? 2 : 1, 0});
-                        m.groups.push_back({q.yoff + q.k, q.k, k0, k0 + 1, MV_SETNEG, 0});
-                    } else {
-                        m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.yoff + q.k, 0, 1, 0});
-                        m.groups.push_back({q.xoff, q.m, k0, k0 + 1, MV_SUB, 0});
-                    }
-                }
-                return m;
-            };
-            fwd.push_back(trsv_phase(sv_elim, false));
-            fwd.push_back(gemv_phase(fwd_groups, fwd_t));
-            fwd.push_back(trsv_phase(sv_scale, false));
-            for (int q = 0; q < 3; q++) fwd.push_back(q_phase(q, false));
-            for (int q = 0; q < 3; q++) bwd.push_back(q_phase(q, true));
-            bwd.push_back(trsv_phase(sv_scale, true));
-            bwd.push_back(gemv_phase(bwd_groups, bwd_terms));
-            bwd.push_back(trsv_phase(sv_elim, true));
-            Packer ps;
-            std::vector<std::array<size_t, 5>> offs;
-            auto add_all = [&](std::vector<MvPlan>& v) {
-                for (auto& m : v) {
-                    m.split(sizeof(T));
-                    max_pieces_ = std::max(max_pieces_, (long long)m.pieces.size());
-                    offs.push_back({ps.add(m.groups), ps.add(m.terms), ps.add(m.chunks), ps.add(m.pieces),
-                                    ps.add(m.rchunks)});
-                }
-            };
-            add_all(fwd);
-            add_all(bwd);
-            size_t s_m = ps.add(mt), s_mc = ps.add(mchildren);
-            void* dsv = upload_tasks(ps);
-            size_t oi = 0;
-            auto to_dev = [&](std::vector<MvPlan>& v, std::vector<MvPhase>& out) {
-                out.clear();
-                for (auto& m : v) {
-                    const auto& o = offs[oi++];
-                    if (m.pieces.empty()) continue;
-                    MvPhase ph;
-                    ph.groups = Packer::at<MvGroup>(dsv, o[0]);
-                    ph.terms = Packer::at<MvTerm>(dsv, o[1]);
-                    ph.chunks = Packer::at<MvChunk>(dsv, o[2]);
-                    ph.pieces = Packer::at<MvPiece>(dsv, o[3]);
-                    ph.rchunks = Packer::at<int>(dsv, o[4]);
-                    ph.npieces = (int)m.pieces.size();
-                    ph.nreduce = (int)m.rchunks.size();
-                    ph.family = m.family;
-                    ph.w = m.work;
-                    out.push_back(ph);
-                }
-            };
-            to_dev(fwd, lsv.fwd);
-            to_dev(bwd, lsv.bwd);
-            for (auto& c : mchildren) lsv.w_merge.vec_elems += 2.0 * c.y;
-            lsv.merge = Packer::at<MergeTask>(dsv, s_m);
-            lsv.n_merge = (int)mt.size();
-            lsv.children = Packer::at<int2>(dsv, s_mc);
+            pending_solve[ilvl_] = std::async(
+                std::launch::async,
+                [esz = sizeof(T), sv_elim = std::move(sv_elim), sv_scale = std::move(sv_scale),
+                 fwd_groups = std::move(fwd_groups), fwd_t = std::move(fwd_t), bwd_groups = std::move(bwd_groups),
+                 bwd_terms = std::move(bwd_terms), qops = std::move(qops), mt = std::move(mt),
+                 mchildren = std::move(mchildren)]() mutable {
+                    return build_solve_plan(esz, sv_elim, sv_scale, fwd_groups, fwd_t, bwd_groups, bwd_terms, qops,
+                                            std::move(mt), std::move(mchildren));
+                });
         }
         lst.t_solve_plan = wall() - t_sp0;
         if (verb_)
             printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, potrfE.size(),
                    potrfS.size(), plist.size(), nwaves, left);
     }
+    for (int l = 0; l < nlevels_; l++)
+        if (pending_solve[l].valid()) {
+            SolvePlanHost h = pending_solve[l].get();
+            upload_solve_plan(l, &h);
+        }
     hip_check(hipDeviceSynchronize(), "factorize");
     t_factor_ = wall() - t_start;
     price_waves(sizeof(T));

@@ -200,6 +200,7 @@ class Tree {
     // step launch to start, [1..] one done flag per stepped cluster of the current level.
     void signal_reserve(size_t n);
     void price_waves(size_t esz);
+    void upload_solve_plan(int level, void* host_plan);  // host_plan: SolvePlanHost (tree.cpp)
     void wait_signal(long long target);
     // Profiled stepped-RRQR waves, priced after the factorization from the clusters' final
     // dims (5 ints each, one block per level), so that no copy waits inside the level loop.
