@@ -586,7 +586,7 @@ static double geqp3_flops(double m, double n) {
 // launches the next one, few enough that launches past every cluster's rank stay rare.
 constexpr int RRQR_STEPS_AHEAD = 12;
 
-// Clusters with at least 16 rows and more than 16384 Asn elements take the stepped
+// Clusters with at least 16 rows and more than 10000 Asn elements take the stepped
 // multi-workgroup RRQR; smaller ones run in one workgroup. Both paths compute the same
 // numbers; SPAND_RRQR_STEPPED_MIN_ROWS / SPAND_RRQR_STEPPED_MIN_ELEMS override the split.
 static long long env_or(const char* name, long long dflt) {
@@ -767,7 +767,7 @@ void Tree::factorize_impl() {
     // its largest resident task
     const long long lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
-    const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 16384);
+    const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 10000);
     const bool gen = symm_ == 2;
 
     for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
