@@ -86,10 +86,8 @@ __global__ void __launch_bounds__(NT) k_mv_piece(const MvGroup* groups, const Mv
     __shared__ T tile[TR][SC + 1];      // A^T tile: inner rows x outputs
     __shared__ VT xl[TR][RG_MAX];       // the source entries of the current inner tile
     __shared__ VT red[NP][SC][RG_MAX];
-    const MvPiece pc = pieces[blockIdx.x];
-    const MvChunk ch = chunks[pc.chunk];
-    const MvGroup g = groups[ch.group];
-    const int o0 = ch.o0, no = min(SC, g.n - o0);
+    const MvPiece pc = pieces[blockIdx.x];  // carries its chunk's and group's fields
+    const int o0 = pc.o0, no = pc.no;
     const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
     const bool one = pc.k1 == pc.k0 + 1;
     for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
@@ -150,10 +148,10 @@ __global__ void __launch_bounds__(NT) k_mv_piece(const MvGroup* groups, const Mv
         __syncthreads();
         if (kp == 0 && lane < no) {
             if (pc.direct) {
-                VT* y = ws + (size_t)(g.xdst + o0 + lane) * nrhs + r0;
+                VT* y = ws + (size_t)(pc.xdst + o0 + lane) * nrhs + r0;
                 for (int r = 0; r < rg; r++) {
                     const VT sum = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
-                    y[r] = g.mode == MV_SUB ? y[r] - sum : (g.mode == MV_SETNEG ? -sum : sum);
+                    y[r] = pc.mode == MV_SUB ? y[r] - sum : (pc.mode == MV_SETNEG ? -sum : sum);
                 }
             } else {
                 VT* y = partial + ((size_t)blockIdx.x * SC + lane) * nrhs + r0;

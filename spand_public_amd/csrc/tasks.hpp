@@ -160,6 +160,9 @@ struct MvPiece {
     int k0, k1;    // terms; when k1 == k0 + 1 only inner indices [j0, j1) of term k0
     int j0, j1;
     int direct;    // the chunk's only piece: writes the destination itself
+    // copies of its chunk / group fields, so that a workgroup reads one record
+    int o0, no;    // the chunk's first output and output count
+    int xdst, mode;
 };
 // Explicit Householder vectors V (unit diagonal, zeros above) and the compact-WY factor of
 // one Orthogonal op, built at factorization. Q = H_0 H_1 ... H_{k-1} = I - V T V^T with T

@@ -462,7 +462,7 @@ struct MvPlan {
                 auto flush = [&](int kend) {
                     if (pk0 >= 0) {
                         const MvTerm& t = terms[pk0];
-                        pieces.push_back({ci, pk0, kend, 0, t.trans ? t.rows : t.cols, 0});
+                        pieces.push_back({ci, pk0, kend, 0, t.trans ? t.rows : t.cols, 0, 0, 0, 0, 0});
                     }
                     pk0 = -1;
                     pwork = 0;
@@ -478,7 +478,7 @@ struct MvPlan {
                     work.vec_elems += len;
                     if (len > W) {  // a long term alone: pieces of W inner indices
                         flush(k);
-                        for (int j = lo; j < hi; j += W) pieces.push_back({ci, k, k + 1, j, std::min(hi, j + W), 0});
+                        for (int j = lo; j < hi; j += W) pieces.push_back({ci, k, k + 1, j, std::min(hi, j + W), 0, 0, 0, 0, 0});
                     } else {
                         if (pk0 >= 0 && pwork + len > W) flush(k);
                         if (pk0 < 0) pk0 = k;
@@ -489,6 +489,12 @@ struct MvPlan {
                 const int p1 = (int)pieces.size();
                 if (p1 == p0) continue;
                 chunks.push_back({gi, o0, p0, p1});
+                for (int q = p0; q < p1; q++) {
+                    pieces[q].o0 = o0;
+                    pieces[q].no = no;
+                    pieces[q].xdst = g.xdst;
+                    pieces[q].mode = g.mode;
+                }
                 work.vec_elems += 2.0 * no;
                 if (p1 - p0 == 1 && !inplace) pieces[p0].direct = 1;
                 else rchunks.push_back(ci);
