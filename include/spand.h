@@ -65,7 +65,9 @@ spand_status spand_set_precision(spand_tree* t, int precision);
  * 6 ints per dof: self.lvl, self.sep, left.lvl, left.sep, right.lvl, right.sep
  * (the returned std::vector<ClusterID>). */
 spand_status spand_partition(spand_tree* t, int N, const int* colptr, const int* rowval, int* part_out);
-/* Tree::assemble(SpMat&)                           include/tree.h:175, src/tree.cpp:505-575 */
+/* Tree::assemble(SpMat&)                           include/tree.h:175, src/tree.cpp:505-575
+ * May also be called after an assembly or a factorization, with new values of the
+ * partitioned pattern (re-assembly; then factorize again). */
 spand_status spand_assemble(spand_tree* t, int N, const int* colptr, const int* rowval, const double* vals);
 /* Tree::factorize()                                include/tree.h:179, src/tree.cpp:1447-1551
  * May be called again after a completed factorization (refactorization of the assembled
@@ -76,6 +78,12 @@ spand_status spand_factorize(spand_tree* t);
 spand_status spand_solve(spand_tree* t, double* x, int nrhs, int ld);
 /* Same on device memory, ordered on `stream` (a hipStream_t; NULL = default stream). */
 spand_status spand_solve_device(spand_tree* t, double* d_x, int nrhs, int ld, void* stream);
+/* x <- M^{-T} x, the adjoint apply of a differentiable solve. No counterpart in the
+ * reference (its Tree has no transposed solve); for callers that differentiate through a
+ * solve. The SPD (LLT) preconditioner is symmetric, so this equals spand_solve; a GEN
+ * (PLU) tree returns SPAND_ERR_UNSUPPORTED. */
+spand_status spand_solve_transpose(spand_tree* t, double* x, int nrhs, int ld);
+spand_status spand_solve_transpose_device(spand_tree* t, double* d_x, int nrhs, int ld, void* stream);
 
 /* Tree::get_N / get_nlevels / get_stop / nnz / get_assembly_perm   include/tree.h:150-160 */
 spand_status spand_get_N(const spand_tree* t, int* N);
@@ -132,6 +140,13 @@ spand_status spand_op_plu(const spand_tree* t, int i, double* U, int* perm);
 spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, const double* vals,
                                  spand_matrix** out);
 void spand_matrix_destroy(spand_matrix* A);
+/* vals_out[k] = alpha * sum_r u[i_k + r*ld] * v[j_k + r*ld] for every stored entry k = (i_k, j_k)
+ * of A, in the CSC order given to spand_matrix_create: with alpha = -1, u the adjoint and v
+ * the solution of A x = b, the gradient of a loss with respect to A's stored values. No
+ * counterpart in the reference; for callers that differentiate through a solve. u, v,
+ * vals_out are device pointers when on_device != 0. */
+spand_status spand_matrix_sampled_outer(const spand_matrix* A, const double* u, const double* v, int nrhs, int ld,
+                                        double alpha, double* vals_out, int on_device);
 
 typedef struct {
     int iterations;       /* max over right-hand sides of the reference's count (i+1) */

@@ -107,6 +107,18 @@ class Tree:
         _check(self._h, self._L.spand_solve_device(self._h, C.c_void_p(d_ptr), int(nrhs), int(ld),
                                                     C.c_void_p(stream)))
 
+    def solve_transpose(self, b) -> np.ndarray:
+        """Returns M^{-T} b (the adjoint apply; equal to solve() for an SPD tree)."""
+        b = np.asarray(b, dtype=np.float64)
+        x = np.asfortranarray(b.reshape(b.shape[0], -1)).copy(order="F")
+        _check(self._h, self._L.spand_solve_transpose(self._h, x.ctypes.data, x.shape[1], x.shape[0]))
+        return x.reshape(b.shape)
+
+    def solve_transpose_device(self, d_ptr: int, nrhs: int, ld: int, stream: int = 0):
+        """In-place M^{-T} on a device buffer."""
+        _check(self._h, self._L.spand_solve_transpose_device(self._h, C.c_void_p(d_ptr), int(nrhs), int(ld),
+                                                              C.c_void_p(stream)))
+
     # -- info --
     def get_N(self) -> int:
         v = C.c_int()
@@ -222,6 +234,24 @@ class Matrix:
         _check(None, self._L.spand_matrix_create(A.n, A.colptr, A.rowval, A.nzval, C.byref(h)))
         self._h = h
         self.n = A.n
+        self.nnz = A.nnz
+
+    def sampled_outer(self, u, v, alpha: float = 1.0) -> np.ndarray:
+        """alpha * sum_r u[i, r] v[j, r] for every stored entry (i, j), in CSC order (host
+        arrays (N,) or (N, nrhs))."""
+        u = np.asfortranarray(np.asarray(u, dtype=np.float64).reshape(self.n, -1))
+        v = np.asfortranarray(np.asarray(v, dtype=np.float64).reshape(self.n, -1))
+        if u.shape != v.shape:
+            raise ValueError("sampled_outer: u and v differ in shape")
+        out = np.zeros(self.nnz)
+        _check(None, self._L.spand_matrix_sampled_outer(self._h, u.ctypes.data, v.ctypes.data, u.shape[1], self.n,
+                                                        float(alpha), out.ctypes.data, 0))
+        return out
+
+    def sampled_outer_device(self, d_u: int, d_v: int, nrhs: int, ld: int, alpha: float, d_out: int):
+        """Device-pointer form of sampled_outer (column-major blocks with leading dimension ld)."""
+        _check(None, self._L.spand_matrix_sampled_outer(self._h, C.c_void_p(d_u), C.c_void_p(d_v), int(nrhs),
+                                                        int(ld), float(alpha), C.c_void_p(d_out), 1))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -44,6 +44,8 @@ SIGNATURES = {
     "spand_factorize": (C.c_int, [vp]),
     "spand_solve": (C.c_int, [vp, vp, C.c_int, C.c_int]),
     "spand_solve_device": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
+    "spand_solve_transpose": (C.c_int, [vp, vp, C.c_int, C.c_int]),
+    "spand_solve_transpose_device": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "spand_get_N": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "spand_get_nlevels": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "spand_get_stop": (C.c_int, [vp, C.POINTER(C.c_int)]),
@@ -65,6 +67,7 @@ SIGNATURES = {
     "spand_op_plu": (C.c_int, [vp, C.c_int, vp, vp]),
     "spand_matrix_create": (C.c_int, [C.c_int, i32p, i32p, f64p, C.POINTER(vp)]),
     "spand_matrix_destroy": (None, [vp]),
+    "spand_matrix_sampled_outer": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]),
     "spand_pcg": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp,
                             C.POINTER(KrylovStats)]),
     "spand_gmres": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp,

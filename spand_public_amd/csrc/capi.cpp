@@ -132,6 +132,21 @@ spand_status spand_solve_device(spand_tree* t, double* d_x, int nrhs, int ld, vo
     });
 }
 
+spand_status spand_solve_transpose(spand_tree* t, double* x, int nrhs, int ld) {
+    CHECK_T(t);
+    return guarded(t, [&] {
+        if (!x) throw spand::Error(spand::ERR_ARG, "solve_transpose: null x");
+        t->t->solve_transpose(x, nrhs, ld, false, 0);
+    });
+}
+spand_status spand_solve_transpose_device(spand_tree* t, double* d_x, int nrhs, int ld, void* stream) {
+    CHECK_T(t);
+    return guarded(t, [&] {
+        if (!d_x) throw spand::Error(spand::ERR_ARG, "solve_transpose: null x");
+        t->t->solve_transpose(d_x, nrhs, ld, true, (hipStream_t)stream);
+    });
+}
+
 spand_status spand_get_N(const spand_tree* t, int* N) {
     CHECK_T(t);
     *N = t->t->N();
@@ -249,6 +264,14 @@ spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, co
     });
 }
 void spand_matrix_destroy(spand_matrix* A) { delete A; }
+spand_status spand_matrix_sampled_outer(const spand_matrix* A, const double* u, const double* v, int nrhs, int ld,
+                                        double alpha, double* vals_out, int on_device) {
+    if (!A || !A->m || !u || !v || !vals_out) {
+        g_err = "matrix_sampled_outer: null argument";
+        return SPAND_ERR_ARG;
+    }
+    return guarded(nullptr, [&] { A->m->sampled_outer(u, v, nrhs, ld, alpha, vals_out, on_device != 0); });
+}
 
 static spand_status krylov_out(spand_tree* t, const spand::KrylovResult& r, int nrhs, int* iters_out,
                                spand_krylov_stats* stats, const char* name) {

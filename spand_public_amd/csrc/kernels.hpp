@@ -105,6 +105,9 @@ hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_chil
 // FP64 Krylov vector kernels (interleaved N x nrhs).
 hipError_t launch_spmv(int N, const int* rowptr, const int* colidx, const double* val, const double* x, double* y,
                        int nrhs, hipStream_t st);
+// out[csc_of[p]] = alpha * sum_r u(i_p, r) v(j_p, r) over the stored entries p of a CSR matrix.
+hipError_t launch_sampled_outer(int N, const int* rowptr, const int* colidx, const int* csc_of, const double* u,
+                                const double* v, int nrhs, int ld, double alpha, double* out, hipStream_t st);
 // partial[b * nrhs + r] = sum over block b of a[i,r] * b[i,r]; out[r] = sum_b partial
 hipError_t launch_dot(int N, int nrhs, const double* a, const double* b, double* partial, double* out,
                       hipStream_t st);

@@ -270,6 +270,22 @@ __global__ void __launch_bounds__(NT) k_spmv(int N, const int* rowptr, const int
     }
 }
 
+// Sampled outer product on A's pattern: for stored entry p = (i, j) (CSR position, row i),
+// out[csc_of[p]] = alpha * sum over r of u(i, r) v(j, r), the right-hand sides in
+// increasing r; u, v column-major with leading dimension ld. One thread per row.
+__global__ void __launch_bounds__(NT) k_sampled_outer(int N, const int* rowptr, const int* colidx, const int* csc_of,
+                                                      const double* u, const double* v, int nrhs, int ld,
+                                                      double alpha, double* out) {
+    for (int i = blockIdx.x * NT + threadIdx.x; i < N; i += gridDim.x * NT) {
+        for (int p = rowptr[i]; p < rowptr[i + 1]; p++) {
+            const int j = colidx[p];
+            double acc = 0.0;
+            for (int r = 0; r < nrhs; r++) acc = fma(u[i + (size_t)r * ld], v[j + (size_t)r * ld], acc);
+            out[csc_of[p]] = alpha * acc;
+        }
+    }
+}
+
 constexpr int DOT_BLOCKS = 1024;
 __global__ void __launch_bounds__(NT) k_dot_partial(int N, int nrhs, const double* a, const double* b,
                                                     double* partial) {
@@ -440,6 +456,12 @@ hipError_t launch_merge(const MergeTask* d_tasks, int ntasks, const int2* d_chil
 hipError_t launch_spmv(int N, const int* rowptr, const int* colidx, const double* val, const double* x, double* y,
                        int nrhs, hipStream_t st) {
     k_spmv<<<grid_for((size_t)N * nrhs), NT, 0, st>>>(N, rowptr, colidx, val, x, y, nrhs);
+    return hipGetLastError();
+}
+hipError_t launch_sampled_outer(int N, const int* rowptr, const int* colidx, const int* csc_of, const double* u,
+                                const double* v, int nrhs, int ld, double alpha, double* out, hipStream_t st) {
+    if (N == 0) return hipSuccess;
+    k_sampled_outer<<<grid_for((size_t)N), NT, 0, st>>>(N, rowptr, colidx, csc_of, u, v, nrhs, ld, alpha, out);
     return hipGetLastError();
 }
 hipError_t launch_dot(int N, int nrhs, const double* a, const double* b, double* partial, double* out,

@@ -19,7 +19,7 @@ static double wall() {
 DevMatrix::DevMatrix(int n, const int* colptr, const int* rowval, const double* vals) : N(n) {
     nnz = colptr[n];
     // CSR of A = CSC of A^T: transpose the CSC arrays
-    std::vector<int> rp(n + 1, 0), ci(nnz);
+    std::vector<int> rp(n + 1, 0), ci(nnz), co(nnz);
     std::vector<double> v(nnz);
     for (long long k = 0; k < nnz; k++) rp[rowval[k] + 1]++;
     for (int i = 0; i < n; i++) rp[i + 1] += rp[i];
@@ -29,15 +29,45 @@ DevMatrix::DevMatrix(int n, const int* colptr, const int* rowval, const double* 
             int p = fill[rowval[k]]++;
             ci[p] = j;
             v[p] = vals[k];
+            co[p] = k;
         }
     hip_check(hipMalloc((void**)&rowptr, (n + 1) * sizeof(int)), "hipMalloc(csr)");
     hip_check(hipMalloc((void**)&colidx, std::max<long long>(nnz, 1) * sizeof(int)), "hipMalloc(csr)");
     hip_check(hipMalloc((void**)&val, std::max<long long>(nnz, 1) * sizeof(double)), "hipMalloc(csr)");
+    hip_check(hipMalloc((void**)&csc_of, std::max<long long>(nnz, 1) * sizeof(int)), "hipMalloc(csr)");
     hip_check(hipMemcpy(rowptr, rp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice), "upload csr");
     if (nnz) {
         hip_check(hipMemcpy(colidx, ci.data(), nnz * sizeof(int), hipMemcpyHostToDevice), "upload csr");
         hip_check(hipMemcpy(val, v.data(), nnz * sizeof(double), hipMemcpyHostToDevice), "upload csr");
+        hip_check(hipMemcpy(csc_of, co.data(), nnz * sizeof(int), hipMemcpyHostToDevice), "upload csr");
     }
+}
+
+void DevMatrix::sampled_outer(const double* u, const double* v, int nrhs, int ld, double alpha, double* vals_out,
+                              bool on_device) const {
+    if (nrhs <= 0 || ld < N) throw Error(ERR_ARG, "sampled_outer: need nrhs >= 1 and ld >= N");
+    if (on_device) {
+        hip_check(launch_sampled_outer(N, rowptr, colidx, csc_of, u, v, nrhs, ld, alpha, vals_out, 0),
+                  "sampled outer");
+        hip_check(hipStreamSynchronize(0), "sampled outer");
+        return;
+    }
+    const size_t vb = (size_t)N * nrhs * sizeof(double);
+    double *du = nullptr, *dv = nullptr, *dout = nullptr;
+    hip_check(hipMalloc((void**)&du, vb), "hipMalloc(sampled outer)");
+    hip_check(hipMalloc((void**)&dv, vb), "hipMalloc(sampled outer)");
+    hip_check(hipMalloc((void**)&dout, std::max<long long>(nnz, 1) * sizeof(double)), "hipMalloc(sampled outer)");
+    hip_check(hipMemcpy2D(du, (size_t)N * sizeof(double), u, (size_t)ld * sizeof(double), (size_t)N * sizeof(double),
+                          nrhs, hipMemcpyHostToDevice),
+              "upload u");
+    hip_check(hipMemcpy2D(dv, (size_t)N * sizeof(double), v, (size_t)ld * sizeof(double), (size_t)N * sizeof(double),
+                          nrhs, hipMemcpyHostToDevice),
+              "upload v");
+    hip_check(launch_sampled_outer(N, rowptr, colidx, csc_of, du, dv, nrhs, N, alpha, dout, 0), "sampled outer");
+    hip_check(hipMemcpy(vals_out, dout, nnz * sizeof(double), hipMemcpyDeviceToHost), "download");
+    (void)hipFree(du);
+    (void)hipFree(dv);
+    (void)hipFree(dout);
 }
 
 void* DevMatrix::workspace(size_t bytes) const {
@@ -56,6 +86,7 @@ DevMatrix::~DevMatrix() {
     (void)hipFree(rowptr);
     (void)hipFree(colidx);
     (void)hipFree(val);
+    (void)hipFree(csc_of);
 }
 
 KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, double tol,

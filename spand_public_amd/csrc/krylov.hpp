@@ -13,6 +13,7 @@ struct DevMatrix {
     int* rowptr = nullptr;
     int* colidx = nullptr;
     double* val = nullptr;
+    int* csc_of = nullptr;  // CSR position -> index in the CSC arrays the matrix was built from
     // Krylov work vectors, kept between calls (grown on demand): repeated solves with the
     // same operator allocate nothing.
     struct Workspace {
@@ -22,6 +23,10 @@ struct DevMatrix {
     mutable Workspace ws;
     void* workspace(size_t bytes) const;
     DevMatrix(int n, const int* colptr, const int* rowval, const double* vals);
+    // vals_out[k] = alpha * sum_r u(i_k, r) v(j_k, r) for the stored entries in CSC order;
+    // u, v, vals_out on the device (on_device) or on the host
+    void sampled_outer(const double* u, const double* v, int nrhs, int ld, double alpha, double* vals_out,
+                       bool on_device) const;
     ~DevMatrix();
     DevMatrix(const DevMatrix&) = delete;
     DevMatrix& operator=(const DevMatrix&) = delete;

@@ -310,7 +310,10 @@ void* Tree::upload_tasks(const Packer& pk) {
 // dense blocks are never materialized on the host: the device zero-fills the level-0
 // region and scatters the nonzeros to precomputed offsets.
 void Tree::assemble(int N, const int* colptr, const int* rowval, const double* vals) {
-    if (state_ != State::Partitioned) throw Error(ERR_STATE, "assemble() must follow partition()");
+    // Called again after an assembly or a factorization, it re-assembles new values (of the
+    // partitioned pattern) from the partition's clusters.
+    if (state_ == State::Assembled || state_ == State::Factorized) cls_ = cls0_;
+    else if (state_ != State::Partitioned) throw Error(ERR_STATE, "assemble() must follow partition()");
     if (N != N_) throw Error(ERR_ARG, "assemble: matrix size differs from the partitioned one");
     std::vector<int> pinv(N);
     for (int i = 0; i < N; i++) pinv[perm_[i]] = i;
@@ -1508,7 +1511,7 @@ void Tree::factorize_impl() {
         if (ilvl_ < nlevels_ - 1) {
             current_bottom_++;
             const std::vector<int>& nb = bottoms_[current_bottom_];
-            std::unordered_map<int, int> pos;
+            std::vector<int> pos(cls_.size(), 0);  // a child's row offset inside its parent
             for (int p : nb) {
                 HCluster& P = cls_[p];
                 int off = 0;
@@ -1530,21 +1533,29 @@ void Tree::factorize_impl() {
             const int first_new = (int)edges_.size();
             struct CT { int dst_edge; long long dst_add; const void* src; int rows, cols; };
             std::vector<CT> cts;
+            // per parent: its new neighbours in order of first appearance over the children's
+            // out-edges (stamped slots instead of searches), each new edge original if any
+            // merged child edge is
+            std::vector<int> stamp(cls_.size(), -1), eid(cls_.size(), -1);
+            std::vector<char> origs;
+            std::vector<int> nnews;
             for (int p : nb) {
-                std::vector<int> nnews;
+                nnews.clear();
+                origs.clear();
                 for (int ch : cls_[p].children)
                     for (int e : cls_[ch].out) {
-                        int nn = cls_[edges_[e].n2].parent;
-                        if (std::find(nnews.begin(), nnews.end(), nn) == nnews.end()) nnews.push_back(nn);
+                        const int nn = cls_[edges_[e].n2].parent;
+                        if (stamp[nn] != p) {
+                            stamp[nn] = p;
+                            eid[nn] = (int)nnews.size();
+                            nnews.push_back(nn);
+                            origs.push_back(0);
+                        }
+                        if (edges_[e].original) origs[eid[nn]] = 1;
                     }
-                std::unordered_map<int, int> eid;
-                for (int nn : nnews) {
-                    bool orig = false;
-                    for (int ch : cls_[p].children)
-                        for (int e : cls_[ch].out)
-                            if (cls_[edges_[e].n2].parent == nn && edges_[e].original) orig = true;
-                    int e = new_edge(p, nn, nullptr, orig);
-                    eid[nn] = e;
+                for (size_t q = 0; q < nnews.size(); q++) {
+                    const int nn = nnews[q];
+                    eid[nn] = new_edge(p, nn, nullptr, origs[q] != 0);
                     moff.push_back(melems);
                     melems += (long long)size(nn) * size(p);
                 }
@@ -1787,6 +1798,14 @@ long long Tree::trailing_coo(int* I, int* J, double* V) const {
 }
 
 // ---------------------------------------------------------------------------------------
+// The SPD apply is its own transpose: every level's backward sweep applies the transposes
+// of its forward sweep's ops in reverse order (L^{-T} after L^{-1}, A_ns^T after A_ns, Q after
+// Q^T), so M^{-1} = W^T W for the forward sweep W.
+void Tree::solve_transpose(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
+    if (symm_ != 0) throw Error(ERR_UNSUPPORTED, "solve_transpose: the transposed GEN (PLU) apply is not built");
+    solve(x, nrhs, ldx, on_device, st);
+}
+
 void Tree::solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
     if (state_ != State::Factorized) throw Error(ERR_STATE, "solve() must follow factorize()");
     if (nrhs <= 0 || nrhs > 64 || ldx < N_) throw Error(ERR_ARG, "solve: need 1 <= nrhs <= 64 and ld >= N");

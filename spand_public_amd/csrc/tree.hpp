@@ -157,6 +157,9 @@ class Tree {
     void factorize();
     // M^{-1} applied in place to nrhs columns (ld >= N) of x, on device or host memory.
     void solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
+    // M^{-T} applied in place (the adjoint apply). The SPD (LLT) preconditioner is
+    // symmetric, so this is solve(); GEN (PLU) is not built.
+    void solve_transpose(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
     // M^{-1} applied in place to an interleaved device block (element (i, r) at x[i*nrhs+r]).
     void solve_interleaved(double* d_x, int nrhs, hipStream_t st);
 

@@ -17,6 +17,9 @@ These produce the matrices spaND is run on, as CSC arrays (``colptr``, ``rowval`
 * ``convdiff(n, d, pe, velocity)``: Dirichlet convection-diffusion, 7-point (5-point in
   2D) diffusion plus first-order upwind convection, scaled by h^2 like ``neglapl``. No
   reference counterpart (a nonsymmetric workload for the GEN / PLU path).
+* ``DiffusionFV(n, d)``: -div(kappa grad u) with cell-wise kappa, 5-point (2D) / 7-point
+  (3D) finite volumes with harmonic-mean face coefficients, the variable-coefficient
+  workload; ``gaussian_field`` draws log-kappa. No reference counterpart.
 * ``symmetric_graph(A)``: |A| + |A^T| + I pattern (src/util.cpp:47-63).
 * ``mmread(path)``: MatrixMarket coordinate reader with the reference's symmetric /
   hermitian expansion (include/mmio.hpp:160-219).
@@ -171,6 +174,86 @@ def convdiff(n: int, d: int, pe: float = 10.0, velocity=(1.0, 0.5, 0.25)) -> CSC
             vals.append(np.full(int(ok.sum()), -1.0 - (pe * abs(v[axis]) if up else 0.0)))
         stride *= n
     return CSC.from_coo(N, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
+
+
+def gaussian_field(n: int, d: int, sigma: float, corr: float, seed: int) -> np.ndarray:
+    """Stationary Gaussian random field on an n^d grid (dimension 0 slowest): white noise
+    N(0,1) from numpy's PCG64 with `seed`, smoothed by a periodic Gaussian kernel of
+    standard deviation `corr` cells (FFT), then shifted and scaled to mean 0 and standard
+    deviation `sigma`. Flattened to n^d values."""
+    w = np.random.default_rng(seed).standard_normal((n,) * d)
+    k = np.meshgrid(*[np.fft.fftfreq(n)] * d, indexing="ij")
+    filt = np.exp(-2.0 * (np.pi * corr) ** 2 * sum(g * g for g in k))
+    g = np.real(np.fft.ifftn(np.fft.fftn(w) * filt))
+    g = (g - g.mean()) / max(g.std(), 1e-300) * sigma
+    return g.reshape(-1)
+
+
+class DiffusionFV:
+    """-div(kappa grad u) on an n^d grid of unit cells, kappa constant per cell, Dirichlet
+    u = 0 one cell outside the grid. A = sum over faces f of t_f (e_a - e_b)(e_a - e_b)^T
+    for the interior face between cells a and b, t_f = 2 kappa_a kappa_b / (kappa_a +
+    kappa_b) (harmonic mean), plus t_f e_a e_a^T for each boundary face of cell a with
+    t_f = kappa_a. kappa = 1 gives ``neglapl(n, d)`` exactly. Stored entries follow the
+    ``neglapl`` pattern in CSC order; entry k of ``values(t)`` is sum_f E[k, f] t_f with
+    E[k, f] in {-1, 0, 1} (``entry_index``, ``entry_face``, ``entry_sign``)."""
+
+    def __init__(self, n: int, d: int):
+        self.n, self.d = int(n), int(d)
+        N = n ** d
+        self.N = N
+        idx = np.arange(N, dtype=np.int64)
+        coords = np.stack(np.unravel_index(idx, (n,) * d), axis=0)
+        fa, fb, bnd = [], [], []
+        stride = 1
+        for axis in range(d - 1, -1, -1):
+            c = coords[axis]
+            ok = c + 1 < n
+            fa.append(idx[ok])
+            fb.append(idx[ok] + stride)
+            bnd.append(idx[c == 0])
+            bnd.append(idx[c == n - 1])
+            stride *= n
+        self.face_a = np.concatenate(fa)
+        self.face_b = np.concatenate(fb)
+        self.bnd_cell = np.concatenate(bnd)
+        self.nfaces = len(self.face_a) + len(self.bnd_cell)
+        # the pattern, with each position's CSC index as its value
+        P = neglapl(n, d)
+        self.colptr, self.rowval = P.colptr, P.rowval
+        pos = {}
+        cols = np.repeat(np.arange(N), np.diff(P.colptr))
+        for k, (r, c) in enumerate(zip(P.rowval.tolist(), cols.tolist())):
+            pos[(r, c)] = k
+        ni = len(self.face_a)
+        ek, ef, es = [], [], []
+        for f, (a, b) in enumerate(zip(self.face_a.tolist(), self.face_b.tolist())):
+            for r, c, sg in ((a, a, 1.0), (b, b, 1.0), (a, b, -1.0), (b, a, -1.0)):
+                ek.append(pos[(r, c)])
+                ef.append(f)
+                es.append(sg)
+        for q, a in enumerate(self.bnd_cell.tolist()):
+            ek.append(pos[(a, a)])
+            ef.append(ni + q)
+            es.append(1.0)
+        self.entry_index = np.asarray(ek, dtype=np.int64)
+        self.entry_face = np.asarray(ef, dtype=np.int64)
+        self.entry_sign = np.asarray(es)
+        self.nnz = int(P.colptr[-1])
+
+    def transmissibility(self, kappa) -> np.ndarray:
+        """Face coefficients t (interior faces, then boundary faces) for cell-wise kappa."""
+        kappa = np.asarray(kappa, dtype=np.float64).reshape(-1)
+        ka, kb = kappa[self.face_a], kappa[self.face_b]
+        return np.concatenate([2.0 * ka * kb / (ka + kb), kappa[self.bnd_cell]])
+
+    def values(self, t) -> np.ndarray:
+        """Stored entries (CSC order) of A for face coefficients t."""
+        return np.bincount(self.entry_index, weights=self.entry_sign * np.asarray(t)[self.entry_face],
+                           minlength=self.nnz)
+
+    def matrix(self, kappa) -> CSC:
+        return CSC(self.N, self.colptr, self.rowval, self.values(self.transmissibility(kappa)))
 
 
 def symmetric_graph(A: CSC) -> CSC:

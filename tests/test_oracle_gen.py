@@ -142,3 +142,23 @@ def test_convdiff_gmres():
     x, it, rel = t.gmres(A, b, iters=200, restart=30, tol=1e-10)
     assert rel < 1e-10 and it <= 30
     assert np.linalg.norm(A.matvec(x) - b) / np.linalg.norm(b) <= 1e-9
+
+
+def test_diffusion_fv_closed_forms():
+    """Constant kappa gives the Laplacian; the matrix is symmetric with zero row sums away
+    from the boundary; one face coefficient is the harmonic mean of its cells."""
+    from spand_public_amd.problems import DiffusionFV, gaussian_field
+    for n, d in ((6, 2), (4, 3)):
+        fv = DiffusionFV(n, d)
+        assert np.array_equal(fv.matrix(np.ones(n ** d)).nzval, neglapl(n, d).nzval)
+    fv = DiffusionFV(5, 2)
+    kappa = np.exp(gaussian_field(5, 2, 1.0, 2.0, 4))
+    M = fv.matrix(kappa).to_scipy().toarray()
+    assert np.allclose(M, M.T, rtol=0, atol=1e-15)
+    i = np.ravel_multi_index((2, 2), (5, 5))
+    assert abs(M[i].sum()) < 1e-13
+    j = i + 1  # the neighbour along the fastest axis
+    assert M[i, j] == -2 * kappa[i] * kappa[j] / (kappa[i] + kappa[j])
+    assert np.all(np.linalg.eigvalsh(M) > 0)
+    g1, g2 = gaussian_field(8, 2, 2.0, 3.0, 7), gaussian_field(8, 2, 2.0, 3.0, 7)
+    assert np.array_equal(g1, g2) and abs(g1.mean()) < 1e-12 and abs(g1.std() - 2.0) < 1e-12
