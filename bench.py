@@ -160,6 +160,66 @@ def cpu_baseline(A, X, B, nl):
             "host": platform.processor() or platform.machine()}
 
 
+def run_differentiable(args):
+    """The differentiable-solve workload: 2D 512^2 variable-coefficient diffusion, kappa =
+    exp(g) with g a Gaussian random field (sigma 1, correlation length 16 cells), 32 RHS
+    N(0,1), loss 1/2 ||x - x*||^2 with x* ~ N(0,1). One step = re-assembly + factorization
+    (FP64, tol 1e-2) + PCG forward + adjoint PCG + sampled outer product + the chain to
+    kappa. Prints one JSON line with the step time and a central-difference check
+    (h = 1e-6) of dL/dkappa and dL/db in random directions."""
+    import torch
+
+    import spand_public_amd.autograd as ad
+    from spand_public_amd.problems import DiffusionFV, auto_nlevels, gaussian_field
+
+    n, nrhs = 512, 32
+    fv = DiffusionFV(n, 2)
+    kappa = np.exp(gaussian_field(n, 2, 1.0, 16.0, SEED))
+    solver = ad.fv_solver(fv, auto_nlevels(fv.N), tol=TOL, pcg_tol=1e-13)
+    rng = np.random.default_rng(SEED + 1)
+    dev = torch.device("cuda:0")
+    b0 = torch.tensor(rng.standard_normal((fv.N, nrhs)), device=dev)
+    xs = torch.tensor(rng.standard_normal((fv.N, nrhs)), device=dev)
+    k0 = torch.tensor(kappa, device=dev)
+
+    def loss(k, b):
+        x = ad.spd_solve(ad.fv_values(fv, k), b, solver)
+        return 0.5 * ((x - xs) ** 2).sum()
+
+    def step():
+        k = k0.clone().requires_grad_()
+        b = b0.clone().requires_grad_()
+        L = loss(k, b)
+        L.backward()
+        torch.cuda.synchronize()
+        return k.grad, b.grad
+
+    for _ in range(args.warmup):
+        step()
+    solver.iters.clear()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gk, gb = step()
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    its = solver.iters[:2]
+    h, errs = 1e-6, []
+    with torch.no_grad():
+        for q in range(4):
+            d = torch.tensor(rng.standard_normal(fv.N), device=dev) * k0
+            fd = float((loss(k0 + h * d, b0) - loss(k0 - h * d, b0)) / (2 * h))
+            errs.append(abs(fd - float((gk * d).sum())) / abs(fd))
+            d = torch.tensor(rng.standard_normal((fv.N, nrhs)), device=dev)
+            fd = float((loss(k0, b0 + h * d) - loss(k0, b0 - h * d)) / (2 * h))
+            errs.append(abs(fd - float((gb * d).sum())) / abs(fd))
+    print(json.dumps({"metric": "differentiable solve step (re-assembly + factorization + PCG + adjoint PCG)",
+                      "value": round(dt, 6), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": f"2D {n}^2 variable-kappa diffusion, {nrhs} RHS, loss 1/2||x-x*||^2",
+                                 "nlevels": auto_nlevels(fv.N), "tol": TOL},
+                      "pcg_iters_forward": its[0] if its else None, "pcg_iters_adjoint": its[1] if len(its) > 1 else None,
+                      "fd_max_rel_err": max(errs)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +230,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-factor comparison")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed steps")
+    ap.add_argument("--differentiable", action="store_true",
+                    help="only the differentiable-solve workload (2D 512^2 variable kappa, 32 RHS)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (the calibration kernel for HBM-traffic counters)")
     args = ap.parse_args()
@@ -187,6 +249,9 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
+    if args.differentiable:
+        run_differentiable(args)
+        return
 
     import torch
 

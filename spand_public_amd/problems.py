@@ -218,27 +218,19 @@ class DiffusionFV:
         self.face_b = np.concatenate(fb)
         self.bnd_cell = np.concatenate(bnd)
         self.nfaces = len(self.face_a) + len(self.bnd_cell)
-        # the pattern, with each position's CSC index as its value
+        # the pattern; an entry's CSC position is the rank of its key col * N + row
         P = neglapl(n, d)
         self.colptr, self.rowval = P.colptr, P.rowval
-        pos = {}
-        cols = np.repeat(np.arange(N), np.diff(P.colptr))
-        for k, (r, c) in enumerate(zip(P.rowval.tolist(), cols.tolist())):
-            pos[(r, c)] = k
-        ni = len(self.face_a)
-        ek, ef, es = [], [], []
-        for f, (a, b) in enumerate(zip(self.face_a.tolist(), self.face_b.tolist())):
-            for r, c, sg in ((a, a, 1.0), (b, b, 1.0), (a, b, -1.0), (b, a, -1.0)):
-                ek.append(pos[(r, c)])
-                ef.append(f)
-                es.append(sg)
-        for q, a in enumerate(self.bnd_cell.tolist()):
-            ek.append(pos[(a, a)])
-            ef.append(ni + q)
-            es.append(1.0)
-        self.entry_index = np.asarray(ek, dtype=np.int64)
-        self.entry_face = np.asarray(ef, dtype=np.int64)
-        self.entry_sign = np.asarray(es)
+        keys = np.repeat(np.arange(N, dtype=np.int64), np.diff(P.colptr)) * N + P.rowval
+        ni, nb = len(self.face_a), len(self.bnd_cell)
+        fa, fb = self.face_a, self.face_b
+        f = np.arange(ni, dtype=np.int64)
+        # interior face f: +t at (a, a) and (b, b), -t at (a, b) and (b, a); boundary: +t at (a, a)
+        r = np.concatenate([fa, fb, fa, fb, self.bnd_cell])
+        c = np.concatenate([fa, fb, fb, fa, self.bnd_cell])
+        self.entry_face = np.concatenate([f, f, f, f, ni + np.arange(nb, dtype=np.int64)])
+        self.entry_sign = np.concatenate([np.ones(2 * ni), -np.ones(2 * ni), np.ones(nb)])
+        self.entry_index = np.searchsorted(keys, c * N + r)
         self.nnz = int(P.colptr[-1])
 
     def transmissibility(self, kappa) -> np.ndarray:
