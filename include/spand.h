@@ -81,7 +81,7 @@ spand_status spand_solve_device(spand_tree* t, double* d_x, int nrhs, int ld, vo
 /* x <- M^{-T} x, the adjoint apply of a differentiable solve. No counterpart in the
  * reference (its Tree has no transposed solve); for callers that differentiate through a
  * solve. The SPD (LLT) preconditioner is symmetric, so this equals spand_solve; a GEN
- * (PLU) tree returns SPAND_ERR_UNSUPPORTED. */
+ * (PLU) tree applies the transposed ops in reverse order. */
 spand_status spand_solve_transpose(spand_tree* t, double* x, int nrhs, int ld);
 spand_status spand_solve_transpose_device(spand_tree* t, double* d_x, int nrhs, int ld, void* stream);
 

@@ -172,7 +172,8 @@ __global__ void __launch_bounds__(NT) k_mv_reduce(const MvGroup* groups, const M
         const int lane = idx / nrhs, r = idx % nrhs;
         VT sum = VT(0);
         for (int p = ch.p0; p < ch.p1; p++) sum += partial[((size_t)p * SC + lane) * nrhs + r];
-        VT* y = ws + (size_t)(g.xdst + ch.o0 + lane) * nrhs + r;
+        const int o = ch.o0 + lane;
+        VT* y = ws + (size_t)(g.xdst + (g.dperm ? g.dperm[o] : o)) * nrhs + r;
         *y = g.mode == MV_SUB ? *y - sum : (g.mode == MV_SETNEG ? -sum : sum);
     }
 }

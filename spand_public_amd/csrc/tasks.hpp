@@ -150,6 +150,8 @@ struct MvGroup {
     int k0, k1;   // term range
     int mode;
     int pad;
+    const int* dperm;  // nullptr, or output o goes to xdst + dperm[o] (then always through
+                       // the partial sums: a permuted destination is never written directly)
 };
 struct MvChunk {
     int group, o0;  // outputs [o0, o0 + 64) of the group

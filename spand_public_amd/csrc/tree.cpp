@@ -174,23 +174,30 @@ void Tree::set_eliminated(int c) {
 // asynchronous on the factorization stream, so the host plans the next batch while the
 // device works. The staging area is reused from the start at every factorization (the
 // previous one ended with a device synchronization); when it runs out, the device is
-// synchronized and the buffer grown.
+// synchronized and the buffer grown to hold everything staged since the factorization
+// began. Such a factorization ends by growing the buffer to its whole upload volume, so
+// that repeated factorizations of one pattern never stall on a growth.
 char* Tree::stage_reserve(size_t bytes) {
     const size_t off = (h_stage_used_ + 255) & ~(size_t)255;
     if (off + bytes > h_stage_cap_) {
         // earlier uploads may still read the old buffer: drain before replacing it
         hip_check(hipDeviceSynchronize(), "staging");
-        const size_t cap = std::max<size_t>(2 * h_stage_cap_, std::max<size_t>(bytes, (size_t)8 << 20));
-        (void)hipHostFree(h_stage_);
-        h_stage_ = nullptr;
-        h_stage_cap_ = 0;
-        hip_check(hipHostMalloc((void**)&h_stage_, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
-        h_stage_cap_ = cap;
-        h_stage_used_ = 0;
+        stage_grow(std::max<size_t>(2 * h_stage_cap_, std::max<size_t>(h_stage_total_ + bytes, (size_t)8 << 20)));
         return stage_reserve(bytes);
     }
+    h_stage_total_ += off + bytes - h_stage_used_;
     h_stage_used_ = off + bytes;
     return h_stage_ + off;
+}
+
+// Replaces the staging buffer by an empty one of cap bytes (no upload may be in flight).
+void Tree::stage_grow(size_t cap) {
+    (void)hipHostFree(h_stage_);
+    h_stage_ = nullptr;
+    h_stage_cap_ = 0;
+    hip_check(hipHostMalloc((void**)&h_stage_, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
+    h_stage_cap_ = cap;
+    h_stage_used_ = 0;
 }
 
 // Grows the signal buffer (between levels: no launch that writes it is in flight).
@@ -499,7 +506,7 @@ struct MvPlan {
                     pieces[q].mode = g.mode;
                 }
                 work.vec_elems += 2.0 * no;
-                if (p1 - p0 == 1 && !inplace) pieces[p0].direct = 1;
+                if (p1 - p0 == 1 && !inplace && !g.dperm) pieces[p0].direct = 1;
                 else rchunks.push_back(ci);
             }
         }
@@ -611,71 +618,132 @@ struct QOp {
     int m, k, xoff, yoff;
 };
 
+// One level's solve-plan inputs, kept on the host for the transposed phases (built and
+// uploaded at the first transposed solve).
+struct HostPhases {
+    std::vector<TrsvTask> sv_elim, sv_scale;
+    std::vector<GemvGroup> fwd_groups, bwd_groups;
+    std::vector<GemvTerm> fwd_t, bwd_terms;
+    std::vector<QOp> qops;
+};
+
 // Host side of one level's preconditioner-apply plan, before upload.
 struct SolvePlanHost {
     std::vector<MvPlan> fwd, bwd;
+    HostPhases inputs;
     std::vector<MergeTask> mt;
     std::vector<int2> mchildren;
     long long max_pieces = 0;
 };
 
+// Phase builders. A triangular sweep (ScalingLLT / ScalingPLUQ of each cluster): forward
+// x := L x with the gathered source P^T x, backward x := Lb^T x.
+static MvPlan trsv_phase(const std::vector<TrsvTask>& v, bool backward) {
+    MvPlan m;
+    m.family = KF_TRMV;
+    m.inplace = true;
+    for (auto& t : v) {
+        const int k0 = (int)m.terms.size();
+        if (!backward) m.terms.push_back({t.L, t.perm, t.n, t.n, t.xoff, 0, 1, 0});
+        else m.terms.push_back({t.Lb, nullptr, t.n, t.n, t.xoff, 1, 2, 0});
+        m.groups.push_back({t.xoff, t.n, k0, k0 + 1, MV_SET, 0, nullptr});
+    }
+    return m;
+}
+
+// x_dst -= sum_k op(A_k) x_src(k) for every group
+static MvPlan gemv_phase(const std::vector<GemvGroup>& gs, const std::vector<GemvTerm>& ts) {
+    MvPlan m;
+    m.family = KF_GEMV;
+    for (auto& g : gs) {
+        const int k0 = (int)m.terms.size();
+        for (int k = g.k0; k < g.k1; k++)
+            m.terms.push_back({ts[k].A, nullptr, ts[k].rows, ts[k].cols, ts[k].xsrc, ts[k].trans, 0, 0});
+        m.groups.push_back({g.xdst, g.n, k0, (int)m.terms.size(), MV_SUB, 0, nullptr});
+    }
+    return m;
+}
+
+// Orthogonal ops, step 0..2: y = -V^T x, z = -X y (forward) or -X^T y (backward), x -= V z
+static MvPlan q_phase(const std::vector<QOp>& qops, int step, bool backward) {
+    MvPlan m;
+    m.family = KF_QAPPLY;
+    for (auto& q : qops) {
+        const int k0 = (int)m.terms.size();
+        if (step == 0) {
+            m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.xoff, 1, 2, 0});
+            m.groups.push_back({q.yoff, q.k, k0, k0 + 1, MV_SETNEG, 0, nullptr});
+        } else if (step == 1) {  // X = T^T lower: z = -X y (fwd), -X^T y (bwd)
+            m.terms.push_back({q.X, nullptr, q.k, q.k, q.yoff, backward ? 1 : 0, backward ? 2 : 1, 0});
+            m.groups.push_back({q.yoff + q.k, q.k, k0, k0 + 1, MV_SETNEG, 0, nullptr});
+        } else {
+            m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.yoff + q.k, 0, 1, 0});
+            m.groups.push_back({q.xoff, q.m, k0, k0 + 1, MV_SUB, 0, nullptr});
+        }
+    }
+    return m;
+}
+
+// The transpose of trsv_phase: (L P^T)^T = P L^T (P as a permuted destination), (Lb^T)^T = Lb.
+static MvPlan trsv_phase_t(const std::vector<TrsvTask>& v, bool backward) {
+    MvPlan m;
+    m.family = KF_TRMV;
+    m.inplace = true;
+    for (auto& t : v) {
+        const int k0 = (int)m.terms.size();
+        if (!backward) {
+            m.terms.push_back({t.L, nullptr, t.n, t.n, t.xoff, 1, 2, 0});
+            m.groups.push_back({t.xoff, t.n, k0, k0 + 1, MV_SET, 0, t.perm});
+        } else {
+            m.terms.push_back({t.Lb, nullptr, t.n, t.n, t.xoff, 0, 1, 0});
+            m.groups.push_back({t.xoff, t.n, k0, k0 + 1, MV_SET, 0, nullptr});
+        }
+    }
+    return m;
+}
+
+// The transpose of gemv_phase: x_src(k) -= op(A_k)^T x_dst, grouped by source segment.
+static MvPlan gemv_phase_t(const std::vector<GemvGroup>& gs, const std::vector<GemvTerm>& ts) {
+    struct TT {
+        int dst, n;
+        GemvTerm t;
+    };
+    std::vector<TT> items;
+    for (auto& g : gs)
+        for (int k = g.k0; k < g.k1; k++) {
+            const GemvTerm& t = ts[k];
+            const int n_in = t.trans ? t.rows : t.cols;  // op(A)'s input length
+            items.push_back({t.xsrc, n_in, GemvTerm{t.A, t.rows, t.cols, g.xdst, t.trans ? 0 : 1}});
+        }
+    std::stable_sort(items.begin(), items.end(), [](const TT& a, const TT& b) { return a.dst < b.dst; });
+    MvPlan m;
+    m.family = KF_GEMV;
+    for (size_t q = 0; q < items.size();) {
+        const int k0 = (int)m.terms.size();
+        const int dst = items[q].dst, n = items[q].n;
+        for (; q < items.size() && items[q].dst == dst; q++) {
+            const GemvTerm& t = items[q].t;
+            m.terms.push_back({t.A, nullptr, t.rows, t.cols, t.xsrc, t.trans, 0, 0});
+        }
+        m.groups.push_back({dst, n, k0, (int)m.terms.size(), MV_SUB, 0, nullptr});
+    }
+    return m;
+}
+
 // The level's mv phases: forward {trsv Elim, gemv, trsv Scale, Q 0..2}, backward
-// {Q 0..2, trsv Scale', gemv, trsv Elim'}, each split into chunks and pieces.
-static SolvePlanHost build_solve_plan(size_t esz, const std::vector<TrsvTask>& sv_elim,
-                                      const std::vector<TrsvTask>& sv_scale, const std::vector<GemvGroup>& fwd_groups,
-                                      const std::vector<GemvTerm>& fwd_t, const std::vector<GemvGroup>& bwd_groups,
-                                      const std::vector<GemvTerm>& bwd_terms, const std::vector<QOp>& qops,
-                                      std::vector<MergeTask> mt, std::vector<int2> mchildren) {
+// {Q 0..2, trsv Scale', gemv, trsv Elim'}, each split into chunks and pieces. The inputs
+// are kept for the transposed phases.
+static SolvePlanHost build_solve_plan(size_t esz, HostPhases in, std::vector<MergeTask> mt,
+                                      std::vector<int2> mchildren) {
     SolvePlanHost h;
-    auto trsv_phase = [&](const std::vector<TrsvTask>& v, bool backward) {
-        MvPlan m;
-        m.family = KF_TRMV;
-        m.inplace = true;
-        for (auto& t : v) {
-            const int k0 = (int)m.terms.size();
-            if (!backward) m.terms.push_back({t.L, t.perm, t.n, t.n, t.xoff, 0, 1, 0});
-            else m.terms.push_back({t.Lb, nullptr, t.n, t.n, t.xoff, 1, 2, 0});
-            m.groups.push_back({t.xoff, t.n, k0, k0 + 1, MV_SET, 0});
-        }
-        return m;
-    };
-    auto gemv_phase = [&](const std::vector<GemvGroup>& gs, const std::vector<GemvTerm>& ts) {
-        MvPlan m;
-        m.family = KF_GEMV;
-        for (auto& g : gs) {
-            const int k0 = (int)m.terms.size();
-            for (int k = g.k0; k < g.k1; k++)
-                m.terms.push_back({ts[k].A, nullptr, ts[k].rows, ts[k].cols, ts[k].xsrc, ts[k].trans, 0, 0});
-            m.groups.push_back({g.xdst, g.n, k0, (int)m.terms.size(), MV_SUB, 0});
-        }
-        return m;
-    };
-    auto q_phase = [&](int step, bool backward) {
-        MvPlan m;
-        m.family = KF_QAPPLY;
-        for (auto& q : qops) {
-            const int k0 = (int)m.terms.size();
-            if (step == 0) {
-                m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.xoff, 1, 2, 0});
-                m.groups.push_back({q.yoff, q.k, k0, k0 + 1, MV_SETNEG, 0});
-            } else if (step == 1) {  // X = T^T lower: z = -X y (fwd), -X^T y (bwd)
-                m.terms.push_back({q.X, nullptr, q.k, q.k, q.yoff, backward ? 1 : 0, backward ? 2 : 1, 0});
-                m.groups.push_back({q.yoff + q.k, q.k, k0, k0 + 1, MV_SETNEG, 0});
-            } else {
-                m.terms.push_back({q.Ve, nullptr, q.m, q.k, q.yoff + q.k, 0, 1, 0});
-                m.groups.push_back({q.xoff, q.m, k0, k0 + 1, MV_SUB, 0});
-            }
-        }
-        return m;
-    };
-    h.fwd.push_back(trsv_phase(sv_elim, false));
-    h.fwd.push_back(gemv_phase(fwd_groups, fwd_t));
-    h.fwd.push_back(trsv_phase(sv_scale, false));
-    for (int q = 0; q < 3; q++) h.fwd.push_back(q_phase(q, false));
-    for (int q = 0; q < 3; q++) h.bwd.push_back(q_phase(q, true));
-    h.bwd.push_back(trsv_phase(sv_scale, true));
-    h.bwd.push_back(gemv_phase(bwd_groups, bwd_terms));
-    h.bwd.push_back(trsv_phase(sv_elim, true));
+    h.fwd.push_back(trsv_phase(in.sv_elim, false));
+    h.fwd.push_back(gemv_phase(in.fwd_groups, in.fwd_t));
+    h.fwd.push_back(trsv_phase(in.sv_scale, false));
+    for (int q = 0; q < 3; q++) h.fwd.push_back(q_phase(in.qops, q, false));
+    for (int q = 0; q < 3; q++) h.bwd.push_back(q_phase(in.qops, q, true));
+    h.bwd.push_back(trsv_phase(in.sv_scale, true));
+    h.bwd.push_back(gemv_phase(in.bwd_groups, in.bwd_terms));
+    h.bwd.push_back(trsv_phase(in.sv_elim, true));
     for (auto* v : {&h.fwd, &h.bwd})
         for (auto& m : *v) {
             m.split(esz);
@@ -683,24 +751,45 @@ static SolvePlanHost build_solve_plan(size_t esz, const std::vector<TrsvTask>& s
         }
     h.mt = std::move(mt);
     h.mchildren = std::move(mchildren);
+    h.inputs = std::move(in);
     return h;
 }
 
-// Uploads one level's solve plan and points the level's phases at the device copy.
-void Tree::upload_solve_plan(int level, void* host_plan) {
-    SolvePlanHost& h = *static_cast<SolvePlanHost*>(host_plan);
-    LevelSolve& lsv = solve_plan_[level];
-    Packer ps;
+// The adjoint apply's phases of one level. Forward level l applies F = Q S G E (E first),
+// backward B = E' G' S' Q' (Q' first); M^{-T} runs B^T (E'^T, G'^T, S'^T, Q'^T = the forward
+// Q) going up and F^T (Q^T = the backward Q, S^T, G^T, E^T) coming down.
+static void build_transposed(size_t esz, const HostPhases& in, std::vector<MvPlan>& up, std::vector<MvPlan>& down,
+                             long long& max_pieces) {
+    up.push_back(trsv_phase_t(in.sv_elim, true));
+    up.push_back(gemv_phase_t(in.bwd_groups, in.bwd_terms));
+    up.push_back(trsv_phase_t(in.sv_scale, true));
+    for (int q = 0; q < 3; q++) up.push_back(q_phase(in.qops, q, false));
+    for (int q = 0; q < 3; q++) down.push_back(q_phase(in.qops, q, true));
+    down.push_back(trsv_phase_t(in.sv_scale, false));
+    down.push_back(gemv_phase_t(in.fwd_groups, in.fwd_t));
+    down.push_back(trsv_phase_t(in.sv_elim, false));
+    for (auto* v : {&up, &down})
+        for (auto& m : *v) {
+            m.split(esz);
+            max_pieces = std::max(max_pieces, (long long)m.pieces.size());
+        }
+}
+
+// Uploads phase lists (one Packer) and returns their device descriptors, in order.
+static std::vector<std::vector<MvPhase>> upload_phases(Tree& tree, std::vector<std::vector<MvPlan>*> lists,
+                                                       void* (Tree::*upload)(const Packer&), Packer& ps,
+                                                       void** base) {
     std::vector<std::array<size_t, 5>> offs;
-    for (auto* v : {&h.fwd, &h.bwd})
+    for (auto* v : lists)
         for (auto& m : *v)
             offs.push_back({ps.add(m.groups), ps.add(m.terms), ps.add(m.chunks), ps.add(m.pieces), ps.add(m.rchunks)});
-    const size_t s_m = ps.add(h.mt), s_mc = ps.add(h.mchildren);
-    void* dsv = upload_tasks(ps);
+    void* dsv = (tree.*upload)(ps);
+    *base = dsv;
+    std::vector<std::vector<MvPhase>> out;
     size_t oi = 0;
-    auto to_dev = [&](std::vector<MvPlan>& v, std::vector<MvPhase>& out) {
-        out.clear();
-        for (auto& m : v) {
+    for (auto* v : lists) {
+        out.emplace_back();
+        for (auto& m : *v) {
             const auto& o = offs[oi++];
             if (m.pieces.empty()) continue;
             MvPhase ph;
@@ -713,16 +802,47 @@ void Tree::upload_solve_plan(int level, void* host_plan) {
             ph.nreduce = (int)m.rchunks.size();
             ph.family = m.family;
             ph.w = m.work;
-            out.push_back(ph);
+            out.back().push_back(ph);
         }
-    };
-    to_dev(h.fwd, lsv.fwd);
-    to_dev(h.bwd, lsv.bwd);
+    }
+    return out;
+}
+
+// Uploads one level's solve plan and points the level's phases at the device copy. A GEN
+// tree keeps the phase inputs on the host until a transposed solve asks for the transposed
+// phases; an SPD tree's transposed apply is its forward apply, and it keeps nothing.
+void Tree::upload_solve_plan(int level, void* host_plan) {
+    SolvePlanHost& h = *static_cast<SolvePlanHost*>(host_plan);
+    LevelSolve& lsv = solve_plan_[level];
+    Packer ps;
+    const size_t s_m = ps.add(h.mt), s_mc = ps.add(h.mchildren);
+    void* dsv = nullptr;
+    auto ph = upload_phases(*this, {&h.fwd, &h.bwd}, &Tree::upload_tasks, ps, &dsv);
+    lsv.fwd = std::move(ph[0]);
+    lsv.bwd = std::move(ph[1]);
     for (auto& c : h.mchildren) lsv.w_merge.vec_elems += 2.0 * c.y;
     lsv.merge = Packer::at<MergeTask>(dsv, s_m);
     lsv.n_merge = (int)h.mt.size();
     lsv.children = Packer::at<int2>(dsv, s_mc);
     max_pieces_ = std::max(max_pieces_, h.max_pieces);
+    if (fsymm_ != 0) tplans_[level].reset(new HostPhases(std::move(h.inputs)));
+}
+
+// Builds and uploads the transposed phases of every level (first transposed solve after a
+// factorization). The partial-sum area grows with them if needed.
+void Tree::upload_transposed() {
+    const size_t esz = prec_ == 0 ? sizeof(double) : sizeof(float);
+    for (int l = 0; l < nlevels_; l++) {
+        if (!tplans_[l]) continue;
+        std::vector<MvPlan> up, down;
+        build_transposed(esz, *tplans_[l], up, down, max_pieces_);
+        Packer ps;
+        void* dsv = nullptr;
+        auto ph = upload_phases(*this, {&up, &down}, &Tree::upload_tasks, ps, &dsv);
+        solve_plan_[l].tfwd = std::move(ph[0]);
+        solve_plan_[l].tbwd = std::move(ph[1]);
+        tplans_[l].reset();
+    }
 }
 
 template <typename T>
@@ -734,8 +854,11 @@ void Tree::factorize_impl() {
     wave_work_.clear();
     level_dims_.clear();
     std::vector<std::future<SolvePlanHost>> pending_solve(nlevels_);  // joined on every exit
+    tplans_.clear();
+    tplans_.resize(nlevels_);
     hip_check(hipDeviceSynchronize(), "factorize");
     h_stage_used_ = 0;
+    h_stage_total_ = 0;
     ops_.clear();
     cls_ = cls0_;
     edges_ = edges0_;
@@ -778,6 +901,7 @@ void Tree::factorize_impl() {
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
     const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 10000);
     const bool gen = symm_ == 2;
+    fsymm_ = symm_;
 
     for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
         LevelStats& lst = stats_[ilvl_];
@@ -1643,14 +1767,12 @@ void Tree::factorize_impl() {
                     fwd_groups.push_back({(int)cls_[gkeys[gi]].xoff, n, k0, (int)fwd_t.size()});
                 }
             }
-            pending_solve[ilvl_] = std::async(
-                std::launch::async,
-                [esz = sizeof(T), sv_elim = std::move(sv_elim), sv_scale = std::move(sv_scale),
-                 fwd_groups = std::move(fwd_groups), fwd_t = std::move(fwd_t), bwd_groups = std::move(bwd_groups),
-                 bwd_terms = std::move(bwd_terms), qops = std::move(qops), mt = std::move(mt),
-                 mchildren = std::move(mchildren)]() mutable {
-                    return build_solve_plan(esz, sv_elim, sv_scale, fwd_groups, fwd_t, bwd_groups, bwd_terms, qops,
-                                            std::move(mt), std::move(mchildren));
+            HostPhases in{std::move(sv_elim),    std::move(sv_scale),  std::move(fwd_groups), std::move(bwd_groups),
+                          std::move(fwd_t),      std::move(bwd_terms), std::move(qops)};
+            pending_solve[ilvl_] =
+                std::async(std::launch::async, [esz = sizeof(T), in = std::move(in), mt = std::move(mt),
+                                                mchildren = std::move(mchildren)]() mutable {
+                    return build_solve_plan(esz, std::move(in), std::move(mt), std::move(mchildren));
                 });
         }
         lst.t_solve_plan = wall() - t_sp0;
@@ -1664,6 +1786,8 @@ void Tree::factorize_impl() {
             upload_solve_plan(l, &h);
         }
     hip_check(hipDeviceSynchronize(), "factorize");
+    // a factorization that outgrew the staging buffer leaves one that holds all its uploads
+    if (h_stage_total_ > h_stage_cap_) stage_grow(h_stage_total_ + h_stage_total_ / 8);
     t_factor_ = wall() - t_start;
     price_waves(sizeof(T));
     prof_.collect();
@@ -1800,13 +1924,24 @@ long long Tree::trailing_coo(int* I, int* J, double* V) const {
 // ---------------------------------------------------------------------------------------
 // The SPD apply is its own transpose: every level's backward sweep applies the transposes
 // of its forward sweep's ops in reverse order (L^{-T} after L^{-1}, A_ns^T after A_ns, Q after
-// Q^T), so M^{-1} = W^T W for the forward sweep W.
+// Q^T), so M^{-1} = W^T W for the forward sweep W. A GEN tree runs the transposed phase lists.
 void Tree::solve_transpose(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
-    if (symm_ != 0) throw Error(ERR_UNSUPPORTED, "solve_transpose: the transposed GEN (PLU) apply is not built");
-    solve(x, nrhs, ldx, on_device, st);
+    solve_any(x, nrhs, ldx, on_device, st, fsymm_ != 0);
 }
 
 void Tree::solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
+    solve_any(x, nrhs, ldx, on_device, st, false);
+}
+
+void Tree::apply_any(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose) {
+    if (prec_ == 0)
+        apply_impl<double>(d_x, nrhs, sx, sr, st, transpose);
+    else
+        apply_impl<float>(d_x, nrhs, sx, sr, st, transpose);
+}
+
+void Tree::solve_any(double* x, int nrhs, int ldx, bool on_device, hipStream_t st, bool transpose) {
+    if (transpose && state_ == State::Factorized) upload_transposed();
     if (state_ != State::Factorized) throw Error(ERR_STATE, "solve() must follow factorize()");
     if (nrhs <= 0 || nrhs > 64 || ldx < N_) throw Error(ERR_ARG, "solve: need 1 <= nrhs <= 64 and ld >= N");
     double* dx = x;
@@ -1820,10 +1955,7 @@ void Tree::solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
         dx = tmp;
         ld = N_;
     }
-    if (prec_ == 0)
-        apply_impl<double>(dx, nrhs, 1, (size_t)ld, st);
-    else
-        apply_impl<float>(dx, nrhs, 1, (size_t)ld, st);
+    apply_any(dx, nrhs, 1, (size_t)ld, st, transpose);
     if (!on_device) {
         hip_check(hipStreamSynchronize(st), "solve");
         hip_check(hipMemcpy2D(x, (size_t)ldx * sizeof(double), tmp, (size_t)N_ * sizeof(double),
@@ -1846,7 +1978,7 @@ void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st) {
 // sweep in reverse, permute out. Split ops need no work: a sibling's segment aliases the
 // tail of its original cluster's segment.
 template <typename T>
-void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st) {
+void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose) {
     // ws: the segments (FP64 whatever the factor's element type T), then the partial sums
     // of the largest mv phase
     const size_t seg = (size_t)ws_len_ * nrhs, part = (size_t)max_pieces_ * SOLVE_CHUNK * nrhs;
@@ -1874,7 +2006,7 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
     run(KF_PERM, wp, "perm in", [&] { return launch_perm_in<T>(dx, sx, sr, nrhs, d_perm_, N_, ws, st); });
     for (int l = 0; l < nlevels_; l++) {
         const LevelSolve& p = solve_plan_[l];
-        for (const MvPhase& ph : p.fwd) run_phase(ph);
+        for (const MvPhase& ph : transpose ? p.tfwd : p.fwd) run_phase(ph);
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 1, st); });
@@ -1884,7 +2016,7 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
         if (p.n_merge)
             run(KF_MERGE, p.w_merge, "merge",
                 [&] { return launch_merge<T>(p.merge, p.n_merge, p.children, ws, nrhs, 0, st); });
-        for (const MvPhase& ph : p.bwd) run_phase(ph);
+        for (const MvPhase& ph : transpose ? p.tbwd : p.bwd) run_phase(ph);
     }
     run(KF_PERM, wp, "perm out", [&] { return launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st); });
 }

@@ -122,8 +122,11 @@ struct MvPhase {
 
 // Per-level solve plan: the forward phases (before the merge), the backward phases (after
 // the split back to the children), and the merge itself.
+struct HostPhases;  // host-side mv phases awaiting upload (tree.cpp)
+
 struct LevelSolve {
     std::vector<MvPhase> fwd, bwd;
+    std::vector<MvPhase> tfwd, tbwd;  // the adjoint apply: transposed bwd ops going up, fwd ops down
     SweepWork w_merge;
     MergeTask* merge = nullptr; int n_merge = 0;
     int2* children = nullptr;
@@ -158,8 +161,9 @@ class Tree {
     // M^{-1} applied in place to nrhs columns (ld >= N) of x, on device or host memory.
     void solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
     // M^{-T} applied in place (the adjoint apply). The SPD (LLT) preconditioner is
-    // symmetric, so this is solve(); GEN (PLU) is not built.
+    // symmetric, so this is solve(); a GEN (PLU) tree runs its transposed phase lists.
     void solve_transpose(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
+    void solve_any(double* x, int nrhs, int ldx, bool on_device, hipStream_t st, bool transpose);
     // M^{-1} applied in place to an interleaved device block (element (i, r) at x[i*nrhs+r]).
     void solve_interleaved(double* d_x, int nrhs, hipStream_t st);
 
@@ -189,21 +193,28 @@ class Tree {
   private:
     enum class State { Created, Partitioned, Assembled, Factorized };
     template <typename T> void factorize_impl();
-    template <typename T> void apply_impl(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st);
+    template <typename T>
+    void apply_impl(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose = false);
+    void apply_any(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose);
     int size(int c) const { return cls_[c].size; }
     int find_out_edge(int from, int to) const;
     int new_edge(int n1, int n2, void* A, bool original);
     void set_eliminated(int c);
     void* upload_tasks(const struct Packer& pk);
     char* stage_reserve(size_t bytes);
+    void stage_grow(size_t cap);
     void stage_upload(void* d, const void* h, size_t bytes);
     char* h_stage_ = nullptr;  // pinned staging of host -> device uploads
     size_t h_stage_cap_ = 0, h_stage_used_ = 0;
+    size_t h_stage_total_ = 0;  // bytes staged since the last factorization began (across growths)
     // Fine-grained pinned signals of the stepped RRQR: [0] the sequence number of the latest
     // step launch to start, [1..] one done flag per stepped cluster of the current level.
     void signal_reserve(size_t n);
     void price_waves(size_t esz);
     void upload_solve_plan(int level, void* host_plan);  // host_plan: SolvePlanHost (tree.cpp)
+    void upload_transposed();
+    // per level, the adjoint apply's phases, uploaded at the first transposed solve
+    std::vector<std::unique_ptr<HostPhases>> tplans_;
     void wait_signal(long long target);
     // Profiled stepped-RRQR waves, priced after the factorization from the clusters' final
     // dims (5 ints each, one block per level), so that no copy waits inside the level loop.
@@ -226,6 +237,7 @@ class Tree {
     int skip_ = 0;
     bool verb_ = false;
     int symm_ = 0;       // 0 SPD (LLT), 2 GEN (PLU)
+    int fsymm_ = 0;      // symm_ of the last factorization
     bool geo_ = false;
     int dim_ = 0;
     std::vector<double> X_;

@@ -1,6 +1,8 @@
 """Adjoint apply and differentiable solve. These have no counterpart in the reference
-(parity unpinned); they are pinned by the symmetry of the SPD apply, by the definition of
-the sampled outer product, and by central finite differences (h = 1e-6)."""
+(parity unpinned); they are pinned by the symmetry of the SPD apply, by the adjoint
+identity <M^{-T} x, y> = <x, M^{-1} y> of the GEN apply (M^{-1} itself being pinned to the
+oracle by the GEN parity tests), by the definition of the sampled outer product, and by
+central finite differences (h = 1e-6)."""
 import numpy as np
 import pytest
 
@@ -60,13 +62,31 @@ def test_spd_apply_is_symmetric(prec):
         assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(Mx[:, r]) * np.linalg.norm(y[:, r])
 
 
-def test_gen_transpose_not_built():
-    say("case gen transpose unsupported")
-    A, X = neglapl_unsym(10, 2, 3), linspace_nd(10, 2)
-    t = tree_for(A, X, 4, 1e-2, symm="GEN")
-    with pytest.raises(S.SpandError) as ei:
-        t.solve_transpose(np.ones(A.n))
-    assert ei.value.status == 8  # SPAND_ERR_UNSUPPORTED
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("kind,n,d,nl,tol", [("unsym", 12, 2, 4, 1e-2), ("unsym", 10, 3, 5, 1e-2),
+                                             ("cd", 14, 3, 6, 1e-2), ("unsym", 20, 2, 5, 0.0)])
+def test_gen_transpose_is_the_adjoint(kind, n, d, nl, tol, prec):
+    """<M^{-T} x, y> = <x, M^{-1} y> for the PLU preconditioner (the transposed apply runs
+    its own phase lists, with P as a permuted destination), and M^{-T} differs from M^{-1}."""
+    from spand_public_amd.problems import convdiff
+    say(f"case gen transpose {kind} {d}D n={n} levels={nl} tol={tol} prec={prec}")
+    A = neglapl_unsym(n, d, 3) if kind == "unsym" else convdiff(n, d)
+    t = tree_for(A, linspace_nd(n, d), nl, tol, prec, symm="GEN")
+    rng = np.random.default_rng(8)
+    x, y = rng.standard_normal((A.n, 3)), rng.standard_normal((A.n, 3))
+    MTx, My = t.solve_transpose(x), t.solve(y)
+    for r in range(3):
+        lhs, rhs = MTx[:, r] @ y[:, r], x[:, r] @ My[:, r]
+        assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(MTx[:, r]) * np.linalg.norm(y[:, r]), (lhs, rhs)
+    assert np.linalg.norm(MTx - t.solve(x)) > 1e-6 * np.linalg.norm(MTx)
+    # the transposed phases are set up once per factorization: again, and after a refactorization
+    assert np.array_equal(t.solve_transpose(x), MTx)
+    t.factorize()
+    assert np.array_equal(t.solve_transpose(x), MTx)
+    if tol == 0.0:  # exact factorization: M^{-T} = A^{-T} (to the factor's precision)
+        At = A.to_scipy().toarray().T
+        bound = 1e-10 if prec == S.FP64 else 1e-4
+        assert np.linalg.norm(At @ MTx - x) <= bound * np.linalg.norm(x)
 
 
 def test_sampled_outer_matches_definition():
