@@ -91,6 +91,17 @@ spand_status spand_get_nlevels(const spand_tree* t, int* nlevels);
 spand_status spand_get_stop(const spand_tree* t, int* stop);
 spand_status spand_nnz(const spand_tree* t, long long* nnz);
 spand_status spand_get_assembly_perm(const spand_tree* t, int* perm);
+/* Tree::get_clusters   include/tree.h:156, src/tree.cpp:265-273 (what write_clusters,
+ * write_merging and write_stats of src/tree.cpp:29-55 print). Rows of 7 ints, one per cluster
+ * of every level's bottom in the reference's list order: order, level, merge level (level -
+ * depth, src/cluster.cpp:178-193), parent's order (-1: none), original size, size (the rank
+ * kept), start. Every level's bottom is listed from partition on (the reference builds them
+ * there); sizes are the factorization's once it has run. Call with out == NULL to get the row
+ * count in *count first. */
+spand_status spand_get_clusters(const spand_tree* t, int* out, int* count);
+/* Tree::get_dof2ID   include/tree.h:157, src/tree.cpp:275-284: ids[dof] = order of the level-0
+ * cluster holding dof (N ints). Valid from partition on. */
+spand_status spand_get_dof2ID(const spand_tree* t, int* ids);
 /* Tree::get_trailing_mat (src/tree.cpp:1730-1763) as COO of the lower triangle (SPD).
  * Call with I == NULL to get the count in *nnz first. */
 spand_status spand_get_trailing_coo(const spand_tree* t, long long* nnz, int* I, int* J, double* V);

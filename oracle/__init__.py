@@ -63,6 +63,7 @@ def lib():
             "or_get_shapes": (None, [vp, _i32p]),
             "or_num_clusters": (C.c_int, [vp]),
             "or_clusters": (None, [vp, _i32p]),
+            "or_num_bottom_clusters": (C.c_int, [vp, C.c_int]),
             "or_cg": (C.c_int, [vp, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int, C.c_double,
                                 C.POINTER(C.c_double)]),
             "or_gmres": (C.c_int, [vp, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_double,
@@ -224,6 +225,15 @@ class OracleTree:
         out = np.zeros(8 * n, dtype=np.int32)
         self._L.or_clusters(self.h, out)
         return out.reshape(n, 8)
+
+    def bottom_clusters(self, level=-1):
+        """The rows of clusters() that Tree::get_clusters lists (every level's bottom), or
+        only level `level`'s bottom (they come in level order)."""
+        rows = self.clusters()
+        if level < 0:
+            return rows[: self._L.or_num_bottom_clusters(self.h, -1)]
+        k0 = sum(self._L.or_num_bottom_clusters(self.h, l) for l in range(level))
+        return rows[k0: k0 + self._L.or_num_bottom_clusters(self.h, level)]
 
     def cg(self, A, b, x=None, iters=100, tol=1e-12):
         b = np.ascontiguousarray(b, dtype=np.float64)

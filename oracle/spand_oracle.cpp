@@ -1273,6 +1273,15 @@ int or_num_clusters(void* h) {
     for (auto& b : t->bottoms) n += (int)b.size();
     return n;
 }
+// clusters listed by Tree::get_clusters (src/tree.cpp:265-273): the bottoms only (or_clusters
+// lists them first, then the sparsification siblings); level >= 0: that bottom's count
+int or_num_bottom_clusters(void* h, int level) {
+    Tree* t = (Tree*)h;
+    if (level >= 0) return level < (int)t->bottoms.size() ? (int)t->bottoms[level].size() : 0;
+    int n = 0;
+    for (auto& b : t->bottoms) n += (int)b.size();
+    return n;
+}
 void or_clusters(void* h, int* out) {
     Tree* t = (Tree*)h;
     int k = 0;

@@ -140,6 +140,26 @@ class Tree:
         _check(self._h, self._L.spand_get_assembly_perm(self._h, p))
         return p
 
+    def get_nlevels(self) -> int:
+        v = C.c_int()
+        _check(self._h, self._L.spand_get_nlevels(self._h, C.byref(v)))
+        return v.value
+
+    def get_clusters(self) -> np.ndarray:
+        """Tree::get_clusters (src/tree.cpp:265-273) as rows of (order, level, merge level,
+        parent order or -1, original size, size, start), in the reference's list order."""
+        k = C.c_int()
+        _check(self._h, self._L.spand_get_clusters(self._h, None, C.byref(k)))
+        out = np.zeros((k.value, 7), dtype=np.int32)
+        _check(self._h, self._L.spand_get_clusters(self._h, out.ctypes.data, C.byref(k)))
+        return out
+
+    def get_dof2ID(self) -> np.ndarray:
+        """Tree::get_dof2ID (src/tree.cpp:275-284): order of each dof's level-0 cluster."""
+        ids = np.zeros(self.get_N(), dtype=np.int32)
+        _check(self._h, self._L.spand_get_dof2ID(self._h, ids))
+        return ids
+
     def get_trailing_mat(self):
         """Current trailing matrix (scipy CSC, both triangles), Tree::get_trailing_mat."""
         import scipy.sparse as sp

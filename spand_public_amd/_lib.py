@@ -52,6 +52,8 @@ SIGNATURES = {
     "spand_nnz": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
     "spand_get_assembly_perm": (C.c_int, [vp, i32p]),
     "spand_get_trailing_coo": (C.c_int, [vp, C.POINTER(C.c_longlong), vp, vp, vp]),
+    "spand_get_clusters": (C.c_int, [vp, vp, C.POINTER(C.c_int)]),
+    "spand_get_dof2ID": (C.c_int, [vp, i32p]),
     "spand_get_level_stats": (C.c_int, [vp, f64p]),
     "spand_get_factor_info": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),

@@ -173,6 +173,16 @@ spand_status spand_get_assembly_perm(const spand_tree* t, int* perm) {
     std::memcpy(perm, p.data(), p.size() * sizeof(int));
     return SPAND_OK;
 }
+spand_status spand_get_clusters(const spand_tree* t, int* out, int* count) {
+    CHECK_T(t);
+    if (!count) return SPAND_ERR_ARG;
+    return guarded(const_cast<spand_tree*>(t), [&] { *count = t->t->clusters_info(out); });
+}
+spand_status spand_get_dof2ID(const spand_tree* t, int* ids) {
+    CHECK_T(t);
+    if (!ids) return SPAND_ERR_ARG;
+    return guarded(const_cast<spand_tree*>(t), [&] { t->t->dof2id(ids); });
+}
 spand_status spand_get_trailing_coo(const spand_tree* t, long long* nnz, int* I, int* J, double* V) {
     CHECK_T(t);
     return guarded(const_cast<spand_tree*>(t), [&] { *nnz = t->t->trailing_coo(I, J, V); });

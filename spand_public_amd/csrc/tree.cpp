@@ -1868,6 +1868,41 @@ void Tree::op_plu(int i, double* U, int* perm) const {
     }
 }
 
+// Columns: order, level, merge level (level - depth, src/cluster.cpp:178-193), parent's
+// order (-1: none), original size (its x segment), size (after sparsification), start.
+// Every level's bottom exists from the partition on, as in the reference
+// (src/tree.cpp:368-412); the sizes are the factorization's once it has run.
+int Tree::clusters_info(int* out) const {
+    if (state_ == State::Created) throw Error(ERR_STATE, "get_clusters() must follow partition()");
+    int k = 0;
+    for (size_t l = 0; l < bottoms_.size(); l++)
+        for (int c : bottoms_[l]) {
+            if (out) {
+                const HCluster& h = cls_[c];
+                int depth = 0;
+                for (int p = h.parent; p >= 0; p = cls_[p].parent) depth++;
+                int* o = out + (size_t)CLUSTER_COLS * k;
+                o[0] = h.order;
+                o[1] = h.level;
+                o[2] = h.level - depth;
+                o[3] = h.parent >= 0 ? cls_[h.parent].order : -1;
+                o[4] = h.osize;
+                o[5] = h.size;
+                o[6] = h.start;
+            }
+            k++;
+        }
+    return k;
+}
+
+void Tree::dof2id(int* ids) const {
+    if (state_ == State::Created) throw Error(ERR_STATE, "get_dof2ID() must follow partition()");
+    for (int c : bottoms_[0]) {
+        const HCluster& h = cls_[c];  // level-0 starts and sizes: fixed by the partition
+        for (int i = 0; i < h.osize; i++) ids[perm_[h.start + i]] = h.order;
+    }
+}
+
 long long Tree::trailing_coo(int* I, int* J, double* V) const {
     // Before factorize(): the assembled matrix, from the scatter map. After: the current
     // bottom's blocks (empty once every level is eliminated).

@@ -187,6 +187,12 @@ class Tree {
     void op_plu(int i, double* U, int* perm) const;
     // Trailing (current) matrix as COO triplets; returns the count (arrays may be null).
     long long trailing_coo(int* I, int* J, double* V) const;
+    // Tree::get_clusters (src/tree.cpp:265-273): every level's bottom, in cluster order, as
+    // rows of CLUSTER_COLS ints; returns the row count (out may be null).
+    static constexpr int CLUSTER_COLS = 7;
+    int clusters_info(int* out) const;
+    // Tree::get_dof2ID (src/tree.cpp:275-284): dof -> order of its level-0 cluster
+    void dof2id(int* ids) const;
     // Per-launch kernel timing (HIP events on the launch stream); off by default.
     KernelProfiler& profiler() { return prof_; }
 

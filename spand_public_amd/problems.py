@@ -273,6 +273,7 @@ def mmread(path: str) -> CSC:
     r = data[:, 0].astype(np.int64) - 1
     c = data[:, 1].astype(np.int64) - 1
     v = data[:, 2].astype(np.float64)
+    prop = prop.replace("_", "-")  # the reference's writer spells it skew_symmetric
     if prop in ("symmetric", "hermitian", "skew-symmetric"):
         off = r != c
         sgn = -1.0 if prop == "skew-symmetric" else 1.0
