@@ -1106,6 +1106,135 @@ void Tree::factorize_impl() {
                     }
             }
         }
+        // ---------------- upload + launch Elim ----------------
+        // Nothing planned below feeds the elimination group (fill-in zeroing, identity inits
+        // of the explicit inverses, pivots, panels, Schur updates): it is launched now, so the
+        // device eliminates while the host plans the level's scaling and sparsification.
+        const size_t esz = sizeof(T);
+        auto small_first = [](std::vector<PotrfTask>& v) {  // <= 64, <= POTRF_SMALL_MAX, larger
+            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= POTRF_SMALL_MAX; });
+            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= 64; });
+        };
+        // LU pivots that fit LDS first (one launch), the larger ones after (another)
+        auto lds_first = [&](std::vector<GetrfTask>& v) {
+            std::stable_partition(v.begin(), v.end(), [&](const GetrfTask& t) {
+                return (size_t)t.n * (t.n + 1) * sizeof(T) <= (size_t)GETRF_LDS_BYTES;
+            });
+        };
+        // pivots up to POTRF_SMALL_MAX in one launch (one workgroup each); larger ones (sorted
+        // last) one 32-column panel per launch pair
+        auto run_potrf = [&](void* dt, const std::vector<PotrfTask>& v, size_t off, const char* what) {
+            if (v.empty()) return;
+            int ntiny = 0, nsmall = 0, nmax = 0;
+            for (auto& t : v) {
+                if (t.n <= 64) ntiny++;
+                if (t.n <= POTRF_SMALL_MAX) nsmall++;
+                nmax = std::max(nmax, t.n);
+            }
+            const PotrfTask* d = Packer::at<PotrfTask>(dt, off);
+            if (ntiny > 0) {
+                std::vector<PotrfTask> vs(v.begin(), v.begin() + ntiny);
+                Work w = work_potrf(vs, esz);
+                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
+                                [&] { return launch_potrf64<T>(d, ntiny, d_info_, st); }),
+                          what);
+            }
+            if (nsmall > ntiny) {
+                std::vector<PotrfTask> vs(v.begin() + ntiny, v.begin() + nsmall);
+                Work w = work_potrf(vs, esz);
+                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
+                                [&] { return launch_potrf<T>(d + ntiny, nsmall - ntiny, d_info_, st); }),
+                          what);
+            }
+            const int nbig = (int)v.size() - nsmall;
+            if (nbig == 0) return;
+            const PotrfTask* db = d + nsmall;
+            for (int kb = 0; kb < nmax; kb += 32) {
+                double pb = 0, pf = 0, tb = 0, tf = 0;
+                for (int k = nsmall; k < (int)v.size(); k++) {
+                    const double n = v[k].n, nb = std::min(32.0, n - kb), rest = n - kb - nb;
+                    if (nb <= 0) continue;
+                    pb += (nb * (nb + 1) + 2.0 * rest * nb) * esz;
+                    pf += nb * nb * nb / 3.0 + rest * nb * nb;
+                    tb += (rest * (rest + 1) + 2.0 * rest * nb) * esz;
+                    tf += rest * (rest + 1) * nb;
+                }
+                hip_check(timed(prof_, KF_POTRF_PANEL, st, pb, pf,
+                                [&] { return launch_potrf_panel<T>(db, nbig, kb, nmax, d_info_, st); }),
+                          what);
+                hip_check(timed(prof_, KF_POTRF_TRAIL, st, tb, tf,
+                                [&] { return launch_potrf_trail<T>(db, nbig, kb, nmax, st); }),
+                          what);
+            }
+        };
+        auto run_trsm = [&](void* dt, const std::vector<TrsmTask>& v, size_t off, const char* what) {
+            if (v.empty()) return;
+            Work w = work_trsm(v, esz);
+            hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
+                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), st); }),
+                      what);
+        };
+        auto run_getrf = [&](void* dt, const std::vector<GetrfTask>& v, size_t off, const char* what) {
+            if (v.empty()) return;
+            int nl = 0, nmax_l = 0;
+            double lb = 0, lf = 0, gb = 0, gf = 0;
+            for (auto& t : v) {
+                const double n = t.n;
+                const double by = (3.0 * n * n) * esz + 4.0 * n, fl = getrf_flops(n);
+                if ((size_t)t.n * (t.n + 1) * esz <= (size_t)GETRF_LDS_BYTES) {
+                    nl++;
+                    nmax_l = std::max(nmax_l, t.n);
+                    lb += by;
+                    lf += fl;
+                } else {
+                    gb += by;
+                    gf += fl;
+                }
+            }
+            const GetrfTask* d = Packer::at<GetrfTask>(dt, off);
+            if (nl > 0)
+                hip_check(timed(prof_, KF_GETRF, st, lb, lf,
+                                [&] { return launch_getrf<T>(d, nl, 1, nmax_l, d_info_, st); }),
+                          what);
+            if (nl < (int)v.size())
+                hip_check(timed(prof_, KF_GETRF, st, gb, gf,
+                                [&] { return launch_getrf<T>(d + nl, (int)v.size() - nl, 0, 0, d_info_, st); }),
+                          what);
+        };
+        const size_t n_potrf_elim = potrfE.size();
+        {
+            small_first(potrfE);
+            for (PotrfTask& t : potrfE)
+                if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
+            lds_first(getrfE);
+            Packer pe;
+            const size_t o_pe = pe.add(potrfE), o_te = pe.add(trsmE), o_ti = pe.add(tiles), o_co = pe.add(contribs);
+            const size_t o_ie = pe.add(inv_init), o_ge = pe.add(getrfE);
+            void* de = upload_tasks(pe);
+            if (fill_elems > 0)
+                hip_check(timed(prof_, KF_FILL, st, (double)fill_elems * esz, 0,
+                                [&] { return launch_fill<T>(fill_base, (size_t)fill_elems, T(0), st); }),
+                          "fill fill-in");
+            if (!inv_init.empty()) {
+                Work w = work_copy(inv_init, esz);
+                hip_check(timed(prof_, KF_COPY, st, w.bytes, 0,
+                                [&] { return launch_copy<T>(Packer::at<CopyTask>(de, o_ie), (int)inv_init.size(), st); }),
+                          "identity blocks");
+            }
+            run_potrf(de, potrfE, o_pe, "potrf elim");
+            run_getrf(de, getrfE, o_ge, "getrf elim");
+            run_trsm(de, trsmE, o_te, "trsm elim");
+            if (!tiles.empty()) {
+                Work w = work_schur(tiles, contribs, esz);
+                hip_check(timed(prof_, KF_SCHUR, st, w.bytes, w.flops,
+                                [&] {
+                                    return launch_schur<T>(Packer::at<SchurTile>(de, o_ti), (int)tiles.size(),
+                                                           Packer::at<SchurContrib>(de, o_co), st);
+                                }),
+                          "schur");
+            }
+            inv_init.clear();  // the scaling's identity inits follow
+        }
         int dofs_left = 0;
         for (int c : bot)
             if (!cls_[c].elim) dofs_left += cls_[c].size;
@@ -1349,28 +1478,14 @@ void Tree::factorize_impl() {
         lst.nwaves = nwaves;
         // ---------------- upload + launch Elim / Scale / Sparsify ----------------
         for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
-        auto small_first = [](std::vector<PotrfTask>& v) {  // <= 64, <= POTRF_SMALL_MAX, larger
-            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= POTRF_SMALL_MAX; });
-            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= 64; });
-        };
-        small_first(potrfE);
         small_first(potrfS);
-        for (auto* v : {&potrfE, &potrfS})
-            for (PotrfTask& t : *v)
-                if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
+        for (PotrfTask& t : potrfS)
+            if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
         Packer pk;
-        size_t o_pe = pk.add(potrfE), o_te = pk.add(trsmE), o_ti = pk.add(tiles), o_co = pk.add(contribs);
         size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
         size_t o_ii = pk.add(inv_init);
-        // LU pivots that fit LDS first (one launch), the larger ones after (another)
-        auto lds_first = [&](std::vector<GetrfTask>& v) {
-            std::stable_partition(v.begin(), v.end(), [&](const GetrfTask& t) {
-                return (size_t)t.n * (t.n + 1) * sizeof(T) <= (size_t)GETRF_LDS_BYTES;
-            });
-        };
-        lds_first(getrfE);
         lds_first(getrfS);
-        size_t o_ge = pk.add(getrfE), o_gs = pk.add(getrfS);
+        size_t o_gs = pk.add(getrfS);
         std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves);
         for (int w = 0; w < nwaves; w++) {
             o_w[w] = pk.add(wtasks[w]);
@@ -1380,113 +1495,16 @@ void Tree::factorize_impl() {
         const double t_plan1 = wall();
         void* dt = upload_tasks(pk);
         stage_upload(d_size_, hsize.data(), cls_.size() * sizeof(int));
-        const size_t esz = sizeof(T);
-        if (fill_elems > 0)
-            hip_check(timed(prof_, KF_FILL, st, (double)fill_elems * esz, 0,
-                            [&] { return launch_fill<T>(fill_base, (size_t)fill_elems, T(0), st); }),
-                      "fill fill-in");
-        {
+        if (!inv_init.empty()) {
             Work w = work_copy(inv_init, esz);
             hip_check(timed(prof_, KF_COPY, st, w.bytes, 0,
                             [&] { return launch_copy<T>(Packer::at<CopyTask>(dt, o_ii), (int)inv_init.size(), st); }),
                       "identity blocks");
         }
-        // pivots up to POTRF_SMALL_MAX in one launch (one workgroup each); larger ones (sorted
-        // last) one 32-column panel per launch pair
-        auto run_potrf = [&](const std::vector<PotrfTask>& v, size_t off, const char* what) {
-            if (v.empty()) return;
-            int ntiny = 0, nsmall = 0, nmax = 0;
-            for (auto& t : v) {
-                if (t.n <= 64) ntiny++;
-                if (t.n <= POTRF_SMALL_MAX) nsmall++;
-                nmax = std::max(nmax, t.n);
-            }
-            const PotrfTask* d = Packer::at<PotrfTask>(dt, off);
-            if (ntiny > 0) {
-                std::vector<PotrfTask> vs(v.begin(), v.begin() + ntiny);
-                Work w = work_potrf(vs, esz);
-                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
-                                [&] { return launch_potrf64<T>(d, ntiny, d_info_, st); }),
-                          what);
-            }
-            if (nsmall > ntiny) {
-                std::vector<PotrfTask> vs(v.begin() + ntiny, v.begin() + nsmall);
-                Work w = work_potrf(vs, esz);
-                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
-                                [&] { return launch_potrf<T>(d + ntiny, nsmall - ntiny, d_info_, st); }),
-                          what);
-            }
-            const int nbig = (int)v.size() - nsmall;
-            if (nbig == 0) return;
-            const PotrfTask* db = d + nsmall;
-            for (int kb = 0; kb < nmax; kb += 32) {
-                double pb = 0, pf = 0, tb = 0, tf = 0;
-                for (int k = nsmall; k < (int)v.size(); k++) {
-                    const double n = v[k].n, nb = std::min(32.0, n - kb), rest = n - kb - nb;
-                    if (nb <= 0) continue;
-                    pb += (nb * (nb + 1) + 2.0 * rest * nb) * esz;
-                    pf += nb * nb * nb / 3.0 + rest * nb * nb;
-                    tb += (rest * (rest + 1) + 2.0 * rest * nb) * esz;
-                    tf += rest * (rest + 1) * nb;
-                }
-                hip_check(timed(prof_, KF_POTRF_PANEL, st, pb, pf,
-                                [&] { return launch_potrf_panel<T>(db, nbig, kb, nmax, d_info_, st); }),
-                          what);
-                hip_check(timed(prof_, KF_POTRF_TRAIL, st, tb, tf,
-                                [&] { return launch_potrf_trail<T>(db, nbig, kb, nmax, st); }),
-                          what);
-            }
-        };
-        auto run_trsm = [&](const std::vector<TrsmTask>& v, size_t off, const char* what) {
-            if (v.empty()) return;
-            Work w = work_trsm(v, esz);
-            hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
-                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), st); }),
-                      what);
-        };
-        auto run_getrf = [&](const std::vector<GetrfTask>& v, size_t off, const char* what) {
-            if (v.empty()) return;
-            int nl = 0, nmax_l = 0;
-            double lb = 0, lf = 0, gb = 0, gf = 0;
-            for (auto& t : v) {
-                const double n = t.n;
-                const double by = (3.0 * n * n) * esz + 4.0 * n, fl = getrf_flops(n);
-                if ((size_t)t.n * (t.n + 1) * esz <= (size_t)GETRF_LDS_BYTES) {
-                    nl++;
-                    nmax_l = std::max(nmax_l, t.n);
-                    lb += by;
-                    lf += fl;
-                } else {
-                    gb += by;
-                    gf += fl;
-                }
-            }
-            const GetrfTask* d = Packer::at<GetrfTask>(dt, off);
-            if (nl > 0)
-                hip_check(timed(prof_, KF_GETRF, st, lb, lf,
-                                [&] { return launch_getrf<T>(d, nl, 1, nmax_l, d_info_, st); }),
-                          what);
-            if (nl < (int)v.size())
-                hip_check(timed(prof_, KF_GETRF, st, gb, gf,
-                                [&] { return launch_getrf<T>(d + nl, (int)v.size() - nl, 0, 0, d_info_, st); }),
-                          what);
-        };
-        run_potrf(potrfE, o_pe, "potrf elim");
-        run_getrf(getrfE, o_ge, "getrf elim");
-        run_trsm(trsmE, o_te, "trsm elim");
-        if (!tiles.empty()) {
-            Work w = work_schur(tiles, contribs, esz);
-            hip_check(timed(prof_, KF_SCHUR, st, w.bytes, w.flops,
-                            [&] {
-                                return launch_schur<T>(Packer::at<SchurTile>(dt, o_ti), (int)tiles.size(),
-                                                       Packer::at<SchurContrib>(dt, o_co), st);
-                            }),
-                      "schur");
-        }
-        run_potrf(potrfS, o_ps, "potrf scale");
-        run_getrf(getrfS, o_gs, "getrf scale");
-        run_trsm(trsmA, o_ta, "trsm scale A");
-        run_trsm(trsmB, o_tb, "trsm scale B");
+        run_potrf(dt, potrfS, o_ps, "potrf scale");
+        run_getrf(dt, getrfS, o_gs, "getrf scale");
+        run_trsm(dt, trsmA, o_ta, "trsm scale A");
+        run_trsm(dt, trsmB, o_tb, "trsm scale B");
         // stepped-path launch slots, for the post-hoc work attribution below
         std::vector<int> slot_init(nwaves, -1), slot_finish(nwaves, -1);
         std::vector<int> slot_step(nwaves, -1);
@@ -1777,7 +1795,7 @@ void Tree::factorize_impl() {
         }
         lst.t_solve_plan = wall() - t_sp0;
         if (verb_)
-            printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, potrfE.size(),
+            printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, n_potrf_elim,
                    potrfS.size(), plist.size(), nwaves, left);
     }
     for (int l = 0; l < nlevels_; l++)
