@@ -69,6 +69,12 @@ spand_status spand_partition(spand_tree* t, int N, const int* colptr, const int*
  * May also be called after an assembly or a factorization, with new values of the
  * partitioned pattern (re-assembly; then factorize again). */
 spand_status spand_assemble(spand_tree* t, int N, const int* colptr, const int* rowval, const double* vals);
+/* Tree::assemble(SpMat&) with new values only (include/tree.h:175), read from device memory:
+ * d_vals holds the nnz stored entries in the CSC order given to the last spand_assemble
+ * (same pattern). The copy is ordered on `stream` and complete on return; the next
+ * spand_factorize uses the new values. For callers whose values live on the GPU (a
+ * differentiable solve's parameters). */
+spand_status spand_assemble_values_device(spand_tree* t, const double* d_vals, long long nnz, void* stream);
 /* Tree::factorize()                                include/tree.h:179, src/tree.cpp:1447-1551
  * May be called again after a completed factorization (refactorization of the assembled
  * matrix; no device allocation once the arenas have grown). */
@@ -151,6 +157,10 @@ spand_status spand_op_plu(const spand_tree* t, int i, double* U, int* perm);
 spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, const double* vals,
                                  spand_matrix** out);
 void spand_matrix_destroy(spand_matrix* A);
+/* New values of the matrix's pattern from device memory (nnz entries, the CSC order given to
+ * spand_matrix_create), ordered on `stream` and complete on return. No counterpart in the
+ * reference (its SpMat is rebuilt on the host). */
+spand_status spand_matrix_set_values_device(spand_matrix* A, const double* d_vals, long long nnz, void* stream);
 /* vals_out[k] = alpha * sum_r u[i_k + r*ld] * v[j_k + r*ld] for every stored entry k = (i_k, j_k)
  * of A, in the CSC order given to spand_matrix_create: with alpha = -1, u the adjoint and v
  * the solution of A x = b, the gradient of a loss with respect to A's stored values. No
@@ -188,6 +198,13 @@ spand_status spand_pcg(spand_tree* t, const spand_matrix* A, const double* b, do
 spand_status spand_gmres(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld,
                          int maxit, int restart, double tol, int on_device, int* iters_out,
                          spand_krylov_stats* stats);
+
+/* The adjoint solve A^T x = b: gmres above on the transposed operator with the transposed
+ * preconditioner M^{-T} (spand_solve_transpose). No counterpart in the reference; for the
+ * backward pass of a differentiable nonsymmetric solve. Same arguments as spand_gmres. */
+spand_status spand_gmres_transpose(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld,
+                                   int maxit, int restart, double tol, int on_device, int* iters_out,
+                                   spand_krylov_stats* stats);
 
 #ifdef __cplusplus
 }

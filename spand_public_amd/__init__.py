@@ -20,7 +20,7 @@ from . import problems
 from ._lib import KrylovStats, SpandError, lib
 from .problems import CSC
 
-__all__ = ["Tree", "Matrix", "cg", "cg_device", "gmres", "problems", "SpandError", "CSC", "FP64", "FP32"]
+__all__ = ["Tree", "Matrix", "cg", "cg_device", "gmres", "gmres_device", "problems", "SpandError", "CSC", "FP64", "FP32"]
 
 FP64 = 0
 FP32 = 1
@@ -91,6 +91,12 @@ class Tree:
 
     def assemble(self, A: CSC):
         _check(self._h, self._L.spand_assemble(self._h, A.n, A.colptr, A.rowval, A.nzval))
+
+    def assemble_values_device(self, d_vals: int, nnz: int, stream: int = 0):
+        """New values of the assembled pattern from a device array (nnz float64 entries in the
+        CSC order given to assemble(), e.g. a torch CUDA tensor's data_ptr())."""
+        _check(self._h, self._L.spand_assemble_values_device(self._h, C.c_void_p(d_vals), int(nnz),
+                                                              C.c_void_p(stream)))
 
     def factorize(self):
         _check(self._h, self._L.spand_factorize(self._h))
@@ -273,6 +279,11 @@ class Matrix:
         _check(None, self._L.spand_matrix_sampled_outer(self._h, C.c_void_p(d_u), C.c_void_p(d_v), int(nrhs),
                                                         int(ld), float(alpha), C.c_void_p(d_out), 1))
 
+    def set_values_device(self, d_vals: int, nnz: int, stream: int = 0):
+        """New values of the same pattern from a device array (CSC order)."""
+        _check(None, self._L.spand_matrix_set_values_device(self._h, C.c_void_p(d_vals), int(nnz),
+                                                            C.c_void_p(stream)))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -291,6 +302,20 @@ def cg_device(A: "Matrix", precond: Tree, d_b: int, d_x: int, nrhs: int, ld: int
     st = KrylovStats()
     status = lib().spand_pcg(precond._h, A._h, C.c_void_p(d_b), C.c_void_p(d_x), int(nrhs), int(ld), int(iters),
                              float(tol), 1, its.ctypes.data, C.byref(st))
+    if status not in (0, 7):
+        _check(precond._h, status)
+    return its, {"status": status, **{f: getattr(st, f) for f, _ in KrylovStats._fields_}}
+
+
+def gmres_device(A: "Matrix", precond: Tree, d_b: int, d_x: int, nrhs: int, ld: int, iters: int = 100,
+                 restart: int = 30, tol: float = 1e-12, transpose: bool = False):
+    """GMRES(restart) on device-resident column-major blocks, like cg_device; transpose=True
+    solves A^T x = b with M^{-T} (spand_gmres_transpose)."""
+    its = np.zeros(nrhs, dtype=np.int32)
+    st = KrylovStats()
+    fn = lib().spand_gmres_transpose if transpose else lib().spand_gmres
+    status = fn(precond._h, A._h, C.c_void_p(d_b), C.c_void_p(d_x), int(nrhs), int(ld), int(iters), int(restart),
+                float(tol), 1, its.ctypes.data, C.byref(st))
     if status not in (0, 7):
         _check(precond._h, status)
     return its, {"status": status, **{f: getattr(st, f) for f, _ in KrylovStats._fields_}}

@@ -41,6 +41,7 @@ SIGNATURES = {
     "spand_set_precision": (C.c_int, [vp, C.c_int]),
     "spand_partition": (C.c_int, [vp, C.c_int, i32p, i32p, vp]),
     "spand_assemble": (C.c_int, [vp, C.c_int, i32p, i32p, f64p]),
+    "spand_assemble_values_device": (C.c_int, [vp, vp, C.c_longlong, vp]),
     "spand_factorize": (C.c_int, [vp]),
     "spand_solve": (C.c_int, [vp, vp, C.c_int, C.c_int]),
     "spand_solve_device": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
@@ -69,11 +70,14 @@ SIGNATURES = {
     "spand_op_plu": (C.c_int, [vp, C.c_int, vp, vp]),
     "spand_matrix_create": (C.c_int, [C.c_int, i32p, i32p, f64p, C.POINTER(vp)]),
     "spand_matrix_destroy": (None, [vp]),
+    "spand_matrix_set_values_device": (C.c_int, [vp, vp, C.c_longlong, vp]),
     "spand_matrix_sampled_outer": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]),
     "spand_pcg": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp,
                             C.POINTER(KrylovStats)]),
     "spand_gmres": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp,
                               C.POINTER(KrylovStats)]),
+    "spand_gmres_transpose": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                        vp, C.POINTER(KrylovStats)]),
 }
 
 

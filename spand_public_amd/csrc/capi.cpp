@@ -113,6 +113,10 @@ spand_status spand_assemble(spand_tree* t, int N, const int* colptr, const int* 
         t->t->assemble(N, colptr, rowval, vals);
     });
 }
+spand_status spand_assemble_values_device(spand_tree* t, const double* d_vals, long long nnz, void* stream) {
+    CHECK_T(t);
+    return guarded(t, [&] { t->t->assemble_values_device(d_vals, nnz, static_cast<hipStream_t>(stream)); });
+}
 spand_status spand_factorize(spand_tree* t) {
     CHECK_T(t);
     return guarded(t, [&] { t->t->factorize(); });
@@ -274,6 +278,17 @@ spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, co
     });
 }
 void spand_matrix_destroy(spand_matrix* A) { delete A; }
+spand_status spand_matrix_set_values_device(spand_matrix* A, const double* d_vals, long long nnz, void* stream) {
+    if (!A || !A->m || !d_vals) {
+        g_err = "matrix_set_values_device: null argument";
+        return SPAND_ERR_ARG;
+    }
+    if (nnz != A->m->nnz) {
+        g_err = "matrix_set_values_device: nnz differs from the matrix pattern";
+        return SPAND_ERR_ARG;
+    }
+    return guarded(nullptr, [&] { A->m->set_values_device(d_vals, static_cast<hipStream_t>(stream)); });
+}
 spand_status spand_matrix_sampled_outer(const spand_matrix* A, const double* u, const double* v, int nrhs, int ld,
                                         double alpha, double* vals_out, int on_device) {
     if (!A || !A->m || !u || !v || !vals_out) {
@@ -334,6 +349,21 @@ spand_status spand_gmres(spand_tree* t, const spand_matrix* A, const double* b, 
         t, [&] { r = spand::gmres(*t->t, *A->m, b, x, nrhs, ld, maxit, restart, tol, on_device != 0); });
     if (s != SPAND_OK) return s;
     return krylov_out(t, r, nrhs, iters_out, stats, "gmres");
+}
+
+spand_status spand_gmres_transpose(spand_tree* t, const spand_matrix* A, const double* b, double* x, int nrhs, int ld,
+                                   int maxit, int restart, double tol, int on_device, int* iters_out,
+                                   spand_krylov_stats* stats) {
+    CHECK_T(t);
+    if (!A || !A->m || !b || !x) {
+        t->err = "gmres_transpose: null argument";
+        return SPAND_ERR_ARG;
+    }
+    spand::KrylovResult r;
+    spand_status s = guarded(
+        t, [&] { r = spand::gmres(*t->t, *A->m, b, x, nrhs, ld, maxit, restart, tol, on_device != 0, true); });
+    if (s != SPAND_OK) return s;
+    return krylov_out(t, r, nrhs, iters_out, stats, "gmres_transpose");
 }
 
 }  // extern "C"

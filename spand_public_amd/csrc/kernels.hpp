@@ -118,6 +118,9 @@ hipError_t launch_axpby(int N, int nrhs, const double* x, const double* ax, doub
 hipError_t launch_ratio(int nrhs, const double* num, const double* den, const int* active, double* out,
                         hipStream_t st);
 hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t st);
+// dst[k] = src[idx[k]] for k < n (FP64 values of a fixed pattern, reordered)
+hipError_t launch_gather_f64(const long long* idx, const double* src, long long n, double* dst, hipStream_t st);
+hipError_t launch_gather_f64(const int* idx, const double* src, long long n, double* dst, hipStream_t st);
 // GMRES: out = base + [ye; 0] - sum_{j<k} V_j z_j; W = V^T v for k blocks (partial holds
 // 256 * k * nrhs doubles); Householder vector of step k.
 hipError_t launch_gm_comb(int N, int nrhs, const double* base, const double* ye, int ny, const double* V, int k,

@@ -405,6 +405,13 @@ __global__ void __launch_bounds__(NT) k_copy_f64(double* dst, const double* src,
     for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) dst[i] = src[i];
 }
 
+// dst[k] = src[idx[k]]: values of a fixed pattern moved from the caller's order to a stored one
+template <typename I>
+__global__ void __launch_bounds__(NT) k_gather_f64(const I* idx, const double* src, long long n, double* dst) {
+    for (long long k = (long long)blockIdx.x * NT + threadIdx.x; k < n; k += (long long)gridDim.x * NT)
+        dst[k] = src[idx[k]];
+}
+
 // ---------------------------------------------------------------------------------------
 template <typename T>
 hipError_t launch_perm_in(const double* b, size_t sx, size_t sr, int nrhs, const int* perm, int N, double* ws,
@@ -502,6 +509,16 @@ hipError_t launch_ratio(int nrhs, const double* num, const double* den, const in
 }
 hipError_t launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t st) {
     k_copy_f64<<<grid_for(n), NT, 0, st>>>(dst, src, n);
+    return hipGetLastError();
+}
+hipError_t launch_gather_f64(const long long* idx, const double* src, long long n, double* dst, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_gather_f64<long long><<<grid_for((size_t)n), NT, 0, st>>>(idx, src, n, dst);
+    return hipGetLastError();
+}
+hipError_t launch_gather_f64(const int* idx, const double* src, long long n, double* dst, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_gather_f64<int><<<grid_for((size_t)n), NT, 0, st>>>(idx, src, n, dst);
     return hipGetLastError();
 }
 

@@ -35,12 +35,26 @@ DevMatrix::DevMatrix(int n, const int* colptr, const int* rowval, const double* 
     hip_check(hipMalloc((void**)&colidx, std::max<long long>(nnz, 1) * sizeof(int)), "hipMalloc(csr)");
     hip_check(hipMalloc((void**)&val, std::max<long long>(nnz, 1) * sizeof(double)), "hipMalloc(csr)");
     hip_check(hipMalloc((void**)&csc_of, std::max<long long>(nnz, 1) * sizeof(int)), "hipMalloc(csr)");
+    hip_check(hipMalloc((void**)&tptr, (n + 1) * sizeof(int)), "hipMalloc(csc)");
+    hip_check(hipMalloc((void**)&tidx, std::max<long long>(nnz, 1) * sizeof(int)), "hipMalloc(csc)");
+    hip_check(hipMalloc((void**)&tval, std::max<long long>(nnz, 1) * sizeof(double)), "hipMalloc(csc)");
     hip_check(hipMemcpy(rowptr, rp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice), "upload csr");
+    hip_check(hipMemcpy(tptr, colptr, (n + 1) * sizeof(int), hipMemcpyHostToDevice), "upload csc");
     if (nnz) {
         hip_check(hipMemcpy(colidx, ci.data(), nnz * sizeof(int), hipMemcpyHostToDevice), "upload csr");
         hip_check(hipMemcpy(val, v.data(), nnz * sizeof(double), hipMemcpyHostToDevice), "upload csr");
         hip_check(hipMemcpy(csc_of, co.data(), nnz * sizeof(int), hipMemcpyHostToDevice), "upload csr");
+        hip_check(hipMemcpy(tidx, rowval, nnz * sizeof(int), hipMemcpyHostToDevice), "upload csc");
+        hip_check(hipMemcpy(tval, vals, nnz * sizeof(double), hipMemcpyHostToDevice), "upload csc");
     }
+}
+
+void DevMatrix::set_values_device(const double* vals, hipStream_t st) {
+    if (nnz == 0) return;
+    if (!vals) throw Error(ERR_ARG, "set_values: null values");
+    hip_check(launch_gather_f64(csc_of, vals, nnz, val, st), "gather values");
+    hip_check(launch_copy_f64(tval, vals, (size_t)nnz, st), "copy values");
+    hip_check(hipStreamSynchronize(st), "set values");
 }
 
 void DevMatrix::sampled_outer(const double* u, const double* v, int nrhs, int ld, double alpha, double* vals_out,
@@ -87,6 +101,9 @@ DevMatrix::~DevMatrix() {
     (void)hipFree(colidx);
     (void)hipFree(val);
     (void)hipFree(csc_of);
+    (void)hipFree(tptr);
+    (void)hipFree(tidx);
+    (void)hipFree(tval);
 }
 
 KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, double tol,
@@ -293,7 +310,7 @@ void apply_givens_adjoint(const Givens& g, double& x, double& y) {  // applyOnTh
 }  // namespace
 
 KrylovResult gmres(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, int restart,
-                   double tol, bool on_device) {
+                   double tol, bool on_device, bool transpose) {
     if (A.N != t.N()) throw Error(ERR_ARG, "gmres: matrix and tree sizes differ");
     if (nrhs <= 0 || nrhs > 64 || ld < A.N) throw Error(ERR_ARG, "gmres: bad nrhs / ld");
     if (restart < 1) throw Error(ERR_ARG, "gmres: restart must be positive");
@@ -329,14 +346,17 @@ KrylovResult gmres(Tree& t, const DevMatrix& A, const double* b, double* x, int 
     int applies = 0;
     auto spmv = [&](const double* in, double* out) {
         double a = wall();
+        const int* rp = transpose ? A.tptr : A.rowptr;
+        const int* ci = transpose ? A.tidx : A.colidx;
+        const double* va = transpose ? A.tval : A.val;
         hip_check(timed(prof, KF_SPMV, st, 12.0 * A.nnz + 4.0 * (N + 1) + 2.0 * vbytes, 2.0 * A.nnz * nrhs,
-                        [&] { return launch_spmv(N, A.rowptr, A.colidx, A.val, in, out, nrhs, st); }),
+                        [&] { return launch_spmv(N, rp, ci, va, in, out, nrhs, st); }),
                   "spmv");
         t_spmv += wall() - a;
     };
     auto precond = [&](double* w) {
         double a = wall();
-        t.solve_interleaved(w, nrhs, st);
+        t.solve_interleaved(w, nrhs, st, transpose);
         applies++;
         t_prec += wall() - a;
     };

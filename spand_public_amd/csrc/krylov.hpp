@@ -14,6 +14,10 @@ struct DevMatrix {
     int* colidx = nullptr;
     double* val = nullptr;
     int* csc_of = nullptr;  // CSR position -> index in the CSC arrays the matrix was built from
+    // the caller's CSC arrays, which are the CSR arrays of A^T (the transposed operator)
+    int* tptr = nullptr;
+    int* tidx = nullptr;
+    double* tval = nullptr;
     // Krylov work vectors, kept between calls (grown on demand): repeated solves with the
     // same operator allocate nothing.
     struct Workspace {
@@ -27,6 +31,8 @@ struct DevMatrix {
     // u, v, vals_out on the device (on_device) or on the host
     void sampled_outer(const double* u, const double* v, int nrhs, int ld, double alpha, double* vals_out,
                        bool on_device) const;
+    // New values of the same pattern (device array, CSC order), ordered on st.
+    void set_values_device(const double* vals, hipStream_t st);
     ~DevMatrix();
     DevMatrix(const DevMatrix&) = delete;
     DevMatrix& operator=(const DevMatrix&) = delete;
@@ -44,7 +50,8 @@ struct KrylovResult {
 KrylovResult pcg(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, double tol,
                  bool on_device);
 // gmres (src/is.cpp:123-300): left-preconditioned GMRES(restart), columns in lockstep.
+// transpose: solves A^T x = b with the adjoint preconditioner M^{-T}.
 KrylovResult gmres(Tree& t, const DevMatrix& A, const double* b, double* x, int nrhs, int ld, int maxit, int restart,
-                   double tol, bool on_device);
+                   double tol, bool on_device, bool transpose = false);
 
 }  // namespace spand

@@ -102,6 +102,7 @@ Tree::Tree(int nlevels) : nlevels_(nlevels), stats_(nlevels) {
 
 Tree::~Tree() {
     (void)hipFree(d_scatter_off_);
+    (void)hipFree(d_scatter_src_);
     (void)hipFree(d_scatter_val_);
     (void)hipFree(d_size_);
     (void)hipFree(d_info_);
@@ -113,8 +114,12 @@ Tree::~Tree() {
 
 // SymmKind (include/spaND.h): SPD (0) with ScalingKind::LLT, GEN (2) with ScalingKind::PLU.
 // SYM (1, LDLT) is not built.
+// The kind decides which blocks assemble() builds (lower only for SPD), so it cannot change
+// once the matrix is assembled.
 void Tree::set_symm_kind(int k) {
     if (k != 0 && k != 2) throw Error(ERR_UNSUPPORTED, "only SPD (LLT) and GEN (PLU) are available");
+    if (k != symm_ && (state_ == State::Assembled || state_ == State::Factorized))
+        throw Error(ERR_STATE, "set_symm_kind() must come before assemble()");
     symm_ = k;
 }
 void Tree::set_Xcoo(int dim, int N, const double* X) {
@@ -329,12 +334,13 @@ void Tree::assemble(int N, const int* colptr, const int* rowval, const double* v
     std::vector<long long> cp(N + 1, 0);
     for (int j = 0; j < N; j++) cp[pinv[j] + 1] += colptr[j + 1] - colptr[j];
     for (int j = 0; j < N; j++) cp[j + 1] += cp[j];
-    std::vector<std::pair<int, double>> ent(nnz);
+    // (permuted row, index of the entry in the caller's CSC arrays)
+    std::vector<std::pair<int, long long>> ent(nnz);
     {
         std::vector<long long> fill(cp.begin(), cp.end() - 1);
         for (int j = 0; j < N; j++) {
             int pj = pinv[j];
-            for (int k = colptr[j]; k < colptr[j + 1]; k++) ent[fill[pj]++] = {pinv[rowval[k]], vals[k]};
+            for (int k = colptr[j]; k < colptr[j + 1]; k++) ent[fill[pj]++] = {pinv[rowval[k]], (long long)k};
         }
         for (int j = 0; j < N; j++)
             std::sort(ent.begin() + cp[j], ent.begin() + cp[j + 1],
@@ -351,9 +357,10 @@ void Tree::assemble(int N, const int* colptr, const int* rowval, const double* v
     }
     edge0_off_.clear();
     level0_elems_ = 0;
-    std::vector<long long> soff;
+    std::vector<long long> soff, ssrc;
     std::vector<double> sval;
     soff.reserve(nnz);
+    ssrc.reserve(nnz);
     sval.reserve(nnz);
     std::vector<int> edge_to(cls_.size(), -1);
     for (int self : bottoms_[0]) {
@@ -385,14 +392,18 @@ void Tree::assemble(int N, const int* colptr, const int* rowval, const double* v
                 int e = edge_to[n];
                 if (e < 0 || edges_[e].n1 != self) continue;
                 soff.push_back(edge0_off_[e] + (i - cls_[n].start) + (long long)(j - s.start) * cls_[n].size);
-                sval.push_back(ent[k].second);
+                ssrc.push_back(ent[k].second);
+                sval.push_back(vals[ent[k].second]);
             }
         for (int n : nbrs) edge_to[n] = -1;
     }
     nscatter_ = (long long)soff.size();
     h_scatter_off_.swap(soff);
+    h_scatter_src_.swap(ssrc);
     h_scatter_val_.swap(sval);
+    nnz_in_ = nnz;
     device_inputs_ = false;
+    host_vals_stale_ = false;
     // ND statistics (src/tree.cpp:336-342)
     for (auto& st : stats_) st = LevelStats();
     for (int i = 0; i < N; i++) stats_[part_[(size_t)6 * i]].dofs_nd++;
@@ -407,15 +418,21 @@ void Tree::assemble(int N, const int* colptr, const int* rowval, const double* v
 void Tree::upload_inputs() {
     if (device_inputs_) return;
     (void)hipFree(d_scatter_off_);
+    (void)hipFree(d_scatter_src_);
     (void)hipFree(d_scatter_val_);
     (void)hipFree(d_perm_);
     d_scatter_off_ = nullptr;
+    d_scatter_src_ = nullptr;
     d_scatter_val_ = nullptr;
     d_perm_ = nullptr;
     if (nscatter_ > 0) {
         hip_check(hipMalloc((void**)&d_scatter_off_, nscatter_ * sizeof(long long)), "hipMalloc(scatter)");
+        hip_check(hipMalloc((void**)&d_scatter_src_, nscatter_ * sizeof(long long)), "hipMalloc(scatter)");
         hip_check(hipMalloc((void**)&d_scatter_val_, nscatter_ * sizeof(double)), "hipMalloc(scatter)");
         hip_check(hipMemcpy(d_scatter_off_, h_scatter_off_.data(), nscatter_ * sizeof(long long),
+                            hipMemcpyHostToDevice),
+                  "upload scatter");
+        hip_check(hipMemcpy(d_scatter_src_, h_scatter_src_.data(), nscatter_ * sizeof(long long),
                             hipMemcpyHostToDevice),
                   "upload scatter");
         hip_check(hipMemcpy(d_scatter_val_, h_scatter_val_.data(), nscatter_ * sizeof(double), hipMemcpyHostToDevice),
@@ -426,10 +443,32 @@ void Tree::upload_inputs() {
     device_inputs_ = true;
 }
 
+// Tree::assemble with new values of the assembled pattern, read on the device: vals holds
+// the nnz entries in the CSC order given to assemble(). The next factorize() uses them.
+void Tree::assemble_values_device(const double* vals, long long nnz, hipStream_t st) {
+    if (state_ != State::Assembled && state_ != State::Factorized)
+        throw Error(ERR_STATE, "assemble_values_device() must follow assemble()");
+    if (nnz != nnz_in_) throw Error(ERR_ARG, "assemble_values_device: nnz differs from the assembled pattern");
+    if (!vals && nscatter_ > 0) throw Error(ERR_ARG, "assemble_values_device: null values");
+    upload_inputs();
+    if (nscatter_ > 0) {
+        hip_check(launch_gather_f64(d_scatter_src_, vals, nscatter_, d_scatter_val_, st), "gather values");
+        hip_check(hipStreamSynchronize(st), "gather values");
+    }
+    host_vals_stale_ = true;
+    cls_ = cls0_;
+    edges_ = edges0_;
+    state_ = State::Assembled;
+}
+
 void Tree::factorize() {
     if (state_ != State::Assembled && state_ != State::Factorized)
         throw Error(ERR_STATE, "factorize() must follow assemble()");
     upload_inputs();
+    // a factorization that fails part-way leaves no usable factor: solve() refuses until the
+    // next factorize() succeeds
+    state_ = State::Assembled;
+    fprec_ = prec_;
     if (prec_ == 0)
         factorize_impl<double>();
     else
@@ -831,7 +870,8 @@ void Tree::upload_solve_plan(int level, void* host_plan) {
 // Builds and uploads the transposed phases of every level (first transposed solve after a
 // factorization). The partial-sum area grows with them if needed.
 void Tree::upload_transposed() {
-    const size_t esz = prec_ == 0 ? sizeof(double) : sizeof(float);
+    const size_t esz = fprec_ == 0 ? sizeof(double) : sizeof(float);
+    bool any = false;
     for (int l = 0; l < nlevels_; l++) {
         if (!tplans_[l]) continue;
         std::vector<MvPlan> up, down;
@@ -842,7 +882,10 @@ void Tree::upload_transposed() {
         solve_plan_[l].tfwd = std::move(ph[0]);
         solve_plan_[l].tbwd = std::move(ph[1]);
         tplans_[l].reset();
+        any = true;
     }
+    // the uploads run on the null stream; the caller's stream may not wait for it
+    if (any) hip_check(hipStreamSynchronize(0), "upload transposed phases");
 }
 
 template <typename T>
@@ -1846,6 +1889,7 @@ static void d2h_as_double(const void* src, size_t n, double* dst) {
 }
 
 void Tree::op_data(int i, double* M, double* tau) const {
+    if (state_ != State::Factorized) throw Error(ERR_STATE, "op data needs a completed factorize()");
     if (i < 0 || i >= (int)ops_.size()) throw Error(ERR_ARG, "op index out of range");
     const OpRec& o = ops_[i];
     const size_t n = (size_t)o.rows * o.cols;
@@ -1853,26 +1897,27 @@ void Tree::op_data(int i, double* M, double* tau) const {
         if (o.M == nullptr) {
             for (size_t k = 0; k < n; k++) M[k] = 0.0;
             for (int k = 0; k < std::min(o.rows, o.cols); k++) M[k + (size_t)k * o.rows] = 1.0;
-        } else if (prec_ == 0) {
+        } else if (fprec_ == 0) {
             d2h_as_double<double>(o.M, n, M);
         } else {
             d2h_as_double<float>(o.M, n, M);
         }
     }
     if (tau && o.tau && o.cols > 0) {
-        if (prec_ == 0) d2h_as_double<double>(o.tau, o.cols, tau);
+        if (fprec_ == 0) d2h_as_double<double>(o.tau, o.cols, tau);
         else d2h_as_double<float>(o.tau, o.cols, tau);
     }
 }
 
 void Tree::op_plu(int i, double* U, int* perm) const {
+    if (state_ != State::Factorized) throw Error(ERR_STATE, "op data needs a completed factorize()");
     if (i < 0 || i >= (int)ops_.size()) throw Error(ERR_ARG, "op index out of range");
     const OpRec& o = ops_[i];
     if (o.type != OP_SCALING_PLUQ) throw Error(ERR_ARG, "op is not a ScalingPLUQ");
     const int n = o.sa;
     std::vector<double> ut((size_t)n * n, 0.0);
     if (o.Ut) {
-        if (prec_ == 0) d2h_as_double<double>(o.Ut, ut.size(), ut.data());
+        if (fprec_ == 0) d2h_as_double<double>(o.Ut, ut.size(), ut.data());
         else d2h_as_double<float>(o.Ut, ut.size(), ut.data());
     } else {
         for (int k = 0; k < n; k++) ut[k + (size_t)k * n] = 1.0;
@@ -1934,7 +1979,7 @@ long long Tree::trailing_coo(int* I, int* J, double* V) const {
                     std::vector<double> blk((size_t)m * n);
                     if (E.ident) {
                         for (int k = 0; k < std::min(m, n); k++) blk[k + (size_t)k * m] = 1.0;
-                    } else if (prec_ == 0) {
+                    } else if (fprec_ == 0) {
                         d2h_as_double<double>(E.A, blk.size(), blk.data());
                     } else {
                         d2h_as_double<float>(E.A, blk.size(), blk.data());
@@ -1951,7 +1996,13 @@ long long Tree::trailing_coo(int* I, int* J, double* V) const {
             return cnt;
         }
         const std::vector<long long>& off = h_scatter_off_;
-        const std::vector<double>& val = h_scatter_val_;
+        std::vector<double> dev_val;
+        if (host_vals_stale_ && nscatter_ > 0) {  // values re-assembled on the device
+            dev_val.resize(nscatter_);
+            hip_check(hipMemcpy(dev_val.data(), d_scatter_val_, nscatter_ * sizeof(double), hipMemcpyDeviceToHost),
+                      "read values");
+        }
+        const std::vector<double>& val = host_vals_stale_ ? dev_val : h_scatter_val_;
         // map offsets back to (edge, i, j)
         std::vector<std::pair<long long, int>> starts;
         for (size_t e = 0; e < edges0_.size(); e++) starts.push_back({edge0_off_[e], (int)e});
@@ -1987,7 +2038,7 @@ void Tree::solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
 }
 
 void Tree::apply_any(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose) {
-    if (prec_ == 0)
+    if (fprec_ == 0)
         apply_impl<double>(d_x, nrhs, sx, sr, st, transpose);
     else
         apply_impl<float>(d_x, nrhs, sx, sr, st, transpose);
@@ -2018,13 +2069,12 @@ void Tree::solve_any(double* x, int nrhs, int ldx, bool on_device, hipStream_t s
     }
 }
 
-void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st) {
+void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st, bool transpose) {
     if (state_ != State::Factorized) throw Error(ERR_STATE, "solve() must follow factorize()");
     if (nrhs <= 0 || nrhs > 64) throw Error(ERR_ARG, "solve: need 1 <= nrhs <= 64");
-    if (prec_ == 0)
-        apply_impl<double>(d_x, nrhs, (size_t)nrhs, 1, st);
-    else
-        apply_impl<float>(d_x, nrhs, (size_t)nrhs, 1, st);
+    transpose = transpose && fsymm_ != 0;  // the SPD apply is its own transpose
+    if (transpose) upload_transposed();
+    apply_any(d_x, nrhs, (size_t)nrhs, 1, st, transpose);
 }
 
 // Tree::solve (src/tree.cpp:1610-1635): permute in, forward sweep over the levels, backward

@@ -157,6 +157,9 @@ class Tree {
 
     void partition(int N, const int* colptr, const int* rowval);
     void assemble(int N, const int* colptr, const int* rowval, const double* vals);
+    // New values (device array, CSC order of the assembled pattern) for the next factorize().
+    void assemble_values_device(const double* vals, long long nnz, hipStream_t st);
+    long long assembled_nnz() const { return nnz_in_; }
     void factorize();
     // M^{-1} applied in place to nrhs columns (ld >= N) of x, on device or host memory.
     void solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
@@ -164,12 +167,14 @@ class Tree {
     // symmetric, so this is solve(); a GEN (PLU) tree runs its transposed phase lists.
     void solve_transpose(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
     void solve_any(double* x, int nrhs, int ldx, bool on_device, hipStream_t st, bool transpose);
-    // M^{-1} applied in place to an interleaved device block (element (i, r) at x[i*nrhs+r]).
-    void solve_interleaved(double* d_x, int nrhs, hipStream_t st);
+    // M^{-1} (or M^{-T}) applied in place to an interleaved device block (element (i, r) at
+    // x[i*nrhs+r]).
+    void solve_interleaved(double* d_x, int nrhs, hipStream_t st, bool transpose = false);
 
     int N() const { return N_; }
     int nlevels() const { return nlevels_; }
     int precision() const { return prec_; }
+    int factor_precision() const { return fprec_; }
     int get_stop() const;
     long long nnz() const;
     const std::vector<int>& perm() const { return perm_; }
@@ -247,7 +252,8 @@ class Tree {
     bool geo_ = false;
     int dim_ = 0;
     std::vector<double> X_;
-    int prec_ = 0;       // 0 FP64, 1 FP32
+    int prec_ = 0;       // 0 FP64, 1 FP32: element type of the next factorization
+    int fprec_ = 0;      // element type of the stored factor (prec_ at the last factorize())
     bool use_want_sparsify_ = true;
 
     State state_ = State::Created;
@@ -270,9 +276,13 @@ class Tree {
     std::vector<long long> edge0_off_;  // element offset of each assembled edge block
     long long level0_elems_ = 0;
     std::vector<long long> h_scatter_off_;
+    std::vector<long long> h_scatter_src_;  // index of each scattered value in the caller's CSC arrays
     std::vector<double> h_scatter_val_;
     long long* d_scatter_off_ = nullptr;
+    long long* d_scatter_src_ = nullptr;
     double* d_scatter_val_ = nullptr;
+    long long nnz_in_ = 0;           // stored entries of the assembled CSC
+    bool host_vals_stale_ = false;   // values last set on the device (assemble_values_device)
     long long nscatter_ = 0;
     bool device_inputs_ = false;  // scatter map and permutation uploaded
     void upload_inputs();

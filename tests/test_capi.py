@@ -59,3 +59,35 @@ def test_python_mirror_errors():
     assert t.get_N() == 25
     with pytest.raises(S.SpandError):
         t.solve(np.ones(25))  # solve before factorize
+
+
+def test_symm_kind_is_fixed_by_assembly():
+    """assemble() builds lower blocks only for SPD, so the kind cannot change after it."""
+    A, X = neglapl(5, 2), linspace_nd(5, 2)
+    t = S.Tree(3)
+    t.set_use_geo(True)
+    t.set_Xcoo(X)
+    t.set_symm_kind("GEN")
+    t.partition(symmetric_graph(A))
+    t.assemble(A)
+    t.set_symm_kind("GEN")  # unchanged: accepted
+    with pytest.raises(S.SpandError) as e:
+        t.set_symm_kind("SPD")
+    assert e.value.status == 5  # SPAND_ERR_STATE
+    # a new assembly of the same tree keeps working with its kind
+    t.assemble(A)
+
+
+def test_device_values_need_an_assembly_and_the_pattern_size():
+    L = _lib.lib()
+    A, X = neglapl(5, 2), linspace_nd(5, 2)
+    t = S.Tree(3)
+    # before assemble: state error, nothing read
+    assert L.spand_assemble_values_device(t._h, None, A.nnz, None) == 5
+    t.set_use_geo(True)
+    t.set_Xcoo(X)
+    t.partition(symmetric_graph(A))
+    t.assemble(A)
+    # wrong entry count: argument error, nothing read
+    assert L.spand_assemble_values_device(t._h, None, A.nnz + 1, None) == 1
+    assert b"nnz" in L.spand_last_error(t._h)
