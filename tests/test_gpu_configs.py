@@ -197,8 +197,11 @@ def test_c5_differentiable_512_fd64():
     say(f"  {ndir} kappa directions: relative difference median {np.median(rels['kappa']):.1e}, "
         f"max {max(rels['kappa']):.1e}; 8 b directions: max {max(rels['b']):.1e}; PCG iterations "
         f"{max(max(i) for i in solver.iters)} at most")
-    # The quotient divides the solves' own error by h: PCG stops at a residual of 1e-14, and
-    # with kappa spanning e^+-4 the error in x can be 1e-6 of that times cond(A) larger, so
-    # single directions may differ by 1e-5; a wrong gradient differs by O(1).
-    assert np.median(rels["kappa"]) <= 1e-6 and max(rels["kappa"]) <= 1e-4, rels["kappa"]
+    # The quotient also carries the difference of the two solves' own errors divided by h.
+    # PCG stops at a residual of 1e-14, but with kappa spanning e^+-4 the error in x is up to
+    # cond(A) times larger, and the two solves do not share their preconditioner exactly:
+    # at tol 1e-2 a kept rank can change between kappa +- h d, so the errors need not
+    # cancel. Single directions then differ by up to 1e-4 or so; a wrong gradient differs by
+    # O(1).
+    assert np.median(rels["kappa"]) <= 1e-5 and max(rels["kappa"]) <= 1e-3, rels["kappa"]
     assert max(rels["b"]) <= 1e-5, rels["b"]
