@@ -84,6 +84,14 @@ spand_status spand_factorize(spand_tree* t);
 spand_status spand_solve(spand_tree* t, double* x, int nrhs, int ld);
 /* Same on device memory, ordered on `stream` (a hipStream_t; NULL = default stream). */
 spand_status spand_solve_device(spand_tree* t, double* d_x, int nrhs, int ld, void* stream);
+/* No counterpart in the reference: spand_solve_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The apply's workspace (segments and partial sums, 8 * nrhs * (N + 64 * pieces)
+ * bytes) lives in a static device array of 64 MiB that every GPU the library runs on holds
+ * until the process ends; a call needing more returns SPAND_ERR_ARG and does nothing. No
+ * device allocation and no event creation happen in the call. Calls that overlap on one
+ * GPU share the array and race: serialize them (one stream, or host synchronization). */
+spand_status spand_solve_device_noalloc(spand_tree* t, double* d_x, int nrhs, int ld, void* stream);
 /* x <- M^{-T} x, the adjoint apply of a differentiable solve. No counterpart in the
  * reference (its Tree has no transposed solve); for callers that differentiate through a
  * solve. The SPD (LLT) preconditioner is symmetric, so this equals spand_solve; a GEN

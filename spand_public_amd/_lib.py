@@ -45,6 +45,7 @@ SIGNATURES = {
     "spand_factorize": (C.c_int, [vp]),
     "spand_solve": (C.c_int, [vp, vp, C.c_int, C.c_int]),
     "spand_solve_device": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
+    "spand_solve_device_noalloc": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "spand_solve_transpose": (C.c_int, [vp, vp, C.c_int, C.c_int]),
     "spand_solve_transpose_device": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "spand_get_N": (C.c_int, [vp, C.POINTER(C.c_int)]),

@@ -136,6 +136,11 @@ spand_status spand_solve_device(spand_tree* t, double* d_x, int nrhs, int ld, vo
     });
 }
 
+spand_status spand_solve_device_noalloc(spand_tree* t, double* d_x, int nrhs, int ld, void* stream) {
+    CHECK_T(t);
+    return guarded(t, [&] { t->t->solve_device_noalloc(d_x, nrhs, ld, (hipStream_t)stream); });
+}
+
 spand_status spand_solve_transpose(spand_tree* t, double* x, int nrhs, int ld) {
     CHECK_T(t);
     return guarded(t, [&] {

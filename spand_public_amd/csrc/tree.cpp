@@ -14,6 +14,8 @@
 // with one host<->device synchronization per level (the ranks decide the next level's sizes).
 #include "tree.hpp"
 
+#include "scratch.hpp"
+
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -2037,11 +2039,27 @@ void Tree::solve(double* x, int nrhs, int ldx, bool on_device, hipStream_t st) {
     solve_any(x, nrhs, ldx, on_device, st, false);
 }
 
-void Tree::apply_any(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose) {
+void Tree::apply_any(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose, double* ws_ext) {
     if (fprec_ == 0)
-        apply_impl<double>(d_x, nrhs, sx, sr, st, transpose);
+        apply_impl<double>(d_x, nrhs, sx, sr, st, transpose, ws_ext);
     else
-        apply_impl<float>(d_x, nrhs, sx, sr, st, transpose);
+        apply_impl<float>(d_x, nrhs, sx, sr, st, transpose, ws_ext);
+}
+
+size_t Tree::apply_workspace_bytes(int nrhs) const {
+    const size_t seg = (size_t)ws_len_ * nrhs, part = (size_t)max_pieces_ * SOLVE_CHUNK * nrhs;
+    return (seg + part) * sizeof(double);
+}
+
+void Tree::solve_device_noalloc(double* d_x, int nrhs, int ldx, hipStream_t st) {
+    if (state_ != State::Factorized) throw Error(ERR_STATE, "solve() must follow factorize()");
+    if (!d_x || nrhs <= 0 || nrhs > 64 || ldx < N_) throw Error(ERR_ARG, "solve: need 1 <= nrhs <= 64 and ld >= N");
+    const size_t need = apply_workspace_bytes(nrhs);
+    if (need > SCRATCH_BYTES)
+        throw Error(ERR_ARG, "solve_device_noalloc: the apply needs " + std::to_string(need >> 20) +
+                                 " MiB of workspace, more than the static " + std::to_string(SCRATCH_BYTES >> 20) +
+                                 " MiB (use spand_solve_device, or fewer right-hand sides per call)");
+    apply_any(d_x, nrhs, 1, (size_t)ldx, st, false, scratch_device_ptr());
 }
 
 void Tree::solve_any(double* x, int nrhs, int ldx, bool on_device, hipStream_t st, bool transpose) {
@@ -2081,21 +2099,25 @@ void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st, bool transpo
 // sweep in reverse, permute out. Split ops need no work: a sibling's segment aliases the
 // tail of its original cluster's segment.
 template <typename T>
-void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose) {
+void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose, double* ws_ext) {
     // ws: the segments (FP64 whatever the factor's element type T), then the partial sums
     // of the largest mv phase
-    const size_t seg = (size_t)ws_len_ * nrhs, part = (size_t)max_pieces_ * SOLVE_CHUNK * nrhs;
-    const size_t need = (seg + part) * sizeof(double);
-    if (need > ws_cap_) {
+    const size_t seg = (size_t)ws_len_ * nrhs;
+    const size_t need = apply_workspace_bytes(nrhs);
+    if (!ws_ext && need > ws_cap_) {
         (void)hipFree(d_ws_);
         d_ws_ = nullptr;
         hip_check(hipMalloc(&d_ws_, need), "hipMalloc(solve workspace)");
         ws_cap_ = need;
     }
-    double* ws = reinterpret_cast<double*>(d_ws_);
+    double* ws = ws_ext ? ws_ext : reinterpret_cast<double*>(d_ws_);
     double* partial = ws + seg;
     const double vsz = sizeof(double);
     auto run = [&](int fam, const SweepWork& w, const char* what, auto&& launch) {
+        if (ws_ext) {  // the allocation-free apply creates no events
+            hip_check(launch(), what);
+            return;
+        }
         hip_check(timed(prof_, fam, st, w.mat_bytes + w.vec_elems * nrhs * vsz, w.flops * nrhs, launch), what);
     };
     auto run_phase = [&](const MvPhase& ph) {

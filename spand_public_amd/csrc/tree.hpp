@@ -167,6 +167,13 @@ class Tree {
     // symmetric, so this is solve(); a GEN (PLU) tree runs its transposed phase lists.
     void solve_transpose(double* x, int nrhs, int ldx, bool on_device, hipStream_t st);
     void solve_any(double* x, int nrhs, int ldx, bool on_device, hipStream_t st, bool transpose);
+    // M^{-1} applied in place to nrhs device columns (ld >= N) with the segments and partial
+    // sums in the library's static device workspace (scratch.hpp): no device allocation, no
+    // event; the launches are ordered on st. Throws ERR_ARG when the apply needs more than
+    // SCRATCH_BYTES.
+    void solve_device_noalloc(double* d_x, int nrhs, int ldx, hipStream_t st);
+    // Bytes of workspace an apply on nrhs columns needs (segments + partial sums).
+    size_t apply_workspace_bytes(int nrhs) const;
     // M^{-1} (or M^{-T}) applied in place to an interleaved device block (element (i, r) at
     // x[i*nrhs+r]).
     void solve_interleaved(double* d_x, int nrhs, hipStream_t st, bool transpose = false);
@@ -204,9 +211,13 @@ class Tree {
   private:
     enum class State { Created, Partitioned, Assembled, Factorized };
     template <typename T> void factorize_impl();
+    // ws_ext: a caller-provided workspace of apply_workspace_bytes(nrhs) (then nothing is
+    // allocated and no launch is timed), or nullptr for the tree's own.
     template <typename T>
-    void apply_impl(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose = false);
-    void apply_any(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose);
+    void apply_impl(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose = false,
+                    double* ws_ext = nullptr);
+    void apply_any(double* d_x, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose,
+                   double* ws_ext = nullptr);
     int size(int c) const { return cls_[c].size; }
     int find_out_edge(int from, int to) const;
     int new_edge(int n1, int n2, void* A, bool original);
