@@ -42,6 +42,14 @@ constexpr int RRQR_LDS_MAX = 98304;
 template <typename T>
 hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
                        int lds_bytes, hipStream_t st);
+// Register-resident RRQR (k_rrqr_reg): one workgroup per cluster, the cluster's Asn held in
+// registers for the whole factorization. rrqr_reg_variant gives the kernel form that holds
+// a rows x cols Asn of elements of esz bytes (-1: none does); a batch is launched with a
+// variant that holds each of its tasks.
+int rrqr_reg_variant(size_t esz, int rows, int cols);
+template <typename T>
+hipError_t launch_rrqr_reg(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
+                           int variant, hipStream_t st);
 // Stepped RRQR of the large clusters of one wave: init, then steps 0..nsteps-1, then finish.
 // lds_bytes is the dynamic LDS of the pivot-column / reflector buffers (2 x max_rows
 // elements for a step, max_rows for finish), or 0 when the tasks carry global buffers

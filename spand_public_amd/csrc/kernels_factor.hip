@@ -971,6 +971,386 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
 }
 
 // ---------------------------------------------------------------------------------------
+// Register-resident RRQR of a mid-size cluster: one workgroup per cluster, thread t holds
+// columns t + NTR k (k < K) of Asn, rows 0 .. RR - 1 in registers, for the whole
+// factorization. Columns never move: each keeps its LAPACK position (column swaps and the
+// first-maximum tie-break of dgeqp3 are reproduced on positions). A pivot step:
+//   1. every free column offers its partial norm; the maximum (first position on ties)
+//      is found by wavefront shuffles and one pass over the per-wavefront winners;
+//   2. the thread holding the pivot column computes dlarfg (reference dlarfg, as in
+//      k_rrqr), stores the reflector in LDS and the scalars, and keeps beta / v in its
+//      registers (they are Q's column and R's diagonal);
+//   3. every free column applies the reflector (w = v^T a, a += -tau w v) and downdates
+//      its norm (LAPACK dlaqp2 rule, with the same recomputation test).
+// Two barriers per step and no global memory traffic between the gather and the final
+// writes of Q, tau, R and the rank. Every sum follows the 16-chain order of k_rrqr (chain
+// of row r = (r - start) mod 16, fma in increasing r, halving tree): rows are indexed
+// statically, so the chains are accumulated by r mod 16 and rotated by the start row.
+
+// segment of Asn column j: the last k with col0[k] <= j
+__device__ inline int seg_find(const int* col0, int nseg, int j) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (col0[mid] <= j) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// (value desc, position asc) maximum over the 64 lanes of a wavefront, carrying a tag
+__device__ inline void wave_best_tag(double& v, int& p, int& tag) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const int p2 = __shfl_xor(p, off);
+        const int t2 = __shfl_xor(tag, off);
+        if (v2 > v || (v2 == v && p2 < p)) {
+            v = v2;
+            p = p2;
+            tag = t2;
+        }
+    }
+}
+
+// tree16 of the chains rotated by S (chain k = e[(k + S) mod 16]), without moving registers
+template <int S, typename A>
+__device__ inline A tree16_rot_s(const A* e) {
+#pragma clang fp contract(off)
+    auto c = [&](int k) { return e[(k + S) & (NCH - 1)]; };
+    const A x0 = ((c(0) + c(8)) + (c(4) + c(12))) + ((c(2) + c(10)) + (c(6) + c(14)));
+    const A x1 = ((c(1) + c(9)) + (c(5) + c(13))) + ((c(3) + c(11)) + (c(7) + c(15)));
+    return x0 + x1;
+}
+template <typename A>
+__device__ inline A tree16_rot(const A* e, int s) {  // s uniform
+    switch (s & (NCH - 1)) {
+        case 0: return tree16_rot_s<0>(e);
+        case 1: return tree16_rot_s<1>(e);
+        case 2: return tree16_rot_s<2>(e);
+        case 3: return tree16_rot_s<3>(e);
+        case 4: return tree16_rot_s<4>(e);
+        case 5: return tree16_rot_s<5>(e);
+        case 6: return tree16_rot_s<6>(e);
+        case 7: return tree16_rot_s<7>(e);
+        case 8: return tree16_rot_s<8>(e);
+        case 9: return tree16_rot_s<9>(e);
+        case 10: return tree16_rot_s<10>(e);
+        case 11: return tree16_rot_s<11>(e);
+        case 12: return tree16_rot_s<12>(e);
+        case 13: return tree16_rot_s<13>(e);
+        case 14: return tree16_rot_s<14>(e);
+        default: return tree16_rot_s<15>(e);
+    }
+}
+
+// Sum of g(r, acc) over rows r0 <= r < RR of a column held in registers, chain (r - r0) mod
+// 16: rows are visited by their static index (chains by r mod 16, rotated by r0 at the end).
+// Padding rows beyond the cluster's rows hold zeros and add nothing; 16-row blocks wholly
+// above r0 or below the last row are skipped (uniform branches).
+template <int RR, typename A, typename G>
+__device__ inline A chain_sum(int r0, int rows, G g) {
+#pragma clang fp contract(off)
+    A e[NCH];
+#pragma unroll
+    for (int m = 0; m < NCH; m++) e[m] = A(0);
+    const int b0 = r0 / NCH, m0 = r0 % NCH;
+#pragma unroll
+    for (int b = 0; b < RR / NCH; b++) {
+        if (b < b0 || NCH * b >= rows) continue;
+        if (b == b0) {
+#pragma unroll
+            for (int m = 0; m < NCH; m++)
+                if (m >= m0) e[m] = g(NCH * b + m, e[m]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < NCH; m++) e[m] = g(NCH * b + m, e[m]);
+        }
+        asm volatile("" ::: "memory");  // LDS operands of g are read block by block
+    }
+    return tree16_rot(e, m0);
+}
+
+// element r (uniform, < RR) of a register column
+template <int RR, typename T>
+__device__ inline T reg_at(const T* a, int r) {
+    T x = T(0);
+    const int b0 = r / NCH, m0 = r % NCH;
+#pragma unroll
+    for (int b = 0; b < RR / NCH; b++) {
+        if (b != b0) continue;
+#pragma unroll
+        for (int m = 0; m < NCH; m++)
+            if (m == m0) x = a[NCH * b + m];
+    }
+    return x;
+}
+
+template <typename T, int RR, int K, int NTR>
+__global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const RrqrSeg* segs, int* dsize, double tol) {
+#pragma clang fp contract(off)
+    static_assert(RR % NCH == 0, "rows in blocks of 16");
+    constexpr int NWV = NTR / 64;
+    __shared__ int s_col0[RRQR_MAXSEG + 1];
+    __shared__ double s_bv[NWV];
+    __shared__ int s_bp[NWV], s_bc[NWV];
+    __shared__ T s_v[RR];
+    __shared__ double s_tau, s_rii;
+    const RrqrTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int rows = dsize[t.cluster];
+    const RrqrSeg* sg = segs + t.seg0;
+    if (tid == 0) {
+        int c = 0;
+        for (int k = 0; k < t.nseg; k++) {
+            s_col0[k] = c;
+            c += dsize[sg[k].nbr];
+        }
+        s_col0[t.nseg] = c;
+    }
+    for (int r = tid; r < RR; r += NTR) s_v[r] = T(0);  // padding rows of the reflector stay zero
+    __syncthreads();
+    const int cols = s_col0[t.nseg];
+    if (rows == 0) return;
+    const int mn = min(rows, cols);
+    T a[K][RR];
+    double vn1[K], vn2[K];
+    int pos[K];  // LAPACK position of the column; -1: no column
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = tid + NTR * k;
+        const bool jin = j < cols;
+        const T* E = reinterpret_cast<const T*>(sg[0].E);
+        int sr = 0;
+        if (jin) {
+            const int sgi = seg_find(s_col0, t.nseg, j);
+            const int jj = j - s_col0[sgi], w = s_col0[sgi + 1] - s_col0[sgi];
+            E = reinterpret_cast<const T*>(sg[sgi].E);
+            if (sg[sgi].out) {  // out-edge block (n x s): row jj of it
+                E += jj;
+                sr = w;
+            } else {  // in-edge block (s x n): column jj of it
+                E += (size_t)jj * rows;
+                sr = 1;
+            }
+        }
+        // 16 rows at a time (bounded loads in flight, 32-bit offsets)
+#pragma unroll
+        for (int b = 0; b < RR / NCH; b++) {
+#pragma unroll
+            for (int m = 0; m < NCH; m++) {
+                const int r = NCH * b + m;
+                a[k][r] = (jin && r < rows) ? E[r * sr] : T(0);
+            }
+            asm volatile("" ::: "memory");
+        }
+        vn1[k] = jin ? sqrt(chain_sum<RR, double>(0, rows, [&](int r, double c) {
+                     const double d = (double)a[k][r];
+                     return fma(d, d, c);
+                 }))
+                     : 0.0;
+        vn2[k] = vn1[k];
+        pos[k] = jin ? j : -1;
+    }
+    const double tol3z = sqrt(DBL_EPSILON * 0.5);
+    const int fixed = rank_fixed(tol, mn);
+    const int nsteps = fixed == 0 ? 0 : mn;
+    int rank = fixed >= 0 ? fixed : mn;
+    double r00 = 0.0;
+    for (int i = 0; i < nsteps; i++) {
+        const int b0 = i / NCH, m0 = i % NCH;
+        // 1. pivot: first maximum of |vn1| over the free positions i .. cols - 1
+        double bv = -1.0;
+        int bp = INT32_MAX, bc = -1;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (pos[k] >= i) {
+                const double v = fabs(vn1[k]);
+                if (v > bv || (v == bv && pos[k] < bp)) {
+                    bv = v;
+                    bp = pos[k];
+                    bc = tid + NTR * k;
+                }
+            }
+        }
+        wave_best_tag(bv, bp, bc);
+        if (lane == 0) {
+            s_bv[wv] = bv;
+            s_bp[wv] = bp;
+            s_bc[wv] = bc;
+        }
+        __syncthreads();
+        double v0 = s_bv[0];
+        int p0 = s_bp[0], c0 = s_bc[0];
+#pragma unroll
+        for (int w = 1; w < NWV; w++) {
+            const double v1 = s_bv[w];
+            const int p1 = s_bp[w];
+            if (v1 > v0 || (v1 == v0 && p1 < p0)) {
+                v0 = v1;
+                p0 = p1;
+                c0 = s_bc[w];
+            }
+        }
+        // no candidate (every remaining norm NaN, after a failed pivot upstream): the column
+        // at position i stays the pivot, so that no position leaves [i, cols)
+        const bool valid = p0 >= i && p0 < cols;
+        const int p = valid ? p0 : i;
+        bool own[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) own[k] = pos[k] >= 0 && (valid ? (tid + NTR * k == c0) : (pos[k] == i));
+        // the column at position i moves to the pivot's position (dgeqp3's swap)
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (!own[k] && pos[k] == i) pos[k] = p;
+        // 2. the pivot column's thread: dlarfg of rows i .. rows - 1 (reference dlarfg)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!own[k]) continue;
+            const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+            const double rs = 1.0 / safmin;
+            double alpha = (double)reg_at<RR>(a[k], i);
+            const int n1 = rows - i - 1;
+            double tau = 0.0, scal = 1.0, beta = alpha;
+            int knt = 0;
+            if (n1 >= 1) {
+                const double xs0 = chain_sum<RR, double>(i + 1, rows, [&](int r, double c) {
+                    const double d = (double)a[k][r];
+                    return fma(d, d, c);
+                });
+                const double xnorm = sqrt(xs0);
+                if (xnorm != 0.0) {
+                    beta = -copysign(lapy2_d(alpha, xnorm), alpha);
+                    if (fabs(beta) < safmin) {
+                        do {
+                            knt++;
+                            beta *= rs;
+                            alpha *= rs;
+                        } while (fabs(beta) < safmin && knt < 20);
+                        // norm of the rescaled vector
+                        const double xs = chain_sum<RR, double>(i + 1, rows, [&](int r, double c) {
+                            T xv = a[k][r];
+                            for (int q = 0; q < knt; q++) xv = (T)((double)xv * rs);
+                            const double d = (double)xv;
+                            return fma(d, d, c);
+                        });
+                        beta = -copysign(lapy2_d(alpha, sqrt(xs)), alpha);
+                        tau = (beta - alpha) / beta;
+                        scal = 1.0 / (alpha - beta);
+                        for (int q = 0; q < knt; q++) beta *= safmin;
+                    } else {
+                        tau = (beta - alpha) / beta;
+                        scal = 1.0 / (alpha - beta);
+                    }
+                }
+            }
+            // the reflector: v_i = 1, v_r = x_r * scal below; the column keeps beta and v
+            // (with tau = 0 it is left as it is, alpha on the diagonal)
+#pragma unroll
+            for (int b = 0; b < RR / NCH; b++) {
+                if (b < b0 || NCH * b >= rows) continue;
+#pragma unroll
+                for (int m = 0; m < NCH; m++) {
+                    const int r = NCH * b + m;
+                    if (b > b0 || m > m0) {
+                        if (tau != 0.0) a[k][r] = larfg_scale<T>(a[k][r], knt, scal);
+                        s_v[r] = a[k][r];
+                    } else if (m == m0) {
+                        if (tau != 0.0) a[k][r] = (T)beta;
+                        s_v[r] = T(1);
+                        s_rii = fabs((double)a[k][r]);
+                    }
+                }
+            }
+            s_tau = tau;
+            pos[k] = i;
+            reinterpret_cast<T*>(t.tau)[i] = (T)tau;
+        }
+        __syncthreads();
+        const double taud = s_tau, rii = s_rii;
+        if (i == 0) {
+            r00 = rii;
+        } else if (fixed < 0 && !(rii / r00 >= tol)) {  // |R_ii| / |R_00| < tol: rank i
+            rank = i;
+            break;
+        }
+        const T tau = (T)taud;
+        // 3. apply H(i) to the free columns (positions > i after the swap), downdate the norms
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!(pos[k] > i)) continue;
+            if (tau != T(0)) {
+                const T w = chain_sum<RR, T>(i, rows, [&](int r, T c) { return fma(a[k][r], s_v[r], c); });
+                if (w != T(0)) {
+                    const T tm = -tau * w;
+                    asm volatile("" ::: "memory");  // re-read v from LDS instead of holding it in registers
+#pragma unroll
+                    for (int b = 0; b < RR / NCH; b++) {
+                        if (b < b0 || NCH * b >= rows) continue;
+#pragma unroll
+                        for (int m = 0; m < NCH; m++) {
+                            const int r = NCH * b + m;
+                            if (b > b0 || m > m0) a[k][r] = fma(s_v[r], tm, a[k][r]);
+                            else if (m == m0) a[k][r] = a[k][r] + tm;
+                        }
+                    }
+                }
+            }
+            if (vn1[k] != 0.0) {
+                const double q = fabs((double)reg_at<RR>(a[k], i)) / vn1[k];
+                double temp = 1.0 - q * q;
+                temp = fmax(temp, 0.0);
+                const double rr = vn1[k] / vn2[k];
+                const double temp2 = temp * rr * rr;
+                if (temp2 <= tol3z) {
+                    if (i < rows - 1) {
+                        vn1[k] = sqrt(chain_sum<RR, double>(i + 1, rows, [&](int r, double c) {
+                            const double d = (double)a[k][r];
+                            return fma(d, d, c);
+                        }));
+                        vn2[k] = vn1[k];
+                    } else {
+                        vn1[k] = 0.0;
+                        vn2[k] = 0.0;
+                    }
+                } else {
+                    vn1[k] = vn1[k] * sqrt(temp);
+                }
+            }
+        }
+    }
+    if (rank >= rows) return;
+    // Q: the reflectors of positions 0 .. rank - 1, column by column (R above, beta, v below)
+    T* Q = reinterpret_cast<T*>(t.Q);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (pos[k] < 0 || pos[k] >= rank) continue;
+        T* q = Q + (size_t)pos[k] * rows;
+#pragma unroll
+        for (int r = 0; r < RR; r++)
+            if (r < rows) q[r] = a[k][r];
+    }
+    // AsnP[:, j] = triu(R)[:rank, pos(j)], scattered into the shrunk edge blocks (every
+    // gather read of these blocks happened before the first barrier)
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (pos[k] < 0) continue;
+        const int j = tid + NTR * k;
+        const int sgi = seg_find(s_col0, t.nseg, j);
+        const int jj = j - s_col0[sgi], w = s_col0[sgi + 1] - s_col0[sgi];
+        T* E = reinterpret_cast<T*>(sg[sgi].E);
+        const bool out = sg[sgi].out;
+#pragma unroll
+        for (int r = 0; r < RR; r++) {
+            if (r < rank) {
+                const T v = (r <= pos[k]) ? a[k][r] : T(0);
+                if (out) E[jj + (size_t)r * w] = v;
+                else E[r + (size_t)jj * rank] = v;
+            }
+        }
+    }
+    if (tid == 0) dsize[t.cluster] = rank;
+}
+
+// ---------------------------------------------------------------------------------------
 // Stepped RRQR for clusters too large for one workgroup. Launch sequence per wave:
 // k_rrqr_init (gather + norms), k_rrqr_step for every pivot step, k_rrqr_finish (rank,
 // Q, scatter). The columns are split into 64-wide slabs, one 1024-thread workgroup each:
@@ -993,15 +1373,6 @@ __device__ inline size_t slab_idx(int rows, int r, int j) {
 template <typename T>
 __device__ inline T* step_buf(const RrqrBig& B, unsigned char* smem, int slab) {
     return B.vbuf ? reinterpret_cast<T*>(B.vbuf) + (size_t)slab * 2 * B.t.max_rows : reinterpret_cast<T*>(smem);
-}
-
-__device__ inline int seg_find(const int* col0, int nseg, int j) {
-    int lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (col0[mid] <= j) lo = mid; else hi = mid - 1;
-    }
-    return lo;
 }
 
 // Host-visible signals of the per-step launches (fine-grained pinned host memory): plain
@@ -1755,6 +2126,64 @@ hipError_t launch_rrqr(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_seg
     return hipGetLastError();
 }
 
+// Register path variants: (rows held per thread RR, columns per thread K, threads NTR). The
+// table keeps to the forms that compile without register spills on gfx950 (1024 threads
+// leave 128 VGPRs per thread, 512 threads 256): FP32 rows <= 32 up to 1024 columns, rows
+// <= 64 up to 1024 (2 columns per thread above 512), rows <= 96 up to 512; FP64 rows <= 32
+// up to 1024 (2 per thread above 512), rows <= 48 up to 512. Unused rows cost registers
+// only: the sweeps skip 16-row blocks beyond a cluster's rows.
+int rrqr_reg_variant(size_t esz, int rows, int cols) {
+    if (rows < 0 || cols < 0) return -1;
+    const int cc = cols <= 256 ? 0 : cols <= 512 ? 1 : cols <= 1024 ? 2 : 3;
+    if (cc == 3) return -1;
+    int rc;
+    if (esz == sizeof(double)) {
+        rc = rows <= 32 ? 0 : rows <= 48 ? 1 : -1;
+        if (rc == 1 && cc == 2) return -1;
+    } else {
+        rc = rows <= 32 ? 0 : rows <= 64 ? 1 : rows <= 96 ? 2 : -1;
+        if (rc == 2 && cc == 2) return -1;
+    }
+    return rc < 0 ? -1 : 3 * rc + cc;
+}
+
+template <typename T, int RR, int K, int NTR>
+static hipError_t launch_reg_variant(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size,
+                                     double tol, hipStream_t st) {
+    k_rrqr_reg<T, RR, K, NTR><<<ntasks, NTR, 0, st>>>(d_tasks, d_segs, d_size, tol);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rrqr_reg(const RrqrTask* d_tasks, int ntasks, const RrqrSeg* d_segs, int* d_size, double tol,
+                           int variant, hipStream_t st) {
+    if (ntasks == 0) return hipSuccess;
+#define SPAND_REG(RR, K, NTR) launch_reg_variant<T, RR, K, NTR>(d_tasks, ntasks, d_segs, d_size, tol, st)
+    if constexpr (sizeof(T) == sizeof(double)) {
+        switch (variant) {
+            case 0: return SPAND_REG(32, 1, 256);
+            case 1: return SPAND_REG(32, 1, 512);
+            case 2: return SPAND_REG(32, 2, 512);
+            case 3: return SPAND_REG(48, 1, 256);
+            case 4: return SPAND_REG(48, 1, 512);
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (variant) {
+            case 0: return SPAND_REG(32, 1, 256);
+            case 1: return SPAND_REG(32, 1, 512);
+            case 2: return SPAND_REG(32, 1, 1024);
+            case 3: return SPAND_REG(64, 1, 256);
+            case 4: return SPAND_REG(64, 1, 512);
+            case 5: return SPAND_REG(64, 2, 512);
+            case 6: return SPAND_REG(96, 1, 256);
+            case 7: return SPAND_REG(96, 1, 512);
+            default: return hipErrorInvalidValue;
+        }
+    }
+#undef SPAND_REG
+}
+
 template <typename T>
 hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
                             const int* d_size, double tol, hipStream_t st) {
@@ -1865,6 +2294,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_getrf<T>(const GetrfTask*, int, int, int, int*, hipStream_t);      \
     template hipError_t launch_schur<T>(const SchurTile*, int, const SchurContrib*, hipStream_t); \
     template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, int, hipStream_t); \
+    template hipError_t launch_rrqr_reg<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, int, hipStream_t); \
     template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);                        \
     template hipError_t launch_rrqr_init<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, const int*,      \
                                             double, hipStream_t);                                                  \

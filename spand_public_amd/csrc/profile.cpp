@@ -6,7 +6,7 @@ namespace spand {
 
 static const char* const kFamilyNames[KF_COUNT] = {
     "fill", "scatter", "copy", "potrf", "potrf_panel", "potrf_trail", "trsm", "schur", "rrqr", "rrqr_init", "rrqr_step", "rrqr_wave",
-    "rrqr_finish", "larft", "perm", "trmv", "gemv", "qapply", "merge", "spmv", "krylov_vec", "getrf",
+    "rrqr_finish", "larft", "perm", "trmv", "gemv", "qapply", "merge", "spmv", "krylov_vec", "getrf", "rrqr_reg",
 };
 
 const char* kernel_family_name(int f) { return (f >= 0 && f < KF_COUNT) ? kFamilyNames[f] : nullptr; }

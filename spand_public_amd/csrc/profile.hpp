@@ -32,6 +32,7 @@ enum KernelFamily {
     KF_SPMV,
     KF_KRYLOV_VEC,
     KF_GETRF,
+    KF_RRQR_REG,
     KF_COUNT
 };
 

@@ -112,6 +112,12 @@ Tree::~Tree() {
     (void)hipFree(d_ws_);
     (void)hipHostFree(h_stage_);
     (void)hipHostFree(h_sig_);
+    if (aux_st_) {
+        (void)hipStreamSynchronize(aux_st_);
+        (void)hipStreamDestroy(aux_st_);
+        (void)hipEventDestroy(ev_fork_);
+        (void)hipEventDestroy(ev_join_);
+    }
 }
 
 // SymmKind (include/spaND.h): SPD (0) with ScalingKind::LLT, GEN (2) with ScalingKind::PLU.
@@ -461,6 +467,17 @@ void Tree::assemble_values_device(const double* vals, long long nnz, hipStream_t
     cls_ = cls0_;
     edges_ = edges0_;
     state_ = State::Assembled;
+}
+
+// Second stream of the factorization (non-blocking, default priority) and its fork / join
+// events, created on first use.
+hipStream_t Tree::aux_stream() {
+    if (!aux_st_) {
+        hip_check(hipStreamCreateWithFlags(&aux_st_, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming), "hipEventCreate");
+    }
+    return aux_st_;
 }
 
 void Tree::factorize() {
@@ -945,6 +962,8 @@ void Tree::factorize_impl() {
     const long long lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
     const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 10000);
+    // clusters whose Asn fits a register-resident kernel form take it (SPAND_RRQR_REG=0: never)
+    const bool reg_path = env_or("SPAND_RRQR_REG", 1) != 0;
     const bool gen = symm_ == 2;
     fsymm_ = symm_;
 
@@ -1384,7 +1403,9 @@ void Tree::factorize_impl() {
         std::vector<std::vector<RrqrBig>> wbig;       // stepped path, per wave
         std::vector<std::vector<RrqrSlab>> wslab;
         std::vector<int> wsteps, wmaxrows;
-        std::vector<std::pair<int, int>> where;      // (0 small | 1 big, index) by plist position
+        std::vector<std::vector<RrqrTask>> wreg;      // register path, per wave (sorted by variant)
+        std::vector<std::vector<int>> wregvar;        // the variant of each wreg task
+        std::vector<std::pair<int, int>> where;      // (0 small | 1 big | 2 register, index) by plist position
         std::vector<RrqrSeg> segs;
         if (do_sparsify_level) {
             std::vector<int> inP(cls_.size(), -1);
@@ -1414,7 +1435,11 @@ void Tree::factorize_impl() {
             wslab.assign(nwaves, {});
             wsteps.assign(nwaves, 0);
             wmaxrows.assign(nwaves, 0);
+            wreg.assign(nwaves, {});
+            wregvar.assign(nwaves, {});
             where.assign(plist.size(), {0, 0});
+            std::vector<std::vector<std::pair<int, int>>> regp(nwaves);  // (variant, plist position)
+            std::vector<RrqrTask> ptask(plist.size());
             widx.assign(plist.size(), 0);
             for (size_t p = 0; p < plist.size(); p++) {
                 const int c = plist[p];
@@ -1441,7 +1466,12 @@ void Tree::factorize_impl() {
                 t.tau = arena_.alloc((size_t)std::max(1, mn) * sizeof(T));
                 lst.flops_rrqr += mn > 0 ? 2.0 * s.size * cols * mn - (2.0 / 3.0) * (double)mn * mn * mn : 0.0;
                 const int w = wave[p];
-                if (s.size >= big_min_rows && (long long)s.size * cols > big_min_elems) {
+                const int reg_var = reg_path ? rrqr_reg_variant(sizeof(T), s.size, cols) : -1;
+                if (reg_var >= 0) {
+                    // register-resident, one workgroup per cluster
+                    ptask[p] = t;
+                    regp[w].push_back({reg_var, (int)p});
+                } else if (s.size >= big_min_rows && (long long)s.size * cols > big_min_elems) {
                     // stepped multi-workgroup path
                     RrqrBig b{};
                     b.nslab = std::max(1, (cols + 63) / 64);
@@ -1469,6 +1499,15 @@ void Tree::factorize_impl() {
                     }
                     where[p] = {0, (int)wtasks[w].size()};
                     wtasks[w].push_back(t);
+                }
+            }
+            for (int w = 0; w < nwaves; w++) {
+                std::stable_sort(regp[w].begin(), regp[w].end(),
+                                 [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
+                for (auto& vp : regp[w]) {
+                    where[vp.second] = {2, (int)wreg[w].size()};
+                    wreg[w].push_back(ptask[vp.second]);
+                    wregvar[w].push_back(vp.first);
                 }
             }
         }
@@ -1531,8 +1570,9 @@ void Tree::factorize_impl() {
         size_t o_ii = pk.add(inv_init);
         lds_first(getrfS);
         size_t o_gs = pk.add(getrfS);
-        std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves);
+        std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves), o_r(nwaves);
         for (int w = 0; w < nwaves; w++) {
+            o_r[w] = pk.add(wreg[w]);
             o_w[w] = pk.add(wtasks[w]);
             o_b[w] = pk.add(wbig[w]);
             o_s[w] = pk.add(wslab[w]);
@@ -1554,6 +1594,54 @@ void Tree::factorize_impl() {
         std::vector<int> slot_init(nwaves, -1), slot_finish(nwaves, -1);
         std::vector<int> slot_step(nwaves, -1);
         for (int w = 0; w < nwaves; w++) {
+            // the register-path clusters of a wave run on the side stream, beside the
+            // one-workgroup and stepped launches on the main stream; both join before the
+            // next wave (whose clusters read the edges this wave rewrites)
+            const bool side = !wreg[w].empty() && (!wtasks[w].empty() || !wbig[w].empty());
+            const hipStream_t rst = side ? aux_stream() : st;
+            if (side) {
+                hip_check(hipEventRecord(ev_fork_, st), "fork");
+                hip_check(hipStreamWaitEvent(rst, ev_fork_, 0), "fork");
+            }
+            if (!wreg[w].empty()) {
+                // one launch per variant run (tasks are sorted by variant); a single launch
+                // when the variant of the largest rows and columns holds every task
+                int mr = 0, mc = 0;
+                for (auto& t : wreg[w]) {
+                    mr = std::max(mr, t.max_rows);
+                    mc = std::max(mc, t.max_cols);
+                }
+                const int cover = rrqr_reg_variant(sizeof(T), mr, mc);
+                size_t a0 = 0;
+                while (a0 < wreg[w].size()) {
+                    size_t a1 = cover >= 0 ? wreg[w].size() : a0 + 1;
+                    if (cover < 0)
+                        while (a1 < wreg[w].size() && wregvar[w][a1] == wregvar[w][a0]) a1++;
+                    const int var = cover >= 0 ? cover : wregvar[w][a0];
+                    Work wk;
+                    for (size_t x = a0; x < a1; x++) {
+                        const RrqrTask& t = wreg[w][x];
+                        wk.bytes += (2.0 * t.max_rows * t.max_cols + (double)t.max_rows * std::min(t.max_rows, t.max_cols)) * esz;
+                        wk.flops += geqp3_flops(t.max_rows, t.max_cols);
+                    }
+                    hip_check(timed(prof_, KF_RRQR_REG, rst, wk.bytes, wk.flops,
+                                    [&] {
+                                        return launch_rrqr_reg<T>(Packer::at<RrqrTask>(dt, o_r[w]) + a0, (int)(a1 - a0),
+                                                                  Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, var, rst);
+                                    }),
+                              "rrqr reg");
+                    a0 = a1;
+                }
+            }
+            // join at the end of the wave (below), also on early exits of the loop body
+            struct Join {
+                bool on;
+                hipStream_t main, side;
+                hipEvent_t ev;
+                ~Join() {
+                    if (on && hipEventRecord(ev, side) == hipSuccess) (void)hipStreamWaitEvent(main, ev, 0);
+                }
+            } join{side, st, rst, ev_join_};
             if (!wtasks[w].empty()) {
                 Work wk;
                 for (auto& t : wtasks[w]) {
@@ -1654,8 +1742,9 @@ void Tree::factorize_impl() {
             lst.rank_before += s0;
             lst.nsparsified++;
             if (r < s0) {
-                const RrqrTask* t = where[p].first ? &wbig[wave[p]][where[p].second].t
-                                                   : &wtasks[wave[p]][where[p].second];
+                const RrqrTask* t = where[p].first == 1   ? &wbig[wave[p]][where[p].second].t
+                                    : where[p].first == 2 ? &wreg[wave[p]][where[p].second]
+                                                          : &wtasks[wave[p]][where[p].second];
                 ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
                 factor_bytes_ += ((size_t)s0 * r + (size_t)r * (r + 1) / 2) * sizeof(T);
                 if (r > 0) {

@@ -310,6 +310,10 @@ class Tree {
     long long ws_len_ = 0;   // segments in the workspace (per right-hand side)
     long long max_pieces_ = 0;  // largest mv phase, in pieces (sizes the partial-sum buffer)
 
+    hipStream_t aux_stream();
+    hipStream_t aux_st_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+
     KernelProfiler prof_;
     std::vector<LevelSolve> solve_plan_;
     std::vector<OpRec> ops_;

@@ -39,10 +39,13 @@ def gpu_tree(A, X, nl, tol, prec=S.FP64, skip=0):
 
 @pytest.fixture
 def stepped_rrqr(request, monkeypatch):
-    """Which RRQR path the clusters take: the library's own size split ("default"), every
-    cluster on the multi-workgroup path run as one persistent launch per wave ("wave"),
-    or as one launch per pivot step ("steps"; "steps8" / "steps4": 8 or 4 wavefronts per
-    64-column slab)."""
+    """Which RRQR path the clusters take: the library's own size split ("default": register
+    path, one workgroup per cluster, for the clusters it holds), the split without the
+    register path ("noreg": LDS one-workgroup path and stepped path), every cluster on the
+    multi-workgroup path run as one persistent launch per wave ("wave"), or as one launch per
+    pivot step ("steps"; "steps8" / "steps4": 8 or 4 wavefronts per 64-column slab)."""
+    if request.param != "default":
+        monkeypatch.setenv("SPAND_RRQR_REG", "0")
     if request.param in ("wave", "steps", "steps8", "steps4"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
@@ -52,7 +55,7 @@ def stepped_rrqr(request, monkeypatch):
     return request.param
 
 
-@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps", "steps8", "steps4"], indirect=True)
+@pytest.mark.parametrize("stepped_rrqr", ["default", "noreg", "wave", "steps", "steps8", "steps4"], indirect=True)
 @pytest.mark.parametrize("n,d,nl,tol", CASES)
 def test_fp64_factor_matches_oracle(n, d, nl, tol, stepped_rrqr):
     say(f"case fp64 {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")
@@ -196,6 +199,7 @@ def test_kernel_profile_stepped_rrqr(monkeypatch):
     """With every cluster on the per-step RRQR path, the stepped families are timed and priced
     from the sizes and ranks the device decided (read once the factorization has finished)."""
     say("case profile stepped rrqr")
+    monkeypatch.setenv("SPAND_RRQR_REG", "0")
     monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
     monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
     A, X = neglapl(12, 3), linspace_nd(12, 3)

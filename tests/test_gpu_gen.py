@@ -40,9 +40,13 @@ def gen_gpu_tree(A, X, nl, tol, prec=S.FP64):
 
 @pytest.fixture
 def stepped_rrqr(request, monkeypatch):
-    """RRQR path: the library's own size split, or every cluster on the multi-workgroup path
-    (one persistent launch per wave, or one launch per pivot step with 16, 8 or 4 wavefronts
-    per 64-column slab)."""
+    """Which RRQR path the clusters take: the library's own size split ("default": register
+    path, one workgroup per cluster, for the clusters it holds), the split without the
+    register path ("noreg": LDS one-workgroup path and stepped path), every cluster on the
+    multi-workgroup path run as one persistent launch per wave ("wave"), or as one launch per
+    pivot step ("steps"; "steps8" / "steps4": 8 or 4 wavefronts per 64-column slab)."""
+    if request.param != "default":
+        monkeypatch.setenv("SPAND_RRQR_REG", "0")
     if request.param in ("wave", "steps", "steps8", "steps4"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
@@ -57,7 +61,7 @@ GEN_CASES = [("unsym", 5, 2, 3, 0.0), ("unsym", 20, 2, 5, 1e-2), ("unsym", 10, 3
              ("unsym", 15, 3, 6, 0.0), ("cd", 16, 3, 7, 1e-2), ("unsym", 40, 2, 3, 1e-2)]
 
 
-@pytest.mark.parametrize("stepped_rrqr", ["default", "wave", "steps", "steps8", "steps4"], indirect=True)
+@pytest.mark.parametrize("stepped_rrqr", ["default", "noreg", "wave", "steps", "steps8", "steps4"], indirect=True)
 @pytest.mark.parametrize("kind,n,d,nl,tol", GEN_CASES)
 def test_gen_fp64_factor_matches_oracle(kind, n, d, nl, tol, stepped_rrqr):
     say(f"case gen fp64 {kind} {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")
