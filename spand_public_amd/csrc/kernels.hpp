@@ -58,15 +58,19 @@ constexpr int STEP_LDS_MAX = 40960;
 template <typename T>
 hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
                             const int* d_size, double tol, hipStream_t st);
-// All steps of a wave in one cooperative launch (clusters of at most WAVE_ROWS_MAX rows,
-// RrqrBig::vbuf holding [2][nslab][max_rows] candidate columns); lds_bytes = max_cols
-// ints + 2 max_rows elements. Returns hipErrorCooperativeLaunchTooLarge, launching
-// nothing, when the wave does not fit the device at once.
+// All steps of a set of whole clusters in one cooperative launch (clusters of at most
+// WAVE_ROWS_MAX rows, RrqrBig::vbuf holding [2][nslab][max_rows] candidate columns,
+// RrqrBig::bar a barrier counter); lds_bytes = max_cols ints + 2 max_rows elements. The
+// slabs of each cluster synchronize among themselves (bounded waits: *fail is set to 1 if
+// one gives up). Returns hipErrorCooperativeLaunchTooLarge, launching nothing, when the
+// slabs do not fit the device at once; rrqr_wave_capacity gives how many do.
 constexpr int WAVE_ROWS_MAX = 512;
 constexpr int WAVE_LDS_MAX = 49152;
 template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
-                            double tol, size_t lds_bytes, hipStream_t st);
+                            double tol, size_t lds_bytes, int* fail, hipStream_t st);
+template <typename T>
+hipError_t rrqr_wave_capacity(int max_rows, size_t lds_bytes, int* cap);
 // One pivot step per launch. waves: wavefronts per slab workgroup (4, 8 or 16; 0 picks by
 // the slab count, with at most 64 cached rows per thread). prog (host-visible, or nullptr)
 // receives seq when the launch starts; each cluster's RrqrBig::done is set by its last

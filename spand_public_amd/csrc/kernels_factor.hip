@@ -1411,6 +1411,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         B.dims[4] = 2;  // 2: the steps store every kept reflector in Q; 1: the wave kernel left them in A
         if (B.dims[3] == 0) signal_store(B.done, 1);
     }
+    if (B.bar && tid == 0) B.bar[sl.slab] = 0;  // the persistent launch's per-slab step count
     T* A = reinterpret_cast<T*>(B.t.scratch);
     const int mc = B.t.max_cols;
     const int j = sl.slab * SLAB + lane;
@@ -1726,8 +1727,48 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     }
 }
 
-// Persistent form of the stepped RRQR: one cooperative launch runs every pivot step of a
-// wave, with a grid-wide barrier between steps. Each thread keeps its column's rows
+// Barrier among the slabs of one cluster in the persistent RRQR: each slab publishes, once
+// per step and after its candidate (release at device scope), the number of steps it has
+// finished in its own flag; a slab waits until every flag of its cluster reaches the step
+// (lanes of wavefront 0 poll one flag each; acquire). The wait is bounded: after about 2^21
+// polls it sets *fail and returns false (the launch is cooperative, so every slab is
+// resident and a correct schedule never comes near the bound; the host turns the flag into
+// an error).
+constexpr long long CLUSTER_SPIN_MAX = 1ll << 21;
+__device__ inline void cluster_arrive(int* flags, int slab, int done) {
+    __syncthreads();  // every thread's stores of this step are issued
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(flags + slab, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+__device__ inline bool cluster_wait(const int* flags, int nslab, int target, int* fail, int* s_ok) {
+    if (threadIdx.x < 64) {
+        int ok = 1;
+        for (int k = threadIdx.x; k < nslab; k += 64) {
+            long long spins = 0;
+            while (__hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (++spins > CLUSTER_SPIN_MAX ||
+                    __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 0;
+                    __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            if (!ok) break;
+        }
+        ok = __all(ok);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (threadIdx.x == 0) *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+// Persistent form of the stepped RRQR: one cooperative launch runs every pivot step of the
+// clusters of a wave (or of a part of it that the device holds at once); the slabs of each
+// cluster synchronize among themselves between steps (cluster_wait), so a cluster's steps
+// are paced by its own slabs only, and a cluster that reaches its rank leaves. Each thread keeps its column's rows
 // r = wv + 16 q (q < WAVE_RW) in registers for the whole factorization; lane l of
 // wavefront wv is column l of the slab, and chain c of every sum at step i is held by
 // wavefront (c + i) % 16, so sums run in the same 16-chain order as k_rrqr_step. Every
@@ -1738,15 +1779,14 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
 // WAVE_RW rows per thread: clusters of up to 16 * WAVE_RW rows (16: two workgroups per CU).
 template <typename T, int WAVE_RW>
 __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
-    k_rrqr_wave(const RrqrBig* bigs, const RrqrSlab* slabs, int nsteps, double tol) {
+    k_rrqr_wave(const RrqrBig* bigs, const RrqrSlab* slabs, int nsteps, double tol, int* fail) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double s_part[NCH][SLAB];
     __shared__ double sh[4], s_ch[NCH + 1];
     __shared__ T s_tm[SLAB], s_rowi[SLAB];
     __shared__ int s_upd[SLAB], s_pos[SLAB];
-    __shared__ int s_p, s_ca, s_cb, s_best;
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    __shared__ int s_p, s_ca, s_cb, s_best, s_ok;
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
     const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
@@ -1791,9 +1831,14 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
             }
         }
     }
+    cluster_arrive(B.bar, sl.slab, 1);
+    bool abandoned = false;
     for (int i = 0; i < nsteps; i++) {
-        grid.sync();
-        if (rows == 0 || i >= mn || i >= stop) continue;
+        if (rows == 0 || i >= mn || i >= stop) break;  // the same decision in every slab of the cluster
+        if (!cluster_wait(B.bar, ns, i + 1, fail, &s_ok)) {
+            abandoned = true;
+            break;
+        }
         const int cur = i & 1, nxt = cur ^ 1;
         if (wv == 0) {  // pivot: first maximum over the slabs' candidates
             double bv = -1.0;
@@ -1817,7 +1862,7 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
         const int p = s_p, ca = s_ca, cb = s_cb;
         if (p < i || p >= cols) {  // cannot happen for i < mn: some column is still free
             stop = i;
-            continue;
+            break;
         }
         const T* pc = ccol + ((size_t)cur * ns + ca / SLAB) * mr;
         for (int r = i + tid; r < rows; r += STEP_NT) s_col[r] = pc[r];
@@ -1829,7 +1874,7 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
             sref = rii;
         } else if (fixed < 0 && !(rii / sref >= tol)) {
             stop = i;
-            continue;
+            break;
         }
         const double taud = sh[0];
         const T tau = (T)taud;
@@ -1940,7 +1985,9 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
                 if (r < rows) dst[r] = a[q];
             }
         }
+        cluster_arrive(B.bar, sl.slab, i + 2);
     }
+    if (abandoned) return;  // the host reports the failed wait; the slab storage is left as init wrote it
     // registers back to the slab storage; final maps after `stop` steps, as k_rrqr_finish reads them
     if (jin) {
 #pragma unroll
@@ -2244,27 +2291,44 @@ hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
 // Persistent wave launch: returns hipErrorCooperativeLaunchTooLarge (and launches nothing)
 // when the wave's workgroups cannot all be resident at once.
 template <typename T, int RW>
-static hipError_t launch_wave_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol,
-                                 size_t lds_bytes, hipStream_t st) {
+static hipError_t wave_capacity_rw(size_t lds_bytes, int* cap) {
     int dev = 0, ncu = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW>);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, STEP_NT, lds_bytes);
+    *cap = e == hipSuccess ? per_cu * ncu : 0;
+    return e;
+}
+
+template <typename T>
+hipError_t rrqr_wave_capacity(int max_rows, size_t lds_bytes, int* cap) {
+    *cap = 0;
+    if (max_rows <= 16 * 16) return wave_capacity_rw<T, 16>(lds_bytes, cap);
+    if (max_rows <= 16 * 32) return wave_capacity_rw<T, 32>(lds_bytes, cap);
+    return hipSuccess;
+}
+
+template <typename T, int RW>
+static hipError_t launch_wave_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol,
+                                 size_t lds_bytes, int* fail, hipStream_t st) {
+    int cap = 0;
+    hipError_t e = wave_capacity_rw<T, RW>(lds_bytes, &cap);
     if (e != hipSuccess) return e;
-    if ((long long)per_cu * ncu < nslabs) return hipErrorCooperativeLaunchTooLarge;
-    void* args[] = {(void*)&d_bigs, (void*)&d_slabs, (void*)&nsteps, (void*)&tol};
+    if (cap < nslabs) return hipErrorCooperativeLaunchTooLarge;
+    const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW>);
+    void* args[] = {(void*)&d_bigs, (void*)&d_slabs, (void*)&nsteps, (void*)&tol, (void*)&fail};
     return hipLaunchCooperativeKernel(f, dim3(nslabs), dim3(STEP_NT), args, (unsigned)lds_bytes, st);
 }
 
-// Persistent wave launch: returns hipErrorCooperativeLaunchTooLarge (and launches nothing)
-// when the wave's workgroups cannot all be resident at once.
+// Persistent launch over nslabs slabs (whole clusters): returns hipErrorCooperativeLaunchTooLarge
+// (and launches nothing) when they cannot all be resident at once.
 template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
-                            double tol, size_t lds_bytes, hipStream_t st) {
+                            double tol, size_t lds_bytes, int* fail, hipStream_t st) {
     if (nslabs == 0 || nsteps == 0) return hipSuccess;
-    if (max_rows <= 16 * 16) return launch_wave_rw<T, 16>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, st);
-    if (max_rows <= 16 * 32) return launch_wave_rw<T, 32>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, st);
+    if (max_rows <= 16 * 16) return launch_wave_rw<T, 16>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, fail, st);
+    if (max_rows <= 16 * 32) return launch_wave_rw<T, 32>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, fail, st);
     return hipErrorCooperativeLaunchTooLarge;
 }
 
@@ -2298,8 +2362,9 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_copy<T>(const CopyTask*, int, hipStream_t);                        \
     template hipError_t launch_rrqr_init<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, const int*,      \
                                             double, hipStream_t);                                                  \
-    template hipError_t launch_rrqr_wave<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t,       \
+    template hipError_t launch_rrqr_wave<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t, int*, \
                                             hipStream_t);                                                          \
+    template hipError_t rrqr_wave_capacity<T>(int, size_t, int*);                                                  \
     template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t, int,  \
                                             long long*, long long, hipStream_t);                                                       \
     template hipError_t launch_rrqr_finish<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, int*, double,  \

@@ -103,6 +103,7 @@ struct RrqrBig {
                      // (persistent wave launch)
     long long* done; // host-visible flag, or nullptr: set to 1 by the last step that does
                      // work (rank reached or all min(rows, cols) steps taken)
+    int* bar;        // persistent launch: steps finished by each slab (set to 0 by init)
     int nslab;
     int pad;
 };

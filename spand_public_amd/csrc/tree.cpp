@@ -952,8 +952,9 @@ void Tree::factorize_impl() {
         hip_check(hipMalloc((void**)&d_size_, cap * sizeof(int)), "hipMalloc(size)");
         d_size_cap_ = cap;
     }
-    if (!d_info_) hip_check(hipMalloc((void**)&d_info_, sizeof(int)), "hipMalloc(info)");
-    hip_check(hipMemset(d_info_, 0, sizeof(int)), "memset info");
+    // [0]: failed pivot (cluster order + 1), [1]: a persistent RRQR wait that gave up
+    if (!d_info_) hip_check(hipMalloc((void**)&d_info_, 2 * sizeof(int)), "hipMalloc(info)");
+    hip_check(hipMemset(d_info_, 0, 2 * sizeof(int)), "memset info");
     std::vector<int> hsize(cap, 0);
     for (int c : bottoms_[0]) cls_[c].xoff = cls_[c].start;
     ws_len_ = N_;
@@ -1480,6 +1481,7 @@ void Tree::factorize_impl() {
                     b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
                     b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)(2 * b.nslab + 1) * sizeof(double)));
                     b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
+                    b.bar = reinterpret_cast<int*>(arena_.alloc((size_t)b.nslab * sizeof(int)));  // per-slab step counts
                     b.t = t;
                     where[p] = {1, (int)wbig[w].size()};
                     for (int k = 0; k < b.nslab; k++) wslab[w].push_back({(int)wbig[w].size(), k});
@@ -1517,7 +1519,10 @@ void Tree::factorize_impl() {
         // launch (candidate-column buffers in vbuf); the per-step launches remain the path
         // for larger clusters and for waves the device cannot hold at once.
         std::vector<size_t> wlds_step(nwaves, 0), wlds_wave(nwaves, 0);
-        // off by default: on gfx950 the per-step launches measure faster than the grid barrier
+        // Opt-in (SPAND_RRQR_PERSISTENT=1): the persistent launch with per-cluster barriers
+        // measured slower on gfx950 than one launch per step (64^3 FP32 factor + PCG: 177 ms
+        // against 125 ms per step), although a flag handshake between two workgroups costs
+        // about 1 us one way: the step body itself is slower in the 1024-thread persistent form.
         const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
         const int step_waves = (int)env_or("SPAND_RRQR_STEP_WAVES", 0);  // 4, 8, 16; 0: by slab count
         for (int w = 0; w < nwaves; w++) {
@@ -1667,16 +1672,33 @@ void Tree::factorize_impl() {
             // the steps of a wave are timed as one batch (two events per wave)
             bool done = false;
             if (wlds_wave[w] > 0) {
-                const int sl = prof_.start(KF_RRQR_WAVE, st);
-                const hipError_t e = launch_rrqr_wave<T>(db, ds, nsl, wsteps[w], wmaxrows[w], tol_, wlds_wave[w], st);
-                prof_.stop(sl, st, 0, 0, e == hipSuccess ? 1 : 0);
-                if (e == hipSuccess) {
+                // whole clusters in chunks the device holds at once, one cooperative launch each
+                int cap = 0;
+                hip_check(rrqr_wave_capacity<T>(wmaxrows[w], wlds_wave[w], &cap), "rrqr wave capacity");
+                int maxslab = 0;
+                for (auto& b : wbig[w]) maxslab = std::max(maxslab, b.nslab);
+                if (cap >= maxslab && cap > 0) {
+                    const int sl = prof_.start(KF_RRQR_WAVE, st);
+                    int launches = 0;
+                    size_t x0 = 0, s0 = 0;
+                    while (x0 < wbig[w].size()) {
+                        size_t x1 = x0, s1 = s0;
+                        int steps = 0;
+                        while (x1 < wbig[w].size() && (int)(s1 - s0) + wbig[w][x1].nslab <= cap) {
+                            steps = std::max(steps, std::min(wbig[w][x1].t.max_rows, wbig[w][x1].t.max_cols));
+                            s1 += wbig[w][x1].nslab;
+                            x1++;
+                        }
+                        const hipError_t e = launch_rrqr_wave<T>(db, ds + s0, (int)(s1 - s0), steps, wmaxrows[w], tol_,
+                                                                 wlds_wave[w], d_info_ + 1, st);
+                        hip_check(e, "rrqr wave");
+                        launches++;
+                        x0 = x1;
+                        s0 = s1;
+                    }
+                    prof_.stop(sl, st, 0, 0, launches);
                     done = true;
                     slot_step[w] = sl;
-                } else if (e == hipErrorCooperativeLaunchTooLarge) {
-                    (void)hipGetLastError();
-                } else {
-                    hip_check(e, "rrqr wave");
                 }
             }
             if (!done) {
@@ -1714,8 +1736,10 @@ void Tree::factorize_impl() {
                             &slot_finish[w]),
                       "rrqr finish");
         }
-        int info = 0;
-        hip_check(hipMemcpy(&info, d_info_, sizeof(int), hipMemcpyDeviceToHost), "read info");
+        int info2[2] = {0, 0};
+        hip_check(hipMemcpy(info2, d_info_, 2 * sizeof(int), hipMemcpyDeviceToHost), "read info");
+        const int info = info2[0];
+        if (info2[1] != 0) throw Error(ERR_HIP, "persistent RRQR: a cluster barrier wait gave up");
         if (info != 0) {
             if (gen) throw Error(ERR_SINGULAR, "Error: Singular Pivot (cluster order " + std::to_string(info - 1) + ")");
             throw Error(ERR_NOT_SPD, "Error: Non-SPD Pivot (cluster order " + std::to_string(info - 1) + ")");

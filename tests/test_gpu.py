@@ -200,6 +200,7 @@ def test_kernel_profile_stepped_rrqr(monkeypatch):
     from the sizes and ranks the device decided (read once the factorization has finished)."""
     say("case profile stepped rrqr")
     monkeypatch.setenv("SPAND_RRQR_REG", "0")
+    monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "0")
     monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
     monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
     A, X = neglapl(12, 3), linspace_nd(12, 3)
