@@ -112,6 +112,8 @@ Tree::~Tree() {
     (void)hipFree(d_ws_);
     (void)hipHostFree(h_stage_);
     (void)hipHostFree(h_sig_);
+    drop_apply_graphs();
+    if (cap_st_) (void)hipStreamDestroy(cap_st_);
     if (aux_st_) {
         (void)hipStreamSynchronize(aux_st_);
         (void)hipStreamDestroy(aux_st_);
@@ -911,6 +913,8 @@ template <typename T>
 void Tree::factorize_impl() {
     const double t_start = wall();
     const hipStream_t st = 0;
+    drop_apply_graphs();
+    graphs_on_ = env_or("SPAND_APPLY_GRAPH", 1) != 0;
     arena_.reset();
     task_arena_.reset();
     wave_work_.clear();
@@ -2211,13 +2215,22 @@ void Tree::solve_interleaved(double* d_x, int nrhs, hipStream_t st, bool transpo
 // Tree::solve (src/tree.cpp:1610-1635): permute in, forward sweep over the levels, backward
 // sweep in reverse, permute out. Split ops need no work: a sibling's segment aliases the
 // tail of its original cluster's segment.
+// Instantiated graphs of the apply, by launch configuration; dropped at every factorization
+// (the plans they launch are rebuilt there).
+void Tree::drop_apply_graphs() {
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+}
+
 template <typename T>
-void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st, bool transpose, double* ws_ext) {
+void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st_caller, bool transpose,
+                      double* ws_ext) {
     // ws: the segments (FP64 whatever the factor's element type T), then the partial sums
     // of the largest mv phase
     const size_t seg = (size_t)ws_len_ * nrhs;
     const size_t need = apply_workspace_bytes(nrhs);
     if (!ws_ext && need > ws_cap_) {
+        drop_apply_graphs();  // they address the old workspace
         (void)hipFree(d_ws_);
         d_ws_ = nullptr;
         hip_check(hipMalloc(&d_ws_, need), "hipMalloc(solve workspace)");
@@ -2226,8 +2239,23 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
     double* ws = ws_ext ? ws_ext : reinterpret_cast<double*>(d_ws_);
     double* partial = ws + seg;
     const double vsz = sizeof(double);
+    // The whole apply (a fixed list of about 100-300 launches once the factorization is done)
+    // is captured once per configuration in a hipGraph and replayed: not while launches are
+    // timed, not for the allocation-free apply (instantiation allocates), and not with
+    // SPAND_APPLY_GRAPH=0.
+    const bool graph = !ws_ext && !prof_.on() && graphs_on_;
+    hipStream_t st = st_caller;
+    if (graph) {
+        for (auto& g : graphs_)
+            if (g.dx == dx && g.nrhs == nrhs && g.sx == sx && g.sr == sr && g.transpose == transpose && g.ws == ws) {
+                hip_check(hipGraphLaunch(g.exec, st_caller), "apply graph launch");
+                return;
+            }
+        st = capture_stream();
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "apply graph capture");
+    }
     auto run = [&](int fam, const SweepWork& w, const char* what, auto&& launch) {
-        if (ws_ext) {  // the allocation-free apply creates no events
+        if (ws_ext || graph) {  // the allocation-free apply creates no events; a capture times nothing
             hip_check(launch(), what);
             return;
         }
@@ -2257,6 +2285,26 @@ void Tree::apply_impl(double* dx, int nrhs, size_t sx, size_t sr, hipStream_t st
         for (const MvPhase& ph : transpose ? p.tbwd : p.bwd) run_phase(ph);
     }
     run(KF_PERM, wp, "perm out", [&] { return launch_perm_out<T>(ws, d_perm_, N_, nrhs, dx, sx, sr, st); });
+    if (graph) {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamEndCapture(st, &g), "apply graph capture");
+        ApplyGraph ag{dx, ws, nrhs, sx, sr, transpose, nullptr};
+        const hipError_t e = hipGraphInstantiate(&ag.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "apply graph instantiate");
+        if (graphs_.size() >= 8) {  // a few configurations per factorization: the oldest goes
+            (void)hipGraphExecDestroy(graphs_.front().exec);
+            graphs_.erase(graphs_.begin());
+        }
+        graphs_.push_back(ag);
+        hip_check(hipGraphLaunch(ag.exec, st_caller), "apply graph launch");
+    }
+}
+
+// Stream that apply graphs are captured on (the null stream cannot be captured).
+hipStream_t Tree::capture_stream() {
+    if (!cap_st_) hip_check(hipStreamCreateWithFlags(&cap_st_, hipStreamNonBlocking), "hipStreamCreate");
+    return cap_st_;
 }
 
 }  // namespace spand

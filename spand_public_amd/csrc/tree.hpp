@@ -312,6 +312,20 @@ class Tree {
 
     hipStream_t aux_stream();
     hipStream_t aux_st_ = nullptr;
+    // the apply as instantiated graphs, one per launch configuration
+    struct ApplyGraph {
+        double* dx;
+        double* ws;
+        int nrhs;
+        size_t sx, sr;
+        bool transpose;
+        hipGraphExec_t exec;
+    };
+    std::vector<ApplyGraph> graphs_;
+    bool graphs_on_ = true;
+    hipStream_t cap_st_ = nullptr;
+    hipStream_t capture_stream();
+    void drop_apply_graphs();
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
 
     KernelProfiler prof_;

@@ -264,3 +264,32 @@ def test_gmres_iterations_and_residual(restart, prec):
     assert res.max() <= 1e-8, res
     for io, ig in zip(its_o, info["iters"]):
         assert abs(int(ig) - io) <= max(1, 0.1 * io), (its_o, info["iters"])
+
+
+def test_apply_graph_replays_match_eager(monkeypatch):
+    """The apply runs as a captured hipGraph after its first call per configuration: every
+    replay, with eager work and synchronizations between them, equals the eager apply
+    (SPAND_APPLY_GRAPH=0) bitwise; a refactorization drops the graphs and captures anew."""
+    torch = pytest.importorskip("torch")
+    say("case apply graph replays")
+    A, X = neglapl(14, 3), linspace_nd(14, 3)
+    B = np.random.default_rng(17).standard_normal((A.n, 3))
+    monkeypatch.setenv("SPAND_APPLY_GRAPH", "0")
+    e = gpu_tree(A, X, 6, 1e-2, prec=S.FP32)
+    e.factorize()
+    ref = e.solve(B)
+    x_ref, info_ref = S.cg(A, B, e, iters=100, tol=1e-12)
+    monkeypatch.setenv("SPAND_APPLY_GRAPH", "1")
+    g = gpu_tree(A, X, 6, 1e-2, prec=S.FP32)
+    for rep in range(2):
+        g.factorize()  # the setting is read at factorize; the second one drops the graphs
+        d = torch.tensor(np.asfortranarray(B).T.copy(), device="cuda")
+        for k in range(4):
+            d.copy_(torch.tensor(np.asfortranarray(B).T.copy(), device="cuda"))
+            torch.cuda.synchronize()
+            g.solve_device(d.data_ptr(), 3, A.n)
+            torch.cuda.synchronize()
+            assert np.array_equal(d.cpu().numpy().T, ref), (rep, k)
+            _ = (d * 2.0).sum().item()  # eager work between replays
+        x, info = S.cg(A, B, g, iters=100, tol=1e-12)
+        assert np.array_equal(x, x_ref) and np.array_equal(info["iters"], info_ref["iters"])
