@@ -28,7 +28,11 @@ hipError_t launch_potrf_panel(const PotrfTask* d_tasks, int ntasks, int kb, int 
                               hipStream_t st);
 template <typename T>
 hipError_t launch_potrf_trail(const PotrfTask* d_tasks, int ntasks, int kb, int nmax, hipStream_t st);
-template <typename T> hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st);
+// FP32 tasks from index mfma_from on (triangles longer than TRSM_MFMA_MIN_N, placed last by
+// the caller) run their trailing updates on the matrix cores; the others, and every FP64
+// task, run the ordered-FMA vector kernel.
+constexpr int TRSM_MFMA_MIN_N = 64;
+template <typename T> hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, int mfma_from, hipStream_t st);
 // LU pivots (GEN / PLU): in_lds = 1 runs the batch in LDS (every task must satisfy
 // n (n + 1) sizeof(T) <= GETRF_LDS_BYTES, nmax the largest n), in_lds = 0 in global memory.
 constexpr int GETRF_LDS_BYTES = 65536;

@@ -13,6 +13,8 @@
 //             (src/tree.cpp:1189-1224, 1292-1347, 980-1104)
 //   copy   -> merge block copies                src/tree.cpp:1133-1184
 #include <hip/hip_cooperative_groups.h>
+
+#include <cstdlib>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -408,6 +410,116 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
 }
 
 // ---------------------------------------------------------------------------------------
+// FP32 trsm with the trailing updates on the matrix cores. Same blocking as k_trsm: the
+// 32 x 32 diagonal block is solved by one thread per independent index (the recurrence of
+// k_trsm), and each 64-row group below it is updated as C -= L[i0:i0+64, kb:kb+32] X by
+// v_mfma_f32_32x32x2f32 with FP32 accumulation: four wavefronts, one 32 x 32 output tile
+// each. The tile is laid out so that the lanes of a wavefront run along the contiguous
+// index of the target block: left solves (C(i, c) at i + c ld) compute the tile transposed,
+// D(c, i) = X^T L^T, right solves (C(c, i) at c + i ld) compute D(i, c) = L X. FP32 only:
+// the FP64 path keeps the ordered-FMA recurrence of k_trsm.
+typedef float mfma_f32x16 __attribute__((ext_vector_type(16)));
+__global__ void __launch_bounds__(NT, 3) k_trsm_f32_mfma(const TrsmTask* tasks) {
+    __shared__ float Bk[PB][64 + 1];
+    __shared__ float Lk[PB][PB + 1];
+    __shared__ float Lr[PB][64 + 1];
+    const TrsmTask t = tasks[blockIdx.x];
+    const int n = t.n, m = t.m, ld = t.ldb;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (n == 0 || m == 0) return;
+    const float* L = reinterpret_cast<const float*>(t.L);
+    float* G = reinterpret_cast<float*>(t.B);
+    const bool left = t.side == 0;
+    auto at = [&](int i, int c) -> float& { return left ? G[i + (size_t)c * ld] : G[c + (size_t)i * ld]; };
+    const float* S = reinterpret_cast<const float*>(t.src);
+    const int* P = t.perm;
+    auto first = [&](int i, int c) -> float { return S ? S[(P ? P[i] : i) + (size_t)c * ld] : at(i, c); };
+    // this wavefront's output tile: rows rh*32.. of the 64-row group, columns ch*32.. of the chunk
+    const int rh = wv & 1, ch = wv >> 1;
+    for (int kb = 0; kb < n; kb += PB) {
+        const int nb = min(PB, n - kb);
+        for (int idx = tid; idx < PB * 64; idx += NT) {  // zero padding: rows >= nb contribute nothing
+            int k, c;
+            if (left) { k = idx % PB; c = idx / PB; } else { c = idx % 64; k = idx / 64; }
+            Bk[k][c] = (k < nb && c < m) ? (kb == 0 ? first(kb + k, c) : at(kb + k, c)) : 0.0f;
+        }
+        for (int idx = tid; idx < nb * nb; idx += NT) {
+            int i = idx % nb, k = idx / nb;
+            Lk[i][k] = L[(kb + i) + (size_t)(kb + k) * n];
+        }
+        __syncthreads();
+        if (tid < m) {  // column c of the diagonal block, held in registers
+            const int c = tid;
+            float x[PB];
+#pragma unroll
+            for (int k = 0; k < PB; k++) x[k] = k < nb ? Bk[k][c] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < PB; k++) {
+                if (k < nb) {
+                    float b;
+                    if (left) {
+                        b = x[k] / Lk[k][k];
+                    } else {
+                        const float r = 1.0f / Lk[k][k];
+                        b = x[k] * r;
+                    }
+                    x[k] = b;
+#pragma unroll
+                    for (int i = k + 1; i < PB; i++)
+                        if (i < nb) x[i] = fmaf(-b, Lk[i][k], x[i]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < PB; k++)
+                if (k < nb) Bk[k][c] = x[k];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nb * m; idx += NT) {
+            int k, c;
+            if (left) { k = idx % nb; c = idx / nb; } else { c = idx % m; k = idx / m; }
+            at(kb + k, c) = Bk[k][c];
+        }
+        for (int i0 = kb + nb; i0 < n; i0 += 64) {
+            const int ni = min(64, n - i0);
+            for (int idx = tid; idx < PB * 64; idx += NT) {
+                const int ii = idx % 64, k = idx / 64;
+                Lr[k][ii] = (ii < ni && k < nb) ? L[(i0 + ii) + (size_t)(kb + k) * n] : 0.0f;
+            }
+            __syncthreads();
+            if (rh * 32 < ni && ch * 32 < m) {
+                // element v of this lane: tile row (8 (v/4) + 4 (lane/32) + v%4), tile column lane%32
+                const int tc = lane & 31, tr0 = 4 * (lane >> 5);
+                mfma_f32x16 acc;
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int tr = 8 * (v >> 2) + tr0 + (v & 3);
+                    // left: tile rows are chunk columns c, tile columns are rows i; right: the reverse
+                    const int ii = rh * 32 + (left ? tc : tr), c = ch * 32 + (left ? tr : tc);
+                    float val = 0.0f;
+                    if (ii < ni && c < m) val = kb == 0 ? first(i0 + ii, c) : at(i0 + ii, c);
+                    acc[v] = val;
+                }
+#pragma unroll
+                for (int k = 0; k < PB; k += 2) {
+                    const int kk = k + (lane >> 5);
+                    const float x = -Bk[kk][ch * 32 + (lane & 31)];
+                    const float l = Lr[kk][rh * 32 + (lane & 31)];
+                    acc = left ? __builtin_amdgcn_mfma_f32_32x32x2f32(x, l, acc, 0, 0, 0)
+                               : __builtin_amdgcn_mfma_f32_32x32x2f32(l, x, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int tr = 8 * (v >> 2) + tr0 + (v & 3);
+                    const int ii = rh * 32 + (left ? tc : tr), c = ch * 32 + (left ? tr : tc);
+                    if (ii < ni && c < m) at(i0 + ii, c) = acc[v];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // getrf + split_LU, one workgroup per pivot (getf_cluster, src/tree.cpp:614-635; getf,
 // src/util.cpp:184-199; split_LU, src/util.cpp:213-227). The steps follow dgetf2: the pivot
 // row is the first one of largest |a_ij| (idamax), rows j and jp are interchanged across
@@ -578,6 +690,63 @@ __global__ void __launch_bounds__(NT) k_schur(const SchurTile* tiles, const Schu
         if (i < t.tm && j < t.tn) {
             int gi = t.r0 + i, gj = t.c0 + j;
             if (!t.lower || gi >= gj) C[gi + (size_t)gj * t.ldc] = acc_c[q];
+        }
+    }
+}
+
+// FP32 Schur complement on the matrix cores: the same 32 x 32 target tiles and staged k
+// chunks as k_schur; the four wavefronts each accumulate every fourth k-pair of the tile's
+// contributions with v_mfma_f32_32x32x2f32 (FP32 accumulation, the tile computed
+// transposed so that a wavefront's lanes run down the target's columns), and the four
+// partial sums are added in LDS and subtracted from the target once at the end.
+__global__ void __launch_bounds__(NT) k_schur_f32_mfma(const SchurTile* tiles, const SchurContrib* contribs) {
+    __shared__ float As[SK][ST + 1];
+    __shared__ float Bs[SK][ST + 1];
+    __shared__ float Ps[NT / 64][ST][ST + 1];
+    const SchurTile t = tiles[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    float* C = reinterpret_cast<float*>(t.C);
+    mfma_f32x16 acc = {};
+    for (int kk = t.k0; kk < t.k1; kk++) {
+        const SchurContrib c = contribs[kk];
+        const float* A = reinterpret_cast<const float*>(c.A);
+        const float* B = reinterpret_cast<const float*>(c.B);
+        for (int l0 = 0; l0 < c.k; l0 += SK) {
+            const int kc = min(SK, c.k - l0);
+            for (int idx = tid; idx < SK * ST; idx += NT) {
+                int i = idx % ST, l = idx / ST;
+                float va = 0.0f, vb = 0.0f;
+                if (l < kc) {
+                    int gi = t.r0 + i, gl = l0 + l, gj = t.c0 + i;
+                    if (i < t.tm) va = c.ta ? A[gl + (size_t)gi * c.lda] : A[gi + (size_t)gl * c.lda];
+                    if (i < t.tn) vb = c.tb ? B[gj + (size_t)gl * c.ldb] : B[gl + (size_t)gj * c.ldb];
+                }
+                As[l][i] = va;
+                Bs[l][i] = vb;
+            }
+            __syncthreads();
+            // D(j, i) += sum_l opB(l, j) opA(i, l): the A operand is opB^T, the B operand opA^T
+#pragma unroll
+            for (int q = 0; q < SK / 8; q++) {
+                const int kp = 2 * (wv + 4 * q) + (lane >> 5);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Bs[kp][lane & 31], As[kp][lane & 31], acc, 0, 0, 0);
+            }
+            __syncthreads();
+        }
+    }
+    // element v of this lane: D row j = 8 (v/4) + 4 (lane/32) + v%4, column i = lane%32
+#pragma unroll
+    for (int v = 0; v < 16; v++) Ps[wv][8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)][lane & 31] = acc[v];
+    __syncthreads();
+    for (int idx = tid; idx < ST * ST; idx += NT) {
+        const int i = idx % ST, j = idx / ST;
+        if (i < t.tm && j < t.tn) {
+            const int gi = t.r0 + i, gj = t.c0 + j;
+            if (!t.lower || gi >= gj) {
+                const float sum = (Ps[0][j][i] + Ps[1][j][i]) + (Ps[2][j][i] + Ps[3][j][i]);
+                float* cp = C + gi + (size_t)gj * t.ldc;
+                *cp = *cp - sum;
+            }
         }
     }
 }
@@ -2105,11 +2274,37 @@ hipError_t launch_potrf(const PotrfTask* d_tasks, int ntasks, int* d_info, hipSt
     return hipGetLastError();
 }
 
+// FP32 updates on the matrix cores. The Schur update takes them unless SPAND_SCHUR_MFMA=0.
+// The trsm's trailing updates only with SPAND_TRSM_MFMA=1: on the 64^3 workload its time is
+// in the sequential diagonal-block solves and the block loads, and the matrix-core form
+// measured slower (12.4 ms against 10.7 ms of trsm per FP32 factorization, kernel trace).
+static bool env_flag(const char* name, bool dflt) {
+    const char* e = std::getenv(name);
+    if (!e || !e[0]) return dflt;
+    return e[0] != '0';
+}
+static bool trsm_mfma_enabled() { return env_flag("SPAND_TRSM_MFMA", false); }
+static bool schur_mfma_enabled() { return env_flag("SPAND_SCHUR_MFMA", true); }
+
 template <typename T>
-hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, hipStream_t st) {
+hipError_t launch_trsm(const TrsmTask* d_tasks, int ntasks, int mfma_from, hipStream_t st) {
     if (ntasks == 0) return hipSuccess;
-    k_trsm<T><<<ntasks, NT, 0, st>>>(d_tasks);
-    return hipGetLastError();
+    int nv = ntasks;  // tasks for the vector kernel
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (trsm_mfma_enabled()) nv = std::max(0, std::min(mfma_from, ntasks));
+    }
+    if (nv > 0) {
+        k_trsm<T><<<nv, NT, 0, st>>>(d_tasks);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (nv < ntasks) {
+            k_trsm_f32_mfma<<<ntasks - nv, NT, 0, st>>>(d_tasks + nv);
+            return hipGetLastError();
+        }
+    }
+    return hipSuccess;
 }
 
 template <typename T>
@@ -2128,6 +2323,12 @@ hipError_t launch_getrf(const GetrfTask* d_tasks, int ntasks, int in_lds, int nm
 template <typename T>
 hipError_t launch_schur(const SchurTile* d_tiles, int ntiles, const SchurContrib* d_contribs, hipStream_t st) {
     if (ntiles == 0) return hipSuccess;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (schur_mfma_enabled()) {
+            k_schur_f32_mfma<<<ntiles, NT, 0, st>>>(d_tiles, d_contribs);
+            return hipGetLastError();
+        }
+    }
     k_schur<T><<<ntiles, NT, 0, st>>>(d_tiles, d_contribs);
     return hipGetLastError();
 }
@@ -2354,7 +2555,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_potrf64<T>(const PotrfTask*, int, int*, hipStream_t);              \
     template hipError_t launch_potrf_panel<T>(const PotrfTask*, int, int, int, int*, hipStream_t);     \
     template hipError_t launch_potrf_trail<T>(const PotrfTask*, int, int, int, hipStream_t);      \
-    template hipError_t launch_trsm<T>(const TrsmTask*, int, hipStream_t);                        \
+    template hipError_t launch_trsm<T>(const TrsmTask*, int, int, hipStream_t);                   \
     template hipError_t launch_getrf<T>(const GetrfTask*, int, int, int, int*, hipStream_t);      \
     template hipError_t launch_schur<T>(const SchurTile*, int, const SchurContrib*, hipStream_t); \
     template hipError_t launch_rrqr<T>(const RrqrTask*, int, const RrqrSeg*, int*, double, int, hipStream_t); \

@@ -596,6 +596,13 @@ static std::vector<int> group_by_first(int n, int nkeys, KeyOf key, std::vector<
     return order;
 }
 
+// Triangles of at most TRSM_MFMA_MIN_N rows first (the vector kernel: they have no trailing
+// update to speak of), longer ones after (FP32: the matrix-core kernel); returns the split.
+static int split_trsm(std::vector<TrsmTask>& v) {
+    auto it = std::stable_partition(v.begin(), v.end(), [](const TrsmTask& t) { return t.n <= TRSM_MFMA_MIN_N; });
+    return (int)(it - v.begin());
+}
+
 static inline double potrf_flops(double n) { return n * n * n / 3.0; }
 static inline double getrf_flops(double n) { return 2.0 * n * n * n / 3.0; }
 
@@ -1236,11 +1243,11 @@ void Tree::factorize_impl() {
                           what);
             }
         };
-        auto run_trsm = [&](void* dt, const std::vector<TrsmTask>& v, size_t off, const char* what) {
+        auto run_trsm = [&](void* dt, const std::vector<TrsmTask>& v, size_t off, int split, const char* what) {
             if (v.empty()) return;
             Work w = work_trsm(v, esz);
             hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
-                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), st); }),
+                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), split, st); }),
                       what);
         };
         auto run_getrf = [&](void* dt, const std::vector<GetrfTask>& v, size_t off, const char* what) {
@@ -1277,6 +1284,7 @@ void Tree::factorize_impl() {
                 if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
             lds_first(getrfE);
             Packer pe;
+            const int sp_te = split_trsm(trsmE);
             const size_t o_pe = pe.add(potrfE), o_te = pe.add(trsmE), o_ti = pe.add(tiles), o_co = pe.add(contribs);
             const size_t o_ie = pe.add(inv_init), o_ge = pe.add(getrfE);
             void* de = upload_tasks(pe);
@@ -1292,7 +1300,7 @@ void Tree::factorize_impl() {
             }
             run_potrf(de, potrfE, o_pe, "potrf elim");
             run_getrf(de, getrfE, o_ge, "getrf elim");
-            run_trsm(de, trsmE, o_te, "trsm elim");
+            run_trsm(de, trsmE, o_te, sp_te, "trsm elim");
             if (!tiles.empty()) {
                 Work w = work_schur(tiles, contribs, esz);
                 hip_check(timed(prof_, KF_SCHUR, st, w.bytes, w.flops,
@@ -1575,6 +1583,7 @@ void Tree::factorize_impl() {
         for (PotrfTask& t : potrfS)
             if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
         Packer pk;
+        const int sp_ta = split_trsm(trsmA), sp_tb = split_trsm(trsmB);
         size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
         size_t o_ii = pk.add(inv_init);
         lds_first(getrfS);
@@ -1597,8 +1606,8 @@ void Tree::factorize_impl() {
         }
         run_potrf(dt, potrfS, o_ps, "potrf scale");
         run_getrf(dt, getrfS, o_gs, "getrf scale");
-        run_trsm(dt, trsmA, o_ta, "trsm scale A");
-        run_trsm(dt, trsmB, o_tb, "trsm scale B");
+        run_trsm(dt, trsmA, o_ta, sp_ta, "trsm scale A");
+        run_trsm(dt, trsmB, o_tb, sp_tb, "trsm scale B");
         // stepped-path launch slots, for the post-hoc work attribution below
         std::vector<int> slot_init(nwaves, -1), slot_finish(nwaves, -1);
         std::vector<int> slot_step(nwaves, -1);
@@ -1903,6 +1912,7 @@ void Tree::factorize_impl() {
         }
         if (!wy.empty()) {
             Packer pl;
+            const int sp_wy = split_trsm(wy_trsm);
             size_t o_w = pl.add(wy), o_t = pl.add(wy_tiles), o_s = pl.add(wy_trsm);
             void* dl = upload_tasks(pl);
             double lb = 0, lf = 0;
@@ -1918,7 +1928,9 @@ void Tree::factorize_impl() {
                       "compact WY");
             Work w = work_trsm(wy_trsm, sizeof(T));
             hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
-                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dl, o_s), (int)wy_trsm.size(), st); }),
+                            [&] {
+                                return launch_trsm<T>(Packer::at<TrsmTask>(dl, o_s), (int)wy_trsm.size(), sp_wy, st);
+                            }),
                       "compact WY inverse");
         }
         lst.t_merge = wall() - t_lvl2;

@@ -96,7 +96,12 @@ def test_fp32_factor_solve_close_to_oracle(n, d, nl, tol):
     g.factorize()
     b = np.random.default_rng(2).standard_normal(A.n)
     xo, xg = o.solve(b), g.solve(b)
-    assert np.linalg.norm(xo - xg) <= 1e-4 * np.linalg.norm(xo)
+    # exact factorization: FP32 rounding only (observed 2e-6 .. 5e-6). At tol 1e-2, FP32
+    # rounding moves a few near-tie |R_ii| / |R_00| across tol (a few ranks out of
+    # thousands), each changing the preconditioner by O(tol) on its cluster (observed
+    # 6e-7 .. 8e-4 with the vector or the matrix-core kernels)
+    bound = 1e-4 if tol == 0.0 else 2e-3
+    assert np.linalg.norm(xo - xg) <= bound * np.linalg.norm(xo)
     # kept ranks: same total within 1% (FP32 rounding can move a singular value across tol)
     ro = o.get_log()[:, 6].sum()
     rg = g.level_stats()[:, 6].sum()
@@ -293,3 +298,33 @@ def test_apply_graph_replays_match_eager(monkeypatch):
             _ = (d * 2.0).sum().item()  # eager work between replays
         x, info = S.cg(A, B, g, iters=100, tol=1e-12)
         assert np.array_equal(x, x_ref) and np.array_equal(info["iters"], info_ref["iters"])
+
+
+@pytest.mark.parametrize("n,d,nl", [(16, 3, 7), (128, 2, 7), (15, 3, 6)])
+def test_fp32_matrix_core_updates_match_the_vector_path(n, d, nl, monkeypatch):
+    """FP32 trsm and Schur updates on the matrix cores (v_mfma_f32_32x32x2f32) against the
+    ordered-FMA vector kernels: same kept ranks (up to FP32 near-ties), solves close to each
+    other and to the FP64 oracle, PCG to 1e-12 in the same iteration counts."""
+    say(f"case fp32 mfma vs valu {d}D n={n}")
+    A, X = neglapl(n, d), linspace_nd(n, d)
+    b = np.random.default_rng(23).standard_normal(A.n)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("SPAND_TRSM_MFMA", mode)
+        monkeypatch.setenv("SPAND_SCHUR_MFMA", mode)
+        g = gpu_tree(A, X, nl, 1e-2, prec=S.FP32)
+        g.factorize()
+        x, info = S.cg(A, b, g, iters=100, tol=1e-12)
+        out[mode] = (g.solve(b), g.level_stats()[:, 6].sum(), info["iters"][0], x)
+    rv, rm = out["0"][1], out["1"][1]
+    assert abs(rv - rm) <= 0.01 * rv, (rv, rm)
+    xv, xm = out["0"][0], out["1"][0]
+    # the two differ by FP32 rounding and, through it, by a few near-tie ranks (see the
+    # tolerance of test_fp32_factor_solve_close_to_oracle)
+    assert np.linalg.norm(xv - xm) <= 2e-3 * np.linalg.norm(xv)
+    assert abs(int(out["0"][2]) - int(out["1"][2])) <= 1
+    np.testing.assert_allclose(out["1"][3], out["0"][3], rtol=0, atol=1e-10 * np.abs(out["0"][3]).max())
+    o, _ = oracle.setup_tree(A, X, nl, 1e-2)
+    o.factorize()
+    xo = o.solve(b)
+    assert np.linalg.norm(xo - xm) <= 2e-3 * np.linalg.norm(xo)
