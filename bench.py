@@ -51,11 +51,22 @@ FLOP_BOUND = {"potrf", "trsm", "schur"}
 # HBM traffic per dispatch of the dominant kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE on this workload and scaled by the same counters on a plain
 # 256 MiB device copy (FETCH_SIZE reads 0.5x of the copied bytes, WRITE_SIZE 1.0x).
-# Keyed by (family, factor dtype, grid edge n, levels); other runs report traffic null.
+# Keyed by (family, factor dtype, grid edge n, levels) and valid only for the kernel source
+# it was measured on (SHA-256 prefix of csrc/kernels_factor.hip): after any change to that
+# file the run reports traffic null until the counters are measured again
+# (profiles/rrqr_step_laplace64_hbm_traffic.md gives the commands).
 TRAFFIC_MEASURED = {
     ("rrqr_step", "f32", 64, 12): {"fetch_kib": 3647.15, "write_kib": 4847.71, "fetch_factor": 2.0,
-                                   "write_factor": 1.0},
+                                   "write_factor": 1.0, "src_sha16": ""},
 }
+
+
+def kernel_source_sha16():
+    import hashlib
+
+    path = os.path.join(ROOT, "spand_public_amd", "csrc", "kernels_factor.hip")
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
 def log(*a):
@@ -125,7 +136,7 @@ def roofline(tprof, wprof, steps, dtype, n=64, nlevels=12):
         peak, unit, bound = HBM_PEAK_GBS, "GB/s", "hbm"
     tm = TRAFFIC_MEASURED.get((fam, dtype, n, nlevels))
     traffic = None
-    if tm:
+    if tm and tm["src_sha16"] == kernel_source_sha16():
         traffic = round((tm["fetch_kib"] * tm["fetch_factor"] + tm["write_kib"] * tm["write_factor"]) * 1024.0)
     out = {
         "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
@@ -162,7 +173,30 @@ def cpu_baseline(A, X, B, nl):
             "sample": f"one oracle factorization of the same {A.n}-unknown problem + PCG on RHS 1 of {NRHS} "
                       f"({it} iterations, residual {res:.1e}); value = factor + {NRHS} x PCG",
             "factor_s": round(t_fact, 3), "pcg_one_rhs_s": round(t_cg, 3), "iters": int(it),
-            "host": platform.processor() or platform.machine()}
+            "host": host_info()}
+
+
+def host_info():
+    """CPU model, clock, cores and the dense arithmetic of the CPU baseline."""
+    model, mhz = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if model is None and line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                elif mhz is None and line.startswith("cpu MHz"):
+                    mhz = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu": model or platform.processor() or platform.machine(), "cpu_mhz": mhz,
+            "cores_available": avail, "cores_used": 1,
+            "dense_backend": "none: the oracle's own unblocked C++ loops (no BLAS or LAPACK), "
+                             "g++ -O3 -march=x86-64-v3 -ffp-contract=off, one thread; the reference "
+                             "links sequential MKL or OpenBLAS, whose blocked kernels would be faster"}
 
 
 def run_differentiable(args):
@@ -283,6 +317,7 @@ def main():
         fi = t.factor_info()
         results[name] = {"s": dt, "iters": its, "relres": res, "wprof": wprof, "tprof": tprof,
                          "factor_s": fi["seconds"]}
+        results[name]["tune"] = None
         log(f"{name}: {dt * 1e3:.2f} ms/step (last factorize {fi['seconds'] * 1e3:.2f} ms), PCG iterations {its}, "
             f"true residual {res:.2e}, partition+assemble {t_setup:.2f} s")
         nw = max(args.warmup, 1)
@@ -290,11 +325,19 @@ def main():
             log(f"   {fam:12s} {k['launches'] / nw:9.1f} launches/step {k['ms'] / nw:9.3f} ms/step "
                 f"{k['bytes'] / max(k['ms'], 1e-9) / 1e6:9.1f} GB/s {k['flops'] / max(k['ms'], 1e-9) / 1e9:9.3f} TFLOP/s"
                 "   (warmup, every launch timed)")
+        tune = t.level_tune()
+        log("   level  rows_med  cols_med  elim_med | reg  small_elems  step_waves  trsm_cw  (launch configuration)")
+        for l, r in enumerate(tune):
+            log(f"   {l:5d} {r['rows_med']:9d} {r['cols_med']:9d} {r['elim_med']:9d} | {r['rrqr_reg']:3d} "
+                f"{r['small_elems']:12d} {r['step_waves']:11d} {r['trsm_cw']:8d}")
+        results_tune = [{k: r[k] for k in ("rows_med", "cols_med", "elim_med", "rrqr_reg", "small_elems",
+                                           "step_waves", "trsm_cw")} for r in tune]
         ls = t.level_stats()
         log("   level  plan_ms  device_ms  ranks_ms  merge_ms  solveplan_ms   (last factorization; host vs device)")
         for l in range(nl):
             log(f"   {l:5d} {ls[l, 10] * 1e3:8.2f} {ls[l, 11] * 1e3:10.2f} {ls[l, 12] * 1e3:9.2f} {ls[l, 13] * 1e3:9.2f}"
                 f" {ls[l, 18] * 1e3:13.2f}")
+        results[name]["tune"] = results_tune
         if not res <= KRYLOV_TOL:
             raise RuntimeError(f"{name}: true residual {res:.2e} above {KRYLOV_TOL}")
         del t
@@ -321,6 +364,7 @@ def main():
         "pcg_iters": head["iters"],
         "relres": head["relres"],
         "roofline": roofline(head["tprof"], head["wprof"], args.steps, "f32", n, nl),
+        "level_config": head["tune"],
     }
     if "f64" in results:
         r64 = results["f64"]

@@ -132,6 +132,18 @@ spand_status spand_get_level_stats(const spand_tree* t, double* out);
 spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long long* factor_bytes,
                                    long long* arena_bytes);
 
+/* No counterpart in the reference: the launch configuration of every level of the last
+ * factorization, chosen from the level's size histogram by a table measured on gfx950 (or
+ * set through the environment). 13 ints per level: the shape (pivots eliminated, median and
+ * largest pivot, clusters sparsified, median and largest Asn rows, median and largest Asn
+ * columns), then the configuration (register-path RRQR on, Asn elements up to which the
+ * one-workgroup RRQR runs, wavefronts per stepped slab (0: by slab count), trsm chunk policy,
+ * source: 0 table, 1 environment). No choice changes the arithmetic. */
+spand_status spand_get_level_tune(const spand_tree* t, int* out);
+/* The table's configuration (5 ints as above) for a level shape (8 ints as above) and a
+ * factor element size of elem_bytes (4 or 8). Host only. */
+spand_status spand_tune_choose(int elem_bytes, const int* shape, int* out);
+
 /* Kernel timing (the reference's timer log, include/util.h:366-401, at kernel granularity):
  * for every kernel family whose bit is set in family_mask (bit f = family f; 0 turns timing
  * off, the default), each launch of the tree (factorization, preconditioner apply) and of

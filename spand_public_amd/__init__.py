@@ -188,6 +188,17 @@ class Tree:
         _check(self._h, self._L.spand_get_level_stats(self._h, out))
         return out.reshape(self.nlevels, 20)
 
+    TUNE_FIELDS = ("n_elim", "elim_med", "elim_max", "n_spars", "rows_med", "rows_max", "cols_med", "cols_max",
+                   "rrqr_reg", "small_elems", "step_waves", "trsm_cw", "source")
+
+    def level_tune(self) -> list:
+        """Per level of the last factorization: its shape and the launch configuration chosen
+        for it (spand_get_level_tune), as dicts."""
+        nf = len(self.TUNE_FIELDS)
+        out = np.zeros(nf * self.nlevels, dtype=np.int32)
+        _check(self._h, self._L.spand_get_level_tune(self._h, out))
+        return [dict(zip(self.TUNE_FIELDS, (int(v) for v in row))) for row in out.reshape(self.nlevels, nf)]
+
     def factor_info(self):
         s, fb, ab = C.c_double(), C.c_longlong(), C.c_longlong()
         _check(self._h, self._L.spand_get_factor_info(self._h, C.byref(s), C.byref(fb), C.byref(ab)))
@@ -305,6 +316,15 @@ def cg_device(A: "Matrix", precond: Tree, d_b: int, d_x: int, nrhs: int, ld: int
     if status not in (0, 7):
         _check(precond._h, status)
     return its, {"status": status, **{f: getattr(st, f) for f, _ in KrylovStats._fields_}}
+
+
+def tune_choose(elem_bytes: int, shape: dict) -> dict:
+    """The measured table's launch configuration for a level shape (host only)."""
+    keys = Tree.TUNE_FIELDS[:8]
+    sh = np.array([int(shape.get(k, 0)) for k in keys], dtype=np.int32)
+    out = np.zeros(5, dtype=np.int32)
+    _check(None, lib().spand_tune_choose(int(elem_bytes), sh, out))
+    return dict(zip(Tree.TUNE_FIELDS[8:], (int(v) for v in out)))
 
 
 def gmres_device(A: "Matrix", precond: Tree, d_b: int, d_x: int, nrhs: int, ld: int, iters: int = 100,

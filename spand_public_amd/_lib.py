@@ -57,6 +57,8 @@ SIGNATURES = {
     "spand_get_clusters": (C.c_int, [vp, vp, C.POINTER(C.c_int)]),
     "spand_get_dof2ID": (C.c_int, [vp, i32p]),
     "spand_get_level_stats": (C.c_int, [vp, f64p]),
+    "spand_get_level_tune": (C.c_int, [vp, i32p]),
+    "spand_tune_choose": (C.c_int, [C.c_int, i32p, i32p]),
     "spand_get_factor_info": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),
     "spand_set_profiling": (C.c_int, [vp, C.c_ulonglong]),

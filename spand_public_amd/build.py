@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspand.so")
 
 SOURCES = ["kernels_factor.hip", "kernels_solve.hip", "partition.cpp", "tree.cpp", "krylov.cpp", "profile.cpp",
-           "scratch.cpp", "capi.cpp"]
+           "scratch.cpp", "tune.cpp", "capi.cpp"]
 ARCH = "gfx950"
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

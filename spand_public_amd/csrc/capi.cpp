@@ -270,6 +270,47 @@ spand_status spand_op_plu(const spand_tree* t, int i, double* U, int* perm) {
     return guarded(const_cast<spand_tree*>(t), [&] { t->t->op_plu(i, U, perm); });
 }
 
+spand_status spand_get_level_tune(const spand_tree* t, int* out) {
+    CHECK_T(t);
+    return guarded(const_cast<spand_tree*>(t), [&] {
+        if (!out) throw spand::Error(spand::ERR_ARG, "get_level_tune: null output");
+        const auto& tu = t->t->tunes();
+        const auto& sh = t->t->shapes();
+        const int nl = t->t->nlevels();
+        for (int l = 0; l < nl; l++) {
+            int* o = out + (size_t)l * (spand::TUNE_SHAPE_INTS + spand::TUNE_INTS);
+            spand::LevelShape s = l < (int)sh.size() ? sh[l] : spand::LevelShape();
+            spand::LevelTune u = l < (int)tu.size() ? tu[l] : spand::LevelTune();
+            const int v[] = {s.n_elim, s.elim_med, s.elim_max, s.n_spars, s.rows_med, s.rows_max, s.cols_med,
+                             s.cols_max, u.rrqr_reg, u.small_elems, u.step_waves, u.trsm_cw, u.source};
+            for (int k = 0; k < spand::TUNE_SHAPE_INTS + spand::TUNE_INTS; k++) o[k] = v[k];
+        }
+    });
+}
+
+spand_status spand_tune_choose(int elem_bytes, const int* shape, int* out) {
+    if (!shape || !out || (elem_bytes != 4 && elem_bytes != 8)) {
+        g_err = "tune_choose: bad arguments";
+        return SPAND_ERR_ARG;
+    }
+    spand::LevelShape s;
+    s.n_elim = shape[0];
+    s.elim_med = shape[1];
+    s.elim_max = shape[2];
+    s.n_spars = shape[3];
+    s.rows_med = shape[4];
+    s.rows_max = shape[5];
+    s.cols_med = shape[6];
+    s.cols_max = shape[7];
+    const spand::LevelTune u = spand::choose_tune((size_t)elem_bytes, s);
+    out[0] = u.rrqr_reg;
+    out[1] = u.small_elems;
+    out[2] = u.step_waves;
+    out[3] = u.trsm_cw;
+    out[4] = u.source;
+    return SPAND_OK;
+}
+
 spand_status spand_matrix_create(int N, const int* colptr, const int* rowval, const double* vals, spand_matrix** out) {
     if (!out || N <= 0 || !colptr || !rowval || !vals) {
         g_err = "matrix_create: bad arguments";

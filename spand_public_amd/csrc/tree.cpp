@@ -15,6 +15,7 @@
 #include "tree.hpp"
 
 #include "scratch.hpp"
+#include "tune.hpp"
 
 #include <algorithm>
 #include <array>
@@ -610,10 +611,10 @@ static inline double getrf_flops(double n) { return 2.0 * n * n * n / 3.0; }
 // block n x m, ld n) or 64 rows (right solve, block m x n, ld m); 32 / 16 for n >= 128 / 256.
 // A left solve may read its right-hand side rows permuted from another block (src, perm).
 static void push_trsm(std::vector<TrsmTask>& v, const void* L, void* B, int n, int m, int side, size_t esz,
-                      const void* src = nullptr, const int* perm = nullptr) {
+                      const void* src = nullptr, const int* perm = nullptr, int cw_policy = 0) {
     if (n == 0 || m == 0) return;
     // long triangles walk many row blocks per chunk: narrower chunks give more workgroups
-    const int cw = n >= 256 ? 16 : (n >= 128 ? 32 : 64);
+    const int cw = trsm_chunk_width(cw_policy, n);
     for (int c0 = 0; c0 < m; c0 += cw) {
         const int w = std::min(cw, m - c0);
         const size_t off = (side == 0 ? (size_t)c0 * n : (size_t)c0) * esz;
@@ -973,9 +974,8 @@ void Tree::factorize_impl() {
     // its largest resident task
     const long long lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
-    const long long big_min_elems = env_or("SPAND_RRQR_STEPPED_MIN_ELEMS", 10000);
-    // clusters whose Asn fits a register-resident kernel form take it (SPAND_RRQR_REG=0: never)
-    const bool reg_path = env_or("SPAND_RRQR_REG", 1) != 0;
+    tunes_.assign(nlevels_, LevelTune());
+    shapes_.assign(nlevels_, LevelShape());
     const bool gen = symm_ == 2;
     fsymm_ = symm_;
 
@@ -1007,6 +1007,27 @@ void Tree::factorize_impl() {
                 fill_elems += (long long)size(n1) * size(n2);
             }
             return e;
+        };
+        // the level's launch configuration: trsm chunks from its pivot sizes now, the
+        // sparsification knobs once the sparsified clusters' Asn shapes are known (below)
+        LevelShape shape;
+        {
+            std::vector<int> piv;
+            for (int c : bot)
+                if (cls_[c].level == ilvl_) piv.push_back(cls_[c].size);
+            shape.n_elim = (int)piv.size();
+            if (!piv.empty()) {
+                std::sort(piv.begin(), piv.end());
+                shape.elim_med = piv[piv.size() / 2];
+                shape.elim_max = piv.back();
+            }
+        }
+        LevelTune tune = choose_tune(sizeof(T), shape);
+        apply_tune_env(tune);
+        const int cwp = tune.trsm_cw;
+        auto push_trsm = [cwp](std::vector<TrsmTask>& v, const void* L, void* B, int n, int m, int side, size_t esz,
+                               const void* src = nullptr, const int* perm = nullptr) {
+            spand::push_trsm(v, L, B, n, m, side, esz, src, perm, cwp);
         };
         for (int c : bot) {
             if (cls_[c].level != ilvl_) continue;
@@ -1453,6 +1474,33 @@ void Tree::factorize_impl() {
             where.assign(plist.size(), {0, 0});
             std::vector<std::vector<std::pair<int, int>>> regp(nwaves);  // (variant, plist position)
             std::vector<RrqrTask> ptask(plist.size());
+            {  // the sparsification knobs from the level's Asn shapes
+                std::vector<int> rows, cols;
+                for (int c : plist) {
+                    const HCluster& s = cls_[c];
+                    int nc = 0;
+                    for (int e : s.in) nc += size(edges_[e].n1);
+                    for (size_t k = 1; k < s.out.size(); k++) nc += size(edges_[s.out[k]].n2);
+                    rows.push_back(s.size);
+                    cols.push_back(nc);
+                }
+                shape.n_spars = (int)plist.size();
+                if (!rows.empty()) {
+                    std::sort(rows.begin(), rows.end());
+                    std::sort(cols.begin(), cols.end());
+                    shape.rows_med = rows[rows.size() / 2];
+                    shape.rows_max = rows.back();
+                    shape.cols_med = cols[cols.size() / 2];
+                    shape.cols_max = cols.back();
+                }
+                const LevelTune t = choose_tune(sizeof(T), shape);
+                tune.rrqr_reg = t.rrqr_reg;
+                tune.small_elems = t.small_elems;
+                tune.step_waves = t.step_waves;
+                apply_tune_env(tune);
+            }
+            const bool reg_path = tune.rrqr_reg != 0;
+            const long long big_min_elems = tune.small_elems;
             widx.assign(plist.size(), 0);
             for (size_t p = 0; p < plist.size(); p++) {
                 const int c = plist[p];
@@ -1536,7 +1584,9 @@ void Tree::factorize_impl() {
         // against 125 ms per step), although a flag handshake between two workgroups costs
         // about 1 us one way: the step body itself is slower in the 1024-thread persistent form.
         const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
-        const int step_waves = (int)env_or("SPAND_RRQR_STEP_WAVES", 0);  // 4, 8, 16; 0: by slab count
+        const int step_waves = tune.step_waves;  // 4, 8, 16; 0: by slab count
+        tunes_[ilvl_] = tune;
+        shapes_[ilvl_] = shape;
         for (int w = 0; w < nwaves; w++) {
             const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
             int wmaxcols = 0;

@@ -17,6 +17,7 @@
 
 #include "profile.hpp"
 #include "tasks.hpp"
+#include "tune.hpp"
 
 namespace spand {
 
@@ -189,6 +190,10 @@ class Tree {
     const std::vector<HCluster>& clusters() const { return cls_; }
     const std::vector<int>& bottom(int l) const { return bottoms_[l]; }
     const std::vector<LevelStats>& stats() const { return stats_; }
+    // the launch configuration of each level of the last factorization and the level shape
+    // it was chosen from
+    const std::vector<LevelTune>& tunes() const { return tunes_; }
+    const std::vector<LevelShape>& shapes() const { return shapes_; }
     const std::vector<OpRec>& ops() const { return ops_; }
     double factor_seconds() const { return t_factor_; }
     size_t factor_bytes() const { return factor_bytes_; }
@@ -332,6 +337,8 @@ class Tree {
     std::vector<LevelSolve> solve_plan_;
     std::vector<OpRec> ops_;
     std::vector<LevelStats> stats_;
+    std::vector<LevelTune> tunes_;
+    std::vector<LevelShape> shapes_;
     double t_factor_ = 0;
     size_t factor_bytes_ = 0;
 };

@@ -39,12 +39,15 @@ def gpu_tree(A, X, nl, tol, prec=S.FP64, skip=0):
 
 @pytest.fixture
 def stepped_rrqr(request, monkeypatch):
-    """Which RRQR path the clusters take: the library's own size split ("default": register
-    path, one workgroup per cluster, for the clusters it holds), the split without the
-    register path ("noreg": LDS one-workgroup path and stepped path), every cluster on the
+    """Which RRQR path the clusters take: the library's own choice per level ("default"), the
+    register path (one workgroup per cluster, columns in registers) wherever it holds the
+    cluster ("reg"), the split without the register path ("noreg": LDS one-workgroup path
+    and stepped path), every cluster on the
     multi-workgroup path run as one persistent launch per wave ("wave"), or as one launch per
     pivot step ("steps"; "steps8" / "steps4": 8 or 4 wavefronts per 64-column slab)."""
-    if request.param != "default":
+    if request.param == "reg":  # the register path wherever a kernel form holds the cluster
+        monkeypatch.setenv("SPAND_RRQR_REG", "1")
+    elif request.param != "default":
         monkeypatch.setenv("SPAND_RRQR_REG", "0")
     if request.param in ("wave", "steps", "steps8", "steps4"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
@@ -55,7 +58,8 @@ def stepped_rrqr(request, monkeypatch):
     return request.param
 
 
-@pytest.mark.parametrize("stepped_rrqr", ["default", "noreg", "wave", "steps", "steps8", "steps4"], indirect=True)
+@pytest.mark.parametrize("stepped_rrqr", ["default", "reg", "noreg", "wave", "steps", "steps8", "steps4"],
+                         indirect=True)
 @pytest.mark.parametrize("n,d,nl,tol", CASES)
 def test_fp64_factor_matches_oracle(n, d, nl, tol, stepped_rrqr):
     say(f"case fp64 {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")
@@ -328,3 +332,20 @@ def test_fp32_matrix_core_updates_match_the_vector_path(n, d, nl, monkeypatch):
     o.factorize()
     xo = o.solve(b)
     assert np.linalg.norm(xo - xm) <= 2e-3 * np.linalg.norm(xo)
+
+
+def test_level_tune_reports_the_table_choice(monkeypatch):
+    say("case level tune")
+    A, X = neglapl(24, 3), linspace_nd(24, 3)
+    g = gpu_tree(A, X, 8, 1e-2)
+    g.factorize()
+    lt = g.level_tune()
+    assert len(lt) == 8
+    for row in lt:
+        shape = {k: row[k] for k in S.Tree.TUNE_FIELDS[:8]}
+        want = S.tune_choose(8, shape)
+        assert {k: row[k] for k in want} == want
+    assert lt[0]["n_elim"] > 0 and lt[0]["n_spars"] > 0 and lt[0]["rows_med"] > 0
+    monkeypatch.setenv("SPAND_RRQR_STEP_WAVES", "4")
+    g.factorize()
+    assert all(r["step_waves"] == 4 and r["source"] == 1 for r in g.level_tune())

@@ -40,12 +40,15 @@ def gen_gpu_tree(A, X, nl, tol, prec=S.FP64):
 
 @pytest.fixture
 def stepped_rrqr(request, monkeypatch):
-    """Which RRQR path the clusters take: the library's own size split ("default": register
-    path, one workgroup per cluster, for the clusters it holds), the split without the
-    register path ("noreg": LDS one-workgroup path and stepped path), every cluster on the
+    """Which RRQR path the clusters take: the library's own choice per level ("default"), the
+    register path (one workgroup per cluster, columns in registers) wherever it holds the
+    cluster ("reg"), the split without the register path ("noreg": LDS one-workgroup path
+    and stepped path), every cluster on the
     multi-workgroup path run as one persistent launch per wave ("wave"), or as one launch per
     pivot step ("steps"; "steps8" / "steps4": 8 or 4 wavefronts per 64-column slab)."""
-    if request.param != "default":
+    if request.param == "reg":  # the register path wherever a kernel form holds the cluster
+        monkeypatch.setenv("SPAND_RRQR_REG", "1")
+    elif request.param != "default":
         monkeypatch.setenv("SPAND_RRQR_REG", "0")
     if request.param in ("wave", "steps", "steps8", "steps4"):
         monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
@@ -61,7 +64,8 @@ GEN_CASES = [("unsym", 5, 2, 3, 0.0), ("unsym", 20, 2, 5, 1e-2), ("unsym", 10, 3
              ("unsym", 15, 3, 6, 0.0), ("cd", 16, 3, 7, 1e-2), ("unsym", 40, 2, 3, 1e-2)]
 
 
-@pytest.mark.parametrize("stepped_rrqr", ["default", "noreg", "wave", "steps", "steps8", "steps4"], indirect=True)
+@pytest.mark.parametrize("stepped_rrqr", ["default", "reg", "noreg", "wave", "steps", "steps8", "steps4"],
+                         indirect=True)
 @pytest.mark.parametrize("kind,n,d,nl,tol", GEN_CASES)
 def test_gen_fp64_factor_matches_oracle(kind, n, d, nl, tol, stepped_rrqr):
     say(f"case gen fp64 {kind} {d}D n={n} levels={nl} tol={tol} rrqr={stepped_rrqr}")
