@@ -56,8 +56,10 @@ FLOP_BOUND = {"potrf", "trsm", "schur"}
 # file the run reports traffic null until the counters are measured again
 # (profiles/rrqr_step_laplace64_hbm_traffic.md gives the commands).
 TRAFFIC_MEASURED = {
-    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 3647.15, "write_kib": 4847.71, "fetch_factor": 2.0,
-                                   "write_factor": 1.0, "src_sha16": ""},
+    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 3167.83, "write_kib": 4397.60, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "b935209f168a23ea"},
+    ("rrqr_step", "f64", 64, 12): {"fetch_kib": 5867.05, "write_kib": 9149.93, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "b935209f168a23ea"},
 }
 
 
@@ -268,6 +270,8 @@ def main():
     ap.add_argument("--nlevels", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-factor comparison")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="only the FP64-factor steps (for profiling its kernels); the JSON line carries them")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed steps")
     ap.add_argument("--differentiable", action="store_true",
                     help="only the differentiable-solve workload (2D 512^2 variable kappa, 32 RHS)")
@@ -310,6 +314,8 @@ def main():
     for name, prec in (("f32", S.FP32), ("f64", S.FP64)):
         if name == "f64" and args.no_fp64:
             continue
+        if name == "f32" and args.fp64_only:
+            continue
         t, t_setup = make_tree(S, A, X, nl, prec)
         dt, its, wprof, tprof = run_steps(S, torch, t, Am, d_b, d_x, N, args.steps, args.warmup, not args.no_profile)
         x = d_x.cpu().numpy().T
@@ -342,6 +348,13 @@ def main():
             raise RuntimeError(f"{name}: true residual {res:.2e} above {KRYLOV_TOL}")
         del t
 
+    if args.fp64_only:
+        r64 = results["f64"]
+        print(json.dumps({"metric": METRIC + " (FP64 factor only)", "value": round(r64["s"], 6), "unit": "s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+                          "dtype": "f64", "data": "synthetic", "pcg_iters": r64["iters"], "relres": r64["relres"],
+                          "roofline": roofline(r64["tprof"], r64["wprof"], args.steps, "f64", n, nl)}), flush=True)
+        return
     head = results["f32"]
     out = {
         "metric": METRIC,
