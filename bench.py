@@ -344,6 +344,9 @@ def main():
             log(f"   {l:5d} {ls[l, 10] * 1e3:8.2f} {ls[l, 11] * 1e3:10.2f} {ls[l, 12] * 1e3:9.2f} {ls[l, 13] * 1e3:9.2f}"
                 f" {ls[l, 18] * 1e3:13.2f}")
         results[name]["tune"] = results_tune
+        results[name]["status"] = t.status_flags()
+        if results[name]["status"]["step_throttle_gave_up"]:
+            log(f"   {name}: the stepped-RRQR launch throttle gave up after a 2 s wait (steps launched unthrottled)")
         if not res <= KRYLOV_TOL:
             raise RuntimeError(f"{name}: true residual {res:.2e} above {KRYLOV_TOL}")
         del t
@@ -378,11 +381,13 @@ def main():
         "relres": head["relres"],
         "roofline": roofline(head["tprof"], head["wprof"], args.steps, "f32", n, nl),
         "level_config": head["tune"],
+        "status_flags": head["status"],
     }
     if "f64" in results:
         r64 = results["f64"]
         out["fp64_gpu"] = {"value": round(r64["s"], 6), "unit": "s", "pcg_iters": r64["iters"], "relres": r64["relres"],
-                           "roofline": roofline(r64["tprof"], r64["wprof"], args.steps, "f64", n, nl)}
+                           "roofline": roofline(r64["tprof"], r64["wprof"], args.steps, "f64", n, nl),
+                           "status_flags": r64["status"]}
         out["speedup_f32_vs_f64"] = round(r64["s"] / head["s"], 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, X, B, nl)

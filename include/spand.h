@@ -56,6 +56,9 @@ spand_status spand_set_use_geo(spand_tree* t, int use_geo);
 /* X is dim x N, column-major (the reference's Eigen::MatrixXd Xcoo). Copied. */
 spand_status spand_set_Xcoo(spand_tree* t, int dim, int N, const double* X);
 spand_status spand_set_use_sparsify(spand_tree* t, int use);
+/* Tree::set_monitor_flops(bool)   include/tree.h:147, src/tree.cpp:129: record the shape of
+ * every dense call of the next factorizations (spand_get_flops_log). Off by default. */
+spand_status spand_set_monitor_flops(spand_tree* t, int on);
 /* No reference counterpart: element type of the factorization (SPAND_F64 / SPAND_F32).
  * FP32 factors are corrected by the FP64 Krylov iteration. */
 spand_status spand_set_precision(spand_tree* t, int precision);
@@ -128,6 +131,22 @@ spand_status spand_get_trailing_coo(const spand_tree* t, long long* nnz, int* I,
  * 14 potrf flops, 15 trsm flops, 16 schur flops, 17 rrqr flops, 18 t_solve_plan (seconds,
  * host build of the level's preconditioner-apply plan), 19 reserved */
 spand_status spand_get_level_stats(const spand_tree* t, double* out);
+/* Tree::tprof_flops (include/tree.h:165, include/util.h:311-342; written by write_log_flops,
+ * src/tree.cpp:60-77): the shape of every dense call of the last factorization, recorded when
+ * spand_set_monitor_flops is on, in the reference's call order. 5 ints per record: level,
+ * kind (0 Cholesky pivot, 1 panel triangular solve, 2 gemm, 3 syrk, 4 geqp3, 5 LU pivot:
+ * the reference's pivot = 0 and 5, panel = 1, gemm = 2 and 3, rrqr = 4), then the
+ * reference's (rows, cols, inner): pivot (n, 0, 0), panel (neighbour size, pivot size, 0),
+ * gemm / syrk (target rows, target cols, pivot size), rrqr (Asn rows, Asn cols, 0). The
+ * reference's per-call wall time has no counterpart: the calls run batched per level
+ * (spand_kernel_profile gives the kernel families' times). Call with out == NULL to get the
+ * count first. */
+spand_status spand_get_flops_log(const spand_tree* t, long long* count, int* out);
+/* No counterpart in the reference: fallbacks the library took at run time, as bits of *flags.
+ * Bit 0: the stepped RRQR's launch throttle (the host keeps at most 12 step launches ahead of
+ * the device by polling a progress counter) gave up after a wait longer than two seconds;
+ * later steps were launched without it (same results, possibly more empty step launches). */
+spand_status spand_get_status_flags(const spand_tree* t, unsigned* flags);
 /* Wall seconds of the last spand_factorize, device bytes of the stored factor, of the arena. */
 spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long long* factor_bytes,
                                    long long* arena_bytes);

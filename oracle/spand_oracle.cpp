@@ -792,10 +792,8 @@ struct Tree {
         int mn = std::min(rows, cols);
         std::vector<int> jpvt(cols);
         std::vector<double> ht(mn);
-        if (rows > 0 && cols > 0) {
-            shape(4, rows, cols, 0);
-            geqp3(rows, cols, Asn->data(), rows, jpvt.data(), ht.data());
-        }
+        shape(4, rows, cols, 0);  // recorded for empty Asn too, as src/tree.cpp:1312
+        if (rows > 0 && cols > 0) geqp3(rows, cols, Asn->data(), rows, jpvt.data(), ht.data());
         std::vector<double> diag(mn);
         for (int i = 0; i < mn; i++) diag[i] = (*Asn)(i, i);
         int rank = choose_rank(diag.data(), mn, tol);

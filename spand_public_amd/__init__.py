@@ -77,6 +77,23 @@ class Tree:
     def set_use_sparsify(self, use: bool):
         _check(self._h, self._L.spand_set_use_sparsify(self._h, int(bool(use))))
 
+    def set_monitor_flops(self, on: bool = True):
+        """Tree::set_monitor_flops: record the shape of every dense call of the next
+        factorizations (flops_log)."""
+        _check(self._h, self._L.spand_set_monitor_flops(self._h, int(bool(on))))
+
+    FLOPS_KINDS = ("potf", "panel", "gemm", "syrk", "geqp3", "getf")
+
+    def flops_log(self) -> np.ndarray:
+        """The shape log of the last factorization (spand_get_flops_log): an (n, 5) int32 array
+        of (level, kind, rows, cols, inner) in the reference's call order; kind indexes
+        FLOPS_KINDS."""
+        n = C.c_longlong()
+        _check(self._h, self._L.spand_get_flops_log(self._h, C.byref(n), None))
+        out = np.zeros(5 * max(n.value, 1), dtype=np.int32)
+        _check(self._h, self._L.spand_get_flops_log(self._h, C.byref(n), out.ctypes.data_as(C.c_void_p)))
+        return out[: 5 * n.value].reshape(n.value, 5)
+
     def set_precision(self, precision: int):
         """FP64 (default) or FP32 factor storage and arithmetic (no reference counterpart)."""
         _check(self._h, self._L.spand_set_precision(self._h, int(precision)))
@@ -198,6 +215,12 @@ class Tree:
         out = np.zeros(nf * self.nlevels, dtype=np.int32)
         _check(self._h, self._L.spand_get_level_tune(self._h, out))
         return [dict(zip(self.TUNE_FIELDS, (int(v) for v in row))) for row in out.reshape(self.nlevels, nf)]
+
+    def status_flags(self) -> dict:
+        """Run-time fallbacks taken so far (spand_get_status_flags)."""
+        f = C.c_uint()
+        _check(self._h, self._L.spand_get_status_flags(self._h, C.byref(f)))
+        return {"step_throttle_gave_up": bool(f.value & 1)}
 
     def factor_info(self):
         s, fb, ab = C.c_double(), C.c_longlong(), C.c_longlong()

@@ -38,6 +38,7 @@ SIGNATURES = {
     "spand_set_use_geo": (C.c_int, [vp, C.c_int]),
     "spand_set_Xcoo": (C.c_int, [vp, C.c_int, C.c_int, f64p]),
     "spand_set_use_sparsify": (C.c_int, [vp, C.c_int]),
+    "spand_set_monitor_flops": (C.c_int, [vp, C.c_int]),
     "spand_set_precision": (C.c_int, [vp, C.c_int]),
     "spand_partition": (C.c_int, [vp, C.c_int, i32p, i32p, vp]),
     "spand_assemble": (C.c_int, [vp, C.c_int, i32p, i32p, f64p]),
@@ -58,6 +59,8 @@ SIGNATURES = {
     "spand_get_dof2ID": (C.c_int, [vp, i32p]),
     "spand_get_level_stats": (C.c_int, [vp, f64p]),
     "spand_get_level_tune": (C.c_int, [vp, i32p]),
+    "spand_get_flops_log": (C.c_int, [vp, C.POINTER(C.c_longlong), vp]),
+    "spand_get_status_flags": (C.c_int, [vp, C.POINTER(C.c_uint)]),
     "spand_tune_choose": (C.c_int, [C.c_int, i32p, i32p]),
     "spand_get_factor_info": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong)]),

@@ -93,6 +93,10 @@ spand_status spand_set_use_sparsify(spand_tree* t, int use) {
     CHECK_T(t);
     return guarded(t, [&] { t->t->set_use_sparsify(use != 0); });
 }
+spand_status spand_set_monitor_flops(spand_tree* t, int on) {
+    CHECK_T(t);
+    return guarded(t, [&] { t->t->set_monitor_flops(on != 0); });
+}
 spand_status spand_set_precision(spand_tree* t, int p) {
     CHECK_T(t);
     return guarded(t, [&] { t->t->set_precision(p); });
@@ -209,6 +213,26 @@ spand_status spand_get_level_stats(const spand_tree* t, double* out) {
         o[17] = s.flops_rrqr; o[18] = s.t_solve_plan; o[19] = 0;
     }
     return SPAND_OK;
+}
+spand_status spand_get_flops_log(const spand_tree* t, long long* count, int* out) {
+    CHECK_T(t);
+    return guarded(const_cast<spand_tree*>(t), [&] {
+        if (!count) throw spand::Error(spand::ERR_ARG, "get_flops_log: null count");
+        const auto& log = t->t->flops_log();
+        *count = (long long)log.size();
+        if (out)
+            for (size_t i = 0; i < log.size(); i++) {
+                int* o = out + 5 * i;
+                o[0] = log[i].level; o[1] = log[i].kind; o[2] = log[i].m; o[3] = log[i].n; o[4] = log[i].k;
+            }
+    });
+}
+spand_status spand_get_status_flags(const spand_tree* t, unsigned* flags) {
+    CHECK_T(t);
+    return guarded(const_cast<spand_tree*>(t), [&] {
+        if (!flags) throw spand::Error(spand::ERR_ARG, "get_status_flags: null output");
+        *flags = t->t->status_flags();
+    });
 }
 spand_status spand_get_factor_info(const spand_tree* t, double* seconds, long long* fb, long long* ab) {
     CHECK_T(t);

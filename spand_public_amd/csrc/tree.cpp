@@ -241,7 +241,7 @@ static inline void cpu_relax() {
 
 // Waits until the step launch numbered target has started on the device. A wait longer
 // than two seconds turns the throttle off for the life of the tree (every step is then
-// launched up front, as without signals).
+// launched up front, as without signals) and is reported by status_flags().
 void Tree::wait_signal(long long target) {
     if (!sig_throttle_) return;
     const volatile long long* p = h_sig_;
@@ -251,6 +251,7 @@ void Tree::wait_signal(long long target) {
         cpu_relax();
         if ((n & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
             sig_throttle_ = false;
+            throttle_gave_up_ = true;
             return;
         }
     }
@@ -975,6 +976,7 @@ void Tree::factorize_impl() {
     const long long lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
     const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
     tunes_.assign(nlevels_, LevelTune());
+    flops_log_.clear();
     shapes_.assign(nlevels_, LevelShape());
     const bool gen = symm_ == 2;
     fsymm_ = symm_;
@@ -1042,15 +1044,18 @@ void Tree::factorize_impl() {
                 int* pp = reinterpret_cast<int*>(arena_.alloc((size_t)std::max(1, n) * sizeof(int)));
                 getrfE.push_back({piv.A, Ut, pp, n, c});
                 lst.flops_potrf += getrf_flops(n);
+                log_shape(SK_GETF, n, 0, 0);
                 std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
                 for (int e : in) {  // A_sn := L^{-1} P^T A_sn, into a fresh block
                     const int m = size(edges_[e].n1);
+                    log_shape(SK_PANEL, m, n, 0);
                     void* nb = arena_.alloc((size_t)std::max(1, n * m) * sizeof(T));
                     push_trsm(trsmE, piv.A, nb, n, m, 0, sizeof(T), edges_[e].A, pp);
                     edges_[e].A = nb;
                     lst.flops_trsm += (double)n * n * m;
                 }
                 for (int e : out) {  // A_ns := A_ns U^{-1} = A_ns (U^T)^{-T}
+                    log_shape(SK_PANEL, size(edges_[e].n2), n, 0);
                     push_trsm(trsmE, Ut, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
                     lst.flops_trsm += (double)n * n * size(edges_[e].n2);
                 }
@@ -1061,6 +1066,7 @@ void Tree::factorize_impl() {
                         SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
                         pcs.push_back({tgt, sc});
                         lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
+                        log_shape(SK_GEMM, size(n1), size(n2), n);
                     }
                 // record ScalingPLUQ + GemmOut / GemmIn (src/tree.cpp:945-951)
                 ops_.push_back({OP_SCALING_PLUQ, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr, Ut, pp});
@@ -1100,14 +1106,17 @@ void Tree::factorize_impl() {
             }
             potrfE.push_back({piv.A, n, c, nullptr});
             lst.flops_potrf += potrf_flops(n);
+            log_shape(SK_POTF, n, 0, 0);
             std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
             for (int e : in) {
                 push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n1), 0, sizeof(T));
                 lst.flops_trsm += (double)n * n * size(edges_[e].n1);
+                log_shape(SK_PANEL, size(edges_[e].n1), n, 0);
             }
             for (int e : out) {
                 push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
                 lst.flops_trsm += (double)n * n * size(edges_[e].n2);
+                log_shape(SK_PANEL, size(edges_[e].n2), n, 0);
             }
             // schur_symmetric (src/tree.cpp:862-883) with gemm_edges (:748-793)
             for (int e1 : out)
@@ -1118,6 +1127,7 @@ void Tree::factorize_impl() {
                     SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), size(n2), 0, 1, 0};
                     pcs.push_back({tgt, sc});
                     lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
+                    log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
                 }
             for (int e1 : in)
                 for (int e2 : in) {
@@ -1127,6 +1137,7 @@ void Tree::factorize_impl() {
                     SchurContrib sc{edges_[e1].A, edges_[e2].A, n, n, n, 1, 0, 0};
                     pcs.push_back({tgt, sc});
                     lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
+                    log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
                 }
             for (int e1 : out)
                 for (int e2 : in) {
@@ -1135,6 +1146,7 @@ void Tree::factorize_impl() {
                     SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
                     pcs.push_back({tgt, sc});
                     lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
+                    log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
                 }
             // record ScalingLLT + GemmSymmOut/In (src/tree.cpp:909-910, 885-892)
             ops_.push_back({OP_SCALING_LLT, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr});
@@ -1342,6 +1354,13 @@ void Tree::factorize_impl() {
         std::vector<int> live;
         for (int c : bot)
             if (cls_[c].level > ilvl_) live.push_back(c);
+        if (do_sparsify_level && monitor_flops_)
+            for (int c : live) {  // scale_cluster: the pivot, then the panels of its in- and out-edges
+                const HCluster& s = cls_[c];
+                log_shape(gen ? SK_GETF : SK_POTF, s.size, 0, 0);
+                for (int e : s.in) log_shape(SK_PANEL, size(edges_[e].n1), s.size, 0);
+                for (size_t k = 1; k < s.out.size(); k++) log_shape(SK_PANEL, size(edges_[s.out[k]].n2), s.size, 0);
+            }
         if (do_sparsify_level && gen) {
             // scale_cluster, PLU branch (src/tree.cpp:831-849): an edge a -> b (block
             // size(b) x size(a)) receives A U_a^{-1} from a and L_b^{-1} P_b^T A from b, in
@@ -1828,6 +1847,14 @@ void Tree::factorize_impl() {
             const int s0 = cls_[c].size;
             lst.rank_before += s0;
             lst.nsparsified++;
+            if (monitor_flops_) {
+                // Asn as the sequential loop sees it: neighbours earlier in the list already
+                // hold their kept rank
+                int ncols = 0;
+                for (int e : cls_[c].in) ncols += size(edges_[e].n1);
+                for (size_t k = 1; k < cls_[c].out.size(); k++) ncols += size(edges_[cls_[c].out[k]].n2);
+                log_shape(SK_RRQR, s0, ncols, 0);
+            }
             if (r < s0) {
                 const RrqrTask* t = where[p].first == 1   ? &wbig[wave[p]][where[p].second].t
                                     : where[p].first == 2 ? &wreg[wave[p]][where[p].second]
@@ -1862,6 +1889,7 @@ void Tree::factorize_impl() {
                 ops_.push_back({OP_SPLIT, c, sid, r, s0 - r, 0, 0, ilvl_, nullptr, nullptr});
                 ops_.push_back({gen ? OP_SCALING_PLUQ : OP_SCALING_LLT, sid, -1, s0 - r, 0, s0 - r, s0 - r, ilvl_,
                                 nullptr, nullptr});
+                log_shape(gen ? SK_GETF : SK_POTF, s0 - r, 0, 0);  // eliminate_cluster(sibling)
                 // the sibling's pivot is the identity and is never stored (src/tree.cpp:1076-1099)
             }
             lst.rank_after += cls_[c].size;

@@ -195,6 +195,18 @@ class Tree {
     const std::vector<LevelTune>& tunes() const { return tunes_; }
     const std::vector<LevelShape>& shapes() const { return shapes_; }
     const std::vector<OpRec>& ops() const { return ops_; }
+    // Shape log of the dense calls of the last factorization (Tree::tprof_flops,
+    // include/tree.h:165, recorded under set_monitor_flops, src/tree.cpp:592-1312): one
+    // record per reference call, in the reference's call order.
+    enum ShapeKind { SK_POTF = 0, SK_PANEL = 1, SK_GEMM = 2, SK_SYRK = 3, SK_RRQR = 4, SK_GETF = 5 };
+    struct ShapeRec {
+        int level, kind, m, n, k;
+    };
+    void set_monitor_flops(bool on) { monitor_flops_ = on; }
+    // Run-time fallbacks taken so far (bit 0: the stepped-RRQR launch throttle stopped after
+    // a wait for the device longer than two seconds; later steps launched without it).
+    unsigned status_flags() const { return throttle_gave_up_ ? 1u : 0u; }
+    const std::vector<ShapeRec>& flops_log() const { return flops_log_; }
     double factor_seconds() const { return t_factor_; }
     size_t factor_bytes() const { return factor_bytes_; }
     size_t arena_bytes() const { return arena_.bytes_in_use(); }
@@ -257,6 +269,7 @@ class Tree {
     size_t h_sig_cap_ = 0;
     long long sig_seq_ = 0;
     bool sig_throttle_ = true;  // off after a wait that timed out
+    bool throttle_gave_up_ = false;  // a wait timed out (reported by status_flags)
 
     // options
     int nlevels_;
@@ -339,6 +352,11 @@ class Tree {
     std::vector<LevelStats> stats_;
     std::vector<LevelTune> tunes_;
     std::vector<LevelShape> shapes_;
+    bool monitor_flops_ = false;
+    std::vector<ShapeRec> flops_log_;
+    void log_shape(int kind, int m, int n, int k) {
+        if (monitor_flops_) flops_log_.push_back({ilvl_, kind, m, n, k});
+    }
     double t_factor_ = 0;
     size_t factor_bytes_ = 0;
 };

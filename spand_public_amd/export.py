@@ -5,6 +5,9 @@
   ``Tree.get_clusters``. This build records no R diagonals (the reference keeps them only with
   ``set_monitor_Rdiag(true)``, src/tree.cpp:1315), so the Rdiag column of ``write_stats`` is
   empty, as in the reference's default.
+* ``write_log_flops``: the shape log of the dense calls (src/tree.cpp:60-77) from
+  ``Tree.flops_log`` (recorded under ``Tree.set_monitor_flops``). The time column is 0: the
+  calls run batched per level on the GPU, so a per-call wall time does not exist.
 * ``sp_mmwrite`` / ``dense_mmwrite`` / ``dense_mmread``: include/mmio.hpp:221-327. Sparse
   entries are written column by column with 20 significant digits; dense entries with the
   stream's default 6 significant digits, as the reference writes them. The sparse reader is
@@ -16,7 +19,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["write_clustering", "write_merging", "write_clusters", "write_stats", "sp_mmwrite",
+__all__ = ["write_clustering", "write_merging", "write_clusters", "write_stats", "write_log_flops", "sp_mmwrite",
            "dense_mmwrite", "dense_mmread"]
 
 _PROPS = ("general", "symmetric", "hermitian", "skew_symmetric")
@@ -67,6 +70,32 @@ def write_stats(t, fn: str) -> None:
         f.write("id;size;rank;Rdiag\n")
         for r in cl:
             f.write(f"{r[0]};{r[4]};{r[5]};\n")
+
+
+# shape-log kinds (Tree.FLOPS_KINDS order) -> the reference's four lists
+_FLOPS_LIST = {0: "pivot", 5: "pivot", 1: "panel", 2: "gemm", 3: "gemm", 4: "rrqr"}
+
+
+def write_log_flops(log, fn: str, nlevels: int | None = None) -> None:
+    """src/tree.cpp:60-77: per level, the pivot, panel, gemm and rrqr records (each list in
+    call order) as "level;kind;rows;cols;inner;time". ``log`` is ``Tree.flops_log()`` (or an
+    (n, 5) array of (level, kind, rows, cols, inner)); ``nlevels`` defaults to one past the
+    last level in the log."""
+    log = np.asarray(log, dtype=np.int64).reshape(-1, 5)
+    nl = int(log[:, 0].max()) + 1 if nlevels is None and len(log) else int(nlevels or 0)
+    with open(fn, "w") as f:
+        for lvl in range(nl):
+            rows = log[log[:, 0] == lvl]
+            for name in ("pivot", "panel", "gemm", "rrqr"):
+                for _, kind, m, n, k in rows:
+                    if _FLOPS_LIST[int(kind)] != name:
+                        continue
+                    if name == "pivot":
+                        f.write(f"{lvl};pivot;{m};0;0;0\n")
+                    elif name == "gemm":
+                        f.write(f"{lvl};gemm;{m};{n};{k};0\n")
+                    else:
+                        f.write(f"{lvl};{name};{m};{n};0;0\n")
 
 
 def sp_mmwrite(fn: str, A, prop: str = "general") -> None:

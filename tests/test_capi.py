@@ -91,3 +91,21 @@ def test_device_values_need_an_assembly_and_the_pattern_size():
     # wrong entry count: argument error, nothing read
     assert L.spand_assemble_values_device(t._h, None, A.nnz + 1, None) == 1
     assert b"nnz" in L.spand_last_error(t._h)
+
+
+def test_shape_log_and_status_flags_without_gpu():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.spand_create(3, C.byref(h)) == 0
+    n = C.c_longlong(-1)
+    assert L.spand_get_flops_log(h, C.byref(n), None) == 0 and n.value == 0  # nothing recorded yet
+    assert L.spand_get_flops_log(h, None, None) == 1  # SPAND_ERR_ARG
+    assert b"count" in L.spand_last_error(h)
+    assert L.spand_set_monitor_flops(h, 1) == 0
+    f = C.c_uint(7)
+    assert L.spand_get_status_flags(h, C.byref(f)) == 0 and f.value == 0
+    assert L.spand_get_status_flags(h, None) == 1
+    L.spand_destroy(h)
+    t = S.Tree(3)
+    assert t.flops_log().shape == (0, 5)
+    assert t.status_flags() == {"step_throttle_gave_up": False}

@@ -1,0 +1,98 @@
+"""Shape log of the dense calls (Tree::tprof_flops, include/util.h:311-342, recorded under
+set_monitor_flops, src/tree.cpp:592-1312; written by write_log_flops, src/tree.cpp:60-77).
+
+The oracle's log is pinned by a case worked out by hand from the reference's call structure
+(a 1D chain: two leaves and a one-point separator) and, on larger trees, by invariants a
+dropped, duplicated or misplaced record breaks: per level the rrqr records count the
+sparsified clusters and their rows add up to the ranks before sparsification, every gemm's
+inner size is the pivot just eliminated, and the pivot sizes add up to the eliminated, scaled
+and split-off dofs. The GPU log is compared with the oracle's in test_gpu.py."""
+import numpy as np
+import pytest
+
+import oracle
+from spand_public_amd.export import write_log_flops
+from spand_public_amd.problems import convdiff, linspace_nd, neglapl
+
+POTF, PANEL, GEMM, SYRK, RRQR, GETF = range(6)
+
+
+def oracle_log(A, X, nl, tol, skip=0, symm="SPD"):
+    o, _ = oracle.setup_tree(A, X, nl, tol, skip=skip, symm=symm)
+    o.set_record_shapes(True)
+    o.factorize()
+    return o, o.shapes()
+
+
+@pytest.mark.parametrize("n", [9, 15])
+def test_chain_by_hand(n):
+    # geometric ND of a chain of n (odd) points, 2 levels: leaves of h = (n - 1) / 2 points and
+    # a one-point separator. Level 0: eliminate each leaf (potf h; its one out-edge to the
+    # separator: panel (1, h); schur_symmetric: syrk on the separator's pivot (1, 1, h)).
+    # Scale the separator (potf 1, no edges), sparsify it: Asn is 1 x 0, geqp3 is recorded,
+    # choose_rank keeps 0, so the whole cluster splits off and its sibling is eliminated
+    # (potf 1). Level 1: eliminate the emptied separator (potf 0).
+    h = (n - 1) // 2
+    _, s = oracle_log(neglapl(n, 1), linspace_nd(n, 1), 2, 0.0)
+    leaf = [[0, POTF, h, 0, 0], [0, PANEL, 1, h, 0], [0, SYRK, 1, 1, h]]
+    expect = leaf + leaf + [[0, POTF, 1, 0, 0], [0, RRQR, 1, 0, 0], [0, POTF, 1, 0, 0], [1, POTF, 0, 0, 0]]
+    assert s.tolist() == expect
+
+
+def test_chain_by_hand_gen():
+    # the PLU branch: getf, the panels of the in-edge (separator -> leaf) and of the out-edge,
+    # one gemm per (out, in) pair; no syrk in a nonsymmetric tree
+    n, h = 9, 4
+    _, s = oracle_log(neglapl(n, 1), linspace_nd(n, 1), 2, 0.0, symm="GEN")
+    leaf = [[0, GETF, h, 0, 0], [0, PANEL, 1, h, 0], [0, PANEL, 1, h, 0], [0, GEMM, 1, 1, h]]
+    expect = leaf + leaf + [[0, GETF, 1, 0, 0], [0, RRQR, 1, 0, 0], [0, GETF, 1, 0, 0], [1, GETF, 0, 0, 0]]
+    assert s.tolist() == expect
+
+
+@pytest.mark.parametrize("case", ["lapl3d_skip1", "lapl2d", "convdiff_gen"])
+def test_log_invariants(case):
+    if case == "lapl3d_skip1":
+        A, X, nl, tol, skip, symm = neglapl(8, 3), linspace_nd(8, 3), 5, 1e-2, 1, "SPD"
+    elif case == "lapl2d":
+        A, X, nl, tol, skip, symm = neglapl(24, 2), linspace_nd(24, 2), 6, 1e-3, 0, "SPD"
+    else:
+        A, X, nl, tol, skip, symm = convdiff(8, 3), linspace_nd(8, 3), 5, 1e-2, 0, "GEN"
+    o, s = oracle_log(A, X, nl, tol, skip, symm)
+    L = o.get_log()  # 0 dofs_nd, 1 dofs_left_nd, 2 dofs_left_elim, 3 dofs_left_spars, 5/6 ranks, 7 #sparsified
+    piv = GETF if symm == "GEN" else POTF
+    assert set(np.unique(s[:, 1])) <= {piv, PANEL, GEMM, SYRK, RRQR}
+    if symm == "GEN":
+        assert not (s[:, 1] == SYRK).any()
+    before = A.n
+    for lvl in range(nl):
+        r = s[s[:, 0] == lvl]
+        kinds = r[:, 1]
+        spars = lvl >= skip and lvl < nl - 1
+        assert (kinds == RRQR).sum() == (L[lvl, 7] if spars else 0)
+        assert r[kinds == RRQR, 2].sum() == (L[lvl, 5] if spars else 0)
+        # split-off siblings: the pivot right after an rrqr record
+        after = [i + 1 for i in np.flatnonzero(kinds == RRQR) if i + 1 < len(r) and kinds[i + 1] == piv]
+        assert r[after, 2].sum() == (L[lvl, 5] - L[lvl, 6] if spars else 0)
+        # eliminated + scaled + split off
+        want = (before - L[lvl, 2]) + ((L[lvl, 2] + L[lvl, 5] - L[lvl, 6]) if spars else 0)
+        assert r[kinds == piv, 2].sum() == want, (lvl, r[kinds == piv, 2].sum(), want)
+        # a gemm's inner size is the pivot eliminated just before it; a panel's second size too
+        last = None
+        for kind, m, n, k in r[:, 1:]:
+            if kind == piv:
+                last = m
+            elif kind in (GEMM, SYRK):
+                assert k == last
+            elif kind == PANEL:
+                assert n == last
+        before = L[lvl, 3] if spars else L[lvl, 2]
+
+
+def test_write_log_flops_format(tmp_path):
+    log = np.array([[0, POTF, 4, 0, 0], [0, PANEL, 1, 4, 0], [0, SYRK, 1, 1, 4], [0, RRQR, 1, 0, 0],
+                    [0, GETF, 2, 0, 0], [0, GEMM, 3, 2, 5], [1, POTF, 0, 0, 0]])
+    fn = tmp_path / "flops.txt"
+    write_log_flops(log, str(fn))
+    assert fn.read_text().splitlines() == [
+        "0;pivot;4;0;0;0", "0;pivot;2;0;0;0", "0;panel;1;4;0;0", "0;gemm;1;1;4;0", "0;gemm;3;2;5;0",
+        "0;rrqr;1;0;0;0", "1;pivot;0;0;0;0"]

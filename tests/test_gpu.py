@@ -349,3 +349,43 @@ def test_level_tune_reports_the_table_choice(monkeypatch):
     monkeypatch.setenv("SPAND_RRQR_STEP_WAVES", "4")
     g.factorize()
     assert all(r["step_waves"] == 4 and r["source"] == 1 for r in g.level_tune())
+
+
+@pytest.mark.parametrize("case", ["lapl2d", "lapl3d_skip1", "convdiff_gen", "fp32"])
+def test_flops_log_matches_oracle(case):
+    """The shape log of the dense calls (spand_get_flops_log, the reference's tprof_flops)
+    equals the oracle's record for record: the GPU runs the calls batched and sparsifies a
+    level in dependency waves, but records them in the reference's sequential order, with
+    every Asn's columns as the sequential loop sees them (earlier neighbours already shrunk)."""
+    from spand_public_amd.problems import convdiff
+
+    say(f"case flops log {case}")
+    symm, skip, prec = "SPD", 0, S.FP64
+    if case == "lapl2d":
+        A, X, nl, tol = neglapl(40, 2), linspace_nd(40, 2), 6, 1e-2
+    elif case == "lapl3d_skip1":
+        A, X, nl, tol, skip = neglapl(12, 3), linspace_nd(12, 3), 6, 1e-2, 1
+    elif case == "convdiff_gen":
+        A, X, nl, tol, symm = convdiff(10, 3), linspace_nd(10, 3), 6, 1e-2, "GEN"
+    else:
+        A, X, nl, tol, prec = neglapl(12, 3), linspace_nd(12, 3), 6, 0.0, S.FP32
+    o, _ = oracle.setup_tree(A, X, nl, tol, skip=skip, symm=symm)
+    o.set_record_shapes(True)
+    o.factorize()
+    g = gpu_tree(A, X, nl, tol, prec=prec, skip=skip) if symm == "SPD" else None
+    if g is None:
+        g = S.Tree(nl)
+        g.set_use_geo(True)
+        g.set_Xcoo(X)
+        g.set_tol(tol)
+        g.set_symm_kind("GEN")
+        g.partition(symmetric_graph(A))
+        g.assemble(A)
+    assert len(g.flops_log()) == 0  # off by default
+    g.set_monitor_flops(True)
+    g.factorize()
+    lg, lo = g.flops_log(), o.shapes()
+    say(f"  {len(lo)} records")
+    assert np.array_equal(lg, lo)
+    g.factorize()  # a refactorization starts a new log
+    assert np.array_equal(g.flops_log(), lo)
