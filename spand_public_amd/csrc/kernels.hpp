@@ -102,9 +102,11 @@ hipError_t launch_perm_out(const double* ws, const int* perm, int N, int nrhs, d
 hipError_t launch_cm2il(const double* src, int ld, int N, int nrhs, double* dst, hipStream_t st);
 hipError_t launch_il2cm(const double* src, int N, int nrhs, double* dst, int ld, hipStream_t st);
 // mv phases of the preconditioner apply: SOLVE_CHUNK outputs per chunk, pieces of at most
-// MV_PIECE_WORK inner elements; partial holds npieces x SOLVE_CHUNK x nrhs doubles.
+// MV_PIECE_WORK inner elements and MV_PIECE_TERMS terms; partial holds npieces x
+// SOLVE_CHUNK x nrhs doubles.
 constexpr int SOLVE_CHUNK = 64;
-constexpr int MV_PIECE_WORK = 128;
+constexpr int MV_PIECE_WORK = 64;
+constexpr int MV_PIECE_TERMS = 64;
 template <typename T>
 hipError_t launch_mv(const MvGroup* d_groups, const MvChunk* d_chunks, const MvPiece* d_pieces, int npieces,
                      const MvTerm* d_terms, const int* d_rchunks, int nreduce, double* ws, double* partial, int nrhs,

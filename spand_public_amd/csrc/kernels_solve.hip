@@ -70,95 +70,185 @@ __global__ void __launch_bounds__(NT) k_il2cm(const double* src, int N, int nrhs
 }
 
 // ---------------------------------------------------------------------------------------
-// mv phases (task layouts in tasks.hpp). A workgroup is 64 lanes (one output each) x 4
-// partitions of the inner sum, over chunks of up to 8 right-hand sides; the partitions are
-// combined in LDS in a fixed order.
+// mv phases (task layouts in tasks.hpp). A workgroup computes one piece: 64 outputs of one
+// chunk over at most MV_PIECE_WORK inner positions, gathered from at most MV_PIECE_TERMS
+// terms. The work is staged in one round so that its global loads are in flight together:
+//   1. one lane per term reads the term record, clips its inner range, and a wavefront scan
+//      packs the terms' ranges into consecutive positions;
+//   2. every thread loads its share of the piece's matrix entries (MV_EPT of them, all
+//      issued before any is stored) into tile[position][output], and the sources of up to
+//      RG_MAX right-hand sides into xl[position][r];
+//   3. 4 partitions of the positions (one wavefront each, position p to partition p % 4)
+//      accumulate each output, and the partitions are added in a fixed order.
+// A chunk with several pieces writes partial sums, which k_mv_reduce adds in piece order
+// after the launch. (Adding them in the chunk's last piece to finish, found by a counter,
+// measured 3-7x slower on gfx950: the device-scope release fence each piece needs writes its
+// XCD's L2 back.)
 constexpr int RG_MAX = 8;
 constexpr int WAVE = 64;
 constexpr int SC = SOLVE_CHUNK;
-constexpr int NP = NT / SC;  // 4
-constexpr int TR = 64;       // rows of a transposed tile
-constexpr int UNR = 8;       // independent matrix loads in flight per thread (non-transposed)
+constexpr int NP = NT / SC;                      // 4 partitions of the inner positions
+constexpr int PW = MV_PIECE_WORK;                // positions per piece
+constexpr int MV_EPT = PW * SC / NT;             // matrix entries per thread: 16
+static_assert(MV_PIECE_TERMS <= WAVE, "one lane per term record");
+static_assert(PW * RG_MAX <= 2 * NT, "two source loads per thread");
+
+struct MvTermL {  // a term's record in LDS, with its clipped inner range and first position
+    const void* A;
+    const int* perm;
+    int rows, lo, len, off, xsrc, trans;
+};
+
+// y (op)= sum: the chunk's outputs, in the group's mode, through dperm if the group has one
+__device__ inline void mv_apply(VT* y, int mode, VT sum) {
+    *y = mode == MV_SUB ? *y - sum : (mode == MV_SETNEG ? -sum : sum);
+}
+
+// adds the partial sums of one chunk's pieces in piece order and applies them
+__device__ inline void mv_chunk_reduce(const MvChunk& ch, const MvGroup& g, const VT* partial, VT* ws, int nrhs) {
+    const int no = min(SC, g.n - ch.o0);
+    for (int idx = threadIdx.x; idx < no * nrhs; idx += NT) {
+        const int lane = idx / nrhs, r = idx % nrhs;
+        VT sum = VT(0);
+        for (int p = ch.p0; p < ch.p1; p++) sum += partial[((size_t)p * SC + lane) * nrhs + r];
+        const int o = ch.o0 + lane;
+        mv_apply(ws + (size_t)(g.xdst + (g.dperm ? g.dperm[o] : o)) * nrhs + r, g.mode, sum);
+    }
+}
 
 template <typename T>
-__global__ void __launch_bounds__(NT) k_mv_piece(const MvGroup* groups, const MvChunk* chunks, const MvPiece* pieces,
-                                                 const MvTerm* terms, VT* ws, VT* partial, int nrhs) {
-    __shared__ T tile[TR][SC + 1];      // A^T tile: inner rows x outputs
-    __shared__ VT xl[TR][RG_MAX];       // the source entries of the current inner tile
-    __shared__ VT red[NP][SC][RG_MAX];
-    const MvPiece pc = pieces[blockIdx.x];  // carries its chunk's and group's fields
-    const int o0 = pc.o0, no = pc.no;
+__global__ void __launch_bounds__(NT) k_mv_piece(const MvPiece* pieces, const MvTerm* terms, VT* ws, VT* partial,
+                                                 int nrhs) {
+    __shared__ T tile[PW][SC + 1];          // the piece's matrix: inner position x output
+    __shared__ VT xl[PW][RG_MAX];           // the sources of the current right-hand-side group
+    __shared__ VT red[NP - 1][SC][RG_MAX];  // partitions 1..3 (partition 0 keeps its registers)
+    __shared__ MvTermL st[MV_PIECE_TERMS];
+    __shared__ unsigned char p2t[PW];       // position -> term
+    __shared__ int s_total;
+    const MvPiece pc = pieces[blockIdx.x];
+    const int o0 = pc.o0, no = pc.no, nt = pc.k1 - pc.k0;
     const int tid = threadIdx.x, lane = tid % SC, kp = tid / SC;
-    const bool one = pc.k1 == pc.k0 + 1;
+    // 1. term records and their positions
+    if (tid < WAVE) {
+        int len = 0;
+        if (tid < nt) {
+            const MvTerm tm = terms[pc.k0 + tid];
+            int lo = 0, hi = tm.trans ? tm.rows : tm.cols;
+            if (nt == 1) {
+                lo = pc.j0;
+                hi = pc.j1;
+            }
+            if (tm.tri == 1) hi = min(hi, o0 + SC);
+            if (tm.tri == 2) lo = max(lo, o0);
+            len = max(0, hi - lo);
+            st[tid] = MvTermL{tm.A, tm.perm, tm.rows, lo, len, 0, tm.xsrc, tm.trans};
+        }
+        int incl = len;  // inclusive scan over the wavefront
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (tid >= d) incl += v;
+        }
+        if (tid < nt) st[tid].off = incl - len;
+        if (tid == WAVE - 1) s_total = incl;
+    }
+    __syncthreads();
+    const int total = s_total;  // <= PW (the planner's bound)
+    for (int p = tid; p < total; p += NT) {  // the last term starting at or before p holds p
+        int lo = 0, hi = nt - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (st[mid].off <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        p2t[p] = (unsigned char)lo;
+    }
+    __syncthreads();
+    // 2. the matrix entries: entry e of the piece belongs to the term holding position e / SC;
+    // within the term's block q = e - SC * off runs along the outputs of one inner index (A's
+    // column, not transposed) or along the inner indices of one output (A's column, transposed),
+    // so that neighbouring lanes read neighbouring addresses either way
+    const int ne = total * SC;
+    T v[MV_EPT];
+    int dst[MV_EPT];
+#pragma unroll
+    for (int u = 0; u < MV_EPT; u++) {
+        const int e = tid + u * NT;
+        v[u] = T(0);
+        dst[u] = -1;
+        if (e < ne) {
+            const MvTermL& q = st[p2t[e / SC]];
+            const int qe = e - q.off * SC;
+            int p, i;
+            if (q.trans) {
+                p = qe % q.len;
+                i = qe / q.len;
+            } else {
+                p = qe / SC;
+                i = qe % SC;
+            }
+            const T* A = reinterpret_cast<const T*>(q.A);
+            if (i < no) v[u] = q.trans ? A[(q.lo + p) + (size_t)(o0 + i) * q.rows] : A[(o0 + i) + (size_t)(q.lo + p) * q.rows];
+            dst[u] = (q.off + p) * (SC + 1) + i;
+        }
+    }
+    auto stage_x = [&](int r0, int rg) {
+        VT xv[2];
+        int xd[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int idx = tid + u * NT;
+            xd[u] = -1;
+            if (idx < total * rg) {
+                const int pos = idx / rg, r = idx % rg;
+                const MvTermL& q = st[p2t[pos]];
+                const int j = q.lo + (pos - q.off);
+                const int src = (!q.trans && q.perm) ? q.perm[j] : j;
+                xv[u] = ws[(size_t)(q.xsrc + src) * nrhs + r0 + r];
+                xd[u] = pos * RG_MAX + r;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+            if (xd[u] >= 0) (&xl[0][0])[xd[u]] = xv[u];
+    };
+    stage_x(0, min(RG_MAX, nrhs));
+#pragma unroll
+    for (int u = 0; u < MV_EPT; u++)
+        if (dst[u] >= 0) (&tile[0][0])[dst[u]] = v[u];
+    __syncthreads();
+    // 3. accumulate, per group of up to RG_MAX right-hand sides
     for (int r0 = 0; r0 < nrhs; r0 += RG_MAX) {
         const int rg = min(RG_MAX, nrhs - r0);
+        if (r0 > 0) {
+            __syncthreads();  // every read of xl for the previous group is done
+            stage_x(r0, rg);
+            __syncthreads();
+        }
         VT acc[RG_MAX];
 #pragma unroll
         for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
-        for (int k = pc.k0; k < pc.k1; k++) {
-            const MvTerm tm = terms[k];
-            const T* A = reinterpret_cast<const T*>(tm.A);
-            const VT* xs = ws + (size_t)tm.xsrc * nrhs + r0;
-            int lo = 0, hi = tm.trans ? tm.rows : tm.cols;
-            if (one) { lo = pc.j0; hi = pc.j1; }
-            if (tm.tri == 1) hi = min(hi, o0 + SC);
-            if (tm.tri == 2) lo = max(lo, o0);
-            // inner tiles of TR: the sources (and, transposed, the A^T tile) staged in LDS
-            for (int i0 = lo; i0 < hi; i0 += TR) {
-                const int ni = min(TR, hi - i0);
-                for (int idx = tid; idx < ni * rg; idx += NT) {
-                    const int jj = idx / rg, r = idx % rg;
-                    const int src = (!tm.trans && tm.perm) ? tm.perm[i0 + jj] : i0 + jj;
-                    xl[jj][r] = xs[(size_t)src * nrhs + r];
-                }
-                if (tm.trans)
-                    for (int idx = tid; idx < TR * SC; idx += NT) {
-                        const int ii = idx % TR, jj = idx / TR;
-                        tile[ii][jj] = (ii < ni && jj < no) ? A[(i0 + ii) + (size_t)(o0 + jj) * tm.rows] : T(0);
-                    }
-                __syncthreads();
-                if (!tm.trans) {
-                    // output i = row of A, inner j = column; UNR column loads in flight
-                    if (lane < no) {
-                        const T* Ai = A + o0 + lane + (size_t)i0 * tm.rows;
-                        int jj = kp;
-                        for (; jj + (UNR - 1) * NP < ni; jj += UNR * NP) {
-                            VT a[UNR];
+        for (int p = kp; p < total; p += NP) {
+            const VT a = tile[p][lane];
 #pragma unroll
-                            for (int u = 0; u < UNR; u++) a[u] = Ai[(size_t)(jj + u * NP) * tm.rows];
-#pragma unroll
-                            for (int u = 0; u < UNR; u++)
-                                for (int r = 0; r < rg; r++) acc[r] += a[u] * xl[jj + u * NP][r];
-                        }
-                        for (; jj < ni; jj += NP) {
-                            const VT a = Ai[(size_t)jj * tm.rows];
-                            for (int r = 0; r < rg; r++) acc[r] += a * xl[jj][r];
-                        }
-                    }
-                } else {
-                    for (int ii = kp; ii < ni; ii += NP) {
-                        const VT a = tile[ii][lane];
-                        for (int r = 0; r < rg; r++) acc[r] += a * xl[ii][r];
-                    }
-                }
-                __syncthreads();
-            }
+            for (int r = 0; r < RG_MAX; r++)
+                if (r < rg) acc[r] += a * xl[p][r];
         }
-        for (int r = 0; r < rg; r++) red[kp][lane][r] = acc[r];
+        if (kp > 0)
+#pragma unroll
+            for (int r = 0; r < RG_MAX; r++) red[kp - 1][lane][r] = acc[r];
         __syncthreads();
         if (kp == 0 && lane < no) {
             if (pc.direct) {
                 VT* y = ws + (size_t)(pc.xdst + o0 + lane) * nrhs + r0;
-                for (int r = 0; r < rg; r++) {
-                    const VT sum = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
-                    y[r] = pc.mode == MV_SUB ? y[r] - sum : (pc.mode == MV_SETNEG ? -sum : sum);
-                }
+                for (int r = 0; r < rg; r++)
+                    mv_apply(y + r, pc.mode, ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
             } else {
                 VT* y = partial + ((size_t)blockIdx.x * SC + lane) * nrhs + r0;
-                for (int r = 0; r < rg; r++) y[r] = red[0][lane][r] + red[1][lane][r] + red[2][lane][r] + red[3][lane][r];
+                for (int r = 0; r < rg; r++) y[r] = ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
             }
         }
-        __syncthreads();
+        __syncthreads();  // red is rewritten by the next group
     }
 }
 
@@ -166,16 +256,7 @@ __global__ void __launch_bounds__(NT) k_mv_piece(const MvGroup* groups, const Mv
 __global__ void __launch_bounds__(NT) k_mv_reduce(const MvGroup* groups, const MvChunk* chunks, const int* rchunks,
                                                   const VT* partial, VT* ws, int nrhs) {
     const MvChunk ch = chunks[rchunks[blockIdx.x]];
-    const MvGroup g = groups[ch.group];
-    const int no = min(SC, g.n - ch.o0);
-    for (int idx = threadIdx.x; idx < no * nrhs; idx += NT) {
-        const int lane = idx / nrhs, r = idx % nrhs;
-        VT sum = VT(0);
-        for (int p = ch.p0; p < ch.p1; p++) sum += partial[((size_t)p * SC + lane) * nrhs + r];
-        const int o = ch.o0 + lane;
-        VT* y = ws + (size_t)(g.xdst + (g.dperm ? g.dperm[o] : o)) * nrhs + r;
-        *y = g.mode == MV_SUB ? *y - sum : (g.mode == MV_SETNEG ? -sum : sum);
-    }
+    mv_chunk_reduce(ch, groups[ch.group], partial, ws, nrhs);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -438,7 +519,7 @@ hipError_t launch_mv(const MvGroup* d_groups, const MvChunk* d_chunks, const MvP
                      const MvTerm* d_terms, const int* d_rchunks, int nreduce, double* ws, double* partial, int nrhs,
                      hipStream_t st) {
     if (npieces == 0) return hipSuccess;
-    k_mv_piece<T><<<npieces, NT, 0, st>>>(d_groups, d_chunks, d_pieces, d_terms, ws, partial, nrhs);
+    k_mv_piece<T><<<npieces, NT, 0, st>>>(d_pieces, d_terms, ws, partial, nrhs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nreduce == 0) return e;
     k_mv_reduce<<<nreduce, NT, 0, st>>>(d_groups, d_chunks, d_rchunks, partial, ws, nrhs);

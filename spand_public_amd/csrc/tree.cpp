@@ -550,9 +550,12 @@ struct MvPlan {
                     work.vec_elems += len;
                     if (len > W) {  // a long term alone: pieces of W inner indices
                         flush(k);
-                        for (int j = lo; j < hi; j += W) pieces.push_back({ci, k, k + 1, j, std::min(hi, j + W), 0, 0, 0, 0, 0});
+                        for (int j = lo; j < hi; j += W)
+                            pieces.push_back({ci, k, k + 1, j, std::min(hi, j + W), 0, 0, 0, 0, 0});
                     } else {
-                        if (pk0 >= 0 && pwork + len > W) flush(k);
+                        // short terms share a piece: at most W inner indices and
+                        // MV_PIECE_TERMS terms (one lane of a wavefront reads each record)
+                        if (pk0 >= 0 && (pwork + len > W || k - pk0 >= MV_PIECE_TERMS)) flush(k);
                         if (pk0 < 0) pk0 = k;
                         pwork += len;
                     }
