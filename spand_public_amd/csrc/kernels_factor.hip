@@ -177,7 +177,12 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
             if (i < j) continue;
             T* a = A + (kb + nb + i) + (size_t)(kb + nb + j) * n;
             T acc = *a;
-            for (int k = 0; k < nb; k++) acc = fma(-P[i][k], P[j][k], acc);
+            if (nb == PB) {  // a full panel: unrolled, so the LDS reads run ahead of the FMAs
+#pragma unroll
+                for (int k = 0; k < PB; k++) acc = fma(-P[i][k], P[j][k], acc);
+            } else {
+                for (int k = 0; k < nb; k++) acc = fma(-P[i][k], P[j][k], acc);
+            }
             *a = acc;
         }
         __syncthreads();
@@ -401,7 +406,12 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
                 int ii, c;
                 if (left) { ii = idx % ni; c = idx / ni; } else { c = idx % m; ii = idx / m; }
                 T acc = kb == 0 ? first(i0 + ii, c) : at(i0 + ii, c);
-                for (int k = 0; k < nb; k++) acc = fma(-Bk[k][c], Lr[k][ii], acc);
+                if (nb == PB) {  // a full block: unrolled, so the LDS reads run ahead of the FMAs
+#pragma unroll
+                    for (int k = 0; k < PB; k++) acc = fma(-Bk[k][c], Lr[k][ii], acc);
+                } else {
+                    for (int k = 0; k < nb; k++) acc = fma(-Bk[k][c], Lr[k][ii], acc);
+                }
                 at(i0 + ii, c) = acc;
             }
             __syncthreads();
@@ -677,10 +687,19 @@ __global__ void __launch_bounds__(NT) k_schur(const SchurTile* tiles, const Schu
                 Bs[l][i] = vb;
             }
             __syncthreads();
-            for (int l = 0; l < kc; l++) {
-                const T a = -As[l][ti];
+            if (kc == SK) {  // a full chunk: unrolled, so the LDS reads run ahead of the FMAs
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc_c[q] = fma(a, Bs[l][tj0 + q], acc_c[q]);
+                for (int l = 0; l < SK; l++) {
+                    const T a = -As[l][ti];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc_c[q] = fma(a, Bs[l][tj0 + q], acc_c[q]);
+                }
+            } else {
+                for (int l = 0; l < kc; l++) {
+                    const T a = -As[l][ti];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc_c[q] = fma(a, Bs[l][tj0 + q], acc_c[q]);
+                }
             }
             __syncthreads();
         }

@@ -228,11 +228,17 @@ __global__ void __launch_bounds__(NT) k_mv_piece(const MvPiece* pieces, const Mv
         VT acc[RG_MAX];
 #pragma unroll
         for (int r = 0; r < RG_MAX; r++) acc[r] = VT(0);
-        for (int p = kp; p < total; p += NP) {
-            const VT a = tile[p][lane];
+        // positions kp, kp + 4, ... < total (at most 16 per partition), unrolled so that the
+        // LDS reads run ahead; the sum order per output is the position order either way
 #pragma unroll
-            for (int r = 0; r < RG_MAX; r++)
-                if (r < rg) acc[r] += a * xl[p][r];
+        for (int q = 0; q < PW / NP; q++) {
+            const int p = kp + q * NP;
+            if (p < total) {
+                const VT a = tile[p][lane];
+#pragma unroll
+                for (int r = 0; r < RG_MAX; r++)
+                    if (r < rg) acc[r] += a * xl[p][r];
+            }
         }
         if (kp > 0)
 #pragma unroll
