@@ -1,8 +1,9 @@
 """Shape log of the dense calls (Tree::tprof_flops, include/util.h:311-342, recorded under
 set_monitor_flops, src/tree.cpp:592-1312; written by write_log_flops, src/tree.cpp:60-77).
 
-The oracle's log is pinned by a case worked out by hand from the reference's call structure
-(a 1D chain: two leaves and a one-point separator) and, on larger trees, by invariants a
+The oracle's log is pinned by cases worked out by hand from the reference's call structure
+(1D chains: two leaves and a one-point separator; three separators emptied one after the
+other, which pins the sequential order of sparsification) and, on larger trees, by invariants a
 dropped, duplicated or misplaced record breaks: per level the rrqr records count the
 sparsified clusters and their rows add up to the ranks before sparsification, every gemm's
 inner size is the pivot just eliminated, and the pivot sizes add up to the eliminated, scaled
@@ -47,6 +48,26 @@ def test_chain_by_hand_gen():
     leaf = [[0, GETF, h, 0, 0], [0, PANEL, 1, h, 0], [0, PANEL, 1, h, 0], [0, GEMM, 1, 1, h]]
     expect = leaf + leaf + [[0, GETF, 1, 0, 0], [0, RRQR, 1, 0, 0], [0, GETF, 1, 0, 0], [1, GETF, 0, 0, 0]]
     assert s.tolist() == expect
+
+
+def test_sparsification_is_sequential():
+    # src/tree.cpp:1521-1533 sparsifies the clusters of a level one after the other, each
+    # seeing its neighbours as already shrunk. A chain of 15 points, 3 levels: after the four
+    # leaves are eliminated, three one-point separators remain, a and b (each next to c) and c
+    # (next to both). With tol = 10 every rank is 0 (|R_00| / |R_00| = 1 < tol), so each
+    # cluster splits off whole when its turn comes: a's Asn is 1 x 1 (c), b's 1 x 1 (c), and
+    # c's is 1 x 0, because a and b are already empty. Sparsifying all three from the sizes
+    # before the level (a Jacobi order) would give c a 1 x 2 Asn.
+    o, s = oracle_log(neglapl(15, 1), linspace_nd(15, 1), 3, 10.0)
+    lvl0 = s[s[:, 0] == 0].tolist()
+    assert lvl0[-13:] == [
+        [0, POTF, 1, 0, 0], [0, PANEL, 1, 1, 0],                        # scale a
+        [0, POTF, 1, 0, 0], [0, PANEL, 1, 1, 0],                        # scale b
+        [0, POTF, 1, 0, 0], [0, PANEL, 1, 1, 0], [0, PANEL, 1, 1, 0],   # scale c (two edges)
+        [0, RRQR, 1, 1, 0], [0, POTF, 1, 0, 0],                         # a, its sibling eliminated
+        [0, RRQR, 1, 1, 0], [0, POTF, 1, 0, 0],                         # b
+        [0, RRQR, 1, 0, 0], [0, POTF, 1, 0, 0]]                         # c: both neighbours empty
+    assert o.get_log()[0, 6] == 0  # nothing kept
 
 
 @pytest.mark.parametrize("case", ["lapl3d_skip1", "lapl2d", "convdiff_gen"])

@@ -351,7 +351,7 @@ def test_level_tune_reports_the_table_choice(monkeypatch):
     assert all(r["step_waves"] == 4 and r["source"] == 1 for r in g.level_tune())
 
 
-@pytest.mark.parametrize("case", ["lapl2d", "lapl3d_skip1", "convdiff_gen", "fp32"])
+@pytest.mark.parametrize("case", ["lapl2d", "lapl3d_skip1", "convdiff_gen", "fp32", "chain_rank0"])
 def test_flops_log_matches_oracle(case):
     """The shape log of the dense calls (spand_get_flops_log, the reference's tprof_flops)
     equals the oracle's record for record: the GPU runs the calls batched and sparsifies a
@@ -367,8 +367,10 @@ def test_flops_log_matches_oracle(case):
         A, X, nl, tol, skip = neglapl(12, 3), linspace_nd(12, 3), 6, 1e-2, 1
     elif case == "convdiff_gen":
         A, X, nl, tol, symm = convdiff(10, 3), linspace_nd(10, 3), 6, 1e-2, "GEN"
-    else:
+    elif case == "fp32":
         A, X, nl, tol, prec = neglapl(12, 3), linspace_nd(12, 3), 6, 0.0, S.FP32
+    else:  # every rank 0: each cluster sees its earlier neighbours emptied (sequential order)
+        A, X, nl, tol = neglapl(15, 1), linspace_nd(15, 1), 3, 10.0
     o, _ = oracle.setup_tree(A, X, nl, tol, skip=skip, symm=symm)
     o.set_record_shapes(True)
     o.factorize()
