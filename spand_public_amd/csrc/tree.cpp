@@ -921,17 +921,1335 @@ void Tree::upload_transposed() {
     if (any) hip_check(hipStreamSynchronize(0), "upload transposed phases");
 }
 
+// ---------------------------------------------------------------------------------------
+// The factorization (src/tree.cpp:1447-1551), level by level. Each level runs the phases
+// below in order; they share the level's task lists through LevelPlan.
+
+// One factorization's state shared by its levels.
+struct FactorRun {
+    hipStream_t st = 0;  // the library's calls are ordered on the null stream
+    bool gen = false;    // GEN / PLU tree
+    long long lds_elems = 0, big_min_rows = 0;  // RRQR path limits
+    std::vector<int> hsize;                     // cluster sizes read back from the device
+    std::vector<std::future<SolvePlanHost>> pending_solve;  // per level; joined on every exit
+};
+
+// A Schur contribution and its target edge, in recorded order.
+struct SchurPending {
+    int target;
+    SchurContrib c;
+};
+
+// One level's plan: the task lists each phase builds and the later phases launch or record.
+template <typename T>
+struct LevelPlan {
+    FactorRun* run = nullptr;
+    LevelStats* lst = nullptr;
+    LevelSolve* lsv = nullptr;
+    const std::vector<int>* bot = nullptr;  // the current bottom list
+    double t_lvl0 = 0, t_plan1 = 0, t_lvl1 = 0, t_lvl2 = 0;
+    // elimination and scaling
+    std::vector<PotrfTask> potrfE, potrfS;
+    std::vector<GetrfTask> getrfE, getrfS;
+    std::vector<TrsmTask> trsmE, trsmA, trsmB;
+    std::vector<SchurPending> pcs;
+    std::vector<int> fill_edges;
+    long long fill_elems = 0;
+    std::vector<long long> fill_off;  // by position in fill_edges
+    std::vector<TrsvTask> sv_elim, sv_scale;
+    std::vector<CopyTask> inv_init;  // identity blocks that become explicit inverses L^{-1}
+    std::vector<std::pair<int, GemvTerm>> fwd_list;  // (destination cluster, term), in op order
+    std::vector<GemvGroup> bwd_groups;
+    std::vector<GemvTerm> bwd_terms;
+    LevelShape shape;  // the level's size histogram features and launch configuration
+    LevelTune tune;
+    int cwp = 0;       // trsm chunk-width policy
+    T* fill_base = nullptr;
+    std::vector<SchurTile> tiles;
+    std::vector<SchurContrib> contribs;
+    size_t n_potrf_elim = 0;
+    // sparsification
+    bool do_sparsify_level = false;
+    std::vector<int> live;  // clusters above the level (scaled and sparsified)
+    std::vector<int> plist, wave, widx;
+    int nwaves = 0;
+    std::vector<std::vector<RrqrTask>> wtasks;   // one-workgroup LDS path, per wave
+    std::vector<long long> wlds_small;           // dynamic LDS of each wave's small-path launch
+    std::vector<std::vector<RrqrBig>> wbig;      // stepped path, per wave
+    std::vector<std::vector<RrqrSlab>> wslab;
+    std::vector<int> wsteps, wmaxrows;
+    std::vector<std::vector<RrqrTask>> wreg;     // register path, per wave (sorted by variant)
+    std::vector<std::vector<int>> wregvar;       // the variant of each wreg task
+    std::vector<std::pair<int, int>> where;      // (0 small | 1 big | 2 register, index) by plist position
+    std::vector<RrqrSeg> segs;
+    std::vector<size_t> wlds_step, wlds_wave;
+    bool persistent = false;
+    int step_waves = 0;
+    std::vector<size_t> wflag0;
+    // kept ranks and merge
+    std::vector<QOp> qops;
+    std::vector<WyTask> wy;
+    std::vector<int4> wy_tiles;
+    std::vector<TrsmTask> wy_trsm;
+    std::vector<MergeTask> mt;
+    std::vector<int2> mchildren;
+    int left = 0;  // dofs left after the level
+};
+
+// Cholesky batch order: <= 64, <= POTRF_SMALL_MAX, larger
+static void small_first(std::vector<PotrfTask>& v) {
+    std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= POTRF_SMALL_MAX; });
+    std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= 64; });
+}
+// LU pivots that fit LDS first (one launch), the larger ones after (another)
+template <typename T>
+static void lds_first(std::vector<GetrfTask>& v) {
+    std::stable_partition(v.begin(), v.end(), [](const GetrfTask& t) {
+        return (size_t)t.n * (t.n + 1) * sizeof(T) <= (size_t)GETRF_LDS_BYTES;
+    });
+}
+
+// Cholesky pivots of one batch: up to POTRF_SMALL_MAX in one launch (one workgroup each;
+// 64 and below in their own), larger ones (sorted last) one 32-column panel per launch pair.
+template <typename T>
+void Tree::run_potrf(hipStream_t st, void* dt, const std::vector<PotrfTask>& v, size_t off, const char* what) {
+    const size_t esz = sizeof(T);
+    if (v.empty()) return;
+    int ntiny = 0, nsmall = 0, nmax = 0;
+    for (auto& t : v) {
+        if (t.n <= 64) ntiny++;
+        if (t.n <= POTRF_SMALL_MAX) nsmall++;
+        nmax = std::max(nmax, t.n);
+    }
+    const PotrfTask* d = Packer::at<PotrfTask>(dt, off);
+    if (ntiny > 0) {
+        std::vector<PotrfTask> vs(v.begin(), v.begin() + ntiny);
+        Work w = work_potrf(vs, esz);
+        hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
+                        [&] { return launch_potrf64<T>(d, ntiny, d_info_, st); }),
+                  what);
+    }
+    if (nsmall > ntiny) {
+        std::vector<PotrfTask> vs(v.begin() + ntiny, v.begin() + nsmall);
+        Work w = work_potrf(vs, esz);
+        hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
+                        [&] { return launch_potrf<T>(d + ntiny, nsmall - ntiny, d_info_, st); }),
+                  what);
+    }
+    const int nbig = (int)v.size() - nsmall;
+    if (nbig == 0) return;
+    const PotrfTask* db = d + nsmall;
+    for (int kb = 0; kb < nmax; kb += 32) {
+        double pb = 0, pf = 0, tb = 0, tf = 0;
+        for (int k = nsmall; k < (int)v.size(); k++) {
+            const double n = v[k].n, nb = std::min(32.0, n - kb), rest = n - kb - nb;
+            if (nb <= 0) continue;
+            pb += (nb * (nb + 1) + 2.0 * rest * nb) * esz;
+            pf += nb * nb * nb / 3.0 + rest * nb * nb;
+            tb += (rest * (rest + 1) + 2.0 * rest * nb) * esz;
+            tf += rest * (rest + 1) * nb;
+        }
+        hip_check(timed(prof_, KF_POTRF_PANEL, st, pb, pf,
+                        [&] { return launch_potrf_panel<T>(db, nbig, kb, nmax, d_info_, st); }),
+                  what);
+        hip_check(timed(prof_, KF_POTRF_TRAIL, st, tb, tf,
+                        [&] { return launch_potrf_trail<T>(db, nbig, kb, nmax, st); }),
+                  what);
+    }
+}
+
+// One trsm batch (split: the first task that may take the matrix-core kernel).
+template <typename T>
+void Tree::run_trsm(hipStream_t st, void* dt, const std::vector<TrsmTask>& v, size_t off, int split, const char* what) {
+    const size_t esz = sizeof(T);
+    if (v.empty()) return;
+    Work w = work_trsm(v, esz);
+    hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
+                    [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), split, st); }),
+              what);
+}
+
+// LU pivots of one batch: those that fit LDS in one launch, the larger ones in another.
+template <typename T>
+void Tree::run_getrf(hipStream_t st, void* dt, const std::vector<GetrfTask>& v, size_t off, const char* what) {
+    const size_t esz = sizeof(T);
+    if (v.empty()) return;
+    int nl = 0, nmax_l = 0;
+    double lb = 0, lf = 0, gb = 0, gf = 0;
+    for (auto& t : v) {
+        const double n = t.n;
+        const double by = (3.0 * n * n) * esz + 4.0 * n, fl = getrf_flops(n);
+        if ((size_t)t.n * (t.n + 1) * esz <= (size_t)GETRF_LDS_BYTES) {
+            nl++;
+            nmax_l = std::max(nmax_l, t.n);
+            lb += by;
+            lf += fl;
+        } else {
+            gb += by;
+            gf += fl;
+        }
+    }
+    const GetrfTask* d = Packer::at<GetrfTask>(dt, off);
+    if (nl > 0)
+        hip_check(timed(prof_, KF_GETRF, st, lb, lf,
+                        [&] { return launch_getrf<T>(d, nl, 1, nmax_l, d_info_, st); }),
+                  what);
+    if (nl < (int)v.size())
+        hip_check(timed(prof_, KF_GETRF, st, gb, gf,
+                        [&] { return launch_getrf<T>(d + nl, (int)v.size() - nl, 0, 0, d_info_, st); }),
+                  what);
+}
+
+// Elimination plan of level ilvl_ (eliminate_cluster of every cluster of the level,
+// src/tree.cpp:1494-1506, 895-967): pivots, panels, Schur contributions grouped into target
+// tiles, fill-in blocks, the ops and the solve terms.
+template <typename T>
+void Tree::plan_elimination(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const bool gen = R.gen;
+    LevelStats& lst = *L.lst;
+    const std::vector<int>& bot = *L.bot;
+    auto& potrfE = L.potrfE;
+    auto& getrfE = L.getrfE;
+    auto& trsmE = L.trsmE;
+    auto& pcs = L.pcs;
+    auto& fill_edges = L.fill_edges;
+    auto& fill_elems = L.fill_elems;
+    auto& fill_off = L.fill_off;
+    auto& sv_elim = L.sv_elim;
+    auto& inv_init = L.inv_init;
+    auto& fwd_list = L.fwd_list;
+    auto& bwd_groups = L.bwd_groups;
+    auto& bwd_terms = L.bwd_terms;
+    auto& shape = L.shape;
+    auto& tune = L.tune;
+    auto& fill_base = L.fill_base;
+    auto& tiles = L.tiles;
+    auto& contribs = L.contribs;
+    auto push_trsm = [&L](std::vector<TrsmTask>& v, const void* A, void* B, int n, int m, int side, size_t es,
+                          const void* src = nullptr, const int* perm = nullptr) {
+        spand::push_trsm(v, A, B, n, m, side, es, src, perm, L.cwp);
+    };
+    auto target_edge = [&](int n1, int n2) {
+        int e = find_out_edge(n2, n1);
+        if (e < 0) {
+            e = new_edge(n2, n1, nullptr, false);
+            fill_edges.push_back(e);
+            fill_off.push_back(fill_elems);
+            fill_elems += (long long)size(n1) * size(n2);
+        }
+        return e;
+    };
+    // the level's launch configuration: trsm chunks from its pivot sizes now, the
+    // sparsification knobs once the sparsified clusters' Asn shapes are known (below)
+    {
+        std::vector<int> piv;
+        for (int c : bot)
+            if (cls_[c].level == ilvl_) piv.push_back(cls_[c].size);
+        shape.n_elim = (int)piv.size();
+        if (!piv.empty()) {
+            std::sort(piv.begin(), piv.end());
+            shape.elim_med = piv[piv.size() / 2];
+            shape.elim_max = piv.back();
+        }
+    }
+    tune = choose_tune(sizeof(T), shape);
+    apply_tune_env(tune);
+    L.cwp = tune.trsm_cw;
+    for (int c : bot) {
+        if (cls_[c].level != ilvl_) continue;
+        HCluster& s = cls_[c];
+        const int n = s.size;
+        // edges_ may grow (fill-in) below: keep the pivot block pointer, not a reference
+        const HEdge piv = edges_[s.out[0]];
+        if (piv.ident) throw Error(ERR_STATE, "internal: identity pivot on an eliminated cluster");
+        if (gen) {
+            // getf_cluster + panel_getf + gemm_edges, PLU branch (src/tree.cpp:929-956)
+            void* Ut = arena_.alloc((size_t)std::max(1, n * n) * sizeof(T));
+            int* pp = reinterpret_cast<int*>(arena_.alloc((size_t)std::max(1, n) * sizeof(int)));
+            getrfE.push_back({piv.A, Ut, pp, n, c});
+            lst.flops_potrf += getrf_flops(n);
+            log_shape(SK_GETF, n, 0, 0);
+            std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
+            for (int e : in) {  // A_sn := L^{-1} P^T A_sn, into a fresh block
+                const int m = size(edges_[e].n1);
+                log_shape(SK_PANEL, m, n, 0);
+                void* nb = arena_.alloc((size_t)std::max(1, n * m) * sizeof(T));
+                push_trsm(trsmE, piv.A, nb, n, m, 0, sizeof(T), edges_[e].A, pp);
+                edges_[e].A = nb;
+                lst.flops_trsm += (double)n * n * m;
+            }
+            for (int e : out) {  // A_ns := A_ns U^{-1} = A_ns (U^T)^{-T}
+                log_shape(SK_PANEL, size(edges_[e].n2), n, 0);
+                push_trsm(trsmE, Ut, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
+                lst.flops_trsm += (double)n * n * size(edges_[e].n2);
+            }
+            for (int e1 : out)
+                for (int e2 : in) {
+                    int n1 = edges_[e1].n2, n2 = edges_[e2].n1;
+                    int tgt = target_edge(n1, n2);
+                    SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
+                    pcs.push_back({tgt, sc});
+                    lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
+                    log_shape(SK_GEMM, size(n1), size(n2), n);
+                }
+            // record ScalingPLUQ + GemmOut / GemmIn (src/tree.cpp:945-951)
+            ops_.push_back({OP_SCALING_PLUQ, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr, Ut, pp});
+            if (n > 0) {
+                void* Li = arena_.alloc((size_t)n * n * sizeof(T));
+                void* Wi = arena_.alloc((size_t)n * n * sizeof(T));
+                inv_init.push_back({nullptr, Li, n, n, n, 0});
+                inv_init.push_back({nullptr, Wi, n, n, n, 0});
+                push_trsm(trsmE, piv.A, Li, n, n, 0, sizeof(T));
+                push_trsm(trsmE, Ut, Wi, n, n, 0, sizeof(T));
+                lst.flops_trsm += 2.0 * n * n * n;
+                sv_elim.push_back({Li, Wi, pp, n, (int)s.xoff});
+            }
+            factor_bytes_ += (size_t)n * (n + 1) * sizeof(T) + (size_t)n * sizeof(int);
+            std::vector<GemvTerm> mine;
+            for (int e : out) {
+                int nb = edges_[e].n2;
+                ops_.push_back({OP_GEMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
+                factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+                if (n > 0 && size(nb) > 0) {
+                    fwd_list.push_back({nb, GemvTerm{edges_[e].A, size(nb), n, (int)s.xoff, 0}});
+                }
+            }
+            for (int e : in) {
+                int nb = edges_[e].n1;
+                ops_.push_back({OP_GEMM_IN, c, nb, n, size(nb), n, size(nb), ilvl_, edges_[e].A, nullptr});
+                factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+                if (n > 0 && size(nb) > 0) mine.push_back({edges_[e].A, n, size(nb), (int)cls_[nb].xoff, 0});
+            }
+            if (!mine.empty()) {
+                int k0 = (int)bwd_terms.size();
+                for (auto it = mine.rbegin(); it != mine.rend(); ++it) bwd_terms.push_back(*it);
+                bwd_groups.push_back({(int)s.xoff, n, k0, (int)bwd_terms.size()});
+            }
+            set_eliminated(c);
+            continue;
+        }
+        potrfE.push_back({piv.A, n, c, nullptr});
+        lst.flops_potrf += potrf_flops(n);
+        log_shape(SK_POTF, n, 0, 0);
+        std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
+        for (int e : in) {
+            push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n1), 0, sizeof(T));
+            lst.flops_trsm += (double)n * n * size(edges_[e].n1);
+            log_shape(SK_PANEL, size(edges_[e].n1), n, 0);
+        }
+        for (int e : out) {
+            push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
+            lst.flops_trsm += (double)n * n * size(edges_[e].n2);
+            log_shape(SK_PANEL, size(edges_[e].n2), n, 0);
+        }
+        // schur_symmetric (src/tree.cpp:862-883) with gemm_edges (:748-793)
+        for (int e1 : out)
+            for (int e2 : out) {
+                int n1 = edges_[e1].n2, n2 = edges_[e2].n2;
+                if (cls_[n1].order < cls_[n2].order) continue;
+                int tgt = target_edge(n1, n2);
+                SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), size(n2), 0, 1, 0};
+                pcs.push_back({tgt, sc});
+                lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
+                log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
+            }
+        for (int e1 : in)
+            for (int e2 : in) {
+                int n1 = edges_[e1].n1, n2 = edges_[e2].n1;
+                if (cls_[n1].order < cls_[n2].order) continue;
+                int tgt = target_edge(n1, n2);
+                SchurContrib sc{edges_[e1].A, edges_[e2].A, n, n, n, 1, 0, 0};
+                pcs.push_back({tgt, sc});
+                lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
+                log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
+            }
+        for (int e1 : out)
+            for (int e2 : in) {
+                int n1 = edges_[e1].n2, n2 = edges_[e2].n1;
+                int tgt = target_edge(n1, n2);
+                SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
+                pcs.push_back({tgt, sc});
+                lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
+                log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
+            }
+        // record ScalingLLT + GemmSymmOut/In (src/tree.cpp:909-910, 885-892)
+        ops_.push_back({OP_SCALING_LLT, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr});
+        if (n > 0) {
+            // explicit inverse for the solve: L^{-1} = L^{-1} I by the chunked trsm
+            void* Li = arena_.alloc((size_t)n * n * sizeof(T));
+            inv_init.push_back({nullptr, Li, n, n, n, 0});
+            push_trsm(trsmE, piv.A, Li, n, n, 0, sizeof(T));
+            lst.flops_trsm += (double)n * n * n;
+            sv_elim.push_back({Li, Li, nullptr, n, (int)s.xoff});
+        }
+        factor_bytes_ += (size_t)n * (n + 1) / 2 * sizeof(T);
+        std::vector<GemvTerm> mine;
+        for (int e : out) {
+            int nb = edges_[e].n2;
+            ops_.push_back({OP_GEMM_SYMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
+            factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+            if (n > 0 && size(nb) > 0) {
+                fwd_list.push_back({nb, GemvTerm{edges_[e].A, size(nb), n, (int)s.xoff, 0}});
+                mine.push_back({edges_[e].A, size(nb), n, (int)cls_[nb].xoff, 1});
+            }
+        }
+        for (int e : in) {
+            int nb = edges_[e].n1;
+            ops_.push_back({OP_GEMM_SYMM_IN, c, nb, n, size(nb), n, size(nb), ilvl_, edges_[e].A, nullptr});
+            factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
+            if (n > 0 && size(nb) > 0) {
+                fwd_list.push_back({nb, GemvTerm{edges_[e].A, n, size(nb), (int)s.xoff, 1}});
+                mine.push_back({edges_[e].A, n, size(nb), (int)cls_[nb].xoff, 0});
+            }
+        }
+        if (!mine.empty()) {
+            int k0 = (int)bwd_terms.size();
+            for (auto it = mine.rbegin(); it != mine.rend(); ++it) bwd_terms.push_back(*it);
+            bwd_groups.push_back({(int)s.xoff, n, k0, (int)bwd_terms.size()});
+        }
+        set_eliminated(c);
+    }
+    // fill-in blocks: the Schur targets created at this level, zero-initialized
+    if (fill_elems > 0) {
+        fill_base = reinterpret_cast<T*>(arena_.alloc((size_t)fill_elems * sizeof(T)));
+        for (size_t k = 0; k < fill_edges.size(); k++) edges_[fill_edges[k]].A = fill_base + fill_off[k];
+    }
+    // Schur tiles grouped by target, contributions in recorded order (built before the
+    // scale phase below turns the neighbours' pivots into identities)
+    {
+        std::vector<int> tstart, torder;
+        const std::vector<int> korder = group_by_first(
+            (int)pcs.size(), (int)edges_.size(), [&](int k) { return pcs[k].target; }, tstart, torder);
+        for (size_t gi = 0; gi < torder.size(); gi++) {
+            const int tg = torder[gi];
+            const HEdge& e = edges_[tg];
+            const int m = size(e.n2), nn = size(e.n1);
+            const int k0 = (int)contribs.size();
+            for (int q = tstart[gi]; q < tstart[gi + 1]; q++) contribs.push_back(pcs[korder[q]].c);
+            const int k1 = (int)contribs.size();
+            const int lower = !gen && e.n1 == e.n2 ? 1 : 0;  // syrk on symmetric diagonal targets
+            for (int c0 = 0; c0 < nn; c0 += 32)
+                for (int r0 = 0; r0 < m; r0 += 32) {
+                    int tm = std::min(32, m - r0), tn = std::min(32, nn - c0);
+                    if (lower && r0 + tm - 1 < c0) continue;
+                    tiles.push_back({e.A, m, r0, c0, tm, tn, lower, k0, k1});
+                }
+        }
+    }
+}
+
+// Uploads and launches the elimination group, before the scaling and sparsification are
+// planned, so that the device eliminates while the host plans.
+template <typename T>
+void Tree::launch_elimination(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const hipStream_t st = R.st;
+    const size_t esz = sizeof(T);
+    LevelStats& lst = *L.lst;
+    const std::vector<int>& bot = *L.bot;
+    auto& potrfE = L.potrfE;
+    auto& getrfE = L.getrfE;
+    auto& trsmE = L.trsmE;
+    auto& fill_elems = L.fill_elems;
+    auto& inv_init = L.inv_init;
+    auto& fill_base = L.fill_base;
+    auto& tiles = L.tiles;
+    auto& contribs = L.contribs;
+    auto& n_potrf_elim = L.n_potrf_elim;
+    // Nothing planned below feeds the elimination group (fill-in zeroing, identity inits
+    // of the explicit inverses, pivots, panels, Schur updates): it is launched now, so the
+    // device eliminates while the host plans the level's scaling and sparsification.
+    // LU pivots that fit LDS first (one launch), the larger ones after (another)
+    // pivots up to POTRF_SMALL_MAX in one launch (one workgroup each); larger ones (sorted
+    // last) one 32-column panel per launch pair
+    L.n_potrf_elim = potrfE.size();
+    {
+        small_first(potrfE);
+        for (PotrfTask& t : potrfE)
+            if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
+        lds_first<T>(getrfE);
+        Packer pe;
+        const int sp_te = split_trsm(trsmE);
+        const size_t o_pe = pe.add(potrfE), o_te = pe.add(trsmE), o_ti = pe.add(tiles), o_co = pe.add(contribs);
+        const size_t o_ie = pe.add(inv_init), o_ge = pe.add(getrfE);
+        void* de = upload_tasks(pe);
+        if (fill_elems > 0)
+            hip_check(timed(prof_, KF_FILL, st, (double)fill_elems * esz, 0,
+                            [&] { return launch_fill<T>(fill_base, (size_t)fill_elems, T(0), st); }),
+                      "fill fill-in");
+        if (!inv_init.empty()) {
+            Work w = work_copy(inv_init, esz);
+            hip_check(timed(prof_, KF_COPY, st, w.bytes, 0,
+                            [&] { return launch_copy<T>(Packer::at<CopyTask>(de, o_ie), (int)inv_init.size(), st); }),
+                      "identity blocks");
+        }
+        run_potrf<T>(st, de, potrfE, o_pe, "potrf elim");
+        run_getrf<T>(st, de, getrfE, o_ge, "getrf elim");
+        run_trsm<T>(st, de, trsmE, o_te, sp_te, "trsm elim");
+        if (!tiles.empty()) {
+            Work w = work_schur(tiles, contribs, esz);
+            hip_check(timed(prof_, KF_SCHUR, st, w.bytes, w.flops,
+                            [&] {
+                                return launch_schur<T>(Packer::at<SchurTile>(de, o_ti), (int)tiles.size(),
+                                                       Packer::at<SchurContrib>(de, o_co), st);
+                            }),
+                      "schur");
+        }
+        inv_init.clear();  // the scaling's identity inits follow
+    }
+    int dofs_left = 0;
+    for (int c : bot)
+        if (!cls_[c].elim) dofs_left += cls_[c].size;
+    lst.dofs_left_elim = dofs_left;
+}
+
+// Scaling plan (scale_cluster of every remaining cluster, src/tree.cpp:1510-1519, 796-856).
+template <typename T>
+void Tree::plan_scaling(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const bool gen = R.gen;
+    LevelStats& lst = *L.lst;
+    const std::vector<int>& bot = *L.bot;
+    auto& potrfS = L.potrfS;
+    auto& getrfS = L.getrfS;
+    auto& trsmA = L.trsmA;
+    auto& trsmB = L.trsmB;
+    auto& sv_scale = L.sv_scale;
+    auto& inv_init = L.inv_init;
+    auto& do_sparsify_level = L.do_sparsify_level;
+    auto& live = L.live;
+    auto& left = L.left;
+    auto push_trsm = [&L](std::vector<TrsmTask>& v, const void* A, void* B, int n, int m, int side, size_t es,
+                          const void* src = nullptr, const int* perm = nullptr) {
+        spand::push_trsm(v, A, B, n, m, side, es, src, perm, L.cwp);
+    };
+    do_sparsify_level = ilvl_ >= skip_;
+    for (int c : bot)
+        if (cls_[c].level > ilvl_) live.push_back(c);
+    if (do_sparsify_level && monitor_flops_)
+        for (int c : live) {  // scale_cluster: the pivot, then the panels of its in- and out-edges
+            const HCluster& s = cls_[c];
+            log_shape(gen ? SK_GETF : SK_POTF, s.size, 0, 0);
+            for (int e : s.in) log_shape(SK_PANEL, size(edges_[e].n1), s.size, 0);
+            for (size_t k = 1; k < s.out.size(); k++) log_shape(SK_PANEL, size(edges_[s.out[k]].n2), s.size, 0);
+        }
+    if (do_sparsify_level && gen) {
+        // scale_cluster, PLU branch (src/tree.cpp:831-849): an edge a -> b (block
+        // size(b) x size(a)) receives A U_a^{-1} from a and L_b^{-1} P_b^T A from b, in
+        // the order of a and b in the bottom list, into a fresh block at b's step
+        std::unordered_map<int, std::pair<void*, int*>> lu;  // cluster -> (U^T, perm)
+        for (int c : live) {
+            HCluster& s = cls_[c];
+            const int n = s.size;
+            const HEdge& piv = edges_[s.out[0]];
+            void* Ut = arena_.alloc((size_t)std::max(1, n * n) * sizeof(T));
+            int* pp = reinterpret_cast<int*>(arena_.alloc((size_t)std::max(1, n) * sizeof(int)));
+            lu[c] = {Ut, pp};
+            getrfS.push_back({piv.A, Ut, pp, n, c});
+            lst.flops_potrf += getrf_flops(n);
+            ops_.push_back({OP_SCALING_PLUQ, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr, Ut, pp});
+            factor_bytes_ += (size_t)n * (n + 1) * sizeof(T) + (size_t)n * sizeof(int);
+            if (n > 0) {
+                void* Li = arena_.alloc((size_t)n * n * sizeof(T));
+                void* Wi = arena_.alloc((size_t)n * n * sizeof(T));
+                inv_init.push_back({nullptr, Li, n, n, n, 0});
+                inv_init.push_back({nullptr, Wi, n, n, n, 0});
+                push_trsm(trsmA, piv.A, Li, n, n, 0, sizeof(T));
+                push_trsm(trsmA, Ut, Wi, n, n, 0, sizeof(T));
+                lst.flops_trsm += 2.0 * n * n * n;
+                sv_scale.push_back({Li, Wi, pp, n, (int)s.xoff});
+            }
+        }
+        for (int c : live) {
+            for (size_t k = 1; k < cls_[c].out.size(); k++) {
+                HEdge& e = edges_[cls_[c].out[k]];
+                const int a = e.n1, b = e.n2;  // block is size(b) x size(a)
+                const void* Lb = edges_[cls_[b].out[0]].A;
+                lst.flops_trsm += (double)size(a) * size(b) * (size(a) + size(b));
+                const bool a_first = cls_[a].order < cls_[b].order;
+                void* nb = arena_.alloc((size_t)std::max(1, size(a) * size(b)) * sizeof(T));
+                if (a_first) {
+                    push_trsm(trsmA, lu[a].first, e.A, size(a), size(b), 1, sizeof(T));
+                    push_trsm(trsmB, Lb, nb, size(b), size(a), 0, sizeof(T), e.A, lu[b].second);
+                } else {
+                    push_trsm(trsmA, Lb, nb, size(b), size(a), 0, sizeof(T), e.A, lu[b].second);
+                    push_trsm(trsmB, lu[a].first, nb, size(a), size(b), 1, sizeof(T));
+                }
+                // a zero-sized block keeps its (unread) pointer
+                if (size(a) > 0 && size(b) > 0) e.A = nb;
+            }
+        }
+        for (int c : live) {
+            HEdge& piv = edges_[cls_[c].out[0]];
+            piv.A = nullptr;
+            piv.ident = true;
+        }
+    } else if (do_sparsify_level) {
+        for (int c : live) {
+            HCluster& s = cls_[c];
+            const HEdge& piv = edges_[s.out[0]];
+            potrfS.push_back({piv.A, s.size, c, nullptr});
+            lst.flops_potrf += potrf_flops(s.size);
+            ops_.push_back({OP_SCALING_LLT, c, -1, s.size, 0, s.size, s.size, ilvl_, piv.A, nullptr});
+            factor_bytes_ += (size_t)s.size * (s.size + 1) / 2 * sizeof(T);
+            if (s.size > 0) {
+                void* Li = arena_.alloc((size_t)s.size * s.size * sizeof(T));
+                inv_init.push_back({nullptr, Li, s.size, s.size, s.size, 0});
+                push_trsm(trsmA, piv.A, Li, s.size, s.size, 0, sizeof(T));
+                lst.flops_trsm += (double)s.size * s.size * s.size;
+                sv_scale.push_back({Li, Li, nullptr, s.size, (int)s.xoff});
+            }
+        }
+        for (int c : live) {
+            for (size_t k = 1; k < cls_[c].out.size(); k++) {
+                const HEdge& e = edges_[cls_[c].out[k]];
+                const int a = e.n1, b = e.n2;  // block is size(b) x size(a)
+                const void* La = edges_[cls_[a].out[0]].A;
+                const void* Lb = edges_[cls_[b].out[0]].A;
+                lst.flops_trsm += (double)size(a) * size(b) * (size(a) + size(b));
+                const bool a_first = cls_[a].order < cls_[b].order;
+                // right solve by L_a (B is size(b) x size(a)), left solve by L_b
+                push_trsm(a_first ? trsmA : trsmB, La, e.A, size(a), size(b), 1, sizeof(T));
+                push_trsm(a_first ? trsmB : trsmA, Lb, e.A, size(b), size(a), 0, sizeof(T));
+            }
+        }
+        for (int c : live) {
+            HEdge& piv = edges_[cls_[c].out[0]];
+            piv.A = nullptr;
+            piv.ident = true;
+        }
+    }
+}
+
+// Sparsification plan (src/tree.cpp:1521-1533, 1292-1347): dependency waves, the RRQR
+// path of each cluster (register, one-workgroup LDS or stepped slabs) and its buffers.
+template <typename T>
+void Tree::plan_sparsification(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const long long lds_elems = R.lds_elems;
+    const long long big_min_rows = R.big_min_rows;
+    LevelStats& lst = *L.lst;
+    auto& shape = L.shape;
+    auto& tune = L.tune;
+    auto& do_sparsify_level = L.do_sparsify_level;
+    auto& live = L.live;
+    auto& plist = L.plist;
+    auto& wave = L.wave;
+    auto& widx = L.widx;
+    auto& nwaves = L.nwaves;
+    auto& wtasks = L.wtasks;
+    auto& wlds_small = L.wlds_small;
+    auto& wbig = L.wbig;
+    auto& wslab = L.wslab;
+    auto& wsteps = L.wsteps;
+    auto& wmaxrows = L.wmaxrows;
+    auto& wreg = L.wreg;
+    auto& wregvar = L.wregvar;
+    auto& where = L.where;
+    auto& segs = L.segs;
+    auto& wlds_step = L.wlds_step;
+    auto& wlds_wave = L.wlds_wave;
+    auto& persistent = L.persistent;
+    auto& step_waves = L.step_waves;
+    auto& wflag0 = L.wflag0;
+    if (do_sparsify_level) {
+        std::vector<int> inP(cls_.size(), -1);
+        for (int c : live) {
+            if (use_want_sparsify_ && !cls_[c].spars) {
+                lst.nignored++;
+                continue;
+            }
+            inP[c] = (int)plist.size();
+            plist.push_back(c);
+        }
+        wave.assign(plist.size(), 0);
+        for (size_t p = 0; p < plist.size(); p++) {
+            const int c = plist[p];
+            int w = 0;
+            auto visit = [&](int nb) {
+                if (inP[nb] >= 0 && cls_[nb].order < cls_[c].order) w = std::max(w, wave[inP[nb]] + 1);
+            };
+            for (int e : cls_[c].in) visit(edges_[e].n1);
+            for (size_t k = 1; k < cls_[c].out.size(); k++) visit(edges_[cls_[c].out[k]].n2);
+            wave[p] = w;
+            nwaves = std::max(nwaves, w + 1);
+        }
+        wtasks.assign(nwaves, {});
+        wlds_small.assign(nwaves, 0);
+        wbig.assign(nwaves, {});
+        wslab.assign(nwaves, {});
+        wsteps.assign(nwaves, 0);
+        wmaxrows.assign(nwaves, 0);
+        wreg.assign(nwaves, {});
+        wregvar.assign(nwaves, {});
+        where.assign(plist.size(), {0, 0});
+        std::vector<std::vector<std::pair<int, int>>> regp(nwaves);  // (variant, plist position)
+        std::vector<RrqrTask> ptask(plist.size());
+        {  // the sparsification knobs from the level's Asn shapes
+            std::vector<int> rows, cols;
+            for (int c : plist) {
+                const HCluster& s = cls_[c];
+                int nc = 0;
+                for (int e : s.in) nc += size(edges_[e].n1);
+                for (size_t k = 1; k < s.out.size(); k++) nc += size(edges_[s.out[k]].n2);
+                rows.push_back(s.size);
+                cols.push_back(nc);
+            }
+            shape.n_spars = (int)plist.size();
+            if (!rows.empty()) {
+                std::sort(rows.begin(), rows.end());
+                std::sort(cols.begin(), cols.end());
+                shape.rows_med = rows[rows.size() / 2];
+                shape.rows_max = rows.back();
+                shape.cols_med = cols[cols.size() / 2];
+                shape.cols_max = cols.back();
+            }
+            const LevelTune t = choose_tune(sizeof(T), shape);
+            tune.rrqr_reg = t.rrqr_reg;
+            tune.small_elems = t.small_elems;
+            tune.step_waves = t.step_waves;
+            apply_tune_env(tune);
+        }
+        const bool reg_path = tune.rrqr_reg != 0;
+        const long long big_min_elems = tune.small_elems;
+        widx.assign(plist.size(), 0);
+        for (size_t p = 0; p < plist.size(); p++) {
+            const int c = plist[p];
+            const HCluster& s = cls_[c];
+            RrqrTask t{};
+            t.cluster = c;
+            t.seg0 = (int)segs.size();
+            int cols = 0;
+            for (int e : s.in) {
+                segs.push_back({edges_[e].A, edges_[e].n1, 0});
+                cols += size(edges_[e].n1);
+            }
+            for (size_t k = 1; k < s.out.size(); k++) {
+                const HEdge& e = edges_[s.out[k]];
+                segs.push_back({e.A, e.n2, 1});
+                cols += size(e.n2);
+            }
+            t.nseg = (int)segs.size() - t.seg0;
+            if (t.nseg > 1024) throw Error(ERR_UNSUPPORTED, "cluster with more than 1024 neighbours");
+            t.max_rows = s.size;
+            t.max_cols = cols;
+            const int mn = std::min(s.size, cols);
+            t.Q = arena_.alloc((size_t)std::max(1, s.size * mn) * sizeof(T));
+            t.tau = arena_.alloc((size_t)std::max(1, mn) * sizeof(T));
+            lst.flops_rrqr += mn > 0 ? 2.0 * s.size * cols * mn - (2.0 / 3.0) * (double)mn * mn * mn : 0.0;
+            const int w = wave[p];
+            const int reg_var = reg_path ? rrqr_reg_variant(sizeof(T), s.size, cols) : -1;
+            if (reg_var >= 0) {
+                // register-resident, one workgroup per cluster
+                ptask[p] = t;
+                regp[w].push_back({reg_var, (int)p});
+            } else if (s.size >= big_min_rows && (long long)s.size * cols > big_min_elems) {
+                // stepped multi-workgroup path
+                RrqrBig b{};
+                b.nslab = std::max(1, (cols + 63) / 64);
+                t.scratch = arena_.alloc((size_t)s.size * b.nslab * 64 * sizeof(T));
+                t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * std::max(cols, 1) * sizeof(double)));
+                b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
+                b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)(2 * b.nslab + 1) * sizeof(double)));
+                b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
+                b.bar = reinterpret_cast<int*>(arena_.alloc((size_t)b.nslab * sizeof(int)));  // per-slab step counts
+                b.t = t;
+                where[p] = {1, (int)wbig[w].size()};
+                for (int k = 0; k < b.nslab; k++) wslab[w].push_back({(int)wbig[w].size(), k});
+                wbig[w].push_back(b);
+                wsteps[w] = std::max(wsteps[w], mn);
+                wmaxrows[w] = std::max(wmaxrows[w], s.size);
+            } else {
+                // LDS-resident with an odd column stride (rows | 1), else a global scratch
+                if ((long long)(s.size | 1) * cols > lds_elems)
+                    t.scratch = arena_.alloc((size_t)s.size * cols * sizeof(T));
+                else
+                    wlds_small[w] =
+                        std::max(wlds_small[w], (long long)(s.size | 1) * cols * (long long)sizeof(T));
+                if (cols > 1024) {
+                    t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * cols * sizeof(double)));
+                    t.jpvt = reinterpret_cast<int*>(arena_.alloc((size_t)cols * sizeof(int)));
+                }
+                where[p] = {0, (int)wtasks[w].size()};
+                wtasks[w].push_back(t);
+            }
+        }
+        for (int w = 0; w < nwaves; w++) {
+            std::stable_sort(regp[w].begin(), regp[w].end(),
+                             [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
+            for (auto& vp : regp[w]) {
+                where[vp.second] = {2, (int)wreg[w].size()};
+                wreg[w].push_back(ptask[vp.second]);
+                wregvar[w].push_back(vp.first);
+            }
+        }
+    }
+    // pivot-column / reflector buffers of the stepped path: LDS unless a wave's largest
+    // cluster needs more than STEP_LDS_MAX, then global buffers for the whole wave
+    // A wave whose clusters all have at most WAVE_ROWS_MAX rows runs as one persistent
+    // launch (candidate-column buffers in vbuf); the per-step launches remain the path
+    // for larger clusters and for waves the device cannot hold at once.
+    wlds_step.assign(nwaves, 0);
+    wlds_wave.assign(nwaves, 0);
+    // Opt-in (SPAND_RRQR_PERSISTENT=1): the persistent launch with per-cluster barriers
+    // measured slower on gfx950 than one launch per step (64^3 FP32 factor + PCG: 177 ms
+    // against 125 ms per step), although a flag handshake between two workgroups costs
+    // about 1 us one way: the step body itself is slower in the 1024-thread persistent form.
+    persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
+    step_waves = tune.step_waves;  // 4, 8, 16; 0: by slab count
+    tunes_[ilvl_] = tune;
+    shapes_[ilvl_] = shape;
+    for (int w = 0; w < nwaves; w++) {
+        const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
+        int wmaxcols = 0;
+        for (auto& b : wbig[w]) wmaxcols = std::max(wmaxcols, b.t.max_cols);
+        const size_t lw = ((size_t)wmaxcols * sizeof(int) + 15) / 16 * 16 + 2 * vb;
+        if (persistent && !wbig[w].empty() && wmaxrows[w] <= WAVE_ROWS_MAX && lw <= (size_t)WAVE_LDS_MAX) {
+            wlds_wave[w] = lw;
+            for (auto& b : wbig[w])
+                b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
+        } else if (2 * vb <= (size_t)STEP_LDS_MAX) {
+            wlds_step[w] = 2 * vb;
+        } else {
+            for (auto& b : wbig[w])
+                b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
+        }
+    }
+    // dims (one block per level) and done flags of the stepped clusters; the flags are
+    // reset before any of this level's launches
+    wflag0.assign(nwaves, 0);
+    {
+        size_t nbig = 0;
+        for (int w = 0; w < nwaves; w++) nbig += wbig[w].size();
+        int* dblock = nullptr;
+        if (nbig > 0) {
+            signal_reserve(1 + nbig);
+            dblock = reinterpret_cast<int*>(arena_.alloc(nbig * 5 * sizeof(int)));
+            size_t f = 1;
+            for (int w = 0; w < nwaves; w++) {
+                wflag0[w] = f;
+                for (auto& b : wbig[w]) {
+                    h_sig_[f] = 0;
+                    b.done = d_sig_ + f;
+                    b.dims = dblock + 5 * (f - 1);
+                    f++;
+                }
+            }
+        }
+        level_dims_.push_back({dblock, nbig});
+    }
+    lst.nwaves = nwaves;
+}
+
+// Uploads and launches the scaling and the sparsification waves, then reads the level's
+// pivot status and the kept ranks back.
+template <typename T>
+void Tree::launch_level(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const hipStream_t st = R.st;
+    const bool gen = R.gen;
+    const size_t esz = sizeof(T);
+    std::vector<int>& hsize = R.hsize;
+    LevelStats& lst = *L.lst;
+    auto& potrfS = L.potrfS;
+    auto& getrfS = L.getrfS;
+    auto& trsmA = L.trsmA;
+    auto& trsmB = L.trsmB;
+    auto& inv_init = L.inv_init;
+    auto& live = L.live;
+    auto& plist = L.plist;
+    auto& wave = L.wave;
+    auto& nwaves = L.nwaves;
+    auto& wtasks = L.wtasks;
+    auto& wlds_small = L.wlds_small;
+    auto& wbig = L.wbig;
+    auto& wslab = L.wslab;
+    auto& wsteps = L.wsteps;
+    auto& wmaxrows = L.wmaxrows;
+    auto& wreg = L.wreg;
+    auto& wregvar = L.wregvar;
+    auto& segs = L.segs;
+    auto& wlds_step = L.wlds_step;
+    auto& wlds_wave = L.wlds_wave;
+    auto& persistent = L.persistent;
+    auto& step_waves = L.step_waves;
+    auto& wflag0 = L.wflag0;
+    auto& t_lvl0 = L.t_lvl0;
+    auto& t_plan1 = L.t_plan1;
+    auto& t_lvl1 = L.t_lvl1;
+    for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
+    small_first(potrfS);
+    for (PotrfTask& t : potrfS)
+        if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
+    Packer pk;
+    const int sp_ta = split_trsm(trsmA), sp_tb = split_trsm(trsmB);
+    size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
+    size_t o_ii = pk.add(inv_init);
+    lds_first<T>(getrfS);
+    size_t o_gs = pk.add(getrfS);
+    std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves), o_r(nwaves);
+    for (int w = 0; w < nwaves; w++) {
+        o_r[w] = pk.add(wreg[w]);
+        o_w[w] = pk.add(wtasks[w]);
+        o_b[w] = pk.add(wbig[w]);
+        o_s[w] = pk.add(wslab[w]);
+    }
+    L.t_plan1 = wall();
+    void* dt = upload_tasks(pk);
+    stage_upload(d_size_, hsize.data(), cls_.size() * sizeof(int));
+    if (!inv_init.empty()) {
+        Work w = work_copy(inv_init, esz);
+        hip_check(timed(prof_, KF_COPY, st, w.bytes, 0,
+                        [&] { return launch_copy<T>(Packer::at<CopyTask>(dt, o_ii), (int)inv_init.size(), st); }),
+                  "identity blocks");
+    }
+    run_potrf<T>(st, dt, potrfS, o_ps, "potrf scale");
+    run_getrf<T>(st, dt, getrfS, o_gs, "getrf scale");
+    run_trsm<T>(st, dt, trsmA, o_ta, sp_ta, "trsm scale A");
+    run_trsm<T>(st, dt, trsmB, o_tb, sp_tb, "trsm scale B");
+    // stepped-path launch slots, for the post-hoc work attribution below
+    std::vector<int> slot_init(nwaves, -1), slot_finish(nwaves, -1);
+    std::vector<int> slot_step(nwaves, -1);
+    for (int w = 0; w < nwaves; w++) {
+        // the register-path clusters of a wave run on the side stream, beside the
+        // one-workgroup and stepped launches on the main stream; both join before the
+        // next wave (whose clusters read the edges this wave rewrites)
+        const bool side = !wreg[w].empty() && (!wtasks[w].empty() || !wbig[w].empty());
+        const hipStream_t rst = side ? aux_stream() : st;
+        if (side) {
+            hip_check(hipEventRecord(ev_fork_, st), "fork");
+            hip_check(hipStreamWaitEvent(rst, ev_fork_, 0), "fork");
+        }
+        if (!wreg[w].empty()) {
+            // one launch per variant run (tasks are sorted by variant); a single launch
+            // when the variant of the largest rows and columns holds every task
+            int mr = 0, mc = 0;
+            for (auto& t : wreg[w]) {
+                mr = std::max(mr, t.max_rows);
+                mc = std::max(mc, t.max_cols);
+            }
+            const int cover = rrqr_reg_variant(sizeof(T), mr, mc);
+            size_t a0 = 0;
+            while (a0 < wreg[w].size()) {
+                size_t a1 = cover >= 0 ? wreg[w].size() : a0 + 1;
+                if (cover < 0)
+                    while (a1 < wreg[w].size() && wregvar[w][a1] == wregvar[w][a0]) a1++;
+                const int var = cover >= 0 ? cover : wregvar[w][a0];
+                Work wk;
+                for (size_t x = a0; x < a1; x++) {
+                    const RrqrTask& t = wreg[w][x];
+                    wk.bytes += (2.0 * t.max_rows * t.max_cols + (double)t.max_rows * std::min(t.max_rows, t.max_cols)) * esz;
+                    wk.flops += geqp3_flops(t.max_rows, t.max_cols);
+                }
+                hip_check(timed(prof_, KF_RRQR_REG, rst, wk.bytes, wk.flops,
+                                [&] {
+                                    return launch_rrqr_reg<T>(Packer::at<RrqrTask>(dt, o_r[w]) + a0, (int)(a1 - a0),
+                                                              Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, var, rst);
+                                }),
+                          "rrqr reg");
+                a0 = a1;
+            }
+        }
+        // join at the end of the wave (below), also on early exits of the loop body
+        struct Join {
+            bool on;
+            hipStream_t main, side;
+            hipEvent_t ev;
+            ~Join() {
+                if (on && hipEventRecord(ev, side) == hipSuccess) (void)hipStreamWaitEvent(main, ev, 0);
+            }
+        } join{side, st, rst, ev_join_};
+        if (!wtasks[w].empty()) {
+            Work wk;
+            for (auto& t : wtasks[w]) {
+                wk.bytes += (2.0 * t.max_rows * t.max_cols + (double)t.max_rows * std::min(t.max_rows, t.max_cols)) * esz;
+                wk.flops += geqp3_flops(t.max_rows, t.max_cols);
+            }
+            hip_check(timed(prof_, KF_RRQR, st, wk.bytes, wk.flops,
+                            [&] {
+                                return launch_rrqr<T>(Packer::at<RrqrTask>(dt, o_w[w]), (int)wtasks[w].size(),
+                                                      Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
+                                                      (int)std::max(wlds_small[w], (long long)LDS_BYTES), st);
+                            }),
+                      "rrqr");
+        }
+        if (wbig[w].empty()) continue;
+        const RrqrBig* db = Packer::at<RrqrBig>(dt, o_b[w]);
+        const RrqrSlab* ds = Packer::at<RrqrSlab>(dt, o_s[w]);
+        const int nsl = (int)wslab[w].size();
+        hip_check(timed(prof_, KF_RRQR_INIT, st, 0, 0,
+                        [&] { return launch_rrqr_init<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st); },
+                        &slot_init[w]),
+                  "rrqr init");
+        // the steps of a wave are timed as one batch (two events per wave)
+        bool done = false;
+        if (wlds_wave[w] > 0) {
+            // whole clusters in chunks the device holds at once, one cooperative launch each
+            int cap = 0;
+            hip_check(rrqr_wave_capacity<T>(wmaxrows[w], wlds_wave[w], &cap), "rrqr wave capacity");
+            int maxslab = 0;
+            for (auto& b : wbig[w]) maxslab = std::max(maxslab, b.nslab);
+            if (cap >= maxslab && cap > 0) {
+                const int sl = prof_.start(KF_RRQR_WAVE, st);
+                int launches = 0;
+                size_t x0 = 0, s0 = 0;
+                while (x0 < wbig[w].size()) {
+                    size_t x1 = x0, s1 = s0;
+                    int steps = 0;
+                    while (x1 < wbig[w].size() && (int)(s1 - s0) + wbig[w][x1].nslab <= cap) {
+                        steps = std::max(steps, std::min(wbig[w][x1].t.max_rows, wbig[w][x1].t.max_cols));
+                        s1 += wbig[w][x1].nslab;
+                        x1++;
+                    }
+                    const hipError_t e = launch_rrqr_wave<T>(db, ds + s0, (int)(s1 - s0), steps, wmaxrows[w], tol_,
+                                                             wlds_wave[w], d_info_ + 1, st);
+                    hip_check(e, "rrqr wave");
+                    launches++;
+                    x0 = x1;
+                    s0 = s1;
+                }
+                prof_.stop(sl, st, 0, 0, launches);
+                done = true;
+                slot_step[w] = sl;
+            }
+        }
+        if (!done) {
+            // Steps are launched at most RRQR_STEPS_AHEAD ahead of the device, and none
+            // once every cluster of the wave has flagged its last working step: the
+            // ranks (decided on the device) end most clusters well before min(rows, cols).
+            slot_step[w] = prof_.start(KF_RRQR_STEP, st);
+            std::vector<int> live(wbig[w].size());
+            std::iota(live.begin(), live.end(), 0);
+            std::vector<long long> seq(wsteps[w], 0);
+            int launched = 0;
+            for (int i = 0; i < wsteps[w]; i++) {
+                if (i >= RRQR_STEPS_AHEAD) wait_signal(seq[i - RRQR_STEPS_AHEAD]);
+                size_t k = 0;
+                for (int x : live) {
+                    const RrqrBig& b = wbig[w][x];
+                    const volatile long long* f = h_sig_ + wflag0[w] + x;
+                    if (*f == 0 && i < std::min(b.t.max_rows, b.t.max_cols)) live[k++] = x;
+                }
+                live.resize(k);
+                if (live.empty()) break;
+                seq[i] = ++sig_seq_;
+                hip_check(launch_rrqr_step<T>(db, ds, nsl, i, wmaxrows[w], tol_, wlds_step[w], step_waves, d_sig_,
+                                              seq[i], st),
+                          "rrqr step");
+                launched++;
+            }
+            prof_.stop(slot_step[w], st, 0, 0, launched);
+        }
+        hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,
+                        [&] {
+                            return launch_rrqr_finish<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
+                                                         0, st);
+                        },
+                        &slot_finish[w]),
+                  "rrqr finish");
+    }
+    int info2[2] = {0, 0};
+    hip_check(hipMemcpy(info2, d_info_, 2 * sizeof(int), hipMemcpyDeviceToHost), "read info");
+    const int info = info2[0];
+    if (info2[1] != 0) throw Error(ERR_HIP, "persistent RRQR: a cluster barrier wait gave up");
+    if (info != 0) {
+        if (gen) throw Error(ERR_SINGULAR, "Error: Singular Pivot (cluster order " + std::to_string(info - 1) + ")");
+        throw Error(ERR_NOT_SPD, "Error: Non-SPD Pivot (cluster order " + std::to_string(info - 1) + ")");
+    }
+    if (!plist.empty())
+        hip_check(hipMemcpy(hsize.data(), d_size_, cls_.size() * sizeof(int), hipMemcpyDeviceToHost), "read ranks");
+    if (prof_.on())  // the stepped path's sizes are decided on the device: priced at the end
+        for (int w = 0; w < nwaves; w++)
+            if (!wbig[w].empty())
+                wave_work_.push_back({slot_init[w], slot_finish[w], slot_step[w], wsteps[w], wflag0[w] - 1,
+                                      wbig[w].size(), ilvl_});
+    L.t_lvl1 = wall();
+    lst.t_plan = t_plan1 - t_lvl0;
+    lst.t_elim_scale = t_lvl1 - t_plan1;
+}
+
+// Records the Orthogonal, Split and sibling ops of the kept ranks (src/tree.cpp:1317-1342)
+// and plans the compact-WY factors of the solve.
+template <typename T>
+void Tree::apply_ranks(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const bool gen = R.gen;
+    std::vector<int>& hsize = R.hsize;
+    LevelStats& lst = *L.lst;
+    auto& plist = L.plist;
+    auto& wave = L.wave;
+    auto& wtasks = L.wtasks;
+    auto& wbig = L.wbig;
+    auto& wreg = L.wreg;
+    auto& where = L.where;
+    auto& qops = L.qops;
+    auto& wy = L.wy;
+    auto& wy_tiles = L.wy_tiles;
+    auto& wy_trsm = L.wy_trsm;
+    auto& t_lvl1 = L.t_lvl1;
+    auto& t_lvl2 = L.t_lvl2;
+    auto push_trsm = [&L](std::vector<TrsmTask>& v, const void* A, void* B, int n, int m, int side, size_t es,
+                          const void* src = nullptr, const int* perm = nullptr) {
+        spand::push_trsm(v, A, B, n, m, side, es, src, perm, L.cwp);
+    };
+    for (size_t p = 0; p < plist.size(); p++) {
+        const int c = plist[p];
+        const int r = hsize[c];
+        const int s0 = cls_[c].size;
+        lst.rank_before += s0;
+        lst.nsparsified++;
+        if (monitor_flops_) {
+            // Asn as the sequential loop sees it: neighbours earlier in the list already
+            // hold their kept rank
+            int ncols = 0;
+            for (int e : cls_[c].in) ncols += size(edges_[e].n1);
+            for (size_t k = 1; k < cls_[c].out.size(); k++) ncols += size(edges_[cls_[c].out[k]].n2);
+            log_shape(SK_RRQR, s0, ncols, 0);
+        }
+        if (r < s0) {
+            const RrqrTask* t = where[p].first == 1   ? &wbig[wave[p]][where[p].second].t
+                                : where[p].first == 2 ? &wreg[wave[p]][where[p].second]
+                                                      : &wtasks[wave[p]][where[p].second];
+            ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
+            factor_bytes_ += ((size_t)s0 * r + (size_t)r * (r + 1) / 2) * sizeof(T);
+            if (r > 0) {
+                // compact WY over all r kept reflectors, for the solve: Q = I - Ve T Ve^T
+                void* Ve = arena_.alloc((size_t)s0 * r * sizeof(T));
+                void* Sm = arena_.alloc((size_t)r * r * sizeof(T));
+                void* X = arena_.alloc((size_t)r * r * sizeof(T));
+                const int tw = (int)wy.size();
+                wy.push_back({t->Q, t->tau, Ve, Sm, X, s0, r});
+                for (int c0 = 0; c0 < r; c0 += 32)
+                    for (int a0 = 0; a0 <= c0; a0 += 32) wy_tiles.push_back(make_int4(tw, a0, c0, 0));
+                push_trsm(wy_trsm, Sm, X, r, r, 0, sizeof(T));
+                qops.push_back({Ve, X, s0, r, (int)cls_[c].xoff, (int)ws_len_});
+                ws_len_ += 2 * r;  // the y and z segments of the apply
+            }
+            HCluster sib;
+            sib.start = cls_[c].start + r;
+            sib.size = sib.osize = s0 - r;
+            sib.level = cls_[c].level;
+            sib.spars = cls_[c].spars;
+            sib.elim = true;
+            sib.order = (int)cls_.size();
+            sib.xoff = cls_[c].xoff + r;
+            const int sid = (int)cls_.size();
+            cls_.push_back(sib);
+            others_.push_back(sid);
+            cls_[c].size = r;
+            ops_.push_back({OP_SPLIT, c, sid, r, s0 - r, 0, 0, ilvl_, nullptr, nullptr});
+            ops_.push_back({gen ? OP_SCALING_PLUQ : OP_SCALING_LLT, sid, -1, s0 - r, 0, s0 - r, s0 - r, ilvl_,
+                            nullptr, nullptr});
+            log_shape(gen ? SK_GETF : SK_POTF, s0 - r, 0, 0);  // eliminate_cluster(sibling)
+            // the sibling's pivot is the identity and is never stored (src/tree.cpp:1076-1099)
+        }
+        lst.rank_after += cls_[c].size;
+    }
+    L.t_lvl2 = wall();
+    lst.t_spars = t_lvl2 - t_lvl1;
+}
+
+// Merge (src/tree.cpp:1435-1445, 1106-1184): parents, their merged edges, and the
+// compact-WY launches of the Orthogonal ops.
+template <typename T>
+void Tree::merge_level(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const hipStream_t st = R.st;
+    LevelStats& lst = *L.lst;
+    auto& wy = L.wy;
+    auto& wy_tiles = L.wy_tiles;
+    auto& wy_trsm = L.wy_trsm;
+    auto& mt = L.mt;
+    auto& mchildren = L.mchildren;
+    auto& left = L.left;
+    auto& t_lvl2 = L.t_lvl2;
+    if (ilvl_ < nlevels_ - 1) {
+        current_bottom_++;
+        const std::vector<int>& nb = bottoms_[current_bottom_];
+        std::vector<int> pos(cls_.size(), 0);  // a child's row offset inside its parent
+        for (int p : nb) {
+            HCluster& P = cls_[p];
+            int off = 0;
+            MergeTask m{(int)ws_len_, (int)mchildren.size(), 0, 0};
+            for (int ch : P.children) {
+                pos[ch] = off;
+                mchildren.push_back(make_int2((int)cls_[ch].xoff, cls_[ch].size));
+                off += cls_[ch].size;
+            }
+            m.c1 = (int)mchildren.size();
+            P.size = P.osize = off;
+            P.xoff = ws_len_;
+            ws_len_ += off;
+            ops_.push_back({OP_MERGE, p, -1, off, 0, 0, 0, ilvl_, nullptr, nullptr});
+            if (off > 0) mt.push_back(m);
+        }
+        long long melems = 0;
+        std::vector<long long> moff;   // by new edge id - first_new
+        const int first_new = (int)edges_.size();
+        struct CT { int dst_edge; long long dst_add; const void* src; int rows, cols; };
+        std::vector<CT> cts;
+        // per parent: its new neighbours in order of first appearance over the children's
+        // out-edges (stamped slots instead of searches), each new edge original if any
+        // merged child edge is
+        std::vector<int> stamp(cls_.size(), -1), eid(cls_.size(), -1);
+        std::vector<char> origs;
+        std::vector<int> nnews;
+        for (int p : nb) {
+            nnews.clear();
+            origs.clear();
+            for (int ch : cls_[p].children)
+                for (int e : cls_[ch].out) {
+                    const int nn = cls_[edges_[e].n2].parent;
+                    if (stamp[nn] != p) {
+                        stamp[nn] = p;
+                        eid[nn] = (int)nnews.size();
+                        nnews.push_back(nn);
+                        origs.push_back(0);
+                    }
+                    if (edges_[e].original) origs[eid[nn]] = 1;
+                }
+            for (size_t q = 0; q < nnews.size(); q++) {
+                const int nn = nnews[q];
+                eid[nn] = new_edge(p, nn, nullptr, origs[q] != 0);
+                moff.push_back(melems);
+                melems += (long long)size(nn) * size(p);
+            }
+            auto& out = cls_[p].out;
+            std::stable_sort(out.begin() + 1, out.end(),
+                             [&](int a, int b) { return cls_[edges_[a].n2].order < cls_[edges_[b].n2].order; });
+            auto& in = cls_[p].in;
+            std::stable_sort(in.begin(), in.end(),
+                             [&](int a, int b) { return cls_[edges_[a].n1].order < cls_[edges_[b].n1].order; });
+            for (int ch : cls_[p].children) {
+                for (int e : cls_[ch].out) {
+                    int nold = edges_[e].n2, nn = cls_[nold].parent;
+                    int de = eid[nn];
+                    long long add = pos[nold] + (long long)pos[ch] * size(nn);
+                    if (size(nold) > 0 && size(ch) > 0)
+                        cts.push_back({de, add, edges_[e].ident ? nullptr : edges_[e].A, size(nold), size(ch)});
+                }
+                cls_[ch].out.clear();
+                cls_[ch].in.clear();
+            }
+        }
+        T* mbase = reinterpret_cast<T*>(arena_.alloc((size_t)std::max<long long>(melems, 1) * sizeof(T)));
+        for (size_t k = first_new; k < edges_.size(); k++) edges_[k].A = mbase + moff[k - first_new];
+        std::vector<CopyTask> copies;
+        copies.reserve(cts.size());
+        for (auto& c : cts) {
+            const HEdge& d = edges_[c.dst_edge];
+            copies.push_back({c.src, reinterpret_cast<T*>(d.A) + c.dst_add, c.rows, c.cols, size(d.n2), 0});
+        }
+        Packer pm;
+        size_t o_c = pm.add(copies);
+        void* dm = upload_tasks(pm);
+        if (melems > 0)
+            hip_check(timed(prof_, KF_FILL, st, (double)melems * sizeof(T), 0,
+                            [&] { return launch_fill<T>(mbase, (size_t)melems, T(0), st); }),
+                      "fill merge");
+        Work wc = work_copy(copies, sizeof(T));
+        hip_check(timed(prof_, KF_COPY, st, wc.bytes, 0,
+                        [&] { return launch_copy<T>(Packer::at<CopyTask>(dm, o_c), (int)copies.size(), st); }),
+                  "merge copy");
+    }
+    if (!wy.empty()) {
+        Packer pl;
+        const int sp_wy = split_trsm(wy_trsm);
+        size_t o_w = pl.add(wy), o_t = pl.add(wy_tiles), o_s = pl.add(wy_trsm);
+        void* dl = upload_tasks(pl);
+        double lb = 0, lf = 0;
+        for (auto& w : wy) {
+            lb += (3.0 * w.m * w.k + 3.0 * w.k * w.k) * sizeof(T);
+            lf += (double)w.m * w.k * w.k;
+        }
+        hip_check(timed(prof_, KF_LARFT, st, lb, lf,
+                        [&] {
+                            return launch_wy<T>(Packer::at<WyTask>(dl, o_w), (int)wy.size(),
+                                                Packer::at<int4>(dl, o_t), (int)wy_tiles.size(), st);
+                        }),
+                  "compact WY");
+        Work w = work_trsm(wy_trsm, sizeof(T));
+        hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
+                        [&] {
+                            return launch_trsm<T>(Packer::at<TrsmTask>(dl, o_s), (int)wy_trsm.size(), sp_wy, st);
+                        }),
+                  "compact WY inverse");
+    }
+    lst.t_merge = wall() - t_lvl2;
+    left = 0;
+    for (int c : bottoms_[current_bottom_])
+        if (!cls_[c].elim) left += cls_[c].size;
+    lst.dofs_left_spars = left;
+    lst.fact_nnz = nnz();
+}
+
+// The level's preconditioner-apply plan: forward terms grouped now, the rest built on a
+// worker thread.
+template <typename T>
+void Tree::plan_level_solve(LevelPlan<T>& L) {
+    FactorRun& R = *L.run;
+    const size_t esz = sizeof(T);
+    auto& pending_solve = R.pending_solve;
+    LevelStats& lst = *L.lst;
+    auto& potrfS = L.potrfS;
+    auto& sv_elim = L.sv_elim;
+    auto& sv_scale = L.sv_scale;
+    auto& fwd_list = L.fwd_list;
+    auto& bwd_groups = L.bwd_groups;
+    auto& bwd_terms = L.bwd_terms;
+    auto& n_potrf_elim = L.n_potrf_elim;
+    auto& plist = L.plist;
+    auto& nwaves = L.nwaves;
+    auto& qops = L.qops;
+    auto& mt = L.mt;
+    auto& mchildren = L.mchildren;
+    auto& left = L.left;
+    // The forward terms are grouped here (the grouping reads cluster offsets, and the
+    // cluster array grows at the next merge); the rest is built on a worker thread from
+    // moved copies of this level's lists and uploaded once the factorization is done.
+    const double t_sp0 = wall();
+    {
+        std::vector<GemvGroup> fwd_groups;
+        std::vector<GemvTerm> fwd_t;
+        {
+            std::vector<int> gstart, gkeys;
+            const std::vector<int> forder = group_by_first(
+                (int)fwd_list.size(), (int)cls_.size(), [&](int k) { return fwd_list[k].first; }, gstart, gkeys);
+            for (size_t gi = 0; gi < gkeys.size(); gi++) {
+                const int k0 = (int)fwd_t.size();
+                for (int q = gstart[gi]; q < gstart[gi + 1]; q++) fwd_t.push_back(fwd_list[forder[q]].second);
+                const GemvTerm& t0 = fwd_t[k0];
+                const int n = t0.trans ? t0.cols : t0.rows;
+                fwd_groups.push_back({(int)cls_[gkeys[gi]].xoff, n, k0, (int)fwd_t.size()});
+            }
+        }
+        HostPhases in{std::move(sv_elim),    std::move(sv_scale),  std::move(fwd_groups), std::move(bwd_groups),
+                      std::move(fwd_t),      std::move(bwd_terms), std::move(qops)};
+        pending_solve[ilvl_] =
+            std::async(std::launch::async, [esz = sizeof(T), in = std::move(in), mt = std::move(mt),
+                                            mchildren = std::move(mchildren)]() mutable {
+                return build_solve_plan(esz, std::move(in), std::move(mt), std::move(mchildren));
+            });
+    }
+    lst.t_solve_plan = wall() - t_sp0;
+    if (verb_)
+        printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, n_potrf_elim,
+               potrfS.size(), plist.size(), nwaves, left);
+}
+
 template <typename T>
 void Tree::factorize_impl() {
     const double t_start = wall();
-    const hipStream_t st = 0;
+    FactorRun R;
+    const hipStream_t st = R.st;  // the null stream
     drop_apply_graphs();
     graphs_on_ = env_or("SPAND_APPLY_GRAPH", 1) != 0;
     arena_.reset();
     task_arena_.reset();
     wave_work_.clear();
     level_dims_.clear();
-    std::vector<std::future<SolvePlanHost>> pending_solve(nlevels_);  // joined on every exit
+    R.pending_solve.resize(nlevels_);  // joined on every exit (FactorRun's destructor)
     tplans_.clear();
     tplans_.resize(nlevels_);
     hip_check(hipDeviceSynchronize(), "factorize");
@@ -971,1091 +2289,38 @@ void Tree::factorize_impl() {
     // [0]: failed pivot (cluster order + 1), [1]: a persistent RRQR wait that gave up
     if (!d_info_) hip_check(hipMalloc((void**)&d_info_, 2 * sizeof(int)), "hipMalloc(info)");
     hip_check(hipMemset(d_info_, 0, 2 * sizeof(int)), "memset info");
-    std::vector<int> hsize(cap, 0);
+    R.hsize.assign(cap, 0);
     for (int c : bottoms_[0]) cls_[c].xoff = cls_[c].start;
     ws_len_ = N_;
     // small-path RRQR: Asn in LDS up to RRQR_LDS_MAX; each wave's launch gets the LDS of
     // its largest resident task
-    const long long lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
-    const long long big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
+    R.lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
+    R.big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
     tunes_.assign(nlevels_, LevelTune());
     flops_log_.clear();
     shapes_.assign(nlevels_, LevelShape());
-    const bool gen = symm_ == 2;
+    R.gen = symm_ == 2;
     fsymm_ = symm_;
 
     for (ilvl_ = 0; ilvl_ < nlevels_; ilvl_++) {
-        LevelStats& lst = stats_[ilvl_];
-        LevelSolve& lsv = solve_plan_[ilvl_];
-        const double t_lvl0 = wall();
-        const std::vector<int>& bot = bottoms_[current_bottom_];
-        // ---------------- Elim (src/tree.cpp:1494-1506, 895-967) ----------------
-        std::vector<PotrfTask> potrfE, potrfS;
-        std::vector<GetrfTask> getrfE, getrfS;
-        std::vector<TrsmTask> trsmE, trsmA, trsmB;
-        struct PC { int target; SchurContrib c; };
-        std::vector<PC> pcs;
-        std::vector<int> fill_edges;
-        long long fill_elems = 0;
-        std::vector<long long> fill_off;  // by position in fill_edges
-        std::vector<TrsvTask> sv_elim, sv_scale;
-        std::vector<CopyTask> inv_init;  // identity blocks that become explicit inverses L^{-1}
-        std::vector<std::pair<int, GemvTerm>> fwd_list;  // (destination cluster, term), in op order
-        std::vector<GemvGroup> bwd_groups;
-        std::vector<GemvTerm> bwd_terms;
-        auto target_edge = [&](int n1, int n2) {
-            int e = find_out_edge(n2, n1);
-            if (e < 0) {
-                e = new_edge(n2, n1, nullptr, false);
-                fill_edges.push_back(e);
-                fill_off.push_back(fill_elems);
-                fill_elems += (long long)size(n1) * size(n2);
-            }
-            return e;
-        };
-        // the level's launch configuration: trsm chunks from its pivot sizes now, the
-        // sparsification knobs once the sparsified clusters' Asn shapes are known (below)
-        LevelShape shape;
-        {
-            std::vector<int> piv;
-            for (int c : bot)
-                if (cls_[c].level == ilvl_) piv.push_back(cls_[c].size);
-            shape.n_elim = (int)piv.size();
-            if (!piv.empty()) {
-                std::sort(piv.begin(), piv.end());
-                shape.elim_med = piv[piv.size() / 2];
-                shape.elim_max = piv.back();
-            }
-        }
-        LevelTune tune = choose_tune(sizeof(T), shape);
-        apply_tune_env(tune);
-        const int cwp = tune.trsm_cw;
-        auto push_trsm = [cwp](std::vector<TrsmTask>& v, const void* L, void* B, int n, int m, int side, size_t esz,
-                               const void* src = nullptr, const int* perm = nullptr) {
-            spand::push_trsm(v, L, B, n, m, side, esz, src, perm, cwp);
-        };
-        for (int c : bot) {
-            if (cls_[c].level != ilvl_) continue;
-            HCluster& s = cls_[c];
-            const int n = s.size;
-            // edges_ may grow (fill-in) below: keep the pivot block pointer, not a reference
-            const HEdge piv = edges_[s.out[0]];
-            if (piv.ident) throw Error(ERR_STATE, "internal: identity pivot on an eliminated cluster");
-            if (gen) {
-                // getf_cluster + panel_getf + gemm_edges, PLU branch (src/tree.cpp:929-956)
-                void* Ut = arena_.alloc((size_t)std::max(1, n * n) * sizeof(T));
-                int* pp = reinterpret_cast<int*>(arena_.alloc((size_t)std::max(1, n) * sizeof(int)));
-                getrfE.push_back({piv.A, Ut, pp, n, c});
-                lst.flops_potrf += getrf_flops(n);
-                log_shape(SK_GETF, n, 0, 0);
-                std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
-                for (int e : in) {  // A_sn := L^{-1} P^T A_sn, into a fresh block
-                    const int m = size(edges_[e].n1);
-                    log_shape(SK_PANEL, m, n, 0);
-                    void* nb = arena_.alloc((size_t)std::max(1, n * m) * sizeof(T));
-                    push_trsm(trsmE, piv.A, nb, n, m, 0, sizeof(T), edges_[e].A, pp);
-                    edges_[e].A = nb;
-                    lst.flops_trsm += (double)n * n * m;
-                }
-                for (int e : out) {  // A_ns := A_ns U^{-1} = A_ns (U^T)^{-T}
-                    log_shape(SK_PANEL, size(edges_[e].n2), n, 0);
-                    push_trsm(trsmE, Ut, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
-                    lst.flops_trsm += (double)n * n * size(edges_[e].n2);
-                }
-                for (int e1 : out)
-                    for (int e2 : in) {
-                        int n1 = edges_[e1].n2, n2 = edges_[e2].n1;
-                        int tgt = target_edge(n1, n2);
-                        SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
-                        pcs.push_back({tgt, sc});
-                        lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
-                        log_shape(SK_GEMM, size(n1), size(n2), n);
-                    }
-                // record ScalingPLUQ + GemmOut / GemmIn (src/tree.cpp:945-951)
-                ops_.push_back({OP_SCALING_PLUQ, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr, Ut, pp});
-                if (n > 0) {
-                    void* Li = arena_.alloc((size_t)n * n * sizeof(T));
-                    void* Wi = arena_.alloc((size_t)n * n * sizeof(T));
-                    inv_init.push_back({nullptr, Li, n, n, n, 0});
-                    inv_init.push_back({nullptr, Wi, n, n, n, 0});
-                    push_trsm(trsmE, piv.A, Li, n, n, 0, sizeof(T));
-                    push_trsm(trsmE, Ut, Wi, n, n, 0, sizeof(T));
-                    lst.flops_trsm += 2.0 * n * n * n;
-                    sv_elim.push_back({Li, Wi, pp, n, (int)s.xoff});
-                }
-                factor_bytes_ += (size_t)n * (n + 1) * sizeof(T) + (size_t)n * sizeof(int);
-                std::vector<GemvTerm> mine;
-                for (int e : out) {
-                    int nb = edges_[e].n2;
-                    ops_.push_back({OP_GEMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
-                    factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
-                    if (n > 0 && size(nb) > 0) {
-                        fwd_list.push_back({nb, GemvTerm{edges_[e].A, size(nb), n, (int)s.xoff, 0}});
-                    }
-                }
-                for (int e : in) {
-                    int nb = edges_[e].n1;
-                    ops_.push_back({OP_GEMM_IN, c, nb, n, size(nb), n, size(nb), ilvl_, edges_[e].A, nullptr});
-                    factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
-                    if (n > 0 && size(nb) > 0) mine.push_back({edges_[e].A, n, size(nb), (int)cls_[nb].xoff, 0});
-                }
-                if (!mine.empty()) {
-                    int k0 = (int)bwd_terms.size();
-                    for (auto it = mine.rbegin(); it != mine.rend(); ++it) bwd_terms.push_back(*it);
-                    bwd_groups.push_back({(int)s.xoff, n, k0, (int)bwd_terms.size()});
-                }
-                set_eliminated(c);
-                continue;
-            }
-            potrfE.push_back({piv.A, n, c, nullptr});
-            lst.flops_potrf += potrf_flops(n);
-            log_shape(SK_POTF, n, 0, 0);
-            std::vector<int> out(s.out.begin() + 1, s.out.end()), in(s.in.begin(), s.in.end());
-            for (int e : in) {
-                push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n1), 0, sizeof(T));
-                lst.flops_trsm += (double)n * n * size(edges_[e].n1);
-                log_shape(SK_PANEL, size(edges_[e].n1), n, 0);
-            }
-            for (int e : out) {
-                push_trsm(trsmE, piv.A, edges_[e].A, n, size(edges_[e].n2), 1, sizeof(T));
-                lst.flops_trsm += (double)n * n * size(edges_[e].n2);
-                log_shape(SK_PANEL, size(edges_[e].n2), n, 0);
-            }
-            // schur_symmetric (src/tree.cpp:862-883) with gemm_edges (:748-793)
-            for (int e1 : out)
-                for (int e2 : out) {
-                    int n1 = edges_[e1].n2, n2 = edges_[e2].n2;
-                    if (cls_[n1].order < cls_[n2].order) continue;
-                    int tgt = target_edge(n1, n2);
-                    SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), size(n2), 0, 1, 0};
-                    pcs.push_back({tgt, sc});
-                    lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
-                    log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
-                }
-            for (int e1 : in)
-                for (int e2 : in) {
-                    int n1 = edges_[e1].n1, n2 = edges_[e2].n1;
-                    if (cls_[n1].order < cls_[n2].order) continue;
-                    int tgt = target_edge(n1, n2);
-                    SchurContrib sc{edges_[e1].A, edges_[e2].A, n, n, n, 1, 0, 0};
-                    pcs.push_back({tgt, sc});
-                    lst.flops_schur += (n1 == n2 ? 1.0 : 2.0) * size(n1) * size(n2) * n;
-                    log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
-                }
-            for (int e1 : out)
-                for (int e2 : in) {
-                    int n1 = edges_[e1].n2, n2 = edges_[e2].n1;
-                    int tgt = target_edge(n1, n2);
-                    SchurContrib sc{edges_[e1].A, edges_[e2].A, n, size(n1), n, 0, 0, 0};
-                    pcs.push_back({tgt, sc});
-                    lst.flops_schur += 2.0 * size(n1) * size(n2) * n;
-                    log_shape(n1 == n2 ? SK_SYRK : SK_GEMM, size(n1), size(n2), n);
-                }
-            // record ScalingLLT + GemmSymmOut/In (src/tree.cpp:909-910, 885-892)
-            ops_.push_back({OP_SCALING_LLT, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr});
-            if (n > 0) {
-                // explicit inverse for the solve: L^{-1} = L^{-1} I by the chunked trsm
-                void* Li = arena_.alloc((size_t)n * n * sizeof(T));
-                inv_init.push_back({nullptr, Li, n, n, n, 0});
-                push_trsm(trsmE, piv.A, Li, n, n, 0, sizeof(T));
-                lst.flops_trsm += (double)n * n * n;
-                sv_elim.push_back({Li, Li, nullptr, n, (int)s.xoff});
-            }
-            factor_bytes_ += (size_t)n * (n + 1) / 2 * sizeof(T);
-            std::vector<GemvTerm> mine;
-            for (int e : out) {
-                int nb = edges_[e].n2;
-                ops_.push_back({OP_GEMM_SYMM_OUT, c, nb, n, size(nb), size(nb), n, ilvl_, edges_[e].A, nullptr});
-                factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
-                if (n > 0 && size(nb) > 0) {
-                    fwd_list.push_back({nb, GemvTerm{edges_[e].A, size(nb), n, (int)s.xoff, 0}});
-                    mine.push_back({edges_[e].A, size(nb), n, (int)cls_[nb].xoff, 1});
-                }
-            }
-            for (int e : in) {
-                int nb = edges_[e].n1;
-                ops_.push_back({OP_GEMM_SYMM_IN, c, nb, n, size(nb), n, size(nb), ilvl_, edges_[e].A, nullptr});
-                factor_bytes_ += (size_t)n * size(nb) * sizeof(T);
-                if (n > 0 && size(nb) > 0) {
-                    fwd_list.push_back({nb, GemvTerm{edges_[e].A, n, size(nb), (int)s.xoff, 1}});
-                    mine.push_back({edges_[e].A, n, size(nb), (int)cls_[nb].xoff, 0});
-                }
-            }
-            if (!mine.empty()) {
-                int k0 = (int)bwd_terms.size();
-                for (auto it = mine.rbegin(); it != mine.rend(); ++it) bwd_terms.push_back(*it);
-                bwd_groups.push_back({(int)s.xoff, n, k0, (int)bwd_terms.size()});
-            }
-            set_eliminated(c);
-        }
-        // fill-in blocks: the Schur targets created at this level, zero-initialized
-        T* fill_base = nullptr;
-        if (fill_elems > 0) {
-            fill_base = reinterpret_cast<T*>(arena_.alloc((size_t)fill_elems * sizeof(T)));
-            for (size_t k = 0; k < fill_edges.size(); k++) edges_[fill_edges[k]].A = fill_base + fill_off[k];
-        }
-        // Schur tiles grouped by target, contributions in recorded order (built before the
-        // scale phase below turns the neighbours' pivots into identities)
-        std::vector<SchurTile> tiles;
-        std::vector<SchurContrib> contribs;
-        {
-            std::vector<int> tstart, torder;
-            const std::vector<int> korder = group_by_first(
-                (int)pcs.size(), (int)edges_.size(), [&](int k) { return pcs[k].target; }, tstart, torder);
-            for (size_t gi = 0; gi < torder.size(); gi++) {
-                const int tg = torder[gi];
-                const HEdge& e = edges_[tg];
-                const int m = size(e.n2), nn = size(e.n1);
-                const int k0 = (int)contribs.size();
-                for (int q = tstart[gi]; q < tstart[gi + 1]; q++) contribs.push_back(pcs[korder[q]].c);
-                const int k1 = (int)contribs.size();
-                const int lower = !gen && e.n1 == e.n2 ? 1 : 0;  // syrk on symmetric diagonal targets
-                for (int c0 = 0; c0 < nn; c0 += 32)
-                    for (int r0 = 0; r0 < m; r0 += 32) {
-                        int tm = std::min(32, m - r0), tn = std::min(32, nn - c0);
-                        if (lower && r0 + tm - 1 < c0) continue;
-                        tiles.push_back({e.A, m, r0, c0, tm, tn, lower, k0, k1});
-                    }
-            }
-        }
-        // ---------------- upload + launch Elim ----------------
-        // Nothing planned below feeds the elimination group (fill-in zeroing, identity inits
-        // of the explicit inverses, pivots, panels, Schur updates): it is launched now, so the
-        // device eliminates while the host plans the level's scaling and sparsification.
-        const size_t esz = sizeof(T);
-        auto small_first = [](std::vector<PotrfTask>& v) {  // <= 64, <= POTRF_SMALL_MAX, larger
-            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= POTRF_SMALL_MAX; });
-            std::stable_partition(v.begin(), v.end(), [](const PotrfTask& t) { return t.n <= 64; });
-        };
-        // LU pivots that fit LDS first (one launch), the larger ones after (another)
-        auto lds_first = [&](std::vector<GetrfTask>& v) {
-            std::stable_partition(v.begin(), v.end(), [&](const GetrfTask& t) {
-                return (size_t)t.n * (t.n + 1) * sizeof(T) <= (size_t)GETRF_LDS_BYTES;
-            });
-        };
-        // pivots up to POTRF_SMALL_MAX in one launch (one workgroup each); larger ones (sorted
-        // last) one 32-column panel per launch pair
-        auto run_potrf = [&](void* dt, const std::vector<PotrfTask>& v, size_t off, const char* what) {
-            if (v.empty()) return;
-            int ntiny = 0, nsmall = 0, nmax = 0;
-            for (auto& t : v) {
-                if (t.n <= 64) ntiny++;
-                if (t.n <= POTRF_SMALL_MAX) nsmall++;
-                nmax = std::max(nmax, t.n);
-            }
-            const PotrfTask* d = Packer::at<PotrfTask>(dt, off);
-            if (ntiny > 0) {
-                std::vector<PotrfTask> vs(v.begin(), v.begin() + ntiny);
-                Work w = work_potrf(vs, esz);
-                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
-                                [&] { return launch_potrf64<T>(d, ntiny, d_info_, st); }),
-                          what);
-            }
-            if (nsmall > ntiny) {
-                std::vector<PotrfTask> vs(v.begin() + ntiny, v.begin() + nsmall);
-                Work w = work_potrf(vs, esz);
-                hip_check(timed(prof_, KF_POTRF, st, w.bytes, w.flops,
-                                [&] { return launch_potrf<T>(d + ntiny, nsmall - ntiny, d_info_, st); }),
-                          what);
-            }
-            const int nbig = (int)v.size() - nsmall;
-            if (nbig == 0) return;
-            const PotrfTask* db = d + nsmall;
-            for (int kb = 0; kb < nmax; kb += 32) {
-                double pb = 0, pf = 0, tb = 0, tf = 0;
-                for (int k = nsmall; k < (int)v.size(); k++) {
-                    const double n = v[k].n, nb = std::min(32.0, n - kb), rest = n - kb - nb;
-                    if (nb <= 0) continue;
-                    pb += (nb * (nb + 1) + 2.0 * rest * nb) * esz;
-                    pf += nb * nb * nb / 3.0 + rest * nb * nb;
-                    tb += (rest * (rest + 1) + 2.0 * rest * nb) * esz;
-                    tf += rest * (rest + 1) * nb;
-                }
-                hip_check(timed(prof_, KF_POTRF_PANEL, st, pb, pf,
-                                [&] { return launch_potrf_panel<T>(db, nbig, kb, nmax, d_info_, st); }),
-                          what);
-                hip_check(timed(prof_, KF_POTRF_TRAIL, st, tb, tf,
-                                [&] { return launch_potrf_trail<T>(db, nbig, kb, nmax, st); }),
-                          what);
-            }
-        };
-        auto run_trsm = [&](void* dt, const std::vector<TrsmTask>& v, size_t off, int split, const char* what) {
-            if (v.empty()) return;
-            Work w = work_trsm(v, esz);
-            hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
-                            [&] { return launch_trsm<T>(Packer::at<TrsmTask>(dt, off), (int)v.size(), split, st); }),
-                      what);
-        };
-        auto run_getrf = [&](void* dt, const std::vector<GetrfTask>& v, size_t off, const char* what) {
-            if (v.empty()) return;
-            int nl = 0, nmax_l = 0;
-            double lb = 0, lf = 0, gb = 0, gf = 0;
-            for (auto& t : v) {
-                const double n = t.n;
-                const double by = (3.0 * n * n) * esz + 4.0 * n, fl = getrf_flops(n);
-                if ((size_t)t.n * (t.n + 1) * esz <= (size_t)GETRF_LDS_BYTES) {
-                    nl++;
-                    nmax_l = std::max(nmax_l, t.n);
-                    lb += by;
-                    lf += fl;
-                } else {
-                    gb += by;
-                    gf += fl;
-                }
-            }
-            const GetrfTask* d = Packer::at<GetrfTask>(dt, off);
-            if (nl > 0)
-                hip_check(timed(prof_, KF_GETRF, st, lb, lf,
-                                [&] { return launch_getrf<T>(d, nl, 1, nmax_l, d_info_, st); }),
-                          what);
-            if (nl < (int)v.size())
-                hip_check(timed(prof_, KF_GETRF, st, gb, gf,
-                                [&] { return launch_getrf<T>(d + nl, (int)v.size() - nl, 0, 0, d_info_, st); }),
-                          what);
-        };
-        const size_t n_potrf_elim = potrfE.size();
-        {
-            small_first(potrfE);
-            for (PotrfTask& t : potrfE)
-                if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
-            lds_first(getrfE);
-            Packer pe;
-            const int sp_te = split_trsm(trsmE);
-            const size_t o_pe = pe.add(potrfE), o_te = pe.add(trsmE), o_ti = pe.add(tiles), o_co = pe.add(contribs);
-            const size_t o_ie = pe.add(inv_init), o_ge = pe.add(getrfE);
-            void* de = upload_tasks(pe);
-            if (fill_elems > 0)
-                hip_check(timed(prof_, KF_FILL, st, (double)fill_elems * esz, 0,
-                                [&] { return launch_fill<T>(fill_base, (size_t)fill_elems, T(0), st); }),
-                          "fill fill-in");
-            if (!inv_init.empty()) {
-                Work w = work_copy(inv_init, esz);
-                hip_check(timed(prof_, KF_COPY, st, w.bytes, 0,
-                                [&] { return launch_copy<T>(Packer::at<CopyTask>(de, o_ie), (int)inv_init.size(), st); }),
-                          "identity blocks");
-            }
-            run_potrf(de, potrfE, o_pe, "potrf elim");
-            run_getrf(de, getrfE, o_ge, "getrf elim");
-            run_trsm(de, trsmE, o_te, sp_te, "trsm elim");
-            if (!tiles.empty()) {
-                Work w = work_schur(tiles, contribs, esz);
-                hip_check(timed(prof_, KF_SCHUR, st, w.bytes, w.flops,
-                                [&] {
-                                    return launch_schur<T>(Packer::at<SchurTile>(de, o_ti), (int)tiles.size(),
-                                                           Packer::at<SchurContrib>(de, o_co), st);
-                                }),
-                          "schur");
-            }
-            inv_init.clear();  // the scaling's identity inits follow
-        }
-        int dofs_left = 0;
-        for (int c : bot)
-            if (!cls_[c].elim) dofs_left += cls_[c].size;
-        lst.dofs_left_elim = dofs_left;
-        // ---------------- Scale (src/tree.cpp:1510-1519, 796-856) ----------------
-        const bool do_sparsify_level = ilvl_ >= skip_;
-        std::vector<int> live;
-        for (int c : bot)
-            if (cls_[c].level > ilvl_) live.push_back(c);
-        if (do_sparsify_level && monitor_flops_)
-            for (int c : live) {  // scale_cluster: the pivot, then the panels of its in- and out-edges
-                const HCluster& s = cls_[c];
-                log_shape(gen ? SK_GETF : SK_POTF, s.size, 0, 0);
-                for (int e : s.in) log_shape(SK_PANEL, size(edges_[e].n1), s.size, 0);
-                for (size_t k = 1; k < s.out.size(); k++) log_shape(SK_PANEL, size(edges_[s.out[k]].n2), s.size, 0);
-            }
-        if (do_sparsify_level && gen) {
-            // scale_cluster, PLU branch (src/tree.cpp:831-849): an edge a -> b (block
-            // size(b) x size(a)) receives A U_a^{-1} from a and L_b^{-1} P_b^T A from b, in
-            // the order of a and b in the bottom list, into a fresh block at b's step
-            std::unordered_map<int, std::pair<void*, int*>> lu;  // cluster -> (U^T, perm)
-            for (int c : live) {
-                HCluster& s = cls_[c];
-                const int n = s.size;
-                const HEdge& piv = edges_[s.out[0]];
-                void* Ut = arena_.alloc((size_t)std::max(1, n * n) * sizeof(T));
-                int* pp = reinterpret_cast<int*>(arena_.alloc((size_t)std::max(1, n) * sizeof(int)));
-                lu[c] = {Ut, pp};
-                getrfS.push_back({piv.A, Ut, pp, n, c});
-                lst.flops_potrf += getrf_flops(n);
-                ops_.push_back({OP_SCALING_PLUQ, c, -1, n, 0, n, n, ilvl_, piv.A, nullptr, Ut, pp});
-                factor_bytes_ += (size_t)n * (n + 1) * sizeof(T) + (size_t)n * sizeof(int);
-                if (n > 0) {
-                    void* Li = arena_.alloc((size_t)n * n * sizeof(T));
-                    void* Wi = arena_.alloc((size_t)n * n * sizeof(T));
-                    inv_init.push_back({nullptr, Li, n, n, n, 0});
-                    inv_init.push_back({nullptr, Wi, n, n, n, 0});
-                    push_trsm(trsmA, piv.A, Li, n, n, 0, sizeof(T));
-                    push_trsm(trsmA, Ut, Wi, n, n, 0, sizeof(T));
-                    lst.flops_trsm += 2.0 * n * n * n;
-                    sv_scale.push_back({Li, Wi, pp, n, (int)s.xoff});
-                }
-            }
-            for (int c : live) {
-                for (size_t k = 1; k < cls_[c].out.size(); k++) {
-                    HEdge& e = edges_[cls_[c].out[k]];
-                    const int a = e.n1, b = e.n2;  // block is size(b) x size(a)
-                    const void* Lb = edges_[cls_[b].out[0]].A;
-                    lst.flops_trsm += (double)size(a) * size(b) * (size(a) + size(b));
-                    const bool a_first = cls_[a].order < cls_[b].order;
-                    void* nb = arena_.alloc((size_t)std::max(1, size(a) * size(b)) * sizeof(T));
-                    if (a_first) {
-                        push_trsm(trsmA, lu[a].first, e.A, size(a), size(b), 1, sizeof(T));
-                        push_trsm(trsmB, Lb, nb, size(b), size(a), 0, sizeof(T), e.A, lu[b].second);
-                    } else {
-                        push_trsm(trsmA, Lb, nb, size(b), size(a), 0, sizeof(T), e.A, lu[b].second);
-                        push_trsm(trsmB, lu[a].first, nb, size(a), size(b), 1, sizeof(T));
-                    }
-                    // a zero-sized block keeps its (unread) pointer
-                    if (size(a) > 0 && size(b) > 0) e.A = nb;
-                }
-            }
-            for (int c : live) {
-                HEdge& piv = edges_[cls_[c].out[0]];
-                piv.A = nullptr;
-                piv.ident = true;
-            }
-        } else if (do_sparsify_level) {
-            for (int c : live) {
-                HCluster& s = cls_[c];
-                const HEdge& piv = edges_[s.out[0]];
-                potrfS.push_back({piv.A, s.size, c, nullptr});
-                lst.flops_potrf += potrf_flops(s.size);
-                ops_.push_back({OP_SCALING_LLT, c, -1, s.size, 0, s.size, s.size, ilvl_, piv.A, nullptr});
-                factor_bytes_ += (size_t)s.size * (s.size + 1) / 2 * sizeof(T);
-                if (s.size > 0) {
-                    void* Li = arena_.alloc((size_t)s.size * s.size * sizeof(T));
-                    inv_init.push_back({nullptr, Li, s.size, s.size, s.size, 0});
-                    push_trsm(trsmA, piv.A, Li, s.size, s.size, 0, sizeof(T));
-                    lst.flops_trsm += (double)s.size * s.size * s.size;
-                    sv_scale.push_back({Li, Li, nullptr, s.size, (int)s.xoff});
-                }
-            }
-            for (int c : live) {
-                for (size_t k = 1; k < cls_[c].out.size(); k++) {
-                    const HEdge& e = edges_[cls_[c].out[k]];
-                    const int a = e.n1, b = e.n2;  // block is size(b) x size(a)
-                    const void* La = edges_[cls_[a].out[0]].A;
-                    const void* Lb = edges_[cls_[b].out[0]].A;
-                    lst.flops_trsm += (double)size(a) * size(b) * (size(a) + size(b));
-                    const bool a_first = cls_[a].order < cls_[b].order;
-                    // right solve by L_a (B is size(b) x size(a)), left solve by L_b
-                    push_trsm(a_first ? trsmA : trsmB, La, e.A, size(a), size(b), 1, sizeof(T));
-                    push_trsm(a_first ? trsmB : trsmA, Lb, e.A, size(b), size(a), 0, sizeof(T));
-                }
-            }
-            for (int c : live) {
-                HEdge& piv = edges_[cls_[c].out[0]];
-                piv.A = nullptr;
-                piv.ident = true;
-            }
-        }
-        // ---------------- Sparsify plan (src/tree.cpp:1521-1533, 1292-1347) ----------------
-        std::vector<int> plist;
-        std::vector<int> wave, widx;
-        int nwaves = 0;
-        std::vector<std::vector<RrqrTask>> wtasks;
-        std::vector<long long> wlds_small;  // dynamic LDS of each wave's small-path launch
-        std::vector<std::vector<RrqrBig>> wbig;       // stepped path, per wave
-        std::vector<std::vector<RrqrSlab>> wslab;
-        std::vector<int> wsteps, wmaxrows;
-        std::vector<std::vector<RrqrTask>> wreg;      // register path, per wave (sorted by variant)
-        std::vector<std::vector<int>> wregvar;        // the variant of each wreg task
-        std::vector<std::pair<int, int>> where;      // (0 small | 1 big | 2 register, index) by plist position
-        std::vector<RrqrSeg> segs;
-        if (do_sparsify_level) {
-            std::vector<int> inP(cls_.size(), -1);
-            for (int c : live) {
-                if (use_want_sparsify_ && !cls_[c].spars) {
-                    lst.nignored++;
-                    continue;
-                }
-                inP[c] = (int)plist.size();
-                plist.push_back(c);
-            }
-            wave.assign(plist.size(), 0);
-            for (size_t p = 0; p < plist.size(); p++) {
-                const int c = plist[p];
-                int w = 0;
-                auto visit = [&](int nb) {
-                    if (inP[nb] >= 0 && cls_[nb].order < cls_[c].order) w = std::max(w, wave[inP[nb]] + 1);
-                };
-                for (int e : cls_[c].in) visit(edges_[e].n1);
-                for (size_t k = 1; k < cls_[c].out.size(); k++) visit(edges_[cls_[c].out[k]].n2);
-                wave[p] = w;
-                nwaves = std::max(nwaves, w + 1);
-            }
-            wtasks.assign(nwaves, {});
-            wlds_small.assign(nwaves, 0);
-            wbig.assign(nwaves, {});
-            wslab.assign(nwaves, {});
-            wsteps.assign(nwaves, 0);
-            wmaxrows.assign(nwaves, 0);
-            wreg.assign(nwaves, {});
-            wregvar.assign(nwaves, {});
-            where.assign(plist.size(), {0, 0});
-            std::vector<std::vector<std::pair<int, int>>> regp(nwaves);  // (variant, plist position)
-            std::vector<RrqrTask> ptask(plist.size());
-            {  // the sparsification knobs from the level's Asn shapes
-                std::vector<int> rows, cols;
-                for (int c : plist) {
-                    const HCluster& s = cls_[c];
-                    int nc = 0;
-                    for (int e : s.in) nc += size(edges_[e].n1);
-                    for (size_t k = 1; k < s.out.size(); k++) nc += size(edges_[s.out[k]].n2);
-                    rows.push_back(s.size);
-                    cols.push_back(nc);
-                }
-                shape.n_spars = (int)plist.size();
-                if (!rows.empty()) {
-                    std::sort(rows.begin(), rows.end());
-                    std::sort(cols.begin(), cols.end());
-                    shape.rows_med = rows[rows.size() / 2];
-                    shape.rows_max = rows.back();
-                    shape.cols_med = cols[cols.size() / 2];
-                    shape.cols_max = cols.back();
-                }
-                const LevelTune t = choose_tune(sizeof(T), shape);
-                tune.rrqr_reg = t.rrqr_reg;
-                tune.small_elems = t.small_elems;
-                tune.step_waves = t.step_waves;
-                apply_tune_env(tune);
-            }
-            const bool reg_path = tune.rrqr_reg != 0;
-            const long long big_min_elems = tune.small_elems;
-            widx.assign(plist.size(), 0);
-            for (size_t p = 0; p < plist.size(); p++) {
-                const int c = plist[p];
-                const HCluster& s = cls_[c];
-                RrqrTask t{};
-                t.cluster = c;
-                t.seg0 = (int)segs.size();
-                int cols = 0;
-                for (int e : s.in) {
-                    segs.push_back({edges_[e].A, edges_[e].n1, 0});
-                    cols += size(edges_[e].n1);
-                }
-                for (size_t k = 1; k < s.out.size(); k++) {
-                    const HEdge& e = edges_[s.out[k]];
-                    segs.push_back({e.A, e.n2, 1});
-                    cols += size(e.n2);
-                }
-                t.nseg = (int)segs.size() - t.seg0;
-                if (t.nseg > 1024) throw Error(ERR_UNSUPPORTED, "cluster with more than 1024 neighbours");
-                t.max_rows = s.size;
-                t.max_cols = cols;
-                const int mn = std::min(s.size, cols);
-                t.Q = arena_.alloc((size_t)std::max(1, s.size * mn) * sizeof(T));
-                t.tau = arena_.alloc((size_t)std::max(1, mn) * sizeof(T));
-                lst.flops_rrqr += mn > 0 ? 2.0 * s.size * cols * mn - (2.0 / 3.0) * (double)mn * mn * mn : 0.0;
-                const int w = wave[p];
-                const int reg_var = reg_path ? rrqr_reg_variant(sizeof(T), s.size, cols) : -1;
-                if (reg_var >= 0) {
-                    // register-resident, one workgroup per cluster
-                    ptask[p] = t;
-                    regp[w].push_back({reg_var, (int)p});
-                } else if (s.size >= big_min_rows && (long long)s.size * cols > big_min_elems) {
-                    // stepped multi-workgroup path
-                    RrqrBig b{};
-                    b.nslab = std::max(1, (cols + 63) / 64);
-                    t.scratch = arena_.alloc((size_t)s.size * b.nslab * 64 * sizeof(T));
-                    t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * std::max(cols, 1) * sizeof(double)));
-                    b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
-                    b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)(2 * b.nslab + 1) * sizeof(double)));
-                    b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
-                    b.bar = reinterpret_cast<int*>(arena_.alloc((size_t)b.nslab * sizeof(int)));  // per-slab step counts
-                    b.t = t;
-                    where[p] = {1, (int)wbig[w].size()};
-                    for (int k = 0; k < b.nslab; k++) wslab[w].push_back({(int)wbig[w].size(), k});
-                    wbig[w].push_back(b);
-                    wsteps[w] = std::max(wsteps[w], mn);
-                    wmaxrows[w] = std::max(wmaxrows[w], s.size);
-                } else {
-                    // LDS-resident with an odd column stride (rows | 1), else a global scratch
-                    if ((long long)(s.size | 1) * cols > lds_elems)
-                        t.scratch = arena_.alloc((size_t)s.size * cols * sizeof(T));
-                    else
-                        wlds_small[w] =
-                            std::max(wlds_small[w], (long long)(s.size | 1) * cols * (long long)sizeof(T));
-                    if (cols > 1024) {
-                        t.norms = reinterpret_cast<double*>(arena_.alloc((size_t)2 * cols * sizeof(double)));
-                        t.jpvt = reinterpret_cast<int*>(arena_.alloc((size_t)cols * sizeof(int)));
-                    }
-                    where[p] = {0, (int)wtasks[w].size()};
-                    wtasks[w].push_back(t);
-                }
-            }
-            for (int w = 0; w < nwaves; w++) {
-                std::stable_sort(regp[w].begin(), regp[w].end(),
-                                 [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
-                for (auto& vp : regp[w]) {
-                    where[vp.second] = {2, (int)wreg[w].size()};
-                    wreg[w].push_back(ptask[vp.second]);
-                    wregvar[w].push_back(vp.first);
-                }
-            }
-        }
-        // pivot-column / reflector buffers of the stepped path: LDS unless a wave's largest
-        // cluster needs more than STEP_LDS_MAX, then global buffers for the whole wave
-        // A wave whose clusters all have at most WAVE_ROWS_MAX rows runs as one persistent
-        // launch (candidate-column buffers in vbuf); the per-step launches remain the path
-        // for larger clusters and for waves the device cannot hold at once.
-        std::vector<size_t> wlds_step(nwaves, 0), wlds_wave(nwaves, 0);
-        // Opt-in (SPAND_RRQR_PERSISTENT=1): the persistent launch with per-cluster barriers
-        // measured slower on gfx950 than one launch per step (64^3 FP32 factor + PCG: 177 ms
-        // against 125 ms per step), although a flag handshake between two workgroups costs
-        // about 1 us one way: the step body itself is slower in the 1024-thread persistent form.
-        const bool persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
-        const int step_waves = tune.step_waves;  // 4, 8, 16; 0: by slab count
-        tunes_[ilvl_] = tune;
-        shapes_[ilvl_] = shape;
-        for (int w = 0; w < nwaves; w++) {
-            const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
-            int wmaxcols = 0;
-            for (auto& b : wbig[w]) wmaxcols = std::max(wmaxcols, b.t.max_cols);
-            const size_t lw = ((size_t)wmaxcols * sizeof(int) + 15) / 16 * 16 + 2 * vb;
-            if (persistent && !wbig[w].empty() && wmaxrows[w] <= WAVE_ROWS_MAX && lw <= (size_t)WAVE_LDS_MAX) {
-                wlds_wave[w] = lw;
-                for (auto& b : wbig[w])
-                    b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
-            } else if (2 * vb <= (size_t)STEP_LDS_MAX) {
-                wlds_step[w] = 2 * vb;
-            } else {
-                for (auto& b : wbig[w])
-                    b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
-            }
-        }
-        // dims (one block per level) and done flags of the stepped clusters; the flags are
-        // reset before any of this level's launches
-        std::vector<size_t> wflag0(nwaves, 0);
-        {
-            size_t nbig = 0;
-            for (int w = 0; w < nwaves; w++) nbig += wbig[w].size();
-            int* dblock = nullptr;
-            if (nbig > 0) {
-                signal_reserve(1 + nbig);
-                dblock = reinterpret_cast<int*>(arena_.alloc(nbig * 5 * sizeof(int)));
-                size_t f = 1;
-                for (int w = 0; w < nwaves; w++) {
-                    wflag0[w] = f;
-                    for (auto& b : wbig[w]) {
-                        h_sig_[f] = 0;
-                        b.done = d_sig_ + f;
-                        b.dims = dblock + 5 * (f - 1);
-                        f++;
-                    }
-                }
-            }
-            level_dims_.push_back({dblock, nbig});
-        }
-        lst.nwaves = nwaves;
-        // ---------------- upload + launch Elim / Scale / Sparsify ----------------
-        for (size_t c = 0; c < cls_.size(); c++) hsize[c] = cls_[c].size;
-        small_first(potrfS);
-        for (PotrfTask& t : potrfS)
-            if (t.n > POTRF_SMALL_MAX) t.dbuf = arena_.alloc(32 * 32 * sizeof(T));
-        Packer pk;
-        const int sp_ta = split_trsm(trsmA), sp_tb = split_trsm(trsmB);
-        size_t o_ps = pk.add(potrfS), o_ta = pk.add(trsmA), o_tb = pk.add(trsmB), o_sg = pk.add(segs);
-        size_t o_ii = pk.add(inv_init);
-        lds_first(getrfS);
-        size_t o_gs = pk.add(getrfS);
-        std::vector<size_t> o_w(nwaves), o_b(nwaves), o_s(nwaves), o_r(nwaves);
-        for (int w = 0; w < nwaves; w++) {
-            o_r[w] = pk.add(wreg[w]);
-            o_w[w] = pk.add(wtasks[w]);
-            o_b[w] = pk.add(wbig[w]);
-            o_s[w] = pk.add(wslab[w]);
-        }
-        const double t_plan1 = wall();
-        void* dt = upload_tasks(pk);
-        stage_upload(d_size_, hsize.data(), cls_.size() * sizeof(int));
-        if (!inv_init.empty()) {
-            Work w = work_copy(inv_init, esz);
-            hip_check(timed(prof_, KF_COPY, st, w.bytes, 0,
-                            [&] { return launch_copy<T>(Packer::at<CopyTask>(dt, o_ii), (int)inv_init.size(), st); }),
-                      "identity blocks");
-        }
-        run_potrf(dt, potrfS, o_ps, "potrf scale");
-        run_getrf(dt, getrfS, o_gs, "getrf scale");
-        run_trsm(dt, trsmA, o_ta, sp_ta, "trsm scale A");
-        run_trsm(dt, trsmB, o_tb, sp_tb, "trsm scale B");
-        // stepped-path launch slots, for the post-hoc work attribution below
-        std::vector<int> slot_init(nwaves, -1), slot_finish(nwaves, -1);
-        std::vector<int> slot_step(nwaves, -1);
-        for (int w = 0; w < nwaves; w++) {
-            // the register-path clusters of a wave run on the side stream, beside the
-            // one-workgroup and stepped launches on the main stream; both join before the
-            // next wave (whose clusters read the edges this wave rewrites)
-            const bool side = !wreg[w].empty() && (!wtasks[w].empty() || !wbig[w].empty());
-            const hipStream_t rst = side ? aux_stream() : st;
-            if (side) {
-                hip_check(hipEventRecord(ev_fork_, st), "fork");
-                hip_check(hipStreamWaitEvent(rst, ev_fork_, 0), "fork");
-            }
-            if (!wreg[w].empty()) {
-                // one launch per variant run (tasks are sorted by variant); a single launch
-                // when the variant of the largest rows and columns holds every task
-                int mr = 0, mc = 0;
-                for (auto& t : wreg[w]) {
-                    mr = std::max(mr, t.max_rows);
-                    mc = std::max(mc, t.max_cols);
-                }
-                const int cover = rrqr_reg_variant(sizeof(T), mr, mc);
-                size_t a0 = 0;
-                while (a0 < wreg[w].size()) {
-                    size_t a1 = cover >= 0 ? wreg[w].size() : a0 + 1;
-                    if (cover < 0)
-                        while (a1 < wreg[w].size() && wregvar[w][a1] == wregvar[w][a0]) a1++;
-                    const int var = cover >= 0 ? cover : wregvar[w][a0];
-                    Work wk;
-                    for (size_t x = a0; x < a1; x++) {
-                        const RrqrTask& t = wreg[w][x];
-                        wk.bytes += (2.0 * t.max_rows * t.max_cols + (double)t.max_rows * std::min(t.max_rows, t.max_cols)) * esz;
-                        wk.flops += geqp3_flops(t.max_rows, t.max_cols);
-                    }
-                    hip_check(timed(prof_, KF_RRQR_REG, rst, wk.bytes, wk.flops,
-                                    [&] {
-                                        return launch_rrqr_reg<T>(Packer::at<RrqrTask>(dt, o_r[w]) + a0, (int)(a1 - a0),
-                                                                  Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, var, rst);
-                                    }),
-                              "rrqr reg");
-                    a0 = a1;
-                }
-            }
-            // join at the end of the wave (below), also on early exits of the loop body
-            struct Join {
-                bool on;
-                hipStream_t main, side;
-                hipEvent_t ev;
-                ~Join() {
-                    if (on && hipEventRecord(ev, side) == hipSuccess) (void)hipStreamWaitEvent(main, ev, 0);
-                }
-            } join{side, st, rst, ev_join_};
-            if (!wtasks[w].empty()) {
-                Work wk;
-                for (auto& t : wtasks[w]) {
-                    wk.bytes += (2.0 * t.max_rows * t.max_cols + (double)t.max_rows * std::min(t.max_rows, t.max_cols)) * esz;
-                    wk.flops += geqp3_flops(t.max_rows, t.max_cols);
-                }
-                hip_check(timed(prof_, KF_RRQR, st, wk.bytes, wk.flops,
-                                [&] {
-                                    return launch_rrqr<T>(Packer::at<RrqrTask>(dt, o_w[w]), (int)wtasks[w].size(),
-                                                          Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
-                                                          (int)std::max(wlds_small[w], (long long)LDS_BYTES), st);
-                                }),
-                          "rrqr");
-            }
-            if (wbig[w].empty()) continue;
-            const RrqrBig* db = Packer::at<RrqrBig>(dt, o_b[w]);
-            const RrqrSlab* ds = Packer::at<RrqrSlab>(dt, o_s[w]);
-            const int nsl = (int)wslab[w].size();
-            hip_check(timed(prof_, KF_RRQR_INIT, st, 0, 0,
-                            [&] { return launch_rrqr_init<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_, st); },
-                            &slot_init[w]),
-                      "rrqr init");
-            // the steps of a wave are timed as one batch (two events per wave)
-            bool done = false;
-            if (wlds_wave[w] > 0) {
-                // whole clusters in chunks the device holds at once, one cooperative launch each
-                int cap = 0;
-                hip_check(rrqr_wave_capacity<T>(wmaxrows[w], wlds_wave[w], &cap), "rrqr wave capacity");
-                int maxslab = 0;
-                for (auto& b : wbig[w]) maxslab = std::max(maxslab, b.nslab);
-                if (cap >= maxslab && cap > 0) {
-                    const int sl = prof_.start(KF_RRQR_WAVE, st);
-                    int launches = 0;
-                    size_t x0 = 0, s0 = 0;
-                    while (x0 < wbig[w].size()) {
-                        size_t x1 = x0, s1 = s0;
-                        int steps = 0;
-                        while (x1 < wbig[w].size() && (int)(s1 - s0) + wbig[w][x1].nslab <= cap) {
-                            steps = std::max(steps, std::min(wbig[w][x1].t.max_rows, wbig[w][x1].t.max_cols));
-                            s1 += wbig[w][x1].nslab;
-                            x1++;
-                        }
-                        const hipError_t e = launch_rrqr_wave<T>(db, ds + s0, (int)(s1 - s0), steps, wmaxrows[w], tol_,
-                                                                 wlds_wave[w], d_info_ + 1, st);
-                        hip_check(e, "rrqr wave");
-                        launches++;
-                        x0 = x1;
-                        s0 = s1;
-                    }
-                    prof_.stop(sl, st, 0, 0, launches);
-                    done = true;
-                    slot_step[w] = sl;
-                }
-            }
-            if (!done) {
-                // Steps are launched at most RRQR_STEPS_AHEAD ahead of the device, and none
-                // once every cluster of the wave has flagged its last working step: the
-                // ranks (decided on the device) end most clusters well before min(rows, cols).
-                slot_step[w] = prof_.start(KF_RRQR_STEP, st);
-                std::vector<int> live(wbig[w].size());
-                std::iota(live.begin(), live.end(), 0);
-                std::vector<long long> seq(wsteps[w], 0);
-                int launched = 0;
-                for (int i = 0; i < wsteps[w]; i++) {
-                    if (i >= RRQR_STEPS_AHEAD) wait_signal(seq[i - RRQR_STEPS_AHEAD]);
-                    size_t k = 0;
-                    for (int x : live) {
-                        const RrqrBig& b = wbig[w][x];
-                        const volatile long long* f = h_sig_ + wflag0[w] + x;
-                        if (*f == 0 && i < std::min(b.t.max_rows, b.t.max_cols)) live[k++] = x;
-                    }
-                    live.resize(k);
-                    if (live.empty()) break;
-                    seq[i] = ++sig_seq_;
-                    hip_check(launch_rrqr_step<T>(db, ds, nsl, i, wmaxrows[w], tol_, wlds_step[w], step_waves, d_sig_,
-                                                  seq[i], st),
-                              "rrqr step");
-                    launched++;
-                }
-                prof_.stop(slot_step[w], st, 0, 0, launched);
-            }
-            hip_check(timed(prof_, KF_RRQR_FINISH, st, 0, 0,
-                            [&] {
-                                return launch_rrqr_finish<T>(db, ds, nsl, Packer::at<RrqrSeg>(dt, o_sg), d_size_, tol_,
-                                                             0, st);
-                            },
-                            &slot_finish[w]),
-                      "rrqr finish");
-        }
-        int info2[2] = {0, 0};
-        hip_check(hipMemcpy(info2, d_info_, 2 * sizeof(int), hipMemcpyDeviceToHost), "read info");
-        const int info = info2[0];
-        if (info2[1] != 0) throw Error(ERR_HIP, "persistent RRQR: a cluster barrier wait gave up");
-        if (info != 0) {
-            if (gen) throw Error(ERR_SINGULAR, "Error: Singular Pivot (cluster order " + std::to_string(info - 1) + ")");
-            throw Error(ERR_NOT_SPD, "Error: Non-SPD Pivot (cluster order " + std::to_string(info - 1) + ")");
-        }
-        if (!plist.empty())
-            hip_check(hipMemcpy(hsize.data(), d_size_, cls_.size() * sizeof(int), hipMemcpyDeviceToHost), "read ranks");
-        if (prof_.on())  // the stepped path's sizes are decided on the device: priced at the end
-            for (int w = 0; w < nwaves; w++)
-                if (!wbig[w].empty())
-                    wave_work_.push_back({slot_init[w], slot_finish[w], slot_step[w], wsteps[w], wflag0[w] - 1,
-                                          wbig[w].size(), ilvl_});
-        const double t_lvl1 = wall();
-        lst.t_plan = t_plan1 - t_lvl0;
-        lst.t_elim_scale = t_lvl1 - t_plan1;
-        // ---------------- apply ranks: Orthogonal, Split, sibling (src/tree.cpp:1317-1342) ----------------
-        std::vector<QOp> qops;
-        std::vector<WyTask> wy;
-        std::vector<int4> wy_tiles;
-        std::vector<TrsmTask> wy_trsm;
-        for (size_t p = 0; p < plist.size(); p++) {
-            const int c = plist[p];
-            const int r = hsize[c];
-            const int s0 = cls_[c].size;
-            lst.rank_before += s0;
-            lst.nsparsified++;
-            if (monitor_flops_) {
-                // Asn as the sequential loop sees it: neighbours earlier in the list already
-                // hold their kept rank
-                int ncols = 0;
-                for (int e : cls_[c].in) ncols += size(edges_[e].n1);
-                for (size_t k = 1; k < cls_[c].out.size(); k++) ncols += size(edges_[cls_[c].out[k]].n2);
-                log_shape(SK_RRQR, s0, ncols, 0);
-            }
-            if (r < s0) {
-                const RrqrTask* t = where[p].first == 1   ? &wbig[wave[p]][where[p].second].t
-                                    : where[p].first == 2 ? &wreg[wave[p]][where[p].second]
-                                                          : &wtasks[wave[p]][where[p].second];
-                ops_.push_back({OP_ORTHOGONAL, c, -1, s0, 0, s0, r, ilvl_, t->Q, t->tau});
-                factor_bytes_ += ((size_t)s0 * r + (size_t)r * (r + 1) / 2) * sizeof(T);
-                if (r > 0) {
-                    // compact WY over all r kept reflectors, for the solve: Q = I - Ve T Ve^T
-                    void* Ve = arena_.alloc((size_t)s0 * r * sizeof(T));
-                    void* Sm = arena_.alloc((size_t)r * r * sizeof(T));
-                    void* X = arena_.alloc((size_t)r * r * sizeof(T));
-                    const int tw = (int)wy.size();
-                    wy.push_back({t->Q, t->tau, Ve, Sm, X, s0, r});
-                    for (int c0 = 0; c0 < r; c0 += 32)
-                        for (int a0 = 0; a0 <= c0; a0 += 32) wy_tiles.push_back(make_int4(tw, a0, c0, 0));
-                    push_trsm(wy_trsm, Sm, X, r, r, 0, sizeof(T));
-                    qops.push_back({Ve, X, s0, r, (int)cls_[c].xoff, (int)ws_len_});
-                    ws_len_ += 2 * r;  // the y and z segments of the apply
-                }
-                HCluster sib;
-                sib.start = cls_[c].start + r;
-                sib.size = sib.osize = s0 - r;
-                sib.level = cls_[c].level;
-                sib.spars = cls_[c].spars;
-                sib.elim = true;
-                sib.order = (int)cls_.size();
-                sib.xoff = cls_[c].xoff + r;
-                const int sid = (int)cls_.size();
-                cls_.push_back(sib);
-                others_.push_back(sid);
-                cls_[c].size = r;
-                ops_.push_back({OP_SPLIT, c, sid, r, s0 - r, 0, 0, ilvl_, nullptr, nullptr});
-                ops_.push_back({gen ? OP_SCALING_PLUQ : OP_SCALING_LLT, sid, -1, s0 - r, 0, s0 - r, s0 - r, ilvl_,
-                                nullptr, nullptr});
-                log_shape(gen ? SK_GETF : SK_POTF, s0 - r, 0, 0);  // eliminate_cluster(sibling)
-                // the sibling's pivot is the identity and is never stored (src/tree.cpp:1076-1099)
-            }
-            lst.rank_after += cls_[c].size;
-        }
-        const double t_lvl2 = wall();
-        lst.t_spars = t_lvl2 - t_lvl1;
-        // ---------------- Merge (src/tree.cpp:1435-1445, 1106-1184) ----------------
-        std::vector<MergeTask> mt;
-        std::vector<int2> mchildren;
-        if (ilvl_ < nlevels_ - 1) {
-            current_bottom_++;
-            const std::vector<int>& nb = bottoms_[current_bottom_];
-            std::vector<int> pos(cls_.size(), 0);  // a child's row offset inside its parent
-            for (int p : nb) {
-                HCluster& P = cls_[p];
-                int off = 0;
-                MergeTask m{(int)ws_len_, (int)mchildren.size(), 0, 0};
-                for (int ch : P.children) {
-                    pos[ch] = off;
-                    mchildren.push_back(make_int2((int)cls_[ch].xoff, cls_[ch].size));
-                    off += cls_[ch].size;
-                }
-                m.c1 = (int)mchildren.size();
-                P.size = P.osize = off;
-                P.xoff = ws_len_;
-                ws_len_ += off;
-                ops_.push_back({OP_MERGE, p, -1, off, 0, 0, 0, ilvl_, nullptr, nullptr});
-                if (off > 0) mt.push_back(m);
-            }
-            long long melems = 0;
-            std::vector<long long> moff;   // by new edge id - first_new
-            const int first_new = (int)edges_.size();
-            struct CT { int dst_edge; long long dst_add; const void* src; int rows, cols; };
-            std::vector<CT> cts;
-            // per parent: its new neighbours in order of first appearance over the children's
-            // out-edges (stamped slots instead of searches), each new edge original if any
-            // merged child edge is
-            std::vector<int> stamp(cls_.size(), -1), eid(cls_.size(), -1);
-            std::vector<char> origs;
-            std::vector<int> nnews;
-            for (int p : nb) {
-                nnews.clear();
-                origs.clear();
-                for (int ch : cls_[p].children)
-                    for (int e : cls_[ch].out) {
-                        const int nn = cls_[edges_[e].n2].parent;
-                        if (stamp[nn] != p) {
-                            stamp[nn] = p;
-                            eid[nn] = (int)nnews.size();
-                            nnews.push_back(nn);
-                            origs.push_back(0);
-                        }
-                        if (edges_[e].original) origs[eid[nn]] = 1;
-                    }
-                for (size_t q = 0; q < nnews.size(); q++) {
-                    const int nn = nnews[q];
-                    eid[nn] = new_edge(p, nn, nullptr, origs[q] != 0);
-                    moff.push_back(melems);
-                    melems += (long long)size(nn) * size(p);
-                }
-                auto& out = cls_[p].out;
-                std::stable_sort(out.begin() + 1, out.end(),
-                                 [&](int a, int b) { return cls_[edges_[a].n2].order < cls_[edges_[b].n2].order; });
-                auto& in = cls_[p].in;
-                std::stable_sort(in.begin(), in.end(),
-                                 [&](int a, int b) { return cls_[edges_[a].n1].order < cls_[edges_[b].n1].order; });
-                for (int ch : cls_[p].children) {
-                    for (int e : cls_[ch].out) {
-                        int nold = edges_[e].n2, nn = cls_[nold].parent;
-                        int de = eid[nn];
-                        long long add = pos[nold] + (long long)pos[ch] * size(nn);
-                        if (size(nold) > 0 && size(ch) > 0)
-                            cts.push_back({de, add, edges_[e].ident ? nullptr : edges_[e].A, size(nold), size(ch)});
-                    }
-                    cls_[ch].out.clear();
-                    cls_[ch].in.clear();
-                }
-            }
-            T* mbase = reinterpret_cast<T*>(arena_.alloc((size_t)std::max<long long>(melems, 1) * sizeof(T)));
-            for (size_t k = first_new; k < edges_.size(); k++) edges_[k].A = mbase + moff[k - first_new];
-            std::vector<CopyTask> copies;
-            copies.reserve(cts.size());
-            for (auto& c : cts) {
-                const HEdge& d = edges_[c.dst_edge];
-                copies.push_back({c.src, reinterpret_cast<T*>(d.A) + c.dst_add, c.rows, c.cols, size(d.n2), 0});
-            }
-            Packer pm;
-            size_t o_c = pm.add(copies);
-            void* dm = upload_tasks(pm);
-            if (melems > 0)
-                hip_check(timed(prof_, KF_FILL, st, (double)melems * sizeof(T), 0,
-                                [&] { return launch_fill<T>(mbase, (size_t)melems, T(0), st); }),
-                          "fill merge");
-            Work wc = work_copy(copies, sizeof(T));
-            hip_check(timed(prof_, KF_COPY, st, wc.bytes, 0,
-                            [&] { return launch_copy<T>(Packer::at<CopyTask>(dm, o_c), (int)copies.size(), st); }),
-                      "merge copy");
-        }
-        if (!wy.empty()) {
-            Packer pl;
-            const int sp_wy = split_trsm(wy_trsm);
-            size_t o_w = pl.add(wy), o_t = pl.add(wy_tiles), o_s = pl.add(wy_trsm);
-            void* dl = upload_tasks(pl);
-            double lb = 0, lf = 0;
-            for (auto& w : wy) {
-                lb += (3.0 * w.m * w.k + 3.0 * w.k * w.k) * sizeof(T);
-                lf += (double)w.m * w.k * w.k;
-            }
-            hip_check(timed(prof_, KF_LARFT, st, lb, lf,
-                            [&] {
-                                return launch_wy<T>(Packer::at<WyTask>(dl, o_w), (int)wy.size(),
-                                                    Packer::at<int4>(dl, o_t), (int)wy_tiles.size(), st);
-                            }),
-                      "compact WY");
-            Work w = work_trsm(wy_trsm, sizeof(T));
-            hip_check(timed(prof_, KF_TRSM, st, w.bytes, w.flops,
-                            [&] {
-                                return launch_trsm<T>(Packer::at<TrsmTask>(dl, o_s), (int)wy_trsm.size(), sp_wy, st);
-                            }),
-                      "compact WY inverse");
-        }
-        lst.t_merge = wall() - t_lvl2;
-        int left = 0;
-        for (int c : bottoms_[current_bottom_])
-            if (!cls_[c].elim) left += cls_[c].size;
-        lst.dofs_left_spars = left;
-        lst.fact_nnz = nnz();
-        // ---------------- solve plan for this level ----------------
-        // The forward terms are grouped here (the grouping reads cluster offsets, and the
-        // cluster array grows at the next merge); the rest is built on a worker thread from
-        // moved copies of this level's lists and uploaded once the factorization is done.
-        const double t_sp0 = wall();
-        {
-            std::vector<GemvGroup> fwd_groups;
-            std::vector<GemvTerm> fwd_t;
-            {
-                std::vector<int> gstart, gkeys;
-                const std::vector<int> forder = group_by_first(
-                    (int)fwd_list.size(), (int)cls_.size(), [&](int k) { return fwd_list[k].first; }, gstart, gkeys);
-                for (size_t gi = 0; gi < gkeys.size(); gi++) {
-                    const int k0 = (int)fwd_t.size();
-                    for (int q = gstart[gi]; q < gstart[gi + 1]; q++) fwd_t.push_back(fwd_list[forder[q]].second);
-                    const GemvTerm& t0 = fwd_t[k0];
-                    const int n = t0.trans ? t0.cols : t0.rows;
-                    fwd_groups.push_back({(int)cls_[gkeys[gi]].xoff, n, k0, (int)fwd_t.size()});
-                }
-            }
-            HostPhases in{std::move(sv_elim),    std::move(sv_scale),  std::move(fwd_groups), std::move(bwd_groups),
-                          std::move(fwd_t),      std::move(bwd_terms), std::move(qops)};
-            pending_solve[ilvl_] =
-                std::async(std::launch::async, [esz = sizeof(T), in = std::move(in), mt = std::move(mt),
-                                                mchildren = std::move(mchildren)]() mutable {
-                    return build_solve_plan(esz, std::move(in), std::move(mt), std::move(mchildren));
-                });
-        }
-        lst.t_solve_plan = wall() - t_sp0;
-        if (verb_)
-            printf("Level %d: elim %zu, scale %zu, sparsify %zu in %d waves, %d dofs left\n", ilvl_, n_potrf_elim,
-                   potrfS.size(), plist.size(), nwaves, left);
+        LevelPlan<T> L;
+        L.run = &R;
+        L.lst = &stats_[ilvl_];
+        L.lsv = &solve_plan_[ilvl_];
+        L.t_lvl0 = wall();
+        L.bot = &bottoms_[current_bottom_];
+        plan_elimination(L);
+        launch_elimination(L);
+        plan_scaling(L);
+        plan_sparsification(L);
+        launch_level(L);
+        apply_ranks(L);
+        merge_level(L);
+        plan_level_solve(L);
     }
     for (int l = 0; l < nlevels_; l++)
-        if (pending_solve[l].valid()) {
-            SolvePlanHost h = pending_solve[l].get();
+        if (R.pending_solve[l].valid()) {
+            SolvePlanHost h = R.pending_solve[l].get();
             upload_solve_plan(l, &h);
         }
     hip_check(hipDeviceSynchronize(), "factorize");

@@ -124,6 +124,8 @@ struct MvPhase {
 // Per-level solve plan: the forward phases (before the merge), the backward phases (after
 // the split back to the children), and the merge itself.
 struct HostPhases;  // host-side mv phases awaiting upload (tree.cpp)
+template <typename T>
+struct LevelPlan;   // one level's task lists during factorize (tree.cpp)
 
 struct LevelSolve {
     std::vector<MvPhase> fwd, bwd;
@@ -228,6 +230,21 @@ class Tree {
   private:
     enum class State { Created, Partitioned, Assembled, Factorized };
     template <typename T> void factorize_impl();
+    // the phases of one level of factorize_impl (tree.cpp), in order
+    template <typename T> void plan_elimination(LevelPlan<T>& L);
+    template <typename T> void launch_elimination(LevelPlan<T>& L);
+    template <typename T> void plan_scaling(LevelPlan<T>& L);
+    template <typename T> void plan_sparsification(LevelPlan<T>& L);
+    template <typename T> void launch_level(LevelPlan<T>& L);
+    template <typename T> void apply_ranks(LevelPlan<T>& L);
+    template <typename T> void merge_level(LevelPlan<T>& L);
+    template <typename T> void plan_level_solve(LevelPlan<T>& L);
+    template <typename T>
+    void run_potrf(hipStream_t st, void* dt, const std::vector<PotrfTask>& v, size_t off, const char* what);
+    template <typename T>
+    void run_trsm(hipStream_t st, void* dt, const std::vector<TrsmTask>& v, size_t off, int split, const char* what);
+    template <typename T>
+    void run_getrf(hipStream_t st, void* dt, const std::vector<GetrfTask>& v, size_t off, const char* what);
     // ws_ext: a caller-provided workspace of apply_workspace_bytes(nrhs) (then nothing is
     // allocated and no launch is timed), or nullptr for the tree's own.
     template <typename T>
