@@ -90,13 +90,28 @@ __device__ inline void panel_rows(T* A, int n, int kb, int nb, const T (*D)[PB +
 }
 
 // Unblocked right-looking Cholesky of the nb x nb diagonal block D (nb <= 32) by wavefront
-// 0, lane a holding row a in registers and column j broadcast by shuffles: L(j,j) =
+// 0, lane a holding row a in registers and column j broadcast by v_readlane: L(j,j) =
 // sqrt(d), rdiag[j] = 1 / L(j,j), L(a,j) = A(a,j) * rdiag[j], then A(a,b) = fma(-L(a,j),
 // L(b,j), A(a,b)) for a >= b > j, in j order. On a non-positive pivot at column j, *bad
 // becomes kb + j + 1 and the block is left as it stands. Every thread calls; ends with a
 // barrier.
+// Value of v in lane `lane` (a constant after unrolling) of the calling wavefront, broadcast
+// through v_readlane (no LDS round trip, unlike a general shuffle). Every lane of the
+// wavefront must be active.
+__device__ __attribute__((always_inline)) inline float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __attribute__((always_inline)) inline double lane_value(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Always inlined: called out of line, its LDS accesses go through flat pointers.
 template <typename T>
-__device__ inline void chol_diag_wave(T (*D)[PB + 1], T* rdiag, int nb, int kb, int* bad) {
+__device__ __attribute__((always_inline)) inline void chol_diag_wave(T (*D)[PB + 1], T* rdiag, int nb, int kb,
+                                                                  int* bad) {
 #pragma clang fp contract(off)
     if (threadIdx.x < 64) {
         const int a = threadIdx.x;
@@ -108,7 +123,7 @@ __device__ inline void chol_diag_wave(T (*D)[PB + 1], T* rdiag, int nb, int kb, 
 #pragma unroll
         for (int j = 0; j < PB; j++) {
             if (j < nb) {  // uniform
-                const T d = __shfl(x[j], j);
+                const T d = lane_value(x[j], j);
                 const bool ok = d > T(0);
                 if (!ok && !fail) fail = kb + j + 1;  // the block is not used after a failure
                 const T ljj = ok ? T(sqrt(d)) : T(1);
@@ -116,10 +131,10 @@ __device__ inline void chol_diag_wave(T (*D)[PB + 1], T* rdiag, int nb, int kb, 
                 if (a == j) rdiag[j] = rd;
                 x[j] = a == j ? ljj : (a > j ? x[j] * rd : x[j]);
                 const T laj = x[j];
-                // all shuffles issue together (no branch per b); rows a >= b < nb update
+                // all broadcasts issue together (no branch per b); rows a >= b < nb update
 #pragma unroll
                 for (int b = j + 1; b < PB; b++) {
-                    const T lbj = __shfl(laj, b);
+                    const T lbj = lane_value(laj, b);
                     const T u = fma(-laj, lbj, x[b]);
                     x[b] = (row && a >= b && b < nb) ? u : x[b];
                 }
