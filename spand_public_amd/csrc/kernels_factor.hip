@@ -1566,6 +1566,17 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
 constexpr int SLAB = 64;
 constexpr int STEP_NT = SLAB * NCH;
 
+// Device-memory pointers that the task records carry as plain pointers: accessed through a
+// pointer in the global address space, the compiler emits global_* instead of flat_*
+// instructions (a flat access also holds the LDS wait counter, so an LDS wait would wait
+// for every global load in flight as well).
+template <typename X>
+using gptr = __attribute__((address_space(1))) X*;
+template <typename X>
+__device__ __attribute__((always_inline)) inline gptr<X> gp(X* p) {
+    return (gptr<X>)p;
+}
+
 // Stepped-path storage: 64-column slabs, row-major inside a slab, so a wavefront reading
 // one row of its slab touches one contiguous segment.
 __device__ inline size_t slab_idx(int rows, int r, int j) {
@@ -1678,12 +1689,13 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
     if (blockIdx.x == 0 && tid == 0) signal_store(prog, seq);  // this launch has started
     // dims[3] may be lowered by slab 0 during this launch: read it once per workgroup
-    if (tid == 0) s_stop = B.dims[3];
-    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
+    const gptr<int> dims = gp(B.dims);
+    if (tid == 0) s_stop = dims[3];
+    const int rows = dims[0], cols = dims[1], mn = dims[2];
     const int i = step;
     const int j = sl.slab * SLAB + lane;
-    T* A = reinterpret_cast<T*>(B.t.scratch);
-    T* cj = A + (size_t)sl.slab * rows * SLAB + lane;  // element r of column j at cj[r * SLAB]
+    const gptr<T> A = gp(reinterpret_cast<T*>(B.t.scratch));
+    const gptr<T> cj = A + (size_t)sl.slab * rows * SLAB + lane;  // element r of column j at cj[r * SLAB]
     __syncthreads();
     const int stop = s_stop;
     if (rows == 0 || i >= mn || i >= stop) {
@@ -1704,20 +1716,25 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     }
     const int cur = i & 1, nxt = cur ^ 1;
     const int mc = B.t.max_cols, ns = B.nslab;
-    const int* p2c_cur = B.map + cur * mc;
-    int* p2c_nxt = B.map + nxt * mc;
-    const int* c2p_cur = B.map + 2 * mc + cur * mc;
-    int* c2p_nxt = B.map + 2 * mc + nxt * mc;
-    double* vn1 = B.t.norms;
-    double* vn2 = B.t.norms + mc;
-    T* s_col = step_buf<T>(B, smem_raw, sl.slab);  // the pivot column
-    T* s_v = s_col + B.t.max_rows;                 // its reflector, v[i] = 1
+    const gptr<int> map = gp(B.map);
+    const gptr<const int> p2c_cur = map + cur * mc;
+    const gptr<int> p2c_nxt = map + nxt * mc;
+    const gptr<const int> c2p_cur = map + 2 * mc + cur * mc;
+    const gptr<int> c2p_nxt = map + 2 * mc + nxt * mc;
+    const gptr<double> vn1 = gp(B.t.norms);
+    const gptr<double> vn2 = vn1 + mc;
+    const gptr<double> cand_v = gp(B.cand_v);
+    const gptr<int> cand_p = gp(B.cand_p);
+    // the pivot column and its reflector (v[i] = 1): always in LDS when rows are cached in
+    // registers (at most 16 RW rows; the launcher checks the LDS size)
+    T* s_col = RW > 0 ? reinterpret_cast<T*>(smem_raw) : step_buf<T>(B, smem_raw, sl.slab);
+    T* s_v = s_col + B.t.max_rows;
     if (wv == 0) {  // pivot: first maximum over the slabs' candidates
         double bv = -1.0;
         int bp = INT32_MAX;
         for (int k = lane; k < ns; k += SLAB) {
-            const double v = B.cand_v[cur * ns + k];
-            const int pp = B.cand_p[cur * ns + k];
+            const double v = cand_v[cur * ns + k];
+            const int pp = cand_p[cur * ns + k];
             if (v > bv || (v == bv && pp < bp)) {
                 bv = v;
                 bp = pp;
@@ -1753,10 +1770,10 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     // R_ii as stored (tau = 0 leaves alpha on the diagonal)
     const double rii = fabs((double)(T)sh[1]);
     if (i == 0) {
-        if (sl.slab == 0 && tid == 0) B.cand_v[2 * ns] = rii;
-    } else if (rank_fixed(tol, mn) < 0 && !(rii / B.cand_v[2 * ns] >= tol)) {
+        if (sl.slab == 0 && tid == 0) cand_v[2 * ns] = rii;
+    } else if (rank_fixed(tol, mn) < 0 && !(rii / cand_v[2 * ns] >= tol)) {
         if (sl.slab == 0 && tid == 0) {
-            B.dims[3] = i;  // rank = i: step i and later are not kept
+            dims[3] = i;  // rank = i: step i and later are not kept
             signal_store(B.done, 1);
         }
         return;
@@ -1771,7 +1788,7 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     if (tid == 0) s_v[i] = T(1);
     __syncthreads();
     if (ca / SLAB == sl.slab) {  // the owner stores the reflector: Q[:, i] = (R(0:i, ca), beta, v)
-        T* Qi = reinterpret_cast<T*>(B.t.Q) + (size_t)i * rows;
+        const gptr<T> Qi = gp(reinterpret_cast<T*>(B.t.Q)) + (size_t)i * rows;
         for (int r = tid; r < rows; r += NTH) {
             T v;
             if (r < i) v = A[slab_idx(rows, r, ca)];
@@ -1779,7 +1796,7 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
             else v = r == i ? (T)sh[1] : s_v[r];
             Qi[r] = v;
         }
-        if (tid == 0) reinterpret_cast<T*>(B.t.tau)[i] = tau;
+        if (tid == 0) gp(reinterpret_cast<T*>(B.t.tau))[i] = tau;
     }
     const int pos = (j == cb) ? p : (j < cols ? c2p_cur[j] : 0);
     const bool active = j < cols && j != ca && pos > i;
@@ -1924,8 +1941,8 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
         }
         wave_best(best, bpos);
         if (lane == 0) {
-            B.cand_v[nxt * ns + sl.slab] = best;
-            B.cand_p[nxt * ns + sl.slab] = bpos;
+            cand_v[nxt * ns + sl.slab] = best;
+            cand_p[nxt * ns + sl.slab] = bpos;
         }
     }
 }
@@ -2514,6 +2531,9 @@ template <typename T>
 hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int step, int max_rows,
                             double tol, size_t lds_bytes, int waves, long long* prog, long long seq, hipStream_t st) {
     if (nslabs == 0) return hipSuccess;
+    // the variants that cache rows (max_rows <= 16 * 32) keep the pivot column and its
+    // reflector in the dynamic LDS: it must hold both for the wave's largest cluster
+    if (max_rows <= NCH * 32 && lds_bytes < 2 * (size_t)std::max(max_rows, 1) * sizeof(T)) return hipErrorInvalidValue;
     const StepArgs a{d_bigs, d_slabs, nslabs, step, tol, lds_bytes, prog, seq, st};
     const int nw = rrqr_step_waves(nslabs, max_rows, waves);
     if (max_rows > NCH * 32) step_rw<T, 0, NCH>(a);

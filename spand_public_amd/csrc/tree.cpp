@@ -1707,6 +1707,7 @@ void Tree::plan_sparsification(LevelPlan<T>& L) {
         const size_t lw = ((size_t)wmaxcols * sizeof(int) + 15) / 16 * 16 + 2 * vb;
         if (persistent && !wbig[w].empty() && wmaxrows[w] <= WAVE_ROWS_MAX && lw <= (size_t)WAVE_LDS_MAX) {
             wlds_wave[w] = lw;
+            wlds_step[w] = 2 * vb;  // the per-step fallback (a wave the device cannot hold at once)
             for (auto& b : wbig[w])
                 b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
         } else if (2 * vb <= (size_t)STEP_LDS_MAX) {
