@@ -27,6 +27,17 @@
 
 namespace spand {
 
+// Device-memory pointers that the task records carry as plain pointers: accessed through a
+// pointer in the global address space, the compiler emits global_* instead of flat_*
+// instructions (a flat access also holds the LDS wait counter, so an LDS wait would wait
+// for every global load in flight as well).
+template <typename X>
+using gptr = __attribute__((address_space(1))) X*;
+template <typename X>
+__device__ __attribute__((always_inline)) inline gptr<X> gp(X* p) {
+    return (gptr<X>)p;
+}
+
 constexpr int NT = 256;
 
 // ---------------------------------------------------------------------------------------
@@ -1297,6 +1308,7 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
     __shared__ int s_bp[NWV], s_bc[NWV];
     __shared__ T s_v[RR];
     __shared__ double s_tau, s_rii;
+    __shared__ T s_taus[RR];  // every step's tau, stored to the task once at the end
     const RrqrTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rows = dsize[t.cluster];
@@ -1321,12 +1333,12 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
     for (int k = 0; k < K; k++) {
         const int j = tid + NTR * k;
         const bool jin = j < cols;
-        const T* E = reinterpret_cast<const T*>(sg[0].E);
+        gptr<const T> E = gp(reinterpret_cast<const T*>(sg[0].E));
         int sr = 0;
         if (jin) {
             const int sgi = seg_find(s_col0, t.nseg, j);
             const int jj = j - s_col0[sgi], w = s_col0[sgi + 1] - s_col0[sgi];
-            E = reinterpret_cast<const T*>(sg[sgi].E);
+            E = gp(reinterpret_cast<const T*>(sg[sgi].E));
             if (sg[sgi].out) {  // out-edge block (n x s): row jj of it
                 E += jj;
                 sr = w;
@@ -1465,7 +1477,7 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
             }
             s_tau = tau;
             pos[k] = i;
-            reinterpret_cast<T*>(t.tau)[i] = (T)tau;
+            s_taus[i] = (T)tau;
         }
         __syncthreads();
         const double taud = s_tau, rii = s_rii;
@@ -1521,12 +1533,14 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
         }
     }
     if (rank >= rows) return;
+    // tau of the kept reflectors (each written to LDS in its step, before that step's barrier)
+    for (int r = tid; r < rank; r += NTR) gp(reinterpret_cast<T*>(t.tau))[r] = s_taus[r];
     // Q: the reflectors of positions 0 .. rank - 1, column by column (R above, beta, v below)
-    T* Q = reinterpret_cast<T*>(t.Q);
+    const gptr<T> Q = gp(reinterpret_cast<T*>(t.Q));
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (pos[k] < 0 || pos[k] >= rank) continue;
-        T* q = Q + (size_t)pos[k] * rows;
+        const gptr<T> q = Q + (size_t)pos[k] * rows;
 #pragma unroll
         for (int r = 0; r < RR; r++)
             if (r < rows) q[r] = a[k][r];
@@ -1539,7 +1553,7 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
         const int j = tid + NTR * k;
         const int sgi = seg_find(s_col0, t.nseg, j);
         const int jj = j - s_col0[sgi], w = s_col0[sgi + 1] - s_col0[sgi];
-        T* E = reinterpret_cast<T*>(sg[sgi].E);
+        const gptr<T> E = gp(reinterpret_cast<T*>(sg[sgi].E));
         const bool out = sg[sgi].out;
 #pragma unroll
         for (int r = 0; r < RR; r++) {
@@ -1565,17 +1579,6 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
 // step later, so no data crosses workgroups inside a launch.
 constexpr int SLAB = 64;
 constexpr int STEP_NT = SLAB * NCH;
-
-// Device-memory pointers that the task records carry as plain pointers: accessed through a
-// pointer in the global address space, the compiler emits global_* instead of flat_*
-// instructions (a flat access also holds the LDS wait counter, so an LDS wait would wait
-// for every global load in flight as well).
-template <typename X>
-using gptr = __attribute__((address_space(1))) X*;
-template <typename X>
-__device__ __attribute__((always_inline)) inline gptr<X> gp(X* p) {
-    return (gptr<X>)p;
-}
 
 // Stepped-path storage: 64-column slabs, row-major inside a slab, so a wavefront reading
 // one row of its slab touches one contiguous segment.
