@@ -65,11 +65,11 @@ constexpr int PB = 32;
 // Rows i >= kb + nb of panel kb, one row per thread: column k is scaled by 1/L(k,k), then
 // the later panel columns receive fma(-a_k, L(k2,k), a_k2), in k order: the element
 // recurrence of the unblocked right-looking factorization.
-template <typename T>
-__device__ inline void panel_rows(T* A, int n, int kb, int nb, const T (*D)[PB + 1], const T* rdiag, int i0, int i1) {
+template <typename T, typename P>
+__device__ inline void panel_rows(P A, int n, int kb, int nb, const T (*D)[PB + 1], const T* rdiag, int i0, int i1) {
 #pragma clang fp contract(off)
     for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        T* Ai = A + i + (size_t)kb * n;  // row i of the panel, held in registers
+        const auto Ai = A + i + (size_t)kb * n;  // row i of the panel, held in registers
         T x[PB];
 #pragma unroll
         for (int k = 0; k < PB; k++) x[k] = k < nb ? Ai[(size_t)k * n] : T(0);
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
     const int n = t.n;
     const int tid = threadIdx.x;
     if (n == 0) return;
-    T* A = reinterpret_cast<T*>(t.A);
+    const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
     if (tid == 0) bad = 0;
     for (int kb = 0; kb < n; kb += PB) {
         const int nb = min(PB, n - kb);
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
         for (int idx = tid; idx < mt * mt; idx += NT) {
             const int i = idx % mt, j = idx / mt;
             if (i < j) continue;
-            T* a = A + (kb + nb + i) + (size_t)(kb + nb + j) * n;
+            const auto a = A + (kb + nb + i) + (size_t)(kb + nb + j) * n;
             T acc = *a;
             if (nb == PB) {  // a full panel: unrolled, so the LDS reads run ahead of the FMAs
 #pragma unroll
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(64) k_potrf64(const PotrfTask* tasks, int* inf
     const PotrfTask t = tasks[blockIdx.x];
     const int n = t.n, a = threadIdx.x;
     if (n == 0) return;
-    T* A = reinterpret_cast<T*>(t.A);
+    const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
     if (a == 0) bad = 0;
     for (int b = 0; b < n; b++) L[a][b] = a < n ? A[a + (size_t)b * n] : T(0);
     __syncthreads();
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(NT) k_potrf_panel(const PotrfTask* tasks, int 
     const int nb = min(PB, n - kb);
     const int r0 = kb + nb + blockIdx.x * NT;  // this workgroup's panel rows [r0, r0 + NT)
     if (blockIdx.x > 0 && r0 >= n) return;
-    T* A = reinterpret_cast<T*>(t.A);
+    const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
     if (tid == 0) bad = 0;
     for (int idx = tid; idx < nb * nb; idx += NT) {
         int i = idx % nb, k = idx / nb;
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(NT) k_potrf_panel(const PotrfTask* tasks, int 
         return;
     }
     if (blockIdx.x == 0) {
-        T* db = reinterpret_cast<T*>(t.dbuf);
+        const gptr<T> db = gp(reinterpret_cast<T*>(t.dbuf));
         for (int idx = tid; idx < nb * nb; idx += NT) db[idx] = D[idx % nb][idx / nb];
     }
     panel_rows<T>(A, n, kb, nb, D, rdiag, r0, min(n, r0 + NT));
@@ -312,8 +312,8 @@ __global__ void __launch_bounds__(NT) k_potrf_trail(const PotrfTask* tasks, int 
     if (kb >= n) return;
     if (blockIdx.x == 0) {  // the panel's factored diagonal block, left in dbuf by k_potrf_panel
         const int nb = min(PB, n - kb);
-        T* A = reinterpret_cast<T*>(t.A);
-        const T* db = reinterpret_cast<const T*>(t.dbuf);
+        const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
+        const gptr<const T> db = gp(reinterpret_cast<const T*>(t.dbuf));
         for (int idx = tid; idx < nb * nb; idx += NT) {
             int i = idx % nb, k = idx / nb;
             if (i >= k) A[(kb + i) + (size_t)(kb + k) * n] = db[idx];
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(NT) k_potrf_trail(const PotrfTask* tasks, int 
     const int bj = p;
     const int a0 = r0 + bi * PB, b0 = r0 + bj * PB;
     const int na = min(PB, n - a0), nbb = min(PB, n - b0);
-    T* A = reinterpret_cast<T*>(t.A);
+    const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
     for (int idx = tid; idx < PB * PB; idx += NT) {
         const int i = idx % PB, k = idx / PB;
         As[k][i] = i < na ? A[(a0 + i) + (size_t)(kb + k) * n] : T(0);
@@ -365,15 +365,15 @@ __global__ void __launch_bounds__(NT) k_trsm(const TrsmTask* tasks) {
     const int n = t.n, m = t.m, ld = t.ldb;
     const int tid = threadIdx.x;
     if (n == 0 || m == 0) return;
-    const T* L = reinterpret_cast<const T*>(t.L);
-    T* G = reinterpret_cast<T*>(t.B);
+    const gptr<const T> L = gp(reinterpret_cast<const T*>(t.L));
+    const gptr<T> G = gp(reinterpret_cast<T*>(t.B));
     const bool left = t.side == 0;
     // element (triangular index i, independent index c)
-    auto at = [&](int i, int c) -> T& { return left ? G[i + (size_t)c * ld] : G[c + (size_t)i * ld]; };
+    auto at = [&](int i, int c) -> decltype(G[0]) { return left ? G[i + (size_t)c * ld] : G[c + (size_t)i * ld]; };
     // a left solve from another block reads row perm[i] of it at the first touch of row i
     // (every row is first read in the kb = 0 pass), and its own block afterwards
-    const T* S = reinterpret_cast<const T*>(t.src);
-    const int* P = t.perm;
+    const gptr<const T> S = gp(reinterpret_cast<const T*>(t.src));
+    const gptr<const int> P = gp(t.perm);
     auto first = [&](int i, int c) -> T {
         return S ? S[(P ? P[i] : i) + (size_t)c * ld] : at(i, c);
     };
@@ -576,10 +576,10 @@ __global__ void __launch_bounds__(NTG) k_getrf(const GetrfTask* tasks, int in_ld
     const int n = t.n;
     const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
     if (n == 0) return;
-    T* G = reinterpret_cast<T*>(t.A);
-    T* Ut = reinterpret_cast<T*>(t.Ut);
-    int* perm = t.perm;
-    T* A = in_lds ? reinterpret_cast<T*>(getrf_lds) : G;
+    const gptr<T> G = gp(reinterpret_cast<T*>(t.A));
+    const gptr<T> Ut = gp(reinterpret_cast<T*>(t.Ut));
+    const gptr<int> perm = gp(t.perm);
+    T* A = in_lds ? reinterpret_cast<T*>(getrf_lds) : (T*)G;
     const int ld = in_lds ? n + 1 : n;
     if (in_lds)
         for (int c = wv; c < n; c += NW)
@@ -688,7 +688,7 @@ __global__ void __launch_bounds__(NT) k_schur(const SchurTile* tiles, const Schu
     // thread -> 4 outputs of the tile: row tid % 32, columns (tid / 32) * 4 + 0..3
     const int ti = tid % ST;
     const int tj0 = (tid / ST) * 4;
-    T* C = reinterpret_cast<T*>(t.C);
+    const gptr<T> C = gp(reinterpret_cast<T*>(t.C));
     T acc_c[4];
     for (int q = 0; q < 4; q++) {
         int i = ti, j = tj0 + q;
@@ -696,8 +696,8 @@ __global__ void __launch_bounds__(NT) k_schur(const SchurTile* tiles, const Schu
     }
     for (int kk = t.k0; kk < t.k1; kk++) {
         const SchurContrib c = contribs[kk];
-        const T* A = reinterpret_cast<const T*>(c.A);
-        const T* B = reinterpret_cast<const T*>(c.B);
+        const gptr<const T> A = gp(reinterpret_cast<const T*>(c.A));
+        const gptr<const T> B = gp(reinterpret_cast<const T*>(c.B));
         for (int l0 = 0; l0 < c.k; l0 += SK) {
             const int kc = min(SK, c.k - l0);
             // load opA(r0+i, l0+l) into As[l][i] and opB(l0+l, c0+j) into Bs[l][j]
@@ -750,12 +750,12 @@ __global__ void __launch_bounds__(NT) k_schur_f32_mfma(const SchurTile* tiles, c
     __shared__ float Ps[NT / 64][ST][ST + 1];
     const SchurTile t = tiles[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    float* C = reinterpret_cast<float*>(t.C);
+    const gptr<float> C = gp(reinterpret_cast<float*>(t.C));
     mfma_f32x16 acc = {};
     for (int kk = t.k0; kk < t.k1; kk++) {
         const SchurContrib c = contribs[kk];
-        const float* A = reinterpret_cast<const float*>(c.A);
-        const float* B = reinterpret_cast<const float*>(c.B);
+        const gptr<const float> A = gp(reinterpret_cast<const float*>(c.A));
+        const gptr<const float> B = gp(reinterpret_cast<const float*>(c.B));
         for (int l0 = 0; l0 < c.k; l0 += SK) {
             const int kc = min(SK, c.k - l0);
             for (int idx = tid; idx < SK * ST; idx += NT) {
@@ -789,7 +789,7 @@ __global__ void __launch_bounds__(NT) k_schur_f32_mfma(const SchurTile* tiles, c
             const int gi = t.r0 + i, gj = t.c0 + j;
             if (!t.lower || gi >= gj) {
                 const float sum = (Ps[0][j][i] + Ps[1][j][i]) + (Ps[2][j][i] + Ps[3][j][i]);
-                float* cp = C + gi + (size_t)gj * t.ldc;
+                const auto cp = C + gi + (size_t)gj * t.ldc;
                 *cp = *cp - sum;
             }
         }
@@ -1158,12 +1158,12 @@ __global__ void __launch_bounds__(NT) k_rrqr(const RrqrTask* tasks, const RrqrSe
                 }
             }
         }
-        if (tid == 0) reinterpret_cast<T*>(t.tau)[i] = tau;
+        if (tid == 0) gp(reinterpret_cast<T*>(t.tau))[i] = tau;
         __syncthreads();
     }
     if (rank >= rows) return;
     // Householder vectors v = A[:, :rank] (rows x rank, ld rows)
-    T* Q = reinterpret_cast<T*>(t.Q);
+    const gptr<T> Q = gp(reinterpret_cast<T*>(t.Q));
     for (long long idx = tid; idx < (long long)rows * rank; idx += NT)
         Q[idx] = A[(idx % rows) + (size_t)(idx / rows) * ld];
     // AsnP[:, jpvt[j]] = triu(R)[:rank, j], scattered into the shrunk edge blocks
@@ -1629,7 +1629,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         if (B.dims[3] == 0) signal_store(B.done, 1);
     }
     if (B.bar && tid == 0) B.bar[sl.slab] = 0;  // the persistent launch's per-slab step count
-    T* A = reinterpret_cast<T*>(B.t.scratch);
+    const gptr<T> A = gp(reinterpret_cast<T*>(B.t.scratch));
     const int mc = B.t.max_cols;
     const int j = sl.slab * SLAB + lane;
     const bool active = j < cols && rows > 0;
@@ -1638,7 +1638,7 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         const int k = seg_find(s_col0, B.t.nseg, j);
         const int jj = j - s_col0[k], w = s_col0[k + 1] - s_col0[k];
         const T* E = reinterpret_cast<const T*>(sg[k].E);
-        T* a = A + (size_t)sl.slab * rows * SLAB + lane;
+        const auto a = A + (size_t)sl.slab * rows * SLAB + lane;
         for (int i = wv; i < rows; i += NCH) {
             const T v = sg[k].out ? E[jj + (size_t)i * w] : E[i + (size_t)jj * rows];
             a[(size_t)i * SLAB] = v;
@@ -2257,12 +2257,12 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
     const int fin = stop & 1;  // maps after `stop` steps
     const int* p2c = B.map + fin * mc;
     const int* c2p = B.map + 2 * mc + fin * mc;
-    T* A = reinterpret_cast<T*>(B.t.scratch);
+    const gptr<T> A = gp(reinterpret_cast<T*>(B.t.scratch));
     __syncthreads();
     const int fixed = rank_fixed(tol, mn);
     const int rank = fixed >= 0 ? fixed : stop;
     if (rank >= rows) return;
-    T* Q = reinterpret_cast<T*>(B.t.Q);
+    const gptr<T> Q = gp(reinterpret_cast<T*>(B.t.Q));
     // the wave kernel leaves the reflectors in the pivot columns of A; the per-step path has
     // stored them in Q already, and the pivot columns' R entries come from there
     const bool in_a = B.dims[4] == 1;
@@ -2291,8 +2291,8 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
 template <typename T>
 __global__ void __launch_bounds__(NT) k_copy(const CopyTask* tasks) {
     const CopyTask t = tasks[blockIdx.x];
-    const T* S = reinterpret_cast<const T*>(t.src);
-    T* D = reinterpret_cast<T*>(t.dst);
+    const gptr<const T> S = gp(reinterpret_cast<const T*>(t.src));
+    const gptr<T> D = gp(reinterpret_cast<T*>(t.dst));
     const long long n = (long long)t.rows * t.cols;
     for (long long idx = threadIdx.x; idx < n; idx += NT) {
         int i = (int)(idx % t.rows), j = (int)(idx / t.rows);

@@ -18,6 +18,16 @@
 
 namespace spand {
 
+// Device-memory pointers that the task records carry as plain pointers: accessed through a
+// pointer in the global address space, the compiler emits global_* instead of flat_*
+// instructions (a flat access also holds the LDS wait counter).
+template <typename X>
+using gptr = __attribute__((address_space(1))) X*;
+template <typename X>
+__device__ __attribute__((always_inline)) inline gptr<X> gp(X* p) {
+    return (gptr<X>)p;
+}
+
 constexpr int NT = 256;
 // Segments are FP64 whatever the factor's element type T: an FP32 factor is read as FP32 and
 // applied in FP64 arithmetic, so the preconditioner stays one fixed linear operator.
@@ -112,7 +122,7 @@ __device__ inline void mv_chunk_reduce(const MvChunk& ch, const MvGroup& g, cons
         VT sum = VT(0);
         for (int p = ch.p0; p < ch.p1; p++) sum += partial[((size_t)p * SC + lane) * nrhs + r];
         const int o = ch.o0 + lane;
-        mv_apply(ws + (size_t)(g.xdst + (g.dperm ? g.dperm[o] : o)) * nrhs + r, g.mode, sum);
+        mv_apply(ws + (size_t)(g.xdst + (g.dperm ? gp(g.dperm)[o] : o)) * nrhs + r, g.mode, sum);
     }
 }
 
@@ -187,7 +197,7 @@ __global__ void __launch_bounds__(NT) k_mv_piece(const MvPiece* pieces, const Mv
                 p = qe / SC;
                 i = qe % SC;
             }
-            const T* A = reinterpret_cast<const T*>(q.A);
+            const gptr<const T> A = gp(reinterpret_cast<const T*>(q.A));
             if (i < no) v[u] = q.trans ? A[(q.lo + p) + (size_t)(o0 + i) * q.rows] : A[(o0 + i) + (size_t)(q.lo + p) * q.rows];
             dst[u] = (q.off + p) * (SC + 1) + i;
         }
@@ -203,7 +213,7 @@ __global__ void __launch_bounds__(NT) k_mv_piece(const MvPiece* pieces, const Mv
                 const int pos = idx / rg, r = idx % rg;
                 const MvTermL& q = st[p2t[pos]];
                 const int j = q.lo + (pos - q.off);
-                const int src = (!q.trans && q.perm) ? q.perm[j] : j;
+                const int src = (!q.trans && q.perm) ? gp(q.perm)[j] : j;
                 xv[u] = ws[(size_t)(q.xsrc + src) * nrhs + r0 + r];
                 xd[u] = pos * RG_MAX + r;
             }
