@@ -56,10 +56,10 @@ FLOP_BOUND = {"potrf", "trsm", "schur"}
 # file the run reports traffic null until the counters are measured again
 # (profiles/rrqr_step_laplace64_hbm_traffic.md gives the commands).
 TRAFFIC_MEASURED = {
-    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 3167.83, "write_kib": 4397.60, "fetch_factor": 2.0,
-                                   "write_factor": 1.0, "src_sha16": "b935209f168a23ea"},
-    ("rrqr_step", "f64", 64, 12): {"fetch_kib": 5867.05, "write_kib": 9149.93, "fetch_factor": 2.0,
-                                   "write_factor": 1.0, "src_sha16": "b935209f168a23ea"},
+    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 3162.20, "write_kib": 4399.45, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "bd12eb8f1dca482e"},
+    ("rrqr_step", "f64", 64, 12): {"fetch_kib": 5865.11, "write_kib": 9149.99, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "bd12eb8f1dca482e"},
 }
 
 
