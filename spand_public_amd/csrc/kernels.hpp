@@ -63,13 +63,16 @@ template <typename T>
 hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, const RrqrSeg* d_segs,
                             const int* d_size, double tol, hipStream_t st);
 // All steps of a set of whole clusters in one cooperative launch (clusters of at most
-// WAVE_ROWS_MAX rows, RrqrBig::vbuf holding [2][nslab][max_rows] candidate columns,
-// RrqrBig::bar a barrier counter); lds_bytes = max_cols ints + 2 max_rows elements. The
-// slabs of each cluster synchronize among themselves (bounded waits: *fail is set to 1 if
-// one gives up). Returns hipErrorCooperativeLaunchTooLarge, launching nothing, when the
-// slabs do not fit the device at once; rrqr_wave_capacity gives how many do.
+// WAVE_ROWS_MAX rows, RrqrBig::vbuf holding rrqr_relay_bytes of relay words, relay = 1);
+// lds_bytes = max_cols ints (16-byte aligned) + 2 max_rows elements. The slabs of each
+// cluster exchange their candidates through the relay words (bounded waits: *fail is set
+// to 1 if one gives up). Returns hipErrorCooperativeLaunchTooLarge, launching nothing, when
+// the slabs do not fit the device at once; rrqr_wave_capacity gives how many do.
 constexpr int WAVE_ROWS_MAX = 512;
 constexpr int WAVE_LDS_MAX = 49152;
+inline size_t rrqr_relay_bytes(size_t esz, int nslab, int max_rows) {
+    return ((size_t)2 * nslab * 4 + (size_t)2 * nslab * max_rows * (esz / 4)) * 8;
+}
 template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
                             double tol, size_t lds_bytes, int* fail, hipStream_t st);

@@ -1586,6 +1586,75 @@ __device__ inline size_t slab_idx(int rows, int r, int j) {
     return ((size_t)(j / SLAB) * rows + r) * SLAB + (j % SLAB);
 }
 
+// Relay words of the persistent RRQR: 64-bit words (tag << 32 | 32-bit payload) stored and
+// loaded whole by device-scope relaxed atomics (on gfx950 these bypass the per-XCD L2
+// caches, which are not coherent with each other). A reader that sees the tag it waits for
+// sees the payload of the same store, so no fence orders data and flag: each word is its
+// own flag. Tags are step + 1; k_rrqr_init clears the words to tag 0 before the launch.
+using RelayWord = unsigned long long;
+typedef __attribute__((address_space(1))) RelayWord GRelayWord;
+constexpr long long RELAY_SPIN_MAX = 1ll << 21;
+
+__device__ inline void relay_put(GRelayWord* p, unsigned tag, unsigned v) {
+    __hip_atomic_store(p, ((RelayWord)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a value of T as sizeof(T) / 4 relay words
+template <typename T>
+__device__ inline void relay_put_val(GRelayWord* p, unsigned tag, T x) {
+    if constexpr (sizeof(T) == 4) {
+        relay_put(p, tag, __float_as_uint(x));
+    } else {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+        relay_put(p, tag, (unsigned)b);
+        relay_put(p + 1, tag, (unsigned)(b >> 32));
+    }
+}
+// Waits until the first n of the words p[0..N) all carry `tag` (n loads in flight per poll,
+// one round trip when the data is there); payloads in v. False after about 2^21 polls or
+// once *fail is set (the launch is cooperative, so every slab is resident and a correct
+// schedule never comes near the bound; the host turns the flag into an error).
+template <int N>
+__device__ inline bool relay_get_all(const GRelayWord* const (&p)[N], int n, unsigned tag, unsigned (&v)[N],
+                                     int* fail) {
+    for (long long spins = 0;; spins++) {
+        RelayWord w[N];
+#pragma unroll
+        for (int k = 0; k < N; k++)
+            w[k] = k < n ? __hip_atomic_load(p[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ((RelayWord)tag << 32);
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < N; k++) ok = ok && (unsigned)(w[k] >> 32) == tag;
+        if (ok) {
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = (unsigned)w[k];
+            return true;
+        }
+        if (spins > RELAY_SPIN_MAX ||
+            ((spins & 63) == 63 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+// relay layout of one cluster: candidates [2][nslab][4] (norm low, norm high, position),
+// then candidate columns [2][nslab][max_rows * sizeof(T) / 4]
+template <typename T>
+__device__ inline GRelayWord* relay_cand(const RrqrBig& B, int buf, int slab) {
+    return (GRelayWord*)(B.vbuf) + ((size_t)buf * B.nslab + slab) * 4;
+}
+template <typename T>
+__device__ inline GRelayWord* relay_col(const RrqrBig& B, int buf, int slab) {
+    return (GRelayWord*)(B.vbuf) + (size_t)2 * B.nslab * 4 +
+           ((size_t)buf * B.nslab + slab) * B.t.max_rows * (sizeof(T) / 4);
+}
+__device__ inline void relay_put_cand(GRelayWord* p, unsigned tag, double v, int pos) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    relay_put(p, tag, (unsigned)b);
+    relay_put(p + 1, tag, (unsigned)(b >> 32));
+    relay_put(p + 2, tag, (unsigned)pos);
+}
+
 // pivot column and reflector buffers (2 x max_rows): LDS, or the task's global buffer
 template <typename T>
 __device__ inline T* step_buf(const RrqrBig& B, unsigned char* smem, int slab) {
@@ -1628,7 +1697,14 @@ __global__ void __launch_bounds__(STEP_NT) k_rrqr_init(const RrqrBig* bigs, cons
         B.dims[4] = 2;  // 2: the steps store every kept reflector in Q; 1: the wave kernel left them in A
         if (B.dims[3] == 0) signal_store(B.done, 1);
     }
-    if (B.bar && tid == 0) B.bar[sl.slab] = 0;  // the persistent launch's per-slab step count
+    if (B.relay) {  // the persistent launch's relay words of this slab: tag 0 (no step)
+        for (int b = 0; b < 2; b++) {
+            GRelayWord* c = relay_cand<T>(B, b, sl.slab);
+            if (tid < 4) c[tid] = 0;
+            GRelayWord* col = relay_col<T>(B, b, sl.slab);
+            for (int r = tid; r < B.t.max_rows * (int)(sizeof(T) / 4); r += STEP_NT) col[r] = 0;
+        }
+    }
     const gptr<T> A = gp(reinterpret_cast<T*>(B.t.scratch));
     const int mc = B.t.max_cols;
     const int j = sl.slab * SLAB + lane;
@@ -1950,88 +2026,88 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     }
 }
 
-// Barrier among the slabs of one cluster in the persistent RRQR: each slab publishes, once
-// per step and after its candidate (release at device scope), the number of steps it has
-// finished in its own flag; a slab waits until every flag of its cluster reaches the step
-// (lanes of wavefront 0 poll one flag each; acquire). The wait is bounded: after about 2^21
-// polls it sets *fail and returns false (the launch is cooperative, so every slab is
-// resident and a correct schedule never comes near the bound; the host turns the flag into
-// an error).
-constexpr long long CLUSTER_SPIN_MAX = 1ll << 21;
-__device__ inline void cluster_arrive(int* flags, int slab, int done) {
-    __syncthreads();  // every thread's stores of this step are issued
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(flags + slab, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+// CPW consecutive values of T from 16-byte aligned LDS (one or two 16-byte reads)
+template <typename T, int CPW>
+__device__ inline void lds_row_group(const T* p, T (&out)[CPW]) {
+    static_assert(CPW * sizeof(T) % 16 == 0, "whole 16-byte reads");
+    __builtin_memcpy(out, __builtin_assume_aligned(p, 16), CPW * sizeof(T));
 }
-__device__ inline bool cluster_wait(const int* flags, int nslab, int target, int* fail, int* s_ok) {
-    if (threadIdx.x < 64) {
-        int ok = 1;
-        for (int k = threadIdx.x; k < nslab; k += 64) {
-            long long spins = 0;
-            while (__hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                if (++spins > CLUSTER_SPIN_MAX ||
-                    __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                    ok = 0;
-                    __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-            if (!ok) break;
-        }
-        ok = __all(ok);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (threadIdx.x == 0) *s_ok = ok;
-    }
-    __syncthreads();
-    return *s_ok != 0;
+
+template <typename T, int CPW>
+__device__ inline void lds_store_group(T* p, const T (&in)[CPW]) {
+    static_assert(CPW * sizeof(T) % 16 == 0, "whole 16-byte writes");
+    __builtin_memcpy(__builtin_assume_aligned(p, 16), in, CPW * sizeof(T));
 }
 
 // Persistent form of the stepped RRQR: one cooperative launch runs every pivot step of the
-// clusters of a wave (or of a part of it that the device holds at once); the slabs of each
-// cluster synchronize among themselves between steps (cluster_wait), so a cluster's steps
-// are paced by its own slabs only, and a cluster that reaches its rank leaves. Each thread keeps its column's rows
-// r = wv + 16 q (q < WAVE_RW) in registers for the whole factorization; lane l of
-// wavefront wv is column l of the slab, and chain c of every sum at step i is held by
-// wavefront (c + i) % 16, so sums run in the same 16-chain order as k_rrqr_step. Every
-// slab publishes its best column (candidate for the next pivot) in a double-buffered
-// global buffer; the pivot's dlarfg is recomputed from that copy in every workgroup, and
-// its owner stores the reflector in its registers. The registers and the final maps go
-// back to the scratch at the end, for k_rrqr_finish.
-// WAVE_RW rows per thread: clusters of up to 16 * WAVE_RW rows (16: two workgroups per CU).
-template <typename T, int WAVE_RW>
-__global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
+// clusters of a wave (or of the part of it that the device holds at once), and each
+// cluster's slabs pace each other only. After its update at step i, every slab publishes
+// its best column (candidate for pivot i + 1: partial norm, position, rows i + 1..) in relay
+// words tagged i + 2; at step i + 1 each slab waits for the candidates of its cluster,
+// takes the first maximum and reads the winner's rows, then recomputes the step's dlarfg
+// itself. One round trip through memory per step and no fences.
+// Wavefront wv of NW holds, for its lane's column, rows r = 16 q + CPW wv + k (k < CPW =
+// 16 / NW, q < RW) in registers for the whole factorization. Row r belongs to chain
+// (r - i) % 16 of the w_j sums at step i and to chain (r - i - 1) % 16 of the norm sums,
+// each chain summed in increasing rows, so every sum runs in the 16-chain order of
+// k_rrqr_step. The sums run over every cached row with the reflector zero above row i and
+// past the last row (s_v) and a 0/1 mask for the norms (s_m): a zero term leaves a chain's
+// sum unchanged bit for bit, and the loops test rows only per group of CPW (skipping the
+// groups wholly above row i or past the last row). The pivot's owner keeps the reflector
+// (beta, v) in its registers; the registers and the final maps go back to the slab storage
+// at the end, where k_rrqr_finish copies the reflectors into Q.
+// Wavefronts per SIMD the register budget targets: the cached values plus 96 registers
+// (a few loop-invariant values spill); 128 cached registers take the whole file (the
+// accumulation registers hold what does not fit, no scratch).
+template <typename T, int RW, int NW>
+constexpr int wave_occupancy() {
+    constexpr int c = (NCH / NW) * RW * (int)(sizeof(T) / 4);
+    return c > 64 ? 1 : 512 / (c + 96) > 8 ? 8 : 512 / (c + 96);
+}
+template <typename T, int RW, int NW>
+__global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
     k_rrqr_wave(const RrqrBig* bigs, const RrqrSlab* slabs, int nsteps, double tol, int* fail) {
 #pragma clang fp contract(off)
+    static_assert(NCH % NW == 0, "chains split evenly over the wavefronts");
+    constexpr int NTH = SLAB * NW, CPW = NCH / NW, RMAX = NCH * RW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double s_part[NCH][SLAB];
     __shared__ double sh[4], s_ch[NCH + 1];
+    __shared__ __align__(16) T s_col[RMAX];  // the pivot column
+    __shared__ __align__(16) T s_v[RMAX];    // its reflector; zero above row i and past the last row
+    __shared__ __align__(16) T s_m[RMAX];    // 1 for rows i + 1 .. rows - 1, else 0
     __shared__ T s_tm[SLAB], s_rowi[SLAB];
     __shared__ int s_upd[SLAB], s_pos[SLAB];
-    __shared__ int s_p, s_ca, s_cb, s_best, s_ok;
+    __shared__ int s_p, s_ca, s_cb, s_best, s_ok, s_ok_col;
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
-    const int tid = threadIdx.x, lane = tid % SLAB, wv = tid / SLAB;
-    const int rows = B.dims[0], cols = B.dims[1], mn = B.dims[2];
-    int stop = B.dims[3];
-    const int ns = B.nslab, mc = B.t.max_cols, mr = B.t.max_rows;
+    const int tid = threadIdx.x, lane = tid % SLAB;
+    const int wv = __builtin_amdgcn_readfirstlane(tid / SLAB);  // uniform: row tests are scalar
+    const int r0 = CPW * wv;                                      // first cached row
+    const gptr<const int> dims = gp(reinterpret_cast<const int*>(B.dims));
+    const int rows = dims[0], cols = dims[1], mn = dims[2];
+    int stop = dims[3];
+    const int ns = B.nslab, mc = B.t.max_cols;
     int* p2c = reinterpret_cast<int*>(smem_raw);  // position -> column, this workgroup's copy
-    T* s_col = reinterpret_cast<T*>(smem_raw + ((size_t)mc * sizeof(int) + 15) / 16 * 16);  // pivot column
-    T* s_v = s_col + mr;                                                                     // its reflector
-    T* A = reinterpret_cast<T*>(B.t.scratch);
-    T* ccol = reinterpret_cast<T*>(B.vbuf);  // [2][nslab][max_rows] candidate columns
-    T* tau_out = reinterpret_cast<T*>(B.t.tau);
+    const gptr<T> tau_out = gp(reinterpret_cast<T*>(B.t.tau));
     const int j = sl.slab * SLAB + lane;
     const bool jin = j < cols && rows > 0;
-    T* aslab = A + (size_t)sl.slab * rows * SLAB + lane;
-    T a[WAVE_RW];
+    T a[CPW][RW];
+    {
+        const gptr<const T> aslab = gp(reinterpret_cast<const T*>(B.t.scratch)) + (size_t)sl.slab * rows * SLAB + lane;
 #pragma unroll
-    for (int q = 0; q < WAVE_RW; q++) {
-        const int r = wv + NCH * q;
-        a[q] = (jin && r < rows) ? aslab[(size_t)r * SLAB] : T(0);
+        for (int q = 0; q < RW; q++)
+#pragma unroll
+            for (int k = 0; k < CPW; k++) {
+                const int r = NCH * q + r0 + k;
+                a[k][q] = (jin && r < rows) ? aslab[(size_t)r * SLAB] : T(0);
+            }
     }
-    for (int k = tid; k < cols; k += STEP_NT) p2c[k] = k;
+    for (int k = tid; k < cols; k += NTH) p2c[k] = k;
+    for (int r = tid; r < RMAX; r += NTH) {
+        s_v[r] = T(0);
+        s_m[r] = r < rows ? T(1) : T(0);
+    }
     if (tid < SLAB) s_pos[tid] = j;
     double vn1 = 0.0, vn2 = 0.0;
     if (wv == 0 && jin) {
@@ -2041,55 +2117,102 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
     const double tol3z = sqrt(DBL_EPSILON * 0.5);
     const int fixed = rank_fixed(tol, mn);
     double sref = 0.0;
-    __syncthreads();
-    // candidate column of step 0 (k_rrqr_init left the slab's best in cand_*[0])
-    if (rows > 0) {
-        const int bp = B.cand_p[sl.slab];
-        if (bp != INT32_MAX && lane == bp % SLAB) {
-            T* dst = ccol + (size_t)sl.slab * mr;
+    // candidate of step 0 (k_rrqr_init left the slab's best in cand_*[slab]), tag 1; its
+    // column goes through LDS as in the steps
+    const bool run = rows > 0 && mn > 0 && stop > 0;
+    const int bp0 = run ? B.cand_p[sl.slab] : INT32_MAX;
+    if (run && tid == 0) relay_put_cand(relay_cand<T>(B, 0, sl.slab), 1u, B.cand_v[sl.slab], bp0);
+    if (bp0 != INT32_MAX && lane == bp0 % SLAB) {
 #pragma unroll
-            for (int q = 0; q < WAVE_RW; q++) {
-                const int r = wv + NCH * q;
-                if (r < rows) dst[r] = a[q];
-            }
+        for (int q = 0; q < RW; q++) {
+            T x[CPW];
+#pragma unroll
+            for (int k = 0; k < CPW; k++) x[k] = a[k][q];
+            lds_store_group<T, CPW>(s_col + NCH * q + r0, x);
         }
     }
-    cluster_arrive(B.bar, sl.slab, 1);
+    __syncthreads();
+    if (bp0 != INT32_MAX) {
+        GRelayWord* dst = relay_col<T>(B, 0, sl.slab);
+        for (int r = tid; r < rows; r += NTH) relay_put_val<T>(dst + (size_t)r * (sizeof(T) / 4), 1u, s_col[r]);
+    }
     bool abandoned = false;
     for (int i = 0; i < nsteps; i++) {
         if (rows == 0 || i >= mn || i >= stop) break;  // the same decision in every slab of the cluster
-        if (!cluster_wait(B.bar, ns, i + 1, fail, &s_ok)) {
-            abandoned = true;
-            break;
-        }
         const int cur = i & 1, nxt = cur ^ 1;
+        const unsigned tag = (unsigned)i + 1u;
+        // this wavefront's row window [i, rows) relative to r0, opaque to the compiler so
+        // that no per-row test is hoisted out of the step loop (64 live scalar masks)
+        int iw = i - r0, rw = rows - r0;
+        asm volatile("" : "+s"(rw));
         if (wv == 0) {  // pivot: first maximum over the slabs' candidates
             double bv = -1.0;
             int bp = INT32_MAX;
-            for (int k = lane; k < ns; k += SLAB) {
-                const double v = B.cand_v[cur * ns + k];
-                const int pp = B.cand_p[cur * ns + k];
-                if (v > bv || (v == bv && pp < bp)) {
-                    bv = v;
-                    bp = pp;
+            bool ok = true;
+            for (int k = lane; k < ns && ok; k += SLAB) {
+                const GRelayWord* c = relay_cand<T>(B, cur, k);
+                const GRelayWord* const pc[3] = {c, c + 1, c + 2};
+                unsigned u[3];
+                ok = relay_get_all<3>(pc, 3, tag, u, fail);
+                if (ok) {
+                    const double v = __longlong_as_double((long long)(((unsigned long long)u[1] << 32) | u[0]));
+                    const int pp = (int)u[2];
+                    if (v > bv || (v == bv && pp < bp)) {
+                        bv = v;
+                        bp = pp;
+                    }
                 }
             }
             wave_best(bv, bp);
+            ok = __all(ok);
             if (lane == 0) {
+                s_ok = ok;
+                s_ok_col = 1;  // cleared below by a failed read of the winner's rows
                 s_p = bp;
                 s_ca = (bp >= i && bp < cols) ? p2c[bp] : 0;
                 s_cb = p2c[i];
             }
         }
         __syncthreads();
+        if (!s_ok) {
+            abandoned = true;
+            break;
+        }
         const int p = s_p, ca = s_ca, cb = s_cb;
         if (p < i || p >= cols) {  // cannot happen for i < mn: some column is still free
             stop = i;
             break;
         }
-        const T* pc = ccol + ((size_t)cur * ns + ca / SLAB) * mr;
-        for (int r = i + tid; r < rows; r += STEP_NT) s_col[r] = pc[r];
+        {  // the winner's rows i.. (published by the owner of column ca at step i - 1)
+            static_assert(RMAX <= 2 * NTH, "at most two rows per thread");
+            constexpr int WD = (int)(sizeof(T) / 4), NWD = 2 * WD;
+            const GRelayWord* src = relay_col<T>(B, cur, ca / SLAB);
+            const int r1 = i + tid, r2 = i + tid + NTH;
+            const int nr = (r1 < rows) + (r2 < rows);
+            if (nr > 0) {
+                const GRelayWord* pw[NWD];
+#pragma unroll
+                for (int k = 0; k < NWD; k++) pw[k] = src + (size_t)(k < WD ? r1 : r2) * WD + k % WD;
+                unsigned u[NWD];
+                if (relay_get_all<NWD>(pw, nr * WD, tag, u, fail)) {
+                    T x[2];
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        if constexpr (sizeof(T) == 4) x[h] = __uint_as_float(u[h]);
+                        else x[h] = __longlong_as_double((long long)(((unsigned long long)u[2 * h + 1] << 32) | u[2 * h]));
+                    }
+                    s_col[r1] = x[0];
+                    if (nr > 1) s_col[r2] = x[1];
+                } else {
+                    s_ok_col = 0;
+                }
+            }
+        }
         __syncthreads();
+        if (!s_ok_col) {
+            abandoned = true;
+            break;
+        }
         larfg_block<T>((double)s_col[i], s_col + i + 1, rows - i - 1, sh, s_ch);
         // R_ii as stored (tau = 0 leaves alpha on the diagonal); the QR stops at the rank
         const double rii = fabs((double)(T)sh[1]);
@@ -2103,11 +2226,13 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
         const T tau = (T)taud;
         if (taud != 0.0) {
             const int knt = (int)sh[3];
-            for (int r = i + 1 + tid; r < rows; r += STEP_NT) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
+            for (int r = i + 1 + tid; r < rows; r += NTH) s_v[r] = larfg_scale<T>(s_col[r], knt, sh[2]);
         }
         const bool own_pivot = ca / SLAB == sl.slab;
         if (tid == 0) {
+            if (i > 0) s_v[i - 1] = T(0);
             s_v[i] = T(1);
+            s_m[i] = T(0);
             p2c[i] = ca;
             p2c[p] = cb;
             if (own_pivot) {
@@ -2117,27 +2242,24 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
             if (cb / SLAB == sl.slab && cb != ca) s_pos[cb % SLAB] = p;
         }
         __syncthreads();
-        if (own_pivot && lane == ca % SLAB && taud != 0.0) {  // the reflector: beta on the diagonal, v below
-#pragma unroll
-            for (int q = 0; q < WAVE_RW; q++) {
-                const int r = wv + NCH * q;
-                if (r == i) a[q] = (T)sh[1];
-                else if (r > i && r < rows) a[q] = s_v[r];
-            }
-        }
         const int pos = s_pos[lane];
         const bool active = jin && pos > i;
-        const int ci = i % NCH;
-        if (taud != 0.0) {  // w_j = v^T a_j, this thread's chain
-            T c = T(0);
+        if (taud != 0.0) {  // w_j = v^T a_j, chain (r - i) % 16
+            T c[CPW];
+#pragma unroll
+            for (int k = 0; k < CPW; k++) c[k] = T(0);
             if (active) {
 #pragma unroll
-                for (int q = 0; q < WAVE_RW; q++) {
-                    const int r = wv + NCH * q;
-                    if (r >= i && r < rows) c = fma(a[q], s_v[r], c);
+                for (int q = 0; q < RW; q++) {
+                    if (NCH * q + CPW <= iw || NCH * q >= rw) continue;  // rows all above i or past the last
+                    T v[CPW];
+                    lds_row_group<T, CPW>(s_v + NCH * q + r0, v);
+#pragma unroll
+                    for (int k = 0; k < CPW; k++) c[k] = fma(a[k][q], v[k], c[k]);
                 }
             }
-            s_part[(wv - ci + NCH) % NCH][lane] = (double)c;
+#pragma unroll
+            for (int k = 0; k < CPW; k++) s_part[(r0 + k - i % NCH + NCH) % NCH][lane] = (double)c[k];
             __syncthreads();
             if (wv == 0) {
                 const T w = tree16_lds_t<T>(&s_part[0][lane], SLAB);
@@ -2145,35 +2267,65 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
                 s_upd[lane] = w != T(0);  // dger skips columns with w_j = 0
             }
             __syncthreads();
+            // a_j += tm v (v is 1 on row i and 0 above it)
+            if (active && s_upd[lane]) {
+                const T tm = s_tm[lane];
+#pragma unroll
+                for (int q = 0; q < RW; q++) {
+                    if (NCH * q + CPW <= iw || NCH * q >= rw) continue;
+                    T v[CPW];
+                    lds_row_group<T, CPW>(s_v + NCH * q + r0, v);
+#pragma unroll
+                    for (int k = 0; k < CPW; k++) a[k][q] = fma(v[k], tm, a[k][q]);
+                }
+            }
         }
-        // a_j += tm v, and the chains of the sum of squares over rows > i
-        const bool upd = active && taud != 0.0 && s_upd[lane];
-        const T tm = upd ? s_tm[lane] : T(0);
-        double c2 = 0.0;
+        // row i of every column, for the norm downdate
+        if (iw >= 0 && iw / NCH < RW && iw % NCH < CPW) {
+            const int kq = iw % NCH, qq = iw / NCH;
+            T x = T(0);
+#pragma unroll
+            for (int q = 0; q < RW; q++)
+#pragma unroll
+                for (int k = 0; k < CPW; k++)
+                    if (k == kq && q == qq) x = a[k][q];
+            s_rowi[lane] = x;
+        }
+        // chains (r - i - 1) % 16 of the sum of squares over rows > i
+        double c2[CPW];
+#pragma unroll
+        for (int k = 0; k < CPW; k++) c2[k] = 0.0;
         if (active) {
 #pragma unroll
-            for (int q = 0; q < WAVE_RW; q++) {
-                const int r = wv + NCH * q;
-                if (r >= i && r < rows) {
-                    if (upd) a[q] = (r == i) ? a[q] + tm : fma(s_v[r], tm, a[q]);
-                    if (r == i) {
-                        s_rowi[lane] = a[q];
-                    } else {
-                        const double d = (double)a[q];
-                        c2 = fma(d, d, c2);
+            for (int q = 0; q < RW; q++) {
+                if (NCH * q + CPW <= iw || NCH * q >= rw) continue;
+                if (NCH * q <= iw) {  // the group holding row i: rows <= i masked out
+                    T m[CPW];
+                    lds_row_group<T, CPW>(s_m + NCH * q + r0, m);
+#pragma unroll
+                    for (int k = 0; k < CPW; k++) {
+                        const double d = (double)(a[k][q] * m[k]);
+                        c2[k] = fma(d, d, c2[k]);
+                    }
+                } else {  // rows > i (rows past the last hold 0)
+#pragma unroll
+                    for (int k = 0; k < CPW; k++) {
+                        const double d = (double)a[k][q];
+                        c2[k] = fma(d, d, c2[k]);
                     }
                 }
             }
         }
-        s_part[(wv - ci - 1 + 2 * NCH) % NCH][lane] = c2;
+#pragma unroll
+        for (int k = 0; k < CPW; k++) s_part[(r0 + k - i % NCH - 1 + 2 * NCH) % NCH][lane] = c2[k];
         __syncthreads();
         if (wv == 0) {
             double best = -1.0;
             int bpos = INT32_MAX;
             if (active) {
                 if (vn1 != 0.0) {
-                    const double qq = fabs((double)s_rowi[lane]) / vn1;
-                    double temp = 1.0 - qq * qq;
+                    const double qv = fabs((double)s_rowi[lane]) / vn1;
+                    double temp = 1.0 - qv * qv;
                     temp = fmax(temp, 0.0);
                     const double rr = vn1 / vn2;
                     const double temp2 = temp * rr * rr;
@@ -2194,40 +2346,75 @@ __global__ void __launch_bounds__(STEP_NT, WAVE_RW <= 16 ? 2 : 1)
             }
             wave_best(best, bpos);
             if (lane == 0) {
-                B.cand_v[nxt * ns + sl.slab] = best;
-                B.cand_p[nxt * ns + sl.slab] = bpos;
+                relay_put_cand(relay_cand<T>(B, nxt, sl.slab), tag + 1u, best, bpos);
                 s_best = bpos != INT32_MAX ? p2c[bpos] : -1;
             }
         }
         __syncthreads();
-        if (s_best >= 0 && lane == s_best % SLAB) {  // the slab's candidate column for step i + 1
-            T* dst = ccol + ((size_t)nxt * ns + sl.slab) * mr;
+        // the slab's candidate column for step i + 1: its rows > i go through LDS (s_col is
+        // free until the next step's read), then every thread stores one or two of them
+        const int best = s_best;
+        if (best >= 0 && lane == best % SLAB) {
 #pragma unroll
-            for (int q = 0; q < WAVE_RW; q++) {
-                const int r = wv + NCH * q;
-                if (r < rows) dst[r] = a[q];
+            for (int q = 0; q < RW; q++) {
+                if (NCH * q + CPW <= iw + 1 || NCH * q >= rw) continue;
+                T x[CPW];
+#pragma unroll
+                for (int k = 0; k < CPW; k++) x[k] = a[k][q];
+                lds_store_group<T, CPW>(s_col + NCH * q + r0, x);
             }
         }
-        cluster_arrive(B.bar, sl.slab, i + 2);
+        __syncthreads();
+        if (best >= 0) {
+            GRelayWord* dst = relay_col<T>(B, nxt, sl.slab);
+            for (int r = i + 1 + tid; r < rows; r += NTH) relay_put_val<T>(dst + (size_t)r * (sizeof(T) / 4), tag + 1u, s_col[r]);
+        }
+        // the pivot column is done (never active again): its owner puts the reflector in its
+        // registers, off the path the other slabs wait on (s_v, s_m and sh hold until the
+        // next step's dlarfg)
+        if (own_pivot && lane == ca % SLAB && taud != 0.0) {  // beta on row i, v below
+#pragma unroll
+            for (int q = 0; q < RW; q++) {
+                if (NCH * q + CPW <= iw || NCH * q >= rw) continue;
+                T v[CPW], m[CPW];
+                lds_row_group<T, CPW>(s_v + NCH * q + r0, v);
+                lds_row_group<T, CPW>(s_m + NCH * q + r0, m);
+#pragma unroll
+                for (int k = 0; k < CPW; k++)
+                    if (m[k] != T(0)) a[k][q] = v[k];  // rows i + 1 .. rows - 1
+            }
+            if (iw >= 0 && iw / NCH < RW && iw % NCH < CPW) {
+                const int kq = iw % NCH, qq = iw / NCH;
+#pragma unroll
+                for (int q = 0; q < RW; q++)
+#pragma unroll
+                    for (int k = 0; k < CPW; k++)
+                        if (k == kq && q == qq) a[k][q] = (T)sh[1];
+            }
+        }
     }
     if (abandoned) return;  // the host reports the failed wait; the slab storage is left as init wrote it
     // registers back to the slab storage; final maps after `stop` steps, as k_rrqr_finish reads them
     if (jin) {
+        int rw = rows - r0;
+        asm volatile("" : "+s"(rw));
+        const gptr<T> wb = gp(reinterpret_cast<T*>(B.t.scratch)) + (size_t)sl.slab * rows * SLAB + lane;
 #pragma unroll
-        for (int q = 0; q < WAVE_RW; q++) {
-            const int r = wv + NCH * q;
-            if (r < rows) aslab[(size_t)r * SLAB] = a[q];
-        }
+        for (int q = 0; q < RW; q++)
+#pragma unroll
+            for (int k = 0; k < CPW; k++)
+                if (NCH * q + k < rw) wb[(size_t)(NCH * q + r0 + k) * SLAB] = a[k][q];
     }
     if (sl.slab == 0 && rows > 0) {
         const int fin = stop & 1;
-        for (int k = tid; k < cols; k += STEP_NT) {
-            B.map[fin * mc + k] = p2c[k];
-            B.map[2 * mc + fin * mc + p2c[k]] = k;
+        const gptr<int> map = gp(B.map);
+        for (int k = tid; k < cols; k += NTH) {
+            map[fin * mc + k] = p2c[k];
+            map[2 * mc + fin * mc + p2c[k]] = k;
         }
         if (tid == 0) {
-            B.dims[3] = stop;
-            B.dims[4] = 1;
+            gp(B.dims)[3] = stop;
+            gp(B.dims)[4] = 1;  // the reflectors are in the pivot columns (k_rrqr_finish copies them)
         }
     }
 }
@@ -2546,37 +2733,53 @@ hipError_t launch_rrqr_step(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
     return hipGetLastError();
 }
 
-// Persistent wave launch: returns hipErrorCooperativeLaunchTooLarge (and launches nothing)
-// when the wave's workgroups cannot all be resident at once.
-template <typename T, int RW>
+// Persistent wave launch: rows per thread and wavefronts per slab by the wave's largest
+// cluster (16 / NW * RW values of T per thread, at most 128 registers' worth).
+template <typename T, int RW, int NW>
 static hipError_t wave_capacity_rw(size_t lds_bytes, int* cap) {
     int dev = 0, ncu = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW>);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, STEP_NT, lds_bytes);
+    const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW, NW>);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, SLAB * NW, lds_bytes);
     *cap = e == hipSuccess ? per_cu * ncu : 0;
     return e;
+}
+
+template <typename T, int RW, int NW>
+static hipError_t launch_wave_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol,
+                                 size_t lds_bytes, int* fail, hipStream_t st) {
+    int cap = 0;
+    hipError_t e = wave_capacity_rw<T, RW, NW>(lds_bytes, &cap);
+    if (e != hipSuccess) return e;
+    if (cap < nslabs) return hipErrorCooperativeLaunchTooLarge;
+    const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW, NW>);
+    void* args[] = {(void*)&d_bigs, (void*)&d_slabs, (void*)&nsteps, (void*)&tol, (void*)&fail};
+    return hipLaunchCooperativeKernel(f, dim3(nslabs), dim3(SLAB * NW), args, (unsigned)lds_bytes, st);
+}
+
+// One dispatch table for the capacity query and the launch: the variant for max_rows.
+template <typename T, bool LAUNCH>
+static hipError_t wave_variant(int max_rows, size_t lds_bytes, int* cap, const RrqrBig* d_bigs,
+                               const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol, int* fail, hipStream_t st) {
+#define SPAND_WAVE(RW, NW)                                                                                   \
+    return LAUNCH ? launch_wave_rw<T, RW, NW>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, fail, st) \
+                  : wave_capacity_rw<T, RW, NW>(lds_bytes, cap)
+    if (max_rows <= NCH * 4) SPAND_WAVE(4, 4);
+    if (max_rows <= NCH * 8) SPAND_WAVE(8, 4);
+    if (max_rows <= NCH * 16) SPAND_WAVE(16, 4);
+    if (max_rows <= NCH * 32) {
+        if constexpr (sizeof(T) == 4) SPAND_WAVE(32, 4);
+        else SPAND_WAVE(32, 8);
+    }
+#undef SPAND_WAVE
+    return LAUNCH ? hipErrorCooperativeLaunchTooLarge : hipSuccess;
 }
 
 template <typename T>
 hipError_t rrqr_wave_capacity(int max_rows, size_t lds_bytes, int* cap) {
     *cap = 0;
-    if (max_rows <= 16 * 16) return wave_capacity_rw<T, 16>(lds_bytes, cap);
-    if (max_rows <= 16 * 32) return wave_capacity_rw<T, 32>(lds_bytes, cap);
-    return hipSuccess;
-}
-
-template <typename T, int RW>
-static hipError_t launch_wave_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol,
-                                 size_t lds_bytes, int* fail, hipStream_t st) {
-    int cap = 0;
-    hipError_t e = wave_capacity_rw<T, RW>(lds_bytes, &cap);
-    if (e != hipSuccess) return e;
-    if (cap < nslabs) return hipErrorCooperativeLaunchTooLarge;
-    const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW>);
-    void* args[] = {(void*)&d_bigs, (void*)&d_slabs, (void*)&nsteps, (void*)&tol, (void*)&fail};
-    return hipLaunchCooperativeKernel(f, dim3(nslabs), dim3(STEP_NT), args, (unsigned)lds_bytes, st);
+    return wave_variant<T, false>(max_rows, lds_bytes, cap, nullptr, nullptr, 0, 0, 0.0, nullptr, nullptr);
 }
 
 // Persistent launch over nslabs slabs (whole clusters): returns hipErrorCooperativeLaunchTooLarge
@@ -2585,9 +2788,7 @@ template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
                             double tol, size_t lds_bytes, int* fail, hipStream_t st) {
     if (nslabs == 0 || nsteps == 0) return hipSuccess;
-    if (max_rows <= 16 * 16) return launch_wave_rw<T, 16>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, fail, st);
-    if (max_rows <= 16 * 32) return launch_wave_rw<T, 32>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, fail, st);
-    return hipErrorCooperativeLaunchTooLarge;
+    return wave_variant<T, true>(max_rows, lds_bytes, nullptr, d_bigs, d_slabs, nslabs, nsteps, tol, fail, st);
 }
 
 template <typename T>

@@ -99,13 +99,13 @@ struct RrqrBig {
     int* dims;       // rows, cols, min(rows, cols), steps to run (the rank once it is known),
                      // 1 once every kept reflector is stored
     void* vbuf;      // nullptr, or [nslab][2][max_rows] pivot column / reflector buffers
-                     // (per-step launches), or [2][nslab][max_rows] candidate columns
-                     // (persistent wave launch)
+                     // (per-step launches), or the relay words of the persistent launch
+                     // (relay = 1): candidates [2][nslab][4], then candidate columns
+                     // [2][nslab][max_rows * sizeof(T) / 4], 64 bits each
     long long* done; // host-visible flag, or nullptr: set to 1 by the last step that does
                      // work (rank reached or all min(rows, cols) steps taken)
-    int* bar;        // persistent launch: steps finished by each slab (set to 0 by init)
     int nslab;
-    int pad;
+    int relay;       // 1: vbuf holds relay words (k_rrqr_init clears them)
 };
 struct RrqrSlab {
     int big;   // index into the RrqrBig array

@@ -1653,7 +1653,6 @@ void Tree::plan_sparsification(LevelPlan<T>& L) {
                 b.map = reinterpret_cast<int*>(arena_.alloc((size_t)4 * std::max(cols, 1) * sizeof(int)));
                 b.cand_v = reinterpret_cast<double*>(arena_.alloc((size_t)(2 * b.nslab + 1) * sizeof(double)));
                 b.cand_p = reinterpret_cast<int*>(arena_.alloc((size_t)2 * b.nslab * sizeof(int)));
-                b.bar = reinterpret_cast<int*>(arena_.alloc((size_t)b.nslab * sizeof(int)));  // per-slab step counts
                 b.t = t;
                 where[p] = {1, (int)wbig[w].size()};
                 for (int k = 0; k < b.nslab; k++) wslab[w].push_back({(int)wbig[w].size(), k});
@@ -1692,11 +1691,10 @@ void Tree::plan_sparsification(LevelPlan<T>& L) {
     // for larger clusters and for waves the device cannot hold at once.
     wlds_step.assign(nwaves, 0);
     wlds_wave.assign(nwaves, 0);
-    // Opt-in (SPAND_RRQR_PERSISTENT=1): the persistent launch with per-cluster barriers
-    // measured slower on gfx950 than one launch per step (64^3 FP32 factor + PCG: 177 ms
-    // against 125 ms per step), although a flag handshake between two workgroups costs
-    // about 1 us one way: the step body itself is slower in the 1024-thread persistent form.
-    persistent = env_or("SPAND_RRQR_PERSISTENT", 0) != 0;
+    // The persistent launch is the default (SPAND_RRQR_PERSISTENT=0 returns to one launch
+    // per step): on the 64^3 Laplacian its steps take 6-12 us against about 10.5 us per
+    // step launch, and FP64 and FP32 both gain.
+    persistent = env_or("SPAND_RRQR_PERSISTENT", 1) != 0;
     step_waves = tune.step_waves;  // 4, 8, 16; 0: by slab count
     tunes_[ilvl_] = tune;
     shapes_[ilvl_] = shape;
@@ -1704,12 +1702,14 @@ void Tree::plan_sparsification(LevelPlan<T>& L) {
         const size_t vb = (size_t)std::max(wmaxrows[w], 1) * sizeof(T);
         int wmaxcols = 0;
         for (auto& b : wbig[w]) wmaxcols = std::max(wmaxcols, b.t.max_cols);
-        const size_t lw = ((size_t)wmaxcols * sizeof(int) + 15) / 16 * 16 + 2 * vb;
+        const size_t lw = ((size_t)wmaxcols * sizeof(int) + 15) / 16 * 16;  // the persistent launch's p2c
         if (persistent && !wbig[w].empty() && wmaxrows[w] <= WAVE_ROWS_MAX && lw <= (size_t)WAVE_LDS_MAX) {
             wlds_wave[w] = lw;
             wlds_step[w] = 2 * vb;  // the per-step fallback (a wave the device cannot hold at once)
-            for (auto& b : wbig[w])
-                b.vbuf = arena_.alloc((size_t)b.nslab * 2 * std::max(b.t.max_rows, 1) * sizeof(T));
+            for (auto& b : wbig[w]) {
+                b.vbuf = arena_.alloc(rrqr_relay_bytes(sizeof(T), b.nslab, std::max(b.t.max_rows, 1)));
+                b.relay = 1;
+            }
         } else if (2 * vb <= (size_t)STEP_LDS_MAX) {
             wlds_step[w] = 2 * vb;
         } else {
