@@ -985,16 +985,35 @@ __device__ inline T larfg_scale(T xv, int knt, double scal) {
     return xv * (T)scal;
 }
 
-// (value desc, position asc) maximum over the 64 lanes of a wavefront; every lane gets it
-__device__ inline void wave_best(double& v, int& p) {
-    for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(v, off);
-        const int p2 = __shfl_xor(p, off);
-        if (v2 > v || (v2 == v && p2 < p)) {
-            v = v2;
-            p = p2;
-        }
+// One combining step of wave_best: the pair of the lane that the DPP control CTRL names
+// (lanes of rows outside ROWMASK combine with themselves).
+template <int CTRL, int ROWMASK>
+__device__ inline void best_step(double& v, int& p) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xF, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xF, false);
+    const int p2 = __builtin_amdgcn_update_dpp(p, p, CTRL, ROWMASK, 0xF, false);
+    const double v2 = __hiloint2double(hi2, lo2);
+    if (v2 > v || (v2 == v && p2 < p)) {
+        v = v2;
+        p = p2;
     }
+}
+// (value desc, position asc) maximum over the 64 lanes of a wavefront (all active); every
+// lane gets it. The order is total, so the reduction tree does not change the result: DPP
+// moves within quads (xor 1, xor 2), half-rows and rows (mirrors), then row 0 -> 1, 2 -> 3
+// (row_bcast15) and row 1 -> 2, 3 (row_bcast31) leave the maximum in lane 63.
+__device__ inline void wave_best(double& v, int& p) {
+    best_step<0xB1, 0xF>(v, p);   // quad_perm [1, 0, 3, 2]
+    best_step<0x4E, 0xF>(v, p);   // quad_perm [2, 3, 0, 1]
+    best_step<0x141, 0xF>(v, p);  // row_half_mirror
+    best_step<0x140, 0xF>(v, p);  // row_mirror
+    best_step<0x142, 0xA>(v, p);  // row_bcast15
+    best_step<0x143, 0xC>(v, p);  // row_bcast31
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    v = __hiloint2double(hi, lo);
+    p = __builtin_amdgcn_readlane(p, 63);
 }
 
 // choose_rank (src/util.cpp:434-452) decided during the factorization: -1 while undecided.
