@@ -884,7 +884,15 @@ __device__ inline double sumsq16_block(int n, G g, double* s_ch) {
 #pragma clang fp contract(off)
     if (threadIdx.x < NCH) {
         double c = 0.0;
-        for (int t = threadIdx.x; t < n; t += NCH) {
+        int t = threadIdx.x;
+        for (; t + 3 * NCH < n; t += 4 * NCH) {  // four loads in flight, the sum in order
+            const double v0 = g(t), v1 = g(t + NCH), v2 = g(t + 2 * NCH), v3 = g(t + 3 * NCH);
+            c = fma(v0, v0, c);
+            c = fma(v1, v1, c);
+            c = fma(v2, v2, c);
+            c = fma(v3, v3, c);
+        }
+        for (; t < n; t += NCH) {
             const double v = g(t);
             c = fma(v, v, c);
         }
@@ -2045,17 +2053,17 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
     }
 }
 
-// CPW consecutive values of T from 16-byte aligned LDS (one or two 16-byte reads)
+// CPW consecutive values of T from LDS aligned to their size (8 or 16-byte reads)
 template <typename T, int CPW>
 __device__ inline void lds_row_group(const T* p, T (&out)[CPW]) {
-    static_assert(CPW * sizeof(T) % 16 == 0, "whole 16-byte reads");
-    __builtin_memcpy(out, __builtin_assume_aligned(p, 16), CPW * sizeof(T));
+    static_assert(CPW * sizeof(T) % 8 == 0, "whole 8-byte reads");
+    __builtin_memcpy(out, __builtin_assume_aligned(p, CPW * sizeof(T) >= 16 ? 16 : 8), CPW * sizeof(T));
 }
 
 template <typename T, int CPW>
 __device__ inline void lds_store_group(T* p, const T (&in)[CPW]) {
-    static_assert(CPW * sizeof(T) % 16 == 0, "whole 16-byte writes");
-    __builtin_memcpy(__builtin_assume_aligned(p, 16), in, CPW * sizeof(T));
+    static_assert(CPW * sizeof(T) % 8 == 0, "whole 8-byte writes");
+    __builtin_memcpy(__builtin_assume_aligned(p, CPW * sizeof(T) >= 16 ? 16 : 8), in, CPW * sizeof(T));
 }
 
 // Persistent form of the stepped RRQR: one cooperative launch runs every pivot step of the
@@ -2787,10 +2795,7 @@ static hipError_t wave_variant(int max_rows, size_t lds_bytes, int* cap, const R
     if (max_rows <= NCH * 4) SPAND_WAVE(4, 4);
     if (max_rows <= NCH * 8) SPAND_WAVE(8, 4);
     if (max_rows <= NCH * 16) SPAND_WAVE(16, 4);
-    if (max_rows <= NCH * 32) {
-        if constexpr (sizeof(T) == 4) SPAND_WAVE(32, 4);
-        else SPAND_WAVE(32, 8);
-    }
+    if (max_rows <= NCH * 32) SPAND_WAVE(32, 8);
 #undef SPAND_WAVE
     return LAUNCH ? hipErrorCooperativeLaunchTooLarge : hipSuccess;
 }
