@@ -54,12 +54,12 @@ FLOP_BOUND = {"potrf", "trsm", "schur"}
 # Keyed by (family, factor dtype, grid edge n, levels) and valid only for the kernel source
 # it was measured on (SHA-256 prefix of csrc/kernels_factor.hip): after any change to that
 # file the run reports traffic null until the counters are measured again
-# (profiles/rrqr_step_laplace64_hbm_traffic.md gives the commands).
+# (profiles/rrqr_wave_laplace64_hbm_traffic.md gives the commands).
 TRAFFIC_MEASURED = {
-    ("rrqr_step", "f32", 64, 12): {"fetch_kib": 3162.20, "write_kib": 4399.45, "fetch_factor": 2.0,
-                                   "write_factor": 1.0, "src_sha16": "bd12eb8f1dca482e"},
-    ("rrqr_step", "f64", 64, 12): {"fetch_kib": 5865.11, "write_kib": 9149.99, "fetch_factor": 2.0,
-                                   "write_factor": 1.0, "src_sha16": "bd12eb8f1dca482e"},
+    ("rrqr_wave", "f32", 64, 12): {"fetch_kib": 16107.38, "write_kib": 29219.78, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "9225b05f92bb5501"},
+    ("rrqr_wave", "f64", 64, 12): {"fetch_kib": 22101.30, "write_kib": 67373.63, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "9225b05f92bb5501"},
 }
 
 
