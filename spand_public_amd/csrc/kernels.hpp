@@ -64,10 +64,11 @@ hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
                             const int* d_size, double tol, hipStream_t st);
 // All steps of a set of whole clusters in one cooperative launch (clusters of at most
 // WAVE_ROWS_MAX rows, RrqrBig::vbuf holding rrqr_relay_bytes of relay words, relay = 1);
-// lds_bytes = max_cols ints (16-byte aligned) + 2 max_rows elements. The slabs of each
-// cluster exchange their candidates through the relay words (bounded waits: *fail is set
-// to 1 if one gives up). Returns hipErrorCooperativeLaunchTooLarge, launching nothing, when
-// the slabs do not fit the device at once; rrqr_wave_capacity gives how many do.
+// lds_bytes = max_cols ints (the position map; the rest of the kernel's LDS is static).
+// The slabs of each cluster exchange their candidates through the relay words (bounded
+// waits: *fail is set to 1 if one gives up). Returns hipErrorCooperativeLaunchTooLarge,
+// launching nothing, when the slabs do not fit the device at once; rrqr_wave_capacity
+// gives how many do.
 constexpr int WAVE_ROWS_MAX = 512;
 constexpr int WAVE_LDS_MAX = 49152;
 inline size_t rrqr_relay_bytes(size_t esz, int nslab, int max_rows) {
