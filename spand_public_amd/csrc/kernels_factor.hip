@@ -216,48 +216,67 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
     if (bad && tid == 0) atomicCAS(info, 0, t.cluster + 1);
 }
 
-// Pivots of at most 64 rows: one wavefront each, the whole block in LDS, lane a owning
-// row a. Same element recurrence as k_potrf (column j scaled by 1/L(j,j), then
+// Pivots of at most 64 rows: one wavefront each, lane a owning row a (in registers). Same
+// element recurrence as k_potrf (column j scaled by 1/L(j,j), then
 // A(a,b) = fma(-L(a,j), L(b,j), A(a,b)) for a >= b > j, in j order).
 template <typename T>
 __global__ void __launch_bounds__(64) k_potrf64(const PotrfTask* tasks, int* info) {
 #pragma clang fp contract(off)
-    __shared__ T L[64][65];
+    // r[b] = A[a][b]; step j is unrolled, so every register index is a constant. The
+    // diagonal's lane takes the square root, lane a > j scales its L[a][j] and publishes it
+    // in LDS, and every lane subtracts L[a][j] L[b][j] from its r[b], b > j (entries above a
+    // lane's diagonal are updated too, never read, and not stored).
+    __shared__ T col[64];
     __shared__ T rd;
     __shared__ int bad;
     const PotrfTask t = tasks[blockIdx.x];
     const int n = t.n, a = threadIdx.x;
     if (n == 0) return;
     const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
+    T r[64];
+#pragma unroll
+    for (int b = 0; b < 64; b++) r[b] = (a < n && b < n) ? A[a + (size_t)b * n] : T(0);
     if (a == 0) bad = 0;
-    for (int b = 0; b < n; b++) L[a][b] = a < n ? A[a + (size_t)b * n] : T(0);
     __syncthreads();
-    for (int j = 0; j < n; j++) {
-        if (a == 0) {
-            const T d = L[j][j];
-            if (!(d > T(0))) {
-                bad = j + 1;
-            } else {
-                L[j][j] = sqrt(d);
+    bool stopped = false;  // uniform: a failed pivot or the last column
+#pragma clang loop unroll(full)
+    for (int j = 0; j < 64; j++) {
+        if (stopped || j >= n) {
+            stopped = true;
+        } else {
+            if (a == j) {
+                const T d = r[j];
+                if (!(d > T(0))) {
+                    bad = j + 1;
+                } else {
+                    r[j] = sqrt(d);
+                }
+                rd = T(1) / r[j];
             }
-            rd = T(1) / L[j][j];
+            __syncthreads();
+            if (bad) {
+                stopped = true;
+            } else {
+                if (a > j) r[j] *= rd;
+                col[a] = r[j];
+                __syncthreads();
+                const T laj = r[j];
+#pragma unroll
+                for (int b = j + 1; b < 64; b++)
+                    if (b < n) r[b] = fma(-laj, col[b], r[b]);  // n is uniform: a scalar test
+                __syncthreads();  // col is rewritten in the next step
+            }
         }
-        __syncthreads();
-        if (bad) break;
-        if (a > j && a < n) L[a][j] *= rd;
-        __syncthreads();
-        if (a > j && a < n) {
-            const T laj = L[a][j];
-            for (int b = j + 1; b <= a; b++) L[a][b] = fma(-laj, L[b][j], L[a][b]);
-        }
-        __syncthreads();
     }
     if (bad) {
         if (a == 0) atomicCAS(info, 0, t.cluster + 1);
         return;
     }
-    if (a < n)
-        for (int b = 0; b <= a; b++) A[a + (size_t)b * n] = L[a][b];
+    if (a < n) {
+#pragma unroll
+        for (int b = 0; b < 64; b++)
+            if (b <= a && b < n) A[a + (size_t)b * n] = r[b];
+    }
 }
 
 // Large pivots take the same blocked algorithm one panel per launch pair, so the work of
