@@ -65,18 +65,19 @@ hipError_t launch_rrqr_init(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int 
 // All steps of a set of whole clusters in one cooperative launch (clusters of at most
 // WAVE_ROWS_MAX rows, RrqrBig::vbuf holding rrqr_relay_bytes of relay words, relay = 1);
 // lds_bytes = max_cols ints (the position map; the rest of the kernel's LDS is static).
-// The slabs of each cluster exchange their candidates through the relay words (bounded
-// waits: *fail is set to 1 if one gives up). Returns hipErrorCooperativeLaunchTooLarge,
-// launching nothing, when the slabs do not fit the device at once; rrqr_wave_capacity
-// gives how many do.
+// The slabs of each cluster exchange their candidates through the relay words (waits of at
+// most spin_max polls, RRQR_SPIN_MAX by default: *fail is set to 1 if one gives up).
+// Returns hipErrorCooperativeLaunchTooLarge, launching nothing, when the slabs do not fit
+// the device at once; rrqr_wave_capacity gives how many do.
 constexpr int WAVE_ROWS_MAX = 512;
+constexpr long long RRQR_SPIN_MAX = 1ll << 21;
 constexpr int WAVE_LDS_MAX = 49152;
 inline size_t rrqr_relay_bytes(size_t esz, int nslab, int max_rows) {
     return ((size_t)2 * nslab * 4 + (size_t)2 * nslab * max_rows * (esz / 4)) * 8;
 }
 template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
-                            double tol, size_t lds_bytes, int* fail, hipStream_t st);
+                            double tol, size_t lds_bytes, int* fail, long long spin_max, hipStream_t st);
 template <typename T>
 hipError_t rrqr_wave_capacity(int max_rows, size_t lds_bytes, int* cap);
 // One pivot step per launch. waves: wavefronts per slab workgroup (4, 8 or 16; 0 picks by

@@ -1639,7 +1639,6 @@ __device__ inline size_t slab_idx(int rows, int r, int j) {
 // own flag. Tags are step + 1; k_rrqr_init clears the words to tag 0 before the launch.
 using RelayWord = unsigned long long;
 typedef __attribute__((address_space(1))) RelayWord GRelayWord;
-constexpr long long RELAY_SPIN_MAX = 1ll << 21;
 
 __device__ inline void relay_put(GRelayWord* p, unsigned tag, unsigned v) {
     __hip_atomic_store(p, ((RelayWord)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1656,12 +1655,13 @@ __device__ inline void relay_put_val(GRelayWord* p, unsigned tag, T x) {
     }
 }
 // Waits until the first n of the words p[0..N) all carry `tag` (n loads in flight per poll,
-// one round trip when the data is there); payloads in v. False after about 2^21 polls or
-// once *fail is set (the launch is cooperative, so every slab is resident and a correct
-// schedule never comes near the bound; the host turns the flag into an error).
+// one round trip when the data is there); payloads in v. False after spin_max polls
+// (RRQR_SPIN_MAX unless the caller lowers it) or once *fail is set: the launch is
+// cooperative, so every slab is resident and a correct schedule never comes near the
+// bound; the host turns the flag into an error.
 template <int N>
 __device__ inline bool relay_get_all(const GRelayWord* const (&p)[N], int n, unsigned tag, unsigned (&v)[N],
-                                     int* fail) {
+                                     int* fail, long long spin_max) {
     for (long long spins = 0;; spins++) {
         RelayWord w[N];
 #pragma unroll
@@ -1675,7 +1675,7 @@ __device__ inline bool relay_get_all(const GRelayWord* const (&p)[N], int n, uns
             for (int k = 0; k < N; k++) v[k] = (unsigned)w[k];
             return true;
         }
-        if (spins > RELAY_SPIN_MAX ||
+        if (spins >= spin_max ||
             ((spins & 63) == 63 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
             __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
@@ -2112,7 +2112,7 @@ constexpr int wave_occupancy() {
 }
 template <typename T, int RW, int NW>
 __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
-    k_rrqr_wave(const RrqrBig* bigs, const RrqrSlab* slabs, int nsteps, double tol, int* fail) {
+    k_rrqr_wave(const RrqrBig* bigs, const RrqrSlab* slabs, int nsteps, double tol, int* fail, long long spin_max) {
 #pragma clang fp contract(off)
     static_assert(NCH % NW == 0, "chains split evenly over the wavefronts");
     constexpr int NTH = SLAB * NW, CPW = NCH / NW, RMAX = NCH * RW;
@@ -2199,7 +2199,7 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
                 const GRelayWord* c = relay_cand<T>(B, cur, k);
                 const GRelayWord* const pc[3] = {c, c + 1, c + 2};
                 unsigned u[3];
-                ok = relay_get_all<3>(pc, 3, tag, u, fail);
+                ok = relay_get_all<3>(pc, 3, tag, u, fail, spin_max);
                 if (ok) {
                     const double v = __longlong_as_double((long long)(((unsigned long long)u[1] << 32) | u[0]));
                     const int pp = (int)u[2];
@@ -2240,7 +2240,7 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
 #pragma unroll
                 for (int k = 0; k < NWD; k++) pw[k] = src + (size_t)(k < WD ? r1 : r2) * WD + k % WD;
                 unsigned u[NWD];
-                if (relay_get_all<NWD>(pw, nr * WD, tag, u, fail)) {
+                if (relay_get_all<NWD>(pw, nr * WD, tag, u, fail, spin_max)) {
                     T x[2];
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
@@ -2439,7 +2439,13 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
             }
         }
     }
-    if (abandoned) return;  // the host reports the failed wait; the slab storage is left as init wrote it
+    if (abandoned) {
+        // the host reports the failed wait. The cluster's maps and slab storage are not the
+        // QR's (this slab's registers are dropped; its cluster's other slabs abandon too, or
+        // finish without it): rows = 0 makes k_rrqr_finish leave the cluster alone
+        if (tid == 0) gp(B.dims)[0] = 0;
+        return;
+    }
     // registers back to the slab storage; final maps after `stop` steps, as k_rrqr_finish reads them
     if (jin) {
         int rw = rows - r0;
@@ -2794,22 +2800,23 @@ static hipError_t wave_capacity_rw(size_t lds_bytes, int* cap) {
 
 template <typename T, int RW, int NW>
 static hipError_t launch_wave_rw(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol,
-                                 size_t lds_bytes, int* fail, hipStream_t st) {
+                                 size_t lds_bytes, int* fail, long long spin_max, hipStream_t st) {
     int cap = 0;
     hipError_t e = wave_capacity_rw<T, RW, NW>(lds_bytes, &cap);
     if (e != hipSuccess) return e;
     if (cap < nslabs) return hipErrorCooperativeLaunchTooLarge;
     const void* f = reinterpret_cast<const void*>(k_rrqr_wave<T, RW, NW>);
-    void* args[] = {(void*)&d_bigs, (void*)&d_slabs, (void*)&nsteps, (void*)&tol, (void*)&fail};
+    void* args[] = {(void*)&d_bigs, (void*)&d_slabs, (void*)&nsteps, (void*)&tol, (void*)&fail, (void*)&spin_max};
     return hipLaunchCooperativeKernel(f, dim3(nslabs), dim3(SLAB * NW), args, (unsigned)lds_bytes, st);
 }
 
 // One dispatch table for the capacity query and the launch: the variant for max_rows.
 template <typename T, bool LAUNCH>
 static hipError_t wave_variant(int max_rows, size_t lds_bytes, int* cap, const RrqrBig* d_bigs,
-                               const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol, int* fail, hipStream_t st) {
-#define SPAND_WAVE(RW, NW)                                                                                   \
-    return LAUNCH ? launch_wave_rw<T, RW, NW>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, fail, st) \
+                               const RrqrSlab* d_slabs, int nslabs, int nsteps, double tol, int* fail,
+                               long long spin_max, hipStream_t st) {
+#define SPAND_WAVE(RW, NW)                                                                                             \
+    return LAUNCH ? launch_wave_rw<T, RW, NW>(d_bigs, d_slabs, nslabs, nsteps, tol, lds_bytes, fail, spin_max, st) \
                   : wave_capacity_rw<T, RW, NW>(lds_bytes, cap)
     if (max_rows <= NCH * 4) SPAND_WAVE(4, 4);
     if (max_rows <= NCH * 8) SPAND_WAVE(8, 4);
@@ -2822,16 +2829,18 @@ static hipError_t wave_variant(int max_rows, size_t lds_bytes, int* cap, const R
 template <typename T>
 hipError_t rrqr_wave_capacity(int max_rows, size_t lds_bytes, int* cap) {
     *cap = 0;
-    return wave_variant<T, false>(max_rows, lds_bytes, cap, nullptr, nullptr, 0, 0, 0.0, nullptr, nullptr);
+    return wave_variant<T, false>(max_rows, lds_bytes, cap, nullptr, nullptr, 0, 0, 0.0, nullptr, 0, nullptr);
 }
 
 // Persistent launch over nslabs slabs (whole clusters): returns hipErrorCooperativeLaunchTooLarge
 // (and launches nothing) when they cannot all be resident at once.
 template <typename T>
 hipError_t launch_rrqr_wave(const RrqrBig* d_bigs, const RrqrSlab* d_slabs, int nslabs, int nsteps, int max_rows,
-                            double tol, size_t lds_bytes, int* fail, hipStream_t st) {
+                            double tol, size_t lds_bytes, int* fail, long long spin_max, hipStream_t st) {
     if (nslabs == 0 || nsteps == 0) return hipSuccess;
-    return wave_variant<T, true>(max_rows, lds_bytes, nullptr, d_bigs, d_slabs, nslabs, nsteps, tol, fail, st);
+    if (spin_max < 1) return hipErrorInvalidValue;
+    return wave_variant<T, true>(max_rows, lds_bytes, nullptr, d_bigs, d_slabs, nslabs, nsteps, tol, fail, spin_max,
+                                 st);
 }
 
 template <typename T>
@@ -2865,7 +2874,7 @@ hipError_t launch_copy(const CopyTask* d_tasks, int ntasks, hipStream_t st) {
     template hipError_t launch_rrqr_init<T>(const RrqrBig*, const RrqrSlab*, int, const RrqrSeg*, const int*,      \
                                             double, hipStream_t);                                                  \
     template hipError_t launch_rrqr_wave<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t, int*, \
-                                            hipStream_t);                                                          \
+                                            long long, hipStream_t);                                               \
     template hipError_t rrqr_wave_capacity<T>(int, size_t, int*);                                                  \
     template hipError_t launch_rrqr_step<T>(const RrqrBig*, const RrqrSlab*, int, int, int, double, size_t, int,  \
                                             long long*, long long, hipStream_t);                                                       \

@@ -930,6 +930,7 @@ struct FactorRun {
     hipStream_t st = 0;  // the library's calls are ordered on the null stream
     bool gen = false;    // GEN / PLU tree
     long long lds_elems = 0, big_min_rows = 0;  // RRQR path limits
+    long long spin_max = RRQR_SPIN_MAX;         // polls of a persistent RRQR wait before it gives up
     std::vector<int> hsize;                     // cluster sizes read back from the device
     std::vector<std::future<SolvePlanHost>> pending_solve;  // per level; joined on every exit
 };
@@ -1904,7 +1905,7 @@ void Tree::launch_level(LevelPlan<T>& L) {
                         x1++;
                     }
                     const hipError_t e = launch_rrqr_wave<T>(db, ds + s0, (int)(s1 - s0), steps, wmaxrows[w], tol_,
-                                                             wlds_wave[w], d_info_ + 1, st);
+                                                             wlds_wave[w], d_info_ + 1, R.spin_max, st);
                     hip_check(e, "rrqr wave");
                     launches++;
                     x0 = x1;
@@ -1953,7 +1954,7 @@ void Tree::launch_level(LevelPlan<T>& L) {
     int info2[2] = {0, 0};
     hip_check(hipMemcpy(info2, d_info_, 2 * sizeof(int), hipMemcpyDeviceToHost), "read info");
     const int info = info2[0];
-    if (info2[1] != 0) throw Error(ERR_HIP, "persistent RRQR: a cluster barrier wait gave up");
+    if (info2[1] != 0) throw Error(ERR_HIP, "persistent RRQR: a wait for another slab's candidate gave up");
     if (info != 0) {
         if (gen) throw Error(ERR_SINGULAR, "Error: Singular Pivot (cluster order " + std::to_string(info - 1) + ")");
         throw Error(ERR_NOT_SPD, "Error: Non-SPD Pivot (cluster order " + std::to_string(info - 1) + ")");
@@ -2297,6 +2298,9 @@ void Tree::factorize_impl() {
     // its largest resident task
     R.lds_elems = RRQR_LDS_MAX / (int)sizeof(T);
     R.big_min_rows = env_or("SPAND_RRQR_STEPPED_MIN_ROWS", 16);
+    // SPAND_RRQR_SPIN_MAX lowers the bound on the persistent RRQR's waits (the give-up path's
+    // test runs it at 1 poll: the waits fail, the launch ends, factorize reports the error)
+    R.spin_max = std::max(1LL, env_or("SPAND_RRQR_SPIN_MAX", RRQR_SPIN_MAX));
     tunes_.assign(nlevels_, LevelTune());
     flops_log_.clear();
     shapes_.assign(nlevels_, LevelShape());

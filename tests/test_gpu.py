@@ -204,6 +204,38 @@ def test_not_spd_reports_status():
         g.factorize()
 
 
+def test_persistent_rrqr_wait_give_up_is_reported(monkeypatch):
+    """The persistent RRQR's waits are bounded. With the bound lowered to one poll
+    (SPAND_RRQR_SPIN_MAX, read at factorize) a wait whose data has not arrived gives up, the
+    other slabs of the launch follow through the shared flag, every workgroup returns, and
+    factorize raises instead of returning a half-factorized tree (a launch whose every wait
+    happens to find its data at the first poll completes normally, which is accepted too).
+    The same tree then factorizes with the default bound, equal to the oracle."""
+    say("case persistent rrqr give-up")
+    monkeypatch.setenv("SPAND_RRQR_REG", "0")
+    monkeypatch.setenv("SPAND_RRQR_PERSISTENT", "1")
+    monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ROWS", "0")
+    monkeypatch.setenv("SPAND_RRQR_STEPPED_MIN_ELEMS", "0")
+    A, X = neglapl(20, 3), linspace_nd(20, 3)
+    g = gpu_tree(A, X, 7, 1e-2)
+    monkeypatch.setenv("SPAND_RRQR_SPIN_MAX", "1")
+    try:
+        g.factorize()
+        gave_up = False
+    except S.SpandError as e:
+        assert "gave up" in str(e), str(e)
+        gave_up = True
+    say(f"  give-up reported: {gave_up}")
+    monkeypatch.delenv("SPAND_RRQR_SPIN_MAX")
+    g.factorize()
+    o, _ = oracle.setup_tree(A, X, 7, 1e-2)
+    o.factorize()
+    assert np.array_equal(o.ops(), g.ops())
+    b = np.random.default_rng(5).standard_normal(A.n)
+    xo, xg = o.solve(b), g.solve(b)
+    assert np.linalg.norm(xo - xg) <= 1e-12 * np.linalg.norm(xo)
+
+
 def test_kernel_profile_stepped_rrqr(monkeypatch):
     """With every cluster on the per-step RRQR path, the stepped families are timed and priced
     from the sizes and ranks the device decided (read once the factorization has finished)."""
