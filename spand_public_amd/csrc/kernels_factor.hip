@@ -2091,7 +2091,8 @@ __device__ inline void lds_store_group(T* p, const T (&in)[CPW]) {
 // its best column (candidate for pivot i + 1: partial norm, position, rows i + 1..) in relay
 // words tagged i + 2; at step i + 1 each slab waits for the candidates of its cluster,
 // takes the first maximum and reads the winner's rows, then recomputes the step's dlarfg
-// itself. One round trip through memory per step and no fences.
+// itself: two reads through memory per step (the candidates, then the winner's rows), one
+// publication, and no fences.
 // Wavefront wv of NW holds, for its lane's column, rows r = 16 q + CPW wv + k (k < CPW =
 // 16 / NW, q < RW) in registers for the whole factorization. Row r belongs to chain
 // (r - i) % 16 of the w_j sums at step i and to chain (r - i - 1) % 16 of the norm sums,
@@ -2103,8 +2104,8 @@ __device__ inline void lds_store_group(T* p, const T (&in)[CPW]) {
 // (beta, v) in its registers; the registers and the final maps go back to the slab storage
 // at the end, where k_rrqr_finish copies the reflectors into Q.
 // Wavefronts per SIMD the register budget targets: the cached values plus 96 registers
-// (a few loop-invariant values spill); 128 cached registers take the whole file (the
-// accumulation registers hold what does not fit, no scratch).
+// (a few loop-invariant values may spill); 128 cached registers take the whole file (the
+// accumulation registers hold what does not fit).
 template <typename T, int RW, int NW>
 constexpr int wave_occupancy() {
     constexpr int c = (NCH / NW) * RW * (int)(sizeof(T) / 4);
