@@ -1257,6 +1257,19 @@ __device__ inline int seg_find(const int* col0, int nseg, int j) {
     return lo;
 }
 
+// CPW consecutive values of T from LDS aligned to their size (8 or 16-byte reads)
+template <typename T, int CPW>
+__device__ inline void lds_row_group(const T* p, T (&out)[CPW]) {
+    static_assert(CPW * sizeof(T) % 8 == 0, "whole 8-byte reads");
+    __builtin_memcpy(out, __builtin_assume_aligned(p, CPW * sizeof(T) >= 16 ? 16 : 8), CPW * sizeof(T));
+}
+
+template <typename T, int CPW>
+__device__ inline void lds_store_group(T* p, const T (&in)[CPW]) {
+    static_assert(CPW * sizeof(T) % 8 == 0, "whole 8-byte writes");
+    __builtin_memcpy(__builtin_assume_aligned(p, CPW * sizeof(T) >= 16 ? 16 : 8), in, CPW * sizeof(T));
+}
+
 // (value desc, position asc) maximum over the 64 lanes of a wavefront, carrying a tag
 __device__ inline void wave_best_tag(double& v, int& p, int& tag) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -1353,7 +1366,9 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
     __shared__ double s_bv[NWV];
     __shared__ int s_bp[NWV], s_bc[NWV];
     __shared__ T s_v[RR];
-    __shared__ double s_tau, s_rii;
+    __shared__ __align__(16) T s_x[RR];  // the pivot column as it stands, written by its owner
+    __shared__ double s_tau, s_rii, s_scal;
+    __shared__ int s_knt;
     __shared__ T s_taus[RR];  // every step's tau, stored to the task once at the end
     const RrqrTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1462,13 +1477,25 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
 #pragma unroll
         for (int k = 0; k < K; k++)
             if (!own[k] && pos[k] == i) pos[k] = p;
-        // 2. the pivot column's thread: dlarfg of rows i .. rows - 1 (reference dlarfg)
+        // 2. the pivot column's thread: dlarfg of rows i .. rows - 1 (reference dlarfg). It
+        // publishes the column's rows (whole 16-row blocks in 16-byte writes) and the scalars;
+        // the reflector is formed in LDS by every thread after the barrier.
+        double beta_own = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             if (!own[k]) continue;
+#pragma unroll
+            for (int b = 0; b < RR / NCH; b++) {
+                if (b < b0 || NCH * b >= rows) continue;
+                T x[NCH];
+#pragma unroll
+                for (int m = 0; m < NCH; m++) x[m] = a[k][NCH * b + m];
+                lds_store_group<T, NCH>(s_x + NCH * b, x);
+            }
             const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
             const double rs = 1.0 / safmin;
-            double alpha = (double)reg_at<RR>(a[k], i);
+            const T alpha_t = s_x[i];  // written just above by this thread
+            double alpha = (double)alpha_t;
             const int n1 = rows - i - 1;
             double tau = 0.0, scal = 1.0, beta = alpha;
             int knt = 0;
@@ -1503,26 +1530,12 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
                     }
                 }
             }
-            // the reflector: v_i = 1, v_r = x_r * scal below; the column keeps beta and v
-            // (with tau = 0 it is left as it is, alpha on the diagonal)
-#pragma unroll
-            for (int b = 0; b < RR / NCH; b++) {
-                if (b < b0 || NCH * b >= rows) continue;
-#pragma unroll
-                for (int m = 0; m < NCH; m++) {
-                    const int r = NCH * b + m;
-                    if (b > b0 || m > m0) {
-                        if (tau != 0.0) a[k][r] = larfg_scale<T>(a[k][r], knt, scal);
-                        s_v[r] = a[k][r];
-                    } else if (m == m0) {
-                        if (tau != 0.0) a[k][r] = (T)beta;
-                        s_v[r] = T(1);
-                        s_rii = fabs((double)a[k][r]);
-                    }
-                }
-            }
+            // R_ii as stored: beta, or alpha where tau = 0 leaves the column as it is
+            s_rii = fabs((double)(tau != 0.0 ? (T)beta : alpha_t));
             s_tau = tau;
-            pos[k] = i;
+            s_scal = scal;
+            s_knt = knt;
+            beta_own = beta;
             s_taus[i] = (T)tau;
         }
         __syncthreads();
@@ -1532,6 +1545,35 @@ __global__ void __launch_bounds__(NTR) k_rrqr_reg(const RrqrTask* tasks, const R
         } else if (fixed < 0 && !(rii / r00 >= tol)) {  // |R_ii| / |R_00| < tol: rank i
             rank = i;
             break;
+        }
+        // the reflector v (v_i = 1, v_r = x_r scaled below; used only when tau != 0), one row
+        // per thread; rows past the last stay zero
+        if (taud != 0.0) {
+            const int knt = s_knt;
+            const double scal = s_scal;
+            for (int r = i + tid; r < rows; r += NTR) s_v[r] = r == i ? T(1) : larfg_scale<T>(s_x[r], knt, scal);
+        }
+        __syncthreads();
+        // the owner's column keeps beta and v (with tau = 0 it is left as it is): formed in
+        // its registers beside the other threads' updates (s_knt and s_scal are rewritten
+        // only after the next step's first barrier)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!own[k]) continue;
+            pos[k] = i;
+            if (taud == 0.0) continue;
+            const int knt = s_knt;
+            const double scal = s_scal;
+#pragma unroll
+            for (int b = 0; b < RR / NCH; b++) {
+                if (b < b0 || NCH * b >= rows) continue;
+#pragma unroll
+                for (int m = 0; m < NCH; m++) {
+                    const int r = NCH * b + m;
+                    if (b > b0 || m > m0) a[k][r] = larfg_scale<T>(a[k][r], knt, scal);
+                    else if (m == m0) a[k][r] = (T)beta_own;
+                }
+            }
         }
         const T tau = (T)taud;
         // 3. apply H(i) to the free columns (positions > i after the swap), downdate the norms
@@ -2070,19 +2112,6 @@ __global__ void __launch_bounds__(SLAB * NW) k_rrqr_step(const RrqrBig* bigs, co
             cand_p[nxt * ns + sl.slab] = bpos;
         }
     }
-}
-
-// CPW consecutive values of T from LDS aligned to their size (8 or 16-byte reads)
-template <typename T, int CPW>
-__device__ inline void lds_row_group(const T* p, T (&out)[CPW]) {
-    static_assert(CPW * sizeof(T) % 8 == 0, "whole 8-byte reads");
-    __builtin_memcpy(out, __builtin_assume_aligned(p, CPW * sizeof(T) >= 16 ? 16 : 8), CPW * sizeof(T));
-}
-
-template <typename T, int CPW>
-__device__ inline void lds_store_group(T* p, const T (&in)[CPW]) {
-    static_assert(CPW * sizeof(T) % 8 == 0, "whole 8-byte writes");
-    __builtin_memcpy(__builtin_assume_aligned(p, CPW * sizeof(T) >= 16 ? 16 : 8), in, CPW * sizeof(T));
 }
 
 // Persistent form of the stepped RRQR: one cooperative launch runs every pivot step of the
