@@ -1270,18 +1270,35 @@ __device__ inline void lds_store_group(T* p, const T (&in)[CPW]) {
     __builtin_memcpy(__builtin_assume_aligned(p, CPW * sizeof(T) >= 16 ? 16 : 8), in, CPW * sizeof(T));
 }
 
-// (value desc, position asc) maximum over the 64 lanes of a wavefront, carrying a tag
-__device__ inline void wave_best_tag(double& v, int& p, int& tag) {
-    for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(v, off);
-        const int p2 = __shfl_xor(p, off);
-        const int t2 = __shfl_xor(tag, off);
-        if (v2 > v || (v2 == v && p2 < p)) {
-            v = v2;
-            p = p2;
-            tag = t2;
-        }
+// (value desc, position asc) maximum over the 64 lanes of a wavefront, carrying a tag; the
+// DPP steps of wave_best (positions of real candidates are distinct, so the tag of the
+// maximum does not depend on the reduction tree)
+template <int CTRL, int ROWMASK>
+__device__ inline void best_step_tag(double& v, int& p, int& tag) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xF, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xF, false);
+    const int p2 = __builtin_amdgcn_update_dpp(p, p, CTRL, ROWMASK, 0xF, false);
+    const int t2 = __builtin_amdgcn_update_dpp(tag, tag, CTRL, ROWMASK, 0xF, false);
+    const double v2 = __hiloint2double(hi2, lo2);
+    if (v2 > v || (v2 == v && p2 < p)) {
+        v = v2;
+        p = p2;
+        tag = t2;
     }
+}
+__device__ inline void wave_best_tag(double& v, int& p, int& tag) {
+    best_step_tag<0xB1, 0xF>(v, p, tag);   // quad_perm [1, 0, 3, 2]
+    best_step_tag<0x4E, 0xF>(v, p, tag);   // quad_perm [2, 3, 0, 1]
+    best_step_tag<0x141, 0xF>(v, p, tag);  // row_half_mirror
+    best_step_tag<0x140, 0xF>(v, p, tag);  // row_mirror
+    best_step_tag<0x142, 0xA>(v, p, tag);  // row_bcast15
+    best_step_tag<0x143, 0xC>(v, p, tag);  // row_bcast31
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    v = __hiloint2double(hi, lo);
+    p = __builtin_amdgcn_readlane(p, 63);
+    tag = __builtin_amdgcn_readlane(tag, 63);
 }
 
 // tree16 of the chains rotated by S (chain k = e[(k + S) mod 16]), without moving registers
