@@ -2170,8 +2170,8 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
     __shared__ __align__(16) T s_v[RMAX];    // its reflector; zero above row i and past the last row
     __shared__ __align__(16) T s_m[RMAX];    // 1 for rows i + 1 .. rows - 1, else 0
     __shared__ T s_tm[SLAB], s_rowi[SLAB];
-    __shared__ int s_upd[SLAB], s_pos[SLAB];
-    __shared__ int s_p, s_ca, s_cb, s_best, s_ok, s_ok_col;
+    __shared__ int s_upd[SLAB], s_pos[SLAB], s_rc[SLAB];
+    __shared__ int s_p, s_ca, s_cb, s_best, s_ok, s_ok_col, s_anyrc;
     const RrqrSlab sl = slabs[blockIdx.x];
     const RrqrBig B = bigs[sl.big];
     const int tid = threadIdx.x, lane = tid % SLAB;
@@ -2337,6 +2337,22 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
         __syncthreads();
         const int pos = s_pos[lane];
         const bool active = jin && pos > i;
+        // a_j += tm_j v (v is 1 on row i and 0 above it) on this thread's rows of its column
+        auto update_column = [&](T tmj) {
+#pragma unroll
+            for (int q = 0; q < RW; q++) {
+                if (NCH * q + CPW <= iw || NCH * q >= rw) continue;
+                T v[CPW];
+                lds_row_group<T, CPW>(s_v + NCH * q + r0, v);
+#pragma unroll
+                for (int k = 0; k < CPW; k++) a[k][q] = fma(v[k], tmj, a[k][q]);
+            }
+        };
+        // this column's update of the step, applied once: to the columns whose norm must be
+        // recomputed and to the slab's candidate before the candidate is published, to the
+        // others after it (off the path the other slabs wait on)
+        T tm = T(0);
+        bool pending = false;
         if (taud != 0.0) {  // w_j = v^T a_j, chain (r - i) % 16
             T c[CPW];
 #pragma unroll
@@ -2360,20 +2376,11 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
                 s_upd[lane] = w != T(0);  // dger skips columns with w_j = 0
             }
             __syncthreads();
-            // a_j += tm v (v is 1 on row i and 0 above it)
-            if (active && s_upd[lane]) {
-                const T tm = s_tm[lane];
-#pragma unroll
-                for (int q = 0; q < RW; q++) {
-                    if (NCH * q + CPW <= iw || NCH * q >= rw) continue;
-                    T v[CPW];
-                    lds_row_group<T, CPW>(s_v + NCH * q + r0, v);
-#pragma unroll
-                    for (int k = 0; k < CPW; k++) a[k][q] = fma(v[k], tm, a[k][q]);
-                }
-            }
+            pending = active && s_upd[lane];
+            tm = pending ? s_tm[lane] : T(0);
         }
-        // row i of every column, for the norm downdate
+        // row i of every column after the update (its new value is a_ij + tm: v_i = 1), for
+        // the norm downdate
         if (iw >= 0 && iw / NCH < RW && iw % NCH < CPW) {
             const int kq = iw % NCH, qq = iw / NCH;
             T x = T(0);
@@ -2382,58 +2389,77 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
 #pragma unroll
                 for (int k = 0; k < CPW; k++)
                     if (k == kq && q == qq) x = a[k][q];
-            s_rowi[lane] = x;
+            s_rowi[lane] = pending ? x + tm : x;
         }
-        // chains (r - i - 1) % 16 of the sum of squares over rows > i
-        double c2[CPW];
-#pragma unroll
-        for (int k = 0; k < CPW; k++) c2[k] = 0.0;
-        if (active) {
-#pragma unroll
-            for (int q = 0; q < RW; q++) {
-                if (NCH * q + CPW <= iw || NCH * q >= rw) continue;
-                if (NCH * q <= iw) {  // the group holding row i: rows <= i masked out
-                    T m[CPW];
-                    lds_row_group<T, CPW>(s_m + NCH * q + r0, m);
-#pragma unroll
-                    for (int k = 0; k < CPW; k++) {
-                        const double d = (double)(a[k][q] * m[k]);
-                        c2[k] = fma(d, d, c2[k]);
+        __syncthreads();
+        // downdate (wavefront 0): the columns whose downdated norm has lost its accuracy are
+        // flagged for the recomputation over rows > i of the updated column
+        if (wv == 0) {
+            int rc = 0;
+            if (active && vn1 != 0.0) {
+                const double qv = fabs((double)s_rowi[lane]) / vn1;
+                double temp = 1.0 - qv * qv;
+                temp = fmax(temp, 0.0);
+                const double rr = vn1 / vn2;
+                const double temp2 = temp * rr * rr;
+                if (temp2 <= tol3z) {
+                    if (i < rows - 1) {
+                        rc = 1;
+                    } else {
+                        vn1 = 0.0;
+                        vn2 = 0.0;
                     }
-                } else {  // rows > i (rows past the last hold 0)
+                } else {
+                    vn1 = vn1 * sqrt(temp);
+                }
+            }
+            s_rc[lane] = rc;
+            const int any = __any(rc);
+            if (lane == 0) s_anyrc = any;
+        }
+        __syncthreads();
+        if (s_anyrc) {  // uniform: the flagged columns updated now, their sums of squares in chains (r - i - 1) % 16
+            const bool rc = s_rc[lane] != 0;
+            if (rc && pending) {
+                update_column(tm);
+                pending = false;
+            }
+            double c2[CPW];
 #pragma unroll
-                    for (int k = 0; k < CPW; k++) {
-                        const double d = (double)a[k][q];
-                        c2[k] = fma(d, d, c2[k]);
+            for (int k = 0; k < CPW; k++) c2[k] = 0.0;
+            if (rc) {
+#pragma unroll
+                for (int q = 0; q < RW; q++) {
+                    if (NCH * q + CPW <= iw || NCH * q >= rw) continue;
+                    if (NCH * q <= iw) {  // the group holding row i: rows <= i masked out
+                        T m[CPW];
+                        lds_row_group<T, CPW>(s_m + NCH * q + r0, m);
+#pragma unroll
+                        for (int k = 0; k < CPW; k++) {
+                            const double d = (double)(a[k][q] * m[k]);
+                            c2[k] = fma(d, d, c2[k]);
+                        }
+                    } else {  // rows > i (rows past the last hold 0)
+#pragma unroll
+                        for (int k = 0; k < CPW; k++) {
+                            const double d = (double)a[k][q];
+                            c2[k] = fma(d, d, c2[k]);
+                        }
                     }
                 }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < CPW; k++) s_part[(r0 + k - i % NCH - 1 + 2 * NCH) % NCH][lane] = c2[k];
-        __syncthreads();
-        if (wv == 0) {
+            for (int k = 0; k < CPW; k++) s_part[(r0 + k - i % NCH - 1 + 2 * NCH) % NCH][lane] = c2[k];
+            __syncthreads();
+            if (wv == 0 && rc) {
+                vn1 = sqrt(tree16_lds(&s_part[0][lane], SLAB));
+                vn2 = vn1;
+            }
+        }
+        if (wv == 0) {  // the slab's best column: the candidate for step i + 1
             double best = -1.0;
             int bpos = INT32_MAX;
             if (active) {
-                if (vn1 != 0.0) {
-                    const double qv = fabs((double)s_rowi[lane]) / vn1;
-                    double temp = 1.0 - qv * qv;
-                    temp = fmax(temp, 0.0);
-                    const double rr = vn1 / vn2;
-                    const double temp2 = temp * rr * rr;
-                    if (temp2 <= tol3z) {
-                        if (i < rows - 1) {
-                            vn1 = sqrt(tree16_lds(&s_part[0][lane], SLAB));
-                            vn2 = vn1;
-                        } else {
-                            vn1 = 0.0;
-                            vn2 = 0.0;
-                        }
-                    } else {
-                        vn1 = vn1 * sqrt(temp);
-                    }
-                }
                 best = vn1;
                 bpos = pos;
             }
@@ -2444,10 +2470,14 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
             }
         }
         __syncthreads();
-        // the slab's candidate column for step i + 1: its rows > i go through LDS (s_col is
-        // free until the next step's read), then every thread stores one or two of them
+        // the candidate column, updated, goes through LDS (s_col is free until the next
+        // step's read), then every thread stores one or two of its rows > i
         const int best = s_best;
         if (best >= 0 && lane == best % SLAB) {
+            if (pending) {
+                update_column(tm);
+                pending = false;
+            }
 #pragma unroll
             for (int q = 0; q < RW; q++) {
                 if (NCH * q + CPW <= iw + 1 || NCH * q >= rw) continue;
@@ -2462,6 +2492,7 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
             GRelayWord* dst = relay_col<T>(B, nxt, sl.slab);
             for (int r = i + 1 + tid; r < rows; r += NTH) relay_put_val<T>(dst + (size_t)r * (sizeof(T) / 4), tag + 1u, s_col[r]);
         }
+        if (pending) update_column(tm);  // the other columns' update, after the publication
         // the pivot column is done (never active again): its owner puts the reflector in its
         // registers, off the path the other slabs wait on (s_v, s_m and sh hold until the
         // next step's dlarfg)
