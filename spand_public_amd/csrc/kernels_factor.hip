@@ -64,7 +64,8 @@ constexpr int PB = 32;
 
 // Rows i >= kb + nb of panel kb, one row per thread: column k is scaled by 1/L(k,k), then
 // the later panel columns receive fma(-a_k, L(k2,k), a_k2), in k order: the element
-// recurrence of the unblocked right-looking factorization.
+// recurrence of the unblocked right-looking factorization. D has 2 PB rows, the last PB of
+// them zero.
 template <typename T, typename P>
 __device__ inline void panel_rows(P A, int n, int kb, int nb, const T (*D)[PB + 1], const T* rdiag, int i0, int i1) {
 #pragma clang fp contract(off)
@@ -73,8 +74,25 @@ __device__ inline void panel_rows(P A, int n, int kb, int nb, const T (*D)[PB + 
         T x[PB];
 #pragma unroll
         for (int k = 0; k < PB; k++) x[k] = k < nb ? Ai[(size_t)k * n] : T(0);
-        // a rolled loop over the columns: x is indexed by the unrolled q / k2 only (unrolled
-        // over k, the block's reads would all be scheduled ahead and spill)
+        if (nb == PB) {
+            // a full panel (every panel with rows below it is one): column k is always x[0],
+            // and each step shifts the row down by one register (x[q - 1] takes the updated
+            // x[q]), so every register index is a constant while the loop over the columns
+            // stays rolled. Column k + q reads D[k + q][k]; for k + q >= PB that is a zero
+            // pad row of D, and the slot it updates is never stored.
+#pragma clang loop unroll(disable)
+            for (int k = 0; k < PB; k++) {
+                const T a = x[0] * rdiag[k];
+                Ai[(size_t)k * n] = a;
+                const T* Dk = &D[k][k];  // D[k + q][k] = Dk[q * (PB + 1)]
+#pragma unroll
+                for (int q = 1; q < PB; q++) x[q - 1] = fma(-a, Dk[q * (PB + 1)], x[q]);
+                x[PB - 1] = T(0);
+            }
+            continue;
+        }
+        // a partial panel: a rolled loop over the columns, x indexed by the unrolled q / k2
+        // only
 #pragma clang loop unroll(disable)
         for (int k = 0; k < nb; k++) {
             const T r = rdiag[k];
@@ -165,7 +183,7 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
 #pragma clang fp contract(off)
     static_assert(POTRF_SMALL_MAX <= 128 + PB, "panel staging holds n - 32 rows");
     __shared__ T P[128][PB + 1];  // rows below the diagonal block of the current panel
-    __shared__ T D[PB][PB + 1];
+    __shared__ T D[2 * PB][PB + 1];  // the diagonal block, then panel_rows' zero pad
     __shared__ T rdiag[PB];
     __shared__ int bad;
     const PotrfTask t = tasks[blockIdx.x];
@@ -174,6 +192,7 @@ __global__ void __launch_bounds__(NT) k_potrf(const PotrfTask* tasks, int* info)
     if (n == 0) return;
     const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
     if (tid == 0) bad = 0;
+    for (int idx = tid; idx < PB * (PB + 1); idx += NT) D[PB + idx / (PB + 1)][idx % (PB + 1)] = T(0);
     for (int kb = 0; kb < n; kb += PB) {
         const int nb = min(PB, n - kb);
         for (int idx = tid; idx < nb * nb; idx += NT) {
@@ -288,7 +307,7 @@ __global__ void __launch_bounds__(64) k_potrf64(const PotrfTask* tasks, int* inf
 template <typename T>
 __global__ void __launch_bounds__(NT) k_potrf_panel(const PotrfTask* tasks, int kb, int* info) {
 #pragma clang fp contract(off)
-    __shared__ T D[PB][PB + 1];
+    __shared__ T D[2 * PB][PB + 1];  // the diagonal block, then panel_rows' zero pad
     __shared__ T rdiag[PB];
     __shared__ int bad;
     const PotrfTask t = tasks[blockIdx.y];
@@ -300,6 +319,7 @@ __global__ void __launch_bounds__(NT) k_potrf_panel(const PotrfTask* tasks, int 
     if (blockIdx.x > 0 && r0 >= n) return;
     const gptr<T> A = gp(reinterpret_cast<T*>(t.A));
     if (tid == 0) bad = 0;
+    for (int idx = tid; idx < PB * (PB + 1); idx += NT) D[PB + idx / (PB + 1)][idx % (PB + 1)] = T(0);
     for (int idx = tid; idx < nb * nb; idx += NT) {
         int i = idx % nb, k = idx / nb;
         D[i][k] = A[(kb + i) + (size_t)(kb + k) * n];
