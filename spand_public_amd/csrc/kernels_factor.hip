@@ -2569,9 +2569,9 @@ __global__ void __launch_bounds__(SLAB * NW, (wave_occupancy<T, RW, NW>()))
     }
 }
 
-// One workgroup per slab: the owner of the last pivot column stores its reflector (when
-// the QR ran to min(rows, cols)), every slab copies the reflectors of its own columns into
-// Q and scatters R[:rank, j] of its columns into the shrunk edge blocks.
+// One workgroup per slab: every slab copies the reflectors held in its own columns into Q
+// (persistent path: a column at pivot position pos < rank holds reflector pos) and scatters
+// R[:rank, j] of its columns into the shrunk edge blocks.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const RrqrSlab* slabs, const RrqrSeg* segs,
                                                     int* dsize, double tol) {
@@ -2592,8 +2592,7 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
     }
     const int mc = B.t.max_cols;
     const int fin = stop & 1;  // maps after `stop` steps
-    const int* p2c = B.map + fin * mc;
-    const int* c2p = B.map + 2 * mc + fin * mc;
+    const int* c2p = B.map + 2 * mc + fin * mc;  // column -> pivot position
     const gptr<T> A = gp(reinterpret_cast<T*>(B.t.scratch));
     __syncthreads();
     const int fixed = rank_fixed(tol, mn);
@@ -2603,13 +2602,14 @@ __global__ void __launch_bounds__(NT) k_rrqr_finish(const RrqrBig* bigs, const R
     // the wave kernel leaves the reflectors in the pivot columns of A; the per-step path has
     // stored them in Q already, and the pivot columns' R entries come from there
     const bool in_a = B.dims[4] == 1;
-    if (in_a)
-        for (int k = 0; k < rank; k++) {
-            const int col = p2c[k];
-            if (col / SLAB != sl.slab) continue;
-            for (int r = tid; r < rows; r += NT) Q[r + (size_t)k * rows] = A[slab_idx(rows, r, col)];
-        }
     const int j0 = sl.slab * SLAB;
+    if (in_a)  // column j of this slab at pivot position pos < rank holds reflector pos
+        for (int idx = tid; idx < rows * SLAB; idx += NT) {
+            const int r = idx % rows, j = j0 + idx / rows;
+            if (j >= cols) break;  // j grows with idx
+            const int pos = c2p[j];
+            if (pos < rank) Q[r + (size_t)pos * rows] = A[slab_idx(rows, r, j)];
+        }
     for (int idx = tid; idx < rank * SLAB; idx += NT) {
         const int j = j0 + idx % SLAB, r = idx / SLAB;
         if (j >= cols) continue;
