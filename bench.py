@@ -56,10 +56,10 @@ FLOP_BOUND = {"potrf", "trsm", "schur"}
 # file the run reports traffic null until the counters are measured again
 # (profiles/rrqr_wave_laplace64_hbm_traffic.md gives the commands).
 TRAFFIC_MEASURED = {
-    ("rrqr_wave", "f32", 64, 12): {"fetch_kib": 16298.56, "write_kib": 29219.79, "fetch_factor": 2.0,
-                                   "write_factor": 1.0, "src_sha16": "8941f3164e29af74"},
-    ("rrqr_wave", "f64", 64, 12): {"fetch_kib": 22385.45, "write_kib": 67373.64, "fetch_factor": 2.0,
-                                   "write_factor": 1.0, "src_sha16": "8941f3164e29af74"},
+    ("rrqr_wave", "f32", 64, 12): {"fetch_kib": 16302.47, "write_kib": 29219.79, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "0c433369a10ce38a"},
+    ("rrqr_wave", "f64", 64, 12): {"fetch_kib": 22373.32, "write_kib": 67373.68, "fetch_factor": 2.0,
+                                   "write_factor": 1.0, "src_sha16": "0c433369a10ce38a"},
 }
 
 
